@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit rod-lattice step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A "step" is one LeapFrog explicit step (element Gauss-point loop, material
+update, nodal assembly, central-difference SO(3) update[, contact]) over the
+whole lattice, with all state resident in HBM.  The metric is element-GP
+updates per second = elements x steps / wall time of the timed region.
+
+Prints one JSON line with the metric, a roofline object for the dominant
+kernel (k_strut; achieved = algorithmic bytes per launch / average launch
+duration from HIP events on the library's stream) and the CPU baseline (the
+oracle's C restatement timed on a bounded sample on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
+# algorithmic bytes (per element-GP update / per node), FP64, each array once per step
+B_ELEM = 136.0              # conn 8 + LambdaN 64 + eps0 48 + L0 8 + section 8
+B_ELEM_PLASTIC = 208.0      # eps_p, alpha(6) read+write + 16
+B_NODE = 248.0              # read x, theta, v, M^-1; write x, theta, v
+
+
+def workload(name):
+    from dynlattice_amd import lattice as L
+    if name == "bcc10-elastic":
+        return L.bcc_c2()
+    if name == "bcc40-elastoplastic-contact":
+        return L.bcc_c3(40, contact=True)
+    if name == "bcc40-elastoplastic":
+        return L.bcc_c3(40, contact=False)
+    if name == "bcc20-elastic-contact":
+        return L.bcc_c5(20)
+    if name.startswith("bcc") and name.endswith("-elastic"):
+        n = int(name[3:-8])
+        from dynlattice_amd.lattice import bcc_lattice, crush_bcs, normal, stable_dt
+        R = 8 * n ** 3
+        p = bcc_lattice(n, 4, 10e-3, 0.5e-3 * (1.0 + 0.05 * normal(2, R)), 2e9, 0.3, 1200.0)
+        crush_bcs(p, -5.0)
+        p.dt = stable_dt(p, 0.5)
+        p.name = name
+        return p
+    raise SystemExit(f"unknown workload {name}")
+
+
+def private_node_count(p):
+    """nodes held by exactly one strut-chain position (updated inside k_strut)"""
+    # chain positions: element starts + last node of every strut
+    last = np.r_[p.elem_strut[1:] != p.elem_strut[:-1], True]
+    cnt = np.bincount(p.elem_nodes[:, 0], minlength=p.n_nodes) + \
+        np.bincount(p.elem_nodes[last, 1], minlength=p.n_nodes)
+    return int((cnt == 1).sum())
+
+
+def cpu_baseline(p, budget_s, steps_gpu):
+    """The oracle (single-threaded C restatement) on this host, bounded sample:
+    the same lattice, K steps with K chosen so the timed part takes ~budget_s."""
+    from oracle.ref_cpu import RefModel
+    ref = RefModel(p, grid=True)
+    t0 = time.perf_counter()
+    ref.leapfrog(1)
+    t1 = time.perf_counter() - t0
+    k = int(max(1, min(steps_gpu, budget_s / max(t1, 1e-9))))
+    t0 = time.perf_counter()
+    ref.leapfrog(k)
+    dt = time.perf_counter() - t0
+    return {"value": p.n_elems * k / dt, "unit": "element-GP updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (oracle/ref_cpu.c, gcc -O3 -ffp-contract=off, 1 thread) on the "
+                      f"same {p.name} lattice, {k} LeapFrog steps after 1 warm-up step, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="bcc10-elastic")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="skip the per-kernel event timing pass")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
+
+    p = workload(args.workload)
+    print(f"bench: {p.name}: {p.n_elems} elements, {p.n_nodes} nodes, {p.n_struts} struts, "
+          f"dt {p.dt:.3e}", file=sys.stderr, flush=True)
+    flags = DL_TRACK_DISSIPATION if p.plastic is not None else 0
+    g = GpuModel(p, device=0, flags=flags)
+    if "bottom" in p.groups:
+        g.set_reaction_group(p.groups["bottom"], 2)
+    g.step(args.warmup, out=False)
+    g.sync()
+    t0 = time.perf_counter()
+    o = g.step(args.steps, out=False)
+    g.sync()
+    wall = time.perf_counter() - t0
+    o = g.step(0)  # error check only
+    value = p.n_elems * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+
+    roof = None
+    phases = {}
+    if not args.no_timing:
+        g.set_timing(True)
+        g.step(args.steps, out=False)
+        g.sync()
+        for ph in ("step", "contact", "strut", "joint"):
+            ms, cnt = g.phase_time(ph)
+            phases[ph] = ms / max(cnt, 1)
+        g.set_timing(False)
+        npriv = private_node_count(p)
+        b_elem = B_ELEM + (B_ELEM_PLASTIC if p.plastic is not None else 0.0)
+        bytes_strut = b_elem * p.n_elems + B_NODE * npriv
+        t_strut = phases["strut"] * 1e-3
+        achieved = bytes_strut / t_strut / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_strut", "algorithmic_bytes_per_launch": bytes_strut,
+                "avg_launch_ms": round(phases["strut"], 5)}
+
+    res = {
+        "metric": "element-GP updates/s per explicit step (FP64)",
+        "value": value,
+        "unit": "element-GP updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded BCC lattice generator, random per-strut radii)",
+        "config": {"workload": p.name, "elements": p.n_elems, "nodes": p.n_nodes,
+                   "struts": p.n_struts, "dt": p.dt,
+                   "material": "elastoplastic" if p.plastic is not None else "elastic",
+                   "contact": p.contact is not None, "so3": bool(p.so3)},
+        "roofline": roof,
+        "phase_ms": {k: round(v, 5) for k, v in phases.items()},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(p, args.cpu_budget, args.steps)
+        res["cpu_baseline"] = cb
+        res["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,926 @@
+// dl_api.cpp -- host side of the C ABI (include/dl_gpu.h).
+//
+// dl_create plays the role of the reference's model-tree construction and
+// INIT-time set-up: strut chains, per-(strut,node) reference frames
+// (initRotation_, SpecialCosseratRodModel.cpp:574-658), material tables
+// (ElasticRodMaterial.cpp:265-287), block-lumped mass (assembleM_,
+// SpecialCosseratRodModel.cpp:969-1031, split into a constant per-node part:
+// J_node(theta) = R(theta) J0 R(theta)^T), and the shared-node gather map.
+// Everything is uploaded once; the step runs on resident device state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dl_gpu.h"
+#include "dl_contact.h"
+#include "dl_internal.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t _e = (x);                                                                 \
+    if (_e != hipSuccess)                                                                \
+      return fail(DL_ERR_HIP, std::string("HIP: ") + hipGetErrorString(_e) + " at " #x); \
+  } while (0)
+
+extern "C" const char *dl_last_error(void) { return g_err.c_str(); }
+
+// boundary data kept on the host for re-uploads (constraints, loads, reaction group)
+struct BcHost {
+  std::vector<uint8_t> cmask;
+  std::vector<double> cval, fext;
+  std::vector<int32_t> react;
+};
+
+struct dl_ctx {
+  BcHost bch;
+  std::vector<double> ediss_acc_h; // per-element dissipation of the action path (energyDiss_)
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool so3 = true;
+  bool plastic = false;
+  bool contact = false;
+  bool initialized = false;
+  bool track_diss = false;
+  uint32_t flags = 0;
+  PlasticParams pp{};
+  DevData d{};
+  std::vector<void *> allocs;
+  // host copies needed by actions
+  std::vector<double> w_e;        // Gauss weight (L0) per element
+  std::vector<int32_t> elem_perm; // identity (elements are stored in input order)
+  // reaction group
+  int64_t n_react = 0;
+  // boundary records (host) for re-upload
+  std::vector<int32_t> bc_host;
+  // device buffers for history
+  double *hist_react = nullptr, *hist_diss = nullptr;
+  int64_t *hist_nc = nullptr, *hist_ncand = nullptr;
+  int64_t hist_cap = 0, hist_n = 0;
+  double *diss_total = nullptr;
+  int32_t strut_blocks = 0;
+  // action path energy accumulators
+  double *ediss_acc = nullptr, *epot_e = nullptr, *ediss_inc = nullptr;
+  // timing
+  bool timing = false;
+  double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0;
+  int64_t t_count = 0;
+  // contact
+  ContactState cs{};
+};
+
+template <class T>
+static int dalloc(dl_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+  c->allocs.push_back((void *)*p);
+  return DL_OK;
+}
+
+template <class T>
+static int upload(dl_ctx *c, T **p, const std::vector<T> &h) {
+  int rc = dalloc(c, p, h.size());
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DL_OK;
+}
+
+// ------------------------------------------------------------------
+// host math for set-up (row-major 3x3)
+// ------------------------------------------------------------------
+static void mm3(const double A[9], const double B[9], double C[9]) {
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = A[3 * i] * B[j];
+      s += A[3 * i + 1] * B[3 + j];
+      s += A[3 * i + 2] * B[6 + j];
+      T[3 * i + j] = s;
+    }
+  std::memcpy(C, T, sizeof T);
+}
+
+// rotation matrix -> unit quaternion (Shepperd's method)
+static Quat mat2quat(const double R[9]) {
+  const double tr = R[0] + R[4] + R[8];
+  Quat q;
+  if (tr >= R[0] && tr >= R[4] && tr >= R[8]) {
+    const double s = std::sqrt(1.0 + tr) * 2.0;
+    q.w = 0.25 * s;
+    q.x = (R[7] - R[5]) / s;
+    q.y = (R[2] - R[6]) / s;
+    q.z = (R[3] - R[1]) / s;
+  } else if (R[0] >= R[4] && R[0] >= R[8]) {
+    const double s = std::sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
+    q.w = (R[7] - R[5]) / s;
+    q.x = 0.25 * s;
+    q.y = (R[1] + R[3]) / s;
+    q.z = (R[2] + R[6]) / s;
+  } else if (R[4] >= R[8]) {
+    const double s = std::sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
+    q.w = (R[2] - R[6]) / s;
+    q.x = (R[1] + R[3]) / s;
+    q.y = 0.25 * s;
+    q.z = (R[5] + R[7]) / s;
+  } else {
+    const double s = std::sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
+    q.w = (R[3] - R[1]) / s;
+    q.x = (R[2] + R[6]) / s;
+    q.y = (R[5] + R[7]) / s;
+    q.z = 0.25 * s;
+  }
+  if (q.w < 0) { q.w = -q.w; q.x = -q.x; q.y = -q.y; q.z = -q.z; }
+  return qnormalize(q);
+}
+
+// initRotation_ frame for a unit tangent d: the rotation taking e3 to d
+// (SpecialCosseratRodModel.cpp:641-655), -I for the 180 degree turn.
+static void frame_from_tangent(const double d[3], double Rm[9]) {
+  const double v[3] = {-1 * d[1], 1 * d[0], 0.0};
+  const double cc = d[0] * 0. + d[1] * 0. + d[2] * 1.;
+  for (int i = 0; i < 9; i++) Rm[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  if (std::fabs(cc + 1.0) > 2.220446049250313e-16) {
+    const double Sv[9] = {0.0, -1 * v[2], 1 * v[1], 1 * v[2], 0.0, -1 * v[0], -1 * v[1], 1 * v[0], 0.0};
+    double SS[9];
+    mm3(Sv, Sv, SS);
+    const double f = 1 / (1 + cc);
+    for (int i = 0; i < 9; i++) Rm[i] += Sv[i] + f * SS[i];
+  } else {
+    for (int i = 0; i < 9; i++) Rm[i] *= -1.;
+  }
+}
+
+// ------------------------------------------------------------------
+// dl_create
+// ------------------------------------------------------------------
+extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_material *mat,
+                         const dl_contact *contact, const dl_options *opt, dl_ctx **out) {
+  if (!out) return fail(DL_ERR_INPUT, "out is NULL");
+  *out = nullptr;
+  if (!mesh || !sec) return fail(DL_ERR_INPUT, "mesh and sections are required");
+  const int64_t N = mesh->n_nodes, E = mesh->n_elems;
+  const int32_t R = mesh->n_struts;
+  if (N <= 0 || E <= 0 || R <= 0) return fail(DL_ERR_INPUT, "empty mesh");
+  if (N >= INT32_MAX || E + R >= INT32_MAX) return fail(DL_ERR_INPUT, "mesh too large for int32 indices");
+  const int32_t *en = mesh->elem_nodes, *es = mesh->elem_strut;
+  for (int64_t e = 0; e < E; e++) {
+    if (en[2 * e] < 0 || en[2 * e] >= N || en[2 * e + 1] < 0 || en[2 * e + 1] >= N)
+      return fail(DL_ERR_INPUT, "element node index out of range at element " + std::to_string(e));
+    if (en[2 * e] == en[2 * e + 1]) return fail(DL_ERR_INPUT, "degenerate element " + std::to_string(e));
+    if (es[e] < 0 || es[e] >= R) return fail(DL_ERR_INPUT, "element strut out of range");
+    if (e > 0 && es[e] < es[e - 1])
+      return fail(DL_ERR_INPUT, "elements of a strut must be contiguous (elem_strut non-decreasing)");
+    if (e > 0 && es[e] == es[e - 1] && en[2 * e] != en[2 * e - 1])
+      return fail(DL_ERR_INPUT, "strut elements must form a chain at element " + std::to_string(e));
+  }
+  std::vector<int64_t> eptr(R + 1, 0);
+  for (int64_t e = 0; e < E; e++) eptr[es[e] + 1]++;
+  for (int32_t s = 0; s < R; s++) {
+    if (eptr[s + 1] == 0) return fail(DL_ERR_INPUT, "strut without elements: " + std::to_string(s));
+    eptr[s + 1] += eptr[s];
+  }
+  const bool plastic = mat && mat->model == DL_ELASTOPLASTIC;
+  if (plastic && !mat->weights) return fail(DL_ERR_INPUT, "plastic material needs yield weights");
+  if (plastic)
+    for (int i = 0; i < 6; i++)
+      if (!(mat->exponent[i] >= 1.0)) return fail(DL_ERR_INPUT, "yield exponents must be >= 1");
+  const bool so3 = opt ? opt->so3 != 0 : true;
+
+  dl_ctx *c = new dl_ctx();
+  c->so3 = so3;
+  c->plastic = plastic;
+  c->flags = opt ? opt->flags : 0u;
+  c->track_diss = plastic && (c->flags & DL_TRACK_DISSIPATION);
+  c->device = opt ? opt->device : 0;
+  int rc = DL_OK;
+#define TRY(x)                 \
+  do {                         \
+    rc = (x);                  \
+    if (rc) {                  \
+      dl_destroy(c);           \
+      return rc;               \
+    }                          \
+  } while (0)
+  {
+    hipError_t he = hipSetDevice(c->device);
+    if (he != hipSuccess) {
+      delete c;
+      return fail(DL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(he));
+    }
+  }
+  if (opt && opt->stream) {
+    c->stream = (hipStream_t)opt->stream;
+  } else {
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+      delete c;
+      return fail(DL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    c->own_stream = true;
+  }
+
+  // chains
+  std::vector<int32_t> cnode(E + R);
+  for (int32_t s = 0; s < R; s++) {
+    const int64_t c0 = eptr[s] + s;
+    cnode[c0] = en[2 * eptr[s]];
+    for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) cnode[c0 + (e - eptr[s]) + 1] = en[2 * e + 1];
+  }
+  // node occurrences and gyro multiplicity (distinct struts per node)
+  std::vector<int32_t> occ(N, 0), nstr(N, 0), last(N, -1);
+  for (int32_t s = 0; s < R; s++)
+    for (int64_t p = eptr[s] + s; p <= eptr[s + 1] + s; p++) {
+      const int32_t n = cnode[p];
+      occ[n]++;
+      if (last[n] != s) { nstr[n]++; last[n] = s; }
+    }
+  for (int64_t n = 0; n < N; n++)
+    if (occ[n] == 0) {
+      dl_destroy(c);
+      return fail(DL_ERR_ZERO_MASS, "node " + std::to_string(n) + " belongs to no element (zero mass)");
+    }
+  // slots for shared nodes, in (strut, chain position) order
+  std::vector<int32_t> cslot(E + R, -1);
+  int32_t nslot = 0;
+  for (int64_t p = 0; p < E + R; p++)
+    if (occ[cnode[p]] > 1) cslot[p] = nslot++;
+  std::vector<int32_t> jidx(N, -1), jnode;
+  for (int64_t n = 0; n < N; n++)
+    if (occ[n] > 1) { jidx[n] = (int32_t)jnode.size(); jnode.push_back((int32_t)n); }
+  const int32_t NJ = (int32_t)jnode.size();
+  std::vector<int32_t> jptr(NJ + 1, 0), jslot(nslot);
+  for (int64_t p = 0; p < E + R; p++)
+    if (cslot[p] >= 0) jptr[jidx[cnode[p]] + 1]++;
+  for (int32_t j = 0; j < NJ; j++) jptr[j + 1] += jptr[j];
+  {
+    std::vector<int32_t> fill(jptr.begin(), jptr.end() - 1);
+    for (int64_t p = 0; p < E + R; p++)
+      if (cslot[p] >= 0) jslot[fill[jidx[cnode[p]]]++] = cslot[p];
+  }
+
+  // frames per (strut, node) -> per (element, end) quaternions; masses
+  const double *X = mesh->coords;
+  std::vector<double> qN(8 * E), mt(N, 0.0), J0(6 * N, 0.0), Cd(6 * R), wE(E);
+  std::vector<double> pw, Hk;
+  if (plastic) {
+    pw.resize(6 * R);
+    Hk.resize(6 * R);
+  }
+  const double xi[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
+  std::vector<double> dirs, lam;
+  std::vector<int32_t> lnode;
+  for (int32_t s = 0; s < R; s++) {
+    const double G = sec->shear_modulus[s], Y = sec->young[s], kap = sec->shear_factor[s];
+    const double A = sec->area[s], I1 = sec->I1[s], I2 = sec->I2[s], Jp = sec->J[s], rho = sec->density[s];
+    if (!(A > 0) || !(Y > 0) || !(G > 0) || !(I1 > 0) || !(I2 > 0) || !(Jp > 0) || rho < 0) {
+      dl_destroy(c);
+      return fail(DL_ERR_INPUT, "invalid section/material data for strut " + std::to_string(s));
+    }
+    double *Cs = &Cd[0];
+    Cs[0 * R + s] = G * kap * A;
+    Cs[1 * R + s] = G * kap * A;
+    Cs[2 * R + s] = Y * A;
+    Cs[3 * R + s] = Y * I1;
+    Cs[4 * R + s] = Y * I2;
+    Cs[5 * R + s] = G * Jp;
+    if (plastic)
+      for (int i = 0; i < 6; i++) {
+        pw[i * R + s] = mat->weights[6 * s + i];
+        Hk[i * R + s] = mat->kin_const[i] + mat->kin_ratio * Cs[i * R + s];
+      }
+    double Mm[6] = {rho * A, rho * A, rho * A, rho * I1, rho * I2, rho * Jp};
+    if (sec->inertia_correct != 1.0)
+      for (int i = 3; i < 6; i++) Mm[i] *= sec->inertia_correct;
+    // rod nodes: unique chain nodes of this strut
+    const int64_t c0 = eptr[s] + s, c1 = eptr[s + 1] + s;
+    lnode.clear();
+    for (int64_t p = c0; p <= c1; p++)
+      if (std::find(lnode.begin(), lnode.end(), cnode[p]) == lnode.end()) lnode.push_back(cnode[p]);
+    const size_t nl = lnode.size();
+    dirs.assign(3 * nl, 0.0);
+    lam.assign(9 * nl, 0.0);
+    auto lid = [&](int32_t n) { return (size_t)(std::find(lnode.begin(), lnode.end(), n) - lnode.begin()); };
+    for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) {
+      const int32_t a = en[2 * e], b = en[2 * e + 1];
+      const size_t la = lid(a), lb = lid(b);
+      for (int k = 0; k < 3; k++) {
+        const double t = X[3 * b + k] - X[3 * a + k];
+        dirs[3 * la + k] += t;
+        dirs[3 * lb + k] += t;
+      }
+    }
+    for (size_t k = 0; k < nl; k++) {
+      double *dd = &dirs[3 * k];
+      const double nn = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+      dd[0] = dd[0] / nn; dd[1] = dd[1] / nn; dd[2] = dd[2] / nn;
+      frame_from_tangent(dd, &lam[9 * k]);
+    }
+    for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) {
+      const int32_t a = en[2 * e], b = en[2 * e + 1];
+      const double *La = &lam[9 * lid(a)], *Lb = &lam[9 * lid(b)];
+      const Quat qa = mat2quat(La), qb = mat2quat(Lb);
+      qN[0 * E + e] = qa.w; qN[1 * E + e] = qa.x; qN[2 * E + e] = qa.y; qN[3 * E + e] = qa.z;
+      qN[4 * E + e] = qb.w; qN[5 * E + e] = qb.x; qN[6 * E + e] = qb.y; qN[7 * E + e] = qb.z;
+      // lengths, weights (Line3D.cpp:91-115)
+      double dd[3] = {X[3 * b] - X[3 * a], X[3 * b + 1] - X[3 * a + 1], X[3 * b + 2] - X[3 * a + 2]};
+      double ll = dd[0] * dd[0];
+      ll += dd[1] * dd[1];
+      ll += dd[2] * dd[2];
+      const double Lc = std::sqrt(ll) + 0.;
+      const double Jc = -0.5 * 0. + 0.5 * Lc;
+      wE[e] = 2. * Jc;
+      // mass (assembleM_): translational row sum over Gauss2, rotational lumped / 2
+      const double wts = 1. * Jc;
+      const double l = (wts + wts) / (2 - 1);
+      double sh[2][2];
+      for (int ip = 0; ip < 2; ip++) {
+        sh[0][ip] = 0.5 * (1. - xi[ip]);
+        sh[1][ip] = 0.5 * (1. + xi[ip]);
+      }
+      const int32_t nodes2[2] = {a, b};
+      for (int in = 0; in < 2; in++) {
+        double acc = 0.0;
+        for (int jn = 0; jn < 2; jn++)
+          for (int ip = 0; ip < 2; ip++) acc += wts * sh[in][ip] * sh[jn][ip] * Mm[0];
+        mt[nodes2[in]] += acc;
+      }
+      double Mr[3] = {Mm[3] * l, Mm[4] * l, Mm[5] * l};
+      const double add = A * rho * std::pow(l, 3) / 12.;
+      Mr[0] += add;
+      Mr[1] += add;
+      for (int in = 0; in < 2; in++) {
+        const double *L = in == 0 ? La : Lb;
+        // L diag(Mr) L^T / 2
+        double Jm[9];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            Jm[3 * i + j] = (L[3 * i] * Mr[0] * L[3 * j] + L[3 * i + 1] * Mr[1] * L[3 * j + 1] +
+                             L[3 * i + 2] * Mr[2] * L[3 * j + 2]) / 2.;
+        const int32_t n = nodes2[in];
+        J0[0 * N + n] += Jm[0];
+        J0[1 * N + n] += Jm[4];
+        J0[2 * N + n] += Jm[8];
+        J0[3 * N + n] += Jm[1];
+        J0[4 * N + n] += Jm[2];
+        J0[5 * N + n] += Jm[5];
+      }
+    }
+  }
+  // mass checks (ExplicitModule.cpp:452-454): positive mass and inertia
+  for (int64_t n = 0; n < N; n++) {
+    const double a = J0[n], b = J0[N + n], cc = J0[2 * N + n], xy = J0[3 * N + n],
+                 xz = J0[4 * N + n], yz = J0[5 * N + n];
+    const double det = a * (b * cc - yz * yz) - xy * (xy * cc - yz * xz) + xz * (xy * yz - b * xz);
+    if (!(mt[n] > 2.2250738585072014e-308) || !(a > 0) || !(a * b - xy * xy > 0) || !(det > 0)) {
+      dl_destroy(c);
+      return fail(DL_ERR_ZERO_MASS, "Zero (or negative) masses cannot be inversed! (node " +
+                                        std::to_string(n) + ")");
+    }
+  }
+  c->w_e = wE;
+
+  // upload
+  DevData &d = c->d;
+  d.N = N;
+  d.E = E;
+  d.R = R;
+  {
+    std::vector<double> X0(3 * N);
+    for (int64_t n = 0; n < N; n++)
+      for (int k = 0; k < 3; k++) X0[k * N + n] = X[3 * n + k];
+    double *p;
+    TRY(upload(c, &p, X0));
+    d.X0 = p;
+  }
+  {
+    std::vector<double> z3(3 * N, 0.0), z6(6 * N, 0.0);
+    TRY(upload(c, &d.u, z3));
+    TRY(upload(c, &d.v, z6));
+    std::vector<double> r(so3 ? 4 * N : 3 * N, 0.0);
+    if (so3)
+      for (int64_t n = 0; n < N; n++) r[n] = 1.0;
+    TRY(upload(c, &d.rot, r));
+  }
+  {
+    double *p;
+    TRY(upload(c, &p, mt)); d.mt = p;
+    TRY(upload(c, &p, J0)); d.J0 = p;
+    std::vector<double> g(N);
+    for (int64_t n = 0; n < N; n++) g[n] = (double)nstr[n];
+    TRY(upload(c, &p, g)); d.gm = p;
+    TRY(upload(c, &p, Cd)); d.C = p;
+    TRY(upload(c, &p, qN)); d.qN = p;
+    TRY(dalloc(c, &d.eps0, 6 * E));
+    if (plastic) {
+      TRY(upload(c, &p, pw)); d.pw = p;
+      TRY(upload(c, &p, Hk)); d.Hk = p;
+    }
+  }
+  {
+    std::vector<int32_t> bc(N, -1);
+    int32_t *pi;
+    TRY(upload(c, &pi, bc)); d.bc = pi;
+    c->bc_host = bc;
+    d.nbc = 0;
+    d.bc_mask = nullptr; d.bc_a = nullptr; d.bc_f = nullptr; d.bc_rslot = nullptr;
+  }
+  {
+    int64_t *pl;
+    TRY(upload(c, &pl, eptr)); d.eptr = pl;
+    int32_t *pi;
+    TRY(upload(c, &pi, cnode)); d.cnode = pi;
+    TRY(upload(c, &pi, cslot)); d.cslot = pi;
+    d.NJ = NJ;
+    d.nslot = nslot;
+    TRY(upload(c, &pi, jnode)); d.jnode = pi;
+    TRY(upload(c, &pi, jptr)); d.jptr = pi;
+    TRY(upload(c, &pi, jslot)); d.jslot = pi;
+    TRY(dalloc(c, &d.sbuf, 6 * (size_t)std::max<int32_t>(nslot, 1)));
+  }
+  // material state
+  {
+    std::vector<double> z6e(6 * E, 0.0), z7e(7 * E, 0.0);
+    TRY(upload(c, &d.ecur, z6e));
+    TRY(upload(c, &d.eo, z6e));
+    if (plastic) {
+      TRY(upload(c, &d.ep, z6e));
+      TRY(upload(c, &d.ep_cur, z6e));
+      TRY(upload(c, &d.al, z7e));
+      TRY(upload(c, &d.al_cur, z7e));
+      std::vector<double> zE(E, 0.0);
+      TRY(upload(c, &d.dflow, zE));
+      std::vector<uint8_t> zb(E, 0);
+      TRY(upload(c, &d.active, zb));
+      PlasticParams &P = c->pp;
+      P.yield_level = mat->yield_level;
+      for (int i = 0; i < 6; i++) P.kin_coupling[i] = mat->kin_coupling[i];
+      P.iso_coupling = mat->iso_coupling;
+      P.iso_modulus = mat->iso_modulus;
+      P.precision = mat->precision;
+      P.has_iso = mat->has_iso;
+      P.has_kin = mat->has_kin;
+      P.max_iter = mat->max_iter;
+      for (int i = 0; i < 6; i++) P.exponent[i] = mat->exponent[i];
+    }
+  }
+  TRY(dalloc(c, &d.fout, 6 * N));
+  TRY(dalloc(c, &d.err, 1));
+  HIPCHK(hipMemset(d.err, 0, sizeof(int32_t)));
+  c->strut_blocks = (R + 255) / 256;
+  TRY(dalloc(c, &d.dbuf, (size_t)c->strut_blocks));
+  TRY(dalloc(c, &c->diss_total, 1));
+  HIPCHK(hipMemset(c->diss_total, 0, sizeof(double)));
+  d.rbuf = nullptr;
+  d.react_dof = 2;
+  // contact
+  if (contact && contact->enabled) {
+    c->contact = true;
+    TRY(contact_create(c->cs, d, mesh, contact, [&](void **p, size_t bytes) {
+      hipError_t he = hipMalloc(p, bytes ? bytes : 1);
+      if (he != hipSuccess) return fail(DL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(he));
+      c->allocs.push_back(*p);
+      return (int)DL_OK;
+    }));
+    d.fc = c->cs.fc;
+  }
+#undef TRY
+  *out = c;
+  return DL_OK;
+}
+
+extern "C" int dl_destroy(dl_ctx *c) {
+  if (!c) return DL_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return DL_OK;
+}
+
+extern "C" int dl_sync(dl_ctx *c) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return DL_OK;
+}
+
+static int check_err(dl_ctx *c) {
+  int32_t e = 0;
+  HIPCHK(hipMemcpyAsync(&e, c->d.err, sizeof e, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (e == DL_ERR_PLASTIC_ITER) {
+    HIPCHK(hipMemsetAsync(c->d.err, 0, sizeof(int32_t), c->stream));
+    return fail(DL_ERR_PLASTIC_ITER, "Too many iterations in plasticity loop");
+  }
+  if (e) return fail(e, "device error");
+  return DL_OK;
+}
+
+// ------------------------------------------------------------------
+// state and boundary data
+// ------------------------------------------------------------------
+extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, int64_t ndof) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
+  hipSetDevice(c->device);
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc((void **)&tmp, 2 * ndof * sizeof(double)));
+  hipError_t e1 = hipSuccess, e2 = hipSuccess;
+  if (disp) e1 = hipMemcpyAsync(tmp, disp, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (velo) e2 = hipMemcpyAsync(tmp + ndof, velo, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e1 == hipSuccess && e2 == hipSuccess)
+    launch_state_io(c->d, c->so3, false, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
+  hipError_t e3 = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(DL_ERR_HIP, "dl_set_state copy failed");
+  return DL_OK;
+}
+
+extern "C" int dl_get_state(dl_ctx *c, double *disp, double *velo, int64_t ndof) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
+  hipSetDevice(c->device);
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc((void **)&tmp, 2 * ndof * sizeof(double)));
+  launch_state_io(c->d, c->so3, true, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
+  hipError_t e1 = hipSuccess, e2 = hipSuccess;
+  if (disp) e1 = hipMemcpyAsync(disp, tmp, ndof * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (velo) e2 = hipMemcpyAsync(velo, tmp + ndof, ndof * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e3 = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(DL_ERR_HIP, "dl_get_state copy failed");
+  return DL_OK;
+}
+
+// (re)build the boundary record table from host masks; keeps reaction slots
+static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const double *fext,
+                      const std::vector<int32_t> &react_nodes) {
+  const int64_t N = c->d.N;
+  std::vector<int32_t> bc(N, -1), rs;
+  std::vector<uint32_t> mask;
+  std::vector<double> a, f;
+  std::vector<char> inreact(N, 0);
+  for (int32_t n : react_nodes) inreact[n] = 1;
+  int32_t nbc = 0;
+  for (int64_t n = 0; n < N; n++) {
+    bool any = inreact[n];
+    for (int i = 0; i < 6 && !any; i++)
+      any = (cmask && cmask[6 * n + i]) || (fext && fext[6 * n + i] != 0.0) ||
+            (cval && cmask && cmask[6 * n + i] && cval[6 * n + i] != 0.0);
+    if (any) bc[n] = nbc++;
+  }
+  mask.assign(std::max(nbc, 1), 0u);
+  a.assign(6 * (size_t)std::max(nbc, 1), 0.0);
+  f.assign(6 * (size_t)std::max(nbc, 1), 0.0);
+  rs.assign(std::max(nbc, 1), -1);
+  for (int64_t n = 0; n < N; n++) {
+    const int32_t b = bc[n];
+    if (b < 0) continue;
+    uint32_t m = 0;
+    for (int i = 0; i < 6; i++) {
+      if (cmask && cmask[6 * n + i]) {
+        m |= 1u << i;
+        a[(size_t)i * nbc + b] = cval ? cval[6 * n + i] : 0.0;
+      }
+      f[(size_t)i * nbc + b] = fext ? fext[6 * n + i] : 0.0;
+    }
+    mask[b] = m;
+  }
+  for (size_t k = 0; k < react_nodes.size(); k++) rs[bc[react_nodes[k]]] = (int32_t)k;
+  hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy((void *)c->d.bc, bc.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->bc_host = bc;
+  uint32_t *pm;
+  double *pa, *pf;
+  int32_t *pr;
+  int rc;
+  if ((rc = upload(c, &pm, mask))) return rc;
+  if ((rc = upload(c, &pa, a))) return rc;
+  if ((rc = upload(c, &pf, f))) return rc;
+  if ((rc = upload(c, &pr, rs))) return rc;
+  c->d.bc_mask = pm;
+  c->d.bc_a = pa;
+  c->d.bc_f = pf;
+  c->d.bc_rslot = pr;
+  c->d.nbc = nbc;
+  return DL_OK;
+}
+
+static BcHost &bch(dl_ctx *c) { return c->bch; }
+
+extern "C" int dl_set_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const double *fext,
+                          int64_t ndof) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
+  BcHost &h = bch(c);
+  h.cmask.assign(cmask ? cmask : (const uint8_t *)nullptr, cmask ? cmask + ndof : nullptr);
+  h.cval.assign(cval ? cval : (const double *)nullptr, cval ? cval + ndof : nullptr);
+  h.fext.assign(fext ? fext : (const double *)nullptr, fext ? fext + ndof : nullptr);
+  return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
+                    h.fext.empty() ? nullptr : h.fext.data(), h.react);
+}
+
+extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n, int32_t dof) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (dof < 0 || dof > 5) return fail(DL_ERR_INPUT, "dof must be in [0, 5]");
+  BcHost &h = bch(c);
+  h.react.assign(nodes, nodes + n);
+  for (int32_t x : h.react)
+    if (x < 0 || x >= c->d.N) return fail(DL_ERR_INPUT, "reaction node out of range");
+  std::vector<int32_t> srt = h.react;
+  std::sort(srt.begin(), srt.end());
+  if (std::adjacent_find(srt.begin(), srt.end()) != srt.end())
+    return fail(DL_ERR_INPUT, "duplicate reaction node");
+  c->d.react_dof = dof;
+  c->n_react = n;
+  double *rb;
+  int rc = dalloc(c, &rb, (size_t)std::max<int64_t>(n, 1));
+  if (rc) return rc;
+  c->d.rbuf = rb;
+  HIPCHK(hipMemset(rb, 0, sizeof(double) * std::max<int64_t>(n, 1)));
+  return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
+                    h.fext.empty() ? nullptr : h.fext.data(), h.react);
+}
+
+// ------------------------------------------------------------------
+// actions
+// ------------------------------------------------------------------
+extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  hipSetDevice(c->device);
+  const int64_t N = c->d.N, E = c->d.E;
+  switch (action) {
+  case DL_INIT: {
+    launch_init_strain(c->d, c->stream);
+    HIPCHK(hipGetLastError());
+    if (c->contact) {
+      int rc = contact_init_blacklist(c->cs, c->d, c->stream);
+      if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->initialized = true;
+    return DL_OK;
+  }
+  case DL_GET_INT_VECTOR: {
+    if (!c->initialized) return fail(DL_ERR_STATE, "GET_INT_VECTOR before INIT");
+    if (!p || !p->int_vector) return fail(DL_ERR_INPUT, "params.int_vector is required");
+    StepParams sp{0.0, p->load_case != DL_LOAD_CASE_OUTPUT, 0};
+    if (c->contact) {
+      int rc = contact_force(c->cs, c->d, c->stream);
+      if (rc) return rc;
+    }
+    launch_strut_pp(c->d, sp, c->pp, MODE_FORCE, c->so3, c->plastic, false, c->stream);
+    launch_joint(c->d, sp, MODE_FORCE, c->so3, false, c->stream);
+    launch_force_extras(c->d, c->so3, p->with_gyro && c->so3, c->contact, c->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(6 * N);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d.fout, 6 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = check_err(c);
+    if (rc) return rc;
+    for (int64_t n = 0; n < N; n++)
+      for (int i = 0; i < 6; i++) p->int_vector[6 * n + i] += h[i * N + n];
+    return DL_OK;
+  }
+  case DL_GET_MATRIX2: {
+    if (!p || !p->matrix2) return fail(DL_ERR_INPUT, "params.matrix2 is required");
+    double *blk = nullptr;
+    HIPCHK(hipMalloc((void **)&blk, 36 * N * sizeof(double)));
+    launch_mass_blocks(c->d, c->so3, p->lumped != 0, blk, c->stream);
+    std::vector<double> h(36 * N);
+    hipError_t e1 = hipMemcpyAsync(h.data(), blk, 36 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    hipFree(blk);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(DL_ERR_HIP, "GET_MATRIX2 copy failed");
+    for (int64_t i = 0; i < 36 * N; i++) p->matrix2[i] += h[i];
+    return DL_OK;
+  }
+  case DL_COMMIT: {
+    if (!c->epot_e) {
+      int rc;
+      if ((rc = dalloc(c, &c->epot_e, E))) return rc;
+      if ((rc = dalloc(c, &c->ediss_inc, E))) return rc;
+      c->ediss_acc_h.assign(E, 0.0);
+    }
+    launch_commit_pl(c->d, c->plastic, c->epot_e, c->ediss_inc, c->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> ep(E), ed(E);
+    HIPCHK(hipMemcpyAsync(ep.data(), c->epot_e, E * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ed.data(), c->ediss_inc, E * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> &A = c->ediss_acc_h;
+    // E_pot / E_diss = sum_e sum_I w N_I energy_e  (SpecialCosseratRodModel.cpp:1074-1197)
+    double sp = 0.0, sd = 0.0;
+    for (int64_t e = 0; e < E; e++) {
+      A[e] += ed[e];
+      sp += c->w_e[e] * 0.5 * ep[e];
+      sp += c->w_e[e] * 0.5 * ep[e];
+      sd += c->w_e[e] * 0.5 * A[e];
+      sd += c->w_e[e] * 0.5 * A[e];
+    }
+    if (p) {
+      p->e_pot = sp;
+      p->e_diss = sd;
+    }
+    return DL_OK;
+  }
+  case DL_CANCEL: {
+    if (c->plastic) {
+      HIPCHK(hipMemcpyAsync(c->d.ep_cur, c->d.ep, 6 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->d.al_cur, c->d.al, 7 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->d.dflow, 0, E * sizeof(double), c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(c->d.ecur, c->d.eo, 6 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DL_OK;
+  }
+  default:
+    return fail(DL_ERR_INPUT, "unknown action " + std::to_string(action));
+  }
+}
+
+// ------------------------------------------------------------------
+// explicit step
+// ------------------------------------------------------------------
+extern "C" int dl_set_timing(dl_ctx *c, int32_t enabled) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  c->timing = enabled != 0;
+  return DL_OK;
+}
+
+extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *count) {
+  if (!c || !phase) return fail(DL_ERR_INPUT, "NULL argument");
+  const std::string ph(phase);
+  double v;
+  if (ph == "step") v = c->t_step;
+  else if (ph == "contact") v = c->t_contact;
+  else if (ph == "strut") v = c->t_strut;
+  else if (ph == "joint") v = c->t_joint;
+  else return fail(DL_ERR_INPUT, "unknown phase " + ph);
+  if (ms) *ms = v;
+  if (count) *count = c->t_count;
+  return DL_OK;
+}
+
+extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_out *out) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!c->initialized) return fail(DL_ERR_STATE, "explicit step before INIT");
+  if (nsteps < 0 || !(dt > 0.0)) return fail(DL_ERR_INPUT, "nsteps >= 0 and dt > 0 required");
+  hipSetDevice(c->device);
+  if (nsteps > c->hist_cap) {
+    int rc;
+    const int64_t cap = std::max<int64_t>(nsteps, 16);
+    if ((rc = dalloc(c, &c->hist_react, cap))) return rc;
+    if ((rc = dalloc(c, &c->hist_diss, cap))) return rc;
+    if ((rc = dalloc(c, &c->hist_nc, cap))) return rc;
+    if ((rc = dalloc(c, &c->hist_ncand, cap))) return rc;
+    c->hist_cap = cap;
+  }
+  HIPCHK(hipMemsetAsync(c->hist_react, 0, nsteps * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(c->hist_nc, 0, nsteps * sizeof(int64_t), c->stream));
+  HIPCHK(hipMemsetAsync(c->hist_ncand, 0, nsteps * sizeof(int64_t), c->stream));
+  StepParams sp{dt, 1, c->track_diss ? 1 : 0};
+  std::vector<hipEvent_t> ev;
+  if (c->timing) {
+    ev.resize(4 * (size_t)nsteps + 1);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  }
+  for (int64_t st = 0; st < nsteps; st++) {
+    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st], c->stream));
+    if (c->contact) {
+      int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc + st, c->hist_ncand + st);
+      if (rc) return rc;
+    }
+    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 1], c->stream));
+    launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->plastic, c->contact, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 2], c->stream));
+    launch_joint(c->d, sp, MODE_STEP, c->so3, c->contact, c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 3], c->stream));
+    if (c->n_react > 0) launch_reduce_sum(c->d.rbuf, c->n_react, c->hist_react + st, nullptr, c->stream);
+    if (c->track_diss) launch_reduce_sum(c->d.dbuf, c->strut_blocks, c->hist_diss + st, c->diss_total, c->stream);
+  }
+  if (c->timing) HIPCHK(hipEventRecord(ev[4 * nsteps], c->stream));
+  HIPCHK(hipGetLastError());
+  c->hist_n = nsteps;
+  if (c->timing) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->t_step = c->t_contact = c->t_strut = c->t_joint = 0.0;
+    for (int64_t st = 0; st < nsteps; st++) {
+      float a, b, cc, dd;
+      HIPCHK(hipEventElapsedTime(&a, ev[4 * st], ev[4 * st + 1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[4 * st + 1], ev[4 * st + 2]));
+      HIPCHK(hipEventElapsedTime(&cc, ev[4 * st + 2], ev[4 * st + 3]));
+      HIPCHK(hipEventElapsedTime(&dd, ev[4 * st], ev[4 * st + 4 > 4 * nsteps ? 4 * nsteps : 4 * st + 4]));
+      c->t_contact += a;
+      c->t_strut += b;
+      c->t_joint += cc;
+      c->t_step += dd;
+    }
+    c->t_count = nsteps;
+    for (auto &e : ev) hipEventDestroy(e);
+  }
+  if (out) {
+    int rc = check_err(c);
+    if (rc) return rc;
+    out->steps_done = nsteps;
+    out->reaction = 0.0;
+    out->e_diss = 0.0;
+    out->n_contacts = 0;
+    out->n_candidates = 0;
+    if (nsteps > 0) {
+      HIPCHK(hipMemcpy(&out->reaction, c->hist_react + nsteps - 1, sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&out->n_contacts, c->hist_nc + nsteps - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&out->n_candidates, c->hist_ncand + nsteps - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
+    }
+    HIPCHK(hipMemcpy(&out->e_diss, c->diss_total, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return DL_OK;
+}
+
+extern "C" int dl_get_history(dl_ctx *c, double *reaction, double *e_diss, int64_t *n_contacts,
+                              int64_t cap, int64_t *n) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t k = std::min(cap, c->hist_n);
+  if (n) *n = c->hist_n;
+  if (k <= 0) return DL_OK;
+  if (reaction) HIPCHK(hipMemcpy(reaction, c->hist_react, k * sizeof(double), hipMemcpyDeviceToHost));
+  if (e_diss) {
+    if (c->track_diss) HIPCHK(hipMemcpy(e_diss, c->hist_diss, k * sizeof(double), hipMemcpyDeviceToHost));
+    else std::fill(e_diss, e_diss + k, 0.0);
+  }
+  if (n_contacts) HIPCHK(hipMemcpy(n_contacts, c->hist_nc, k * sizeof(int64_t), hipMemcpyDeviceToHost));
+  return DL_OK;
+}
+
+extern "C" int dl_plastic_active(dl_ctx *c, uint8_t *mask, int64_t n) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (n != c->d.E) return fail(DL_ERR_INPUT, "n must be n_elems");
+  if (!c->plastic) {
+    std::fill(mask, mask + n, 0);
+    return DL_OK;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(mask, c->d.active, n, hipMemcpyDeviceToHost));
+  return DL_OK;
+}
+
+extern "C" int dl_plastic_state(dl_ctx *c, double *eps_p, double *alpha, int64_t n_elems) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  const int64_t E = c->d.E;
+  if (n_elems != E) return fail(DL_ERR_INPUT, "n_elems mismatch");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->plastic) {
+    if (eps_p) std::fill(eps_p, eps_p + 6 * E, 0.0);
+    if (alpha) std::fill(alpha, alpha + 7 * E, 0.0);
+    return DL_OK;
+  }
+  std::vector<double> h(7 * E);
+  if (eps_p) {
+    HIPCHK(hipMemcpy(h.data(), c->d.ep, 6 * E * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < E; e++)
+      for (int i = 0; i < 6; i++) eps_p[6 * e + i] = h[i * E + e];
+  }
+  if (alpha) {
+    HIPCHK(hipMemcpy(h.data(), c->d.al, 7 * E * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < E; e++)
+      for (int i = 0; i < 7; i++) alpha[7 * e + i] = h[i * E + e];
+  }
+  return DL_OK;
+}
+
+extern "C" int dl_contact_pairs(dl_ctx *c, int32_t *a, int32_t *b, int64_t cap, int64_t *n) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!c->contact) {
+    if (n) *n = 0;
+    return DL_OK;
+  }
+  return contact_read_pairs(c->cs, c->stream, true, a, b, cap, n);
+}
+
+extern "C" int dl_contact_candidates(dl_ctx *c, int32_t *a, int32_t *b, int64_t cap, int64_t *n) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!c->contact) {
+    if (n) *n = 0;
+    return DL_OK;
+  }
+  return contact_read_pairs(c->cs, c->stream, false, a, b, cap, n);
+}
+
+int dl_set_error(int code, const char *msg) { return fail(code, msg); }

@@ -1,0 +1,300 @@
+// dl_device.h -- per-element / per-node FP64 math of the explicit rod step.
+//
+// Everything here is __host__ __device__ inline so the kernels in
+// dl_kernels.hip and host-side set-up code share one definition.
+//
+// Rotations are unit quaternions (w, x, y, z) with R(q1*q2) = R(q1) R(q2).
+// The element follows the reference's 2-node, 1-Gauss-point Simo-Reissner
+// element (SpecialCosseratRodModel.cpp:694-771,899-949; Line3D.cpp:102-273)
+// with the Crisfield-Jelenic interpolation evaluated in closed form:
+//   Lambda_k   = exp(theta_k) LambdaN_k                         (:794-796)
+//   phi_ab     = log(Lambda_a^T Lambda_b)                       (Line3D.cpp:258)
+//   Lambda_r   = Lambda_a exp(phi_ab / 2)                       (Line3D.cpp:260-262)
+//   psi_a,b    = log(Lambda_r^T Lambda_a,b) = -+ phi_ab / 2      (Line3D.cpp:271-272)
+//   Lambda_ip  = Lambda_r exp(N . psi) = Lambda_r               (psi_ip = 0)
+//   Lambda'    = Lambda_r skew(psi')  (|psi_ip| < TINY branch of expVecP, helpers.cpp:163-166)
+// so Gamma = Lambda_r^T phi' - Gamma0, K = psi' - K0 with psi' = -phi/2 g_a + phi/2 g_b.
+#ifndef DL_DEVICE_H
+#define DL_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define DL_HD __host__ __device__ __forceinline__
+
+struct Quat {
+  double w, x, y, z;
+};
+
+DL_HD Quat qmul(const Quat a, const Quat b) {
+  Quat r;
+  r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+  r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+  r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+  r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+  return r;
+}
+
+// conj(a) * b
+DL_HD Quat qcmul(const Quat a, const Quat b) {
+  Quat r;
+  r.w = a.w * b.w + a.x * b.x + a.y * b.y + a.z * b.z;
+  r.x = a.w * b.x - a.x * b.w - a.y * b.z + a.z * b.y;
+  r.y = a.w * b.y + a.x * b.z - a.y * b.w - a.z * b.x;
+  r.z = a.w * b.z - a.x * b.y + a.y * b.x - a.z * b.w;
+  return r;
+}
+
+DL_HD Quat qnormalize(Quat q) {
+  const double s = 1.0 / sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  q.w *= s; q.x *= s; q.y *= s; q.z *= s;
+  return q;
+}
+
+// exp of a rotation vector (expVec, helpers.cpp:139-157) as a quaternion
+DL_HD Quat qexp(const double tx, const double ty, const double tz) {
+  const double th = sqrt(tx * tx + ty * ty + tz * tz);
+  Quat q;
+  if (th == 0.0) {
+    q.w = 1.0; q.x = 0.0; q.y = 0.0; q.z = 0.0;
+    return q;
+  }
+  double s, c;
+  sincos(0.5 * th, &s, &c);
+  const double f = s / th;
+  q.w = c; q.x = f * tx; q.y = f * ty; q.z = f * tz;
+  return q;
+}
+
+// log of a quaternion as a rotation vector with angle in [0, pi]
+// (logMat, helpers.cpp:99-111, picks the same branch)
+DL_HD void qlog(Quat q, double v[3]) {
+  if (q.w < 0.0) { q.w = -q.w; q.x = -q.x; q.y = -q.y; q.z = -q.z; }
+  const double s = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+  if (s == 0.0) { v[0] = v[1] = v[2] = 0.0; return; }
+  const double f = 2.0 * atan2(s, q.w) / s;
+  v[0] = f * q.x; v[1] = f * q.y; v[2] = f * q.z;
+}
+
+DL_HD void qmat(const Quat q, double R[9]) {
+  const double xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+  const double xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
+  const double wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+  R[0] = 1.0 - 2.0 * (yy + zz); R[1] = 2.0 * (xy - wz);       R[2] = 2.0 * (xz + wy);
+  R[3] = 2.0 * (xy + wz);       R[4] = 1.0 - 2.0 * (xx + zz); R[5] = 2.0 * (yz - wx);
+  R[6] = 2.0 * (xz - wy);       R[7] = 2.0 * (yz + wx);       R[8] = 1.0 - 2.0 * (xx + yy);
+}
+
+// ---------------------------------------------------------------------
+// element kinematics (material strain), shared by INIT and the step
+// ---------------------------------------------------------------------
+struct ElemKin {
+  double g0, g1, w;   // shape gradients and Gauss1 weight
+  double phiP[3];     // phi' (spatial)
+  double R[9];        // Lambda_ip
+  double eps[6];      // material strain before subtracting the initial strain
+};
+
+// Shape gradients of Line2 with Gauss1 in arclength coordinates
+// (Line3D.cpp:102-115): J = dN0/du * 0 + dN1/du * L0, g = dN/du / J, w = 2 J.
+DL_HD void elem_kinematics(const double Xa[3], const double ua[3], const double Xb[3],
+                           const double ub[3], const Quat qa, const Quat qb, ElemKin &k) {
+#pragma clang fp contract(off)
+  const double d0 = Xb[0] - Xa[0], d1 = Xb[1] - Xa[1], d2 = Xb[2] - Xa[2];
+  double ll = d0 * d0;
+  ll += d1 * d1;
+  ll += d2 * d2;
+  const double L = sqrt(ll) + 0.;
+  const double J = -0.5 * 0. + 0.5 * L;
+  k.g0 = -0.5 / J;
+  k.g1 = 0.5 / J;
+  k.w = 2. * J;
+  // phi' = (X0 + u) . grads (SpecialCosseratRodModel.cpp:720-722)
+  for (int i = 0; i < 3; i++) {
+    double s = (Xa[i] + ua[i]) * k.g0;
+    s += (Xb[i] + ub[i]) * k.g1;
+    k.phiP[i] = s;
+  }
+  // relative rotation and its half (getRefRot_)
+  Quat qr = qcmul(qa, qb);
+  if (qr.w < 0.0) { qr.w = -qr.w; qr.x = -qr.x; qr.y = -qr.y; qr.z = -qr.z; }
+  const double s2 = qr.x * qr.x + qr.y * qr.y + qr.z * qr.z;
+  const double s = sqrt(s2);
+  double phi[3] = {0.0, 0.0, 0.0};
+  if (s > 0.0) {
+    const double f = 2.0 * atan2(s, qr.w) / s;
+    phi[0] = f * qr.x; phi[1] = f * qr.y; phi[2] = f * qr.z;
+  }
+  const double hn = 1.0 / sqrt(2.0 * (1.0 + qr.w));
+  Quat qh;
+  qh.w = (1.0 + qr.w) * hn; qh.x = qr.x * hn; qh.y = qr.y * hn; qh.z = qr.z * hn;
+  const Quat qm = qmul(qa, qh);
+  qmat(qm, k.R);
+  // Gamma = Lambda^T phi'
+  for (int i = 0; i < 3; i++) {
+    double t = k.R[i] * k.phiP[0];
+    t += k.R[3 + i] * k.phiP[1];
+    t += k.R[6 + i] * k.phiP[2];
+    k.eps[i] = t;
+  }
+  // K = psi' = psi_a g0 + psi_b g1, psi_a = -phi/2, psi_b = phi/2
+  for (int i = 0; i < 3; i++) {
+    double t = (-0.5 * phi[i]) * k.g0;
+    t += (0.5 * phi[i]) * k.g1;
+    k.eps[3 + i] = t;
+  }
+}
+
+// internal nodal forces of the element from the material stress
+// fint[Idofs] += w XI sigma_spatial (SpecialCosseratRodModel.cpp:945, Line3D.cpp:141-162)
+DL_HD void elem_forces(const ElemKin &k, const double sm[6], double fa[6], double fb[6]) {
+  double n[3], m[3];
+  for (int i = 0; i < 3; i++) {
+    n[i] = k.R[3 * i] * sm[0] + k.R[3 * i + 1] * sm[1] + k.R[3 * i + 2] * sm[2];
+    m[i] = k.R[3 * i] * sm[3] + k.R[3 * i + 1] * sm[4] + k.R[3 * i + 2] * sm[5];
+  }
+  double c[3];
+  c[0] = k.phiP[1] * n[2] - k.phiP[2] * n[1];
+  c[1] = k.phiP[2] * n[0] - k.phiP[0] * n[2];
+  c[2] = k.phiP[0] * n[1] - k.phiP[1] * n[0];
+  for (int i = 0; i < 3; i++) {
+    fa[i] = k.w * (k.g0 * n[i]);
+    fb[i] = k.w * (k.g1 * n[i]);
+    fa[3 + i] = k.w * (-0.5 * c[i] + k.g0 * m[i]);
+    fb[3 + i] = k.w * (-0.5 * c[i] + k.g1 * m[i]);
+  }
+}
+
+// ---------------------------------------------------------------------
+// elastoplastic rod material, power-sum yield family
+//   f = sum_i |w_i (s_i + ck_i q_i)|^p_i - Y (1 + ci q_0),  q = -H alpha
+// (ElastoPlasticRodMaterial.cpp:198-269, cutting plane)
+// ---------------------------------------------------------------------
+struct PlasticParams {
+  double yield_level;
+  double kin_coupling[6];
+  double iso_coupling;
+  double iso_modulus;
+  double precision;
+  double exponent[6];
+  int32_t has_iso, has_kin, max_iter;
+};
+
+DL_HD double dl_sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
+// |x|^p and d|x|^p/d|x| of the power-sum yield family (p = 1: sum-abs)
+DL_HD double yield_pterm(double x, double p) {
+  const double ax = fabs(x);
+  if (p == 1.0) return ax;
+  if (p == 2.0) return ax * ax;
+  return pow(ax, p);
+}
+DL_HD double yield_pderiv(double x, double p) {
+  const double ax = fabs(x);
+  if (p == 1.0) return 1.0;
+  if (p == 2.0) return 2.0 * ax;
+  return p * pow(ax, p - 1.0);
+}
+
+// Returns the number of cutting-plane corrections taken, or -1 when the
+// iteration cap is hit.  ep[6], al[7] (iso, kin x6) in/out; stress out.
+DL_HD int return_map(const PlasticParams &P, const double C[6], const double w[6],
+                     const double Hk[6], const double strain[6], double ep[6], double al[7],
+                     double stress[6], double *dflow_out, int inelastic) {
+  double dflow = 0.0;
+  int liter = 0;
+  for (;;) {
+    for (int i = 0; i < 6; i++) stress[i] = C[i] * (strain[i] - ep[i]);
+    double q[7];
+    q[0] = -1.0 * (P.has_iso ? P.iso_modulus * al[0] : 0.0);
+    for (int i = 0; i < 6; i++) q[1 + i] = -1.0 * (P.has_kin ? Hk[i] * al[1 + i] : 0.0);
+    if (!inelastic) break;
+    double a[6];
+    double f = 0.0;
+    for (int i = 0; i < 6; i++) {
+      a[i] = stress[i] + (P.has_kin ? P.kin_coupling[i] * q[1 + i] : 0.0);
+      f += yield_pterm(w[i] * a[i], P.exponent[i]);
+    }
+    f -= P.yield_level * (1.0 + (P.has_iso ? P.iso_coupling * q[0] : 0.0));
+    if (!(liter < P.max_iter)) { *dflow_out = dflow; return -1; }
+    if (f < P.precision) break;
+    double gs[6], gq[7];
+    for (int i = 0; i < 6; i++) {
+      const double dd = yield_pderiv(w[i] * a[i], P.exponent[i]) * w[i] * dl_sgn(a[i]);
+      gs[i] = dd;
+      gq[1 + i] = P.has_kin ? dd * P.kin_coupling[i] : 0.0;
+      if (fabs(stress[i]) < 2.220446049250313e-16) gs[i] = 0.0;
+    }
+    gq[0] = P.has_iso ? -P.yield_level * P.iso_coupling : 0.0;
+    double den = 0.0;
+    for (int i = 0; i < 6; i++) den += gs[i] * (C[i] * gs[i]);
+    double b = 0.0;
+    if (P.has_iso) b += gq[0] * (P.iso_modulus * gq[0]);
+    if (P.has_kin)
+      for (int i = 0; i < 6; i++) b += gq[1 + i] * (Hk[i] * gq[1 + i]);
+    den += b;
+    const double ddl = f / den;
+    for (int i = 0; i < 6; i++) ep[i] += ddl * gs[i];
+    if (P.has_iso) al[0] += ddl * gq[0];
+    if (P.has_kin)
+      for (int i = 0; i < 6; i++) al[1 + i] += ddl * gq[1 + i];
+    dflow += ddl;
+    liter++;
+  }
+  *dflow_out = dflow;
+  return liter;
+}
+
+// ---------------------------------------------------------------------
+// node: mass, gyro, acceleration with constraints
+// ---------------------------------------------------------------------
+// spatial rotational inertia J = R J0 R^T, J0 packed (xx, yy, zz, xy, xz, yz)
+DL_HD void spatial_inertia(const double R[9], const double J0p[6], double J[9]) {
+  const double J0[9] = {J0p[0], J0p[3], J0p[4], J0p[3], J0p[1], J0p[5], J0p[4], J0p[5], J0p[2]};
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      T[3 * i + j] = R[3 * i] * J0[j] + R[3 * i + 1] * J0[3 + j] + R[3 * i + 2] * J0[6 + j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      J[3 * i + j] = T[3 * i] * R[3 * j] + T[3 * i + 1] * R[3 * j + 1] + T[3 * i + 2] * R[3 * j + 2];
+}
+
+// solve J a = f for the free components (mask bit i set = constrained with
+// value c[i]) -- the reduced system of a Dirichlet-constrained block solve
+DL_HD void solve3_masked(const double J[9], const double f[3], unsigned mask, const double c[3],
+                         double a[3]) {
+  double A[9], b[3];
+  for (int i = 0; i < 3; i++) {
+    b[i] = f[i];
+    for (int j = 0; j < 3; j++) A[3 * i + j] = J[3 * i + j];
+  }
+  for (int j = 0; j < 3; j++) {
+    if (!(mask & (1u << j))) continue;
+    for (int i = 0; i < 3; i++) {
+      if (mask & (1u << i)) continue;
+      b[i] -= A[3 * i + j] * c[j];
+    }
+    for (int i = 0; i < 3; i++) { A[3 * i + j] = 0.0; A[3 * j + i] = 0.0; }
+    A[3 * j + j] = 1.0;
+    b[j] = c[j];
+  }
+  const double c00 = A[4] * A[8] - A[5] * A[7];
+  const double c01 = A[5] * A[6] - A[3] * A[8];
+  const double c02 = A[3] * A[7] - A[4] * A[6];
+  const double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+  const double id = 1.0 / det;
+  const double c10 = A[2] * A[7] - A[1] * A[8];
+  const double c11 = A[0] * A[8] - A[2] * A[6];
+  const double c12 = A[1] * A[6] - A[0] * A[7];
+  const double c20 = A[1] * A[5] - A[2] * A[4];
+  const double c21 = A[2] * A[3] - A[0] * A[5];
+  const double c22 = A[0] * A[4] - A[1] * A[3];
+  a[0] = (c00 * b[0] + c10 * b[1] + c20 * b[2]) * id;
+  a[1] = (c01 * b[0] + c11 * b[1] + c21 * b[2]) * id;
+  a[2] = (c02 * b[0] + c12 * b[1] + c22 * b[2]) * id;
+  for (int j = 0; j < 3; j++)
+    if (mask & (1u << j)) a[j] = c[j];
+}
+
+#endif

@@ -1,0 +1,89 @@
+// dl_internal.h -- device data layout and launcher prototypes shared by the
+// host API (dl_api.cpp) and the kernels (dl_kernels.hip, dl_contact.hip).
+//
+// HBM layout: structure-of-arrays, FP64, component-major ([c][n]) so that
+// consecutive threads touch consecutive addresses.
+#ifndef DL_INTERNAL_H
+#define DL_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dl_device.h"
+
+// Nodes, elements, struts and the shared-node gather map.
+struct DevData {
+  int64_t N, E;
+  int32_t R;
+  // nodes ([c][N])
+  const double *X0;  // 3
+  double *u;         // 3
+  double *rot;       // so3: quaternion 4 (w,x,y,z); additive: rotation vector 3
+  double *v;         // 6
+  const double *mt;  // translational lumped mass
+  const double *J0;  // 6: rotational inertia at theta = 0 (xx yy zz xy xz yz)
+  const double *gm;  // gyro multiplicity (number of struts holding the node)
+  const int32_t *bc; // -1 or record index
+  // boundary records
+  const uint32_t *bc_mask; // bit i: dof i constrained
+  const double *bc_a;      // [6][nbc] prescribed acceleration
+  const double *bc_f;      // [6][nbc] external force
+  const int32_t *bc_rslot; // reaction slot or -1
+  int32_t nbc;
+  int32_t react_dof;
+  // struts
+  const int64_t *eptr;  // [R+1]
+  const int32_t *cnode; // [E+R] chain nodes
+  const int32_t *cslot; // [E+R] shared slot or -1
+  const double *C;      // [6][R] material stiffness diag
+  // elements ([c][E])
+  const double *qN;     // 8: LambdaN at end a (4), end b (4)
+  double *eps0;         // 6
+  // plastic
+  const double *pw;     // [6][R] yield weights
+  const double *Hk;     // [6][R] kinematic hardening diag
+  double *ep;           // [6][E] committed plastic strain
+  double *al;           // [7][E] committed hardening parameters
+  double *ep_cur;       // [6][E] (action path)
+  double *al_cur;       // [7][E]
+  double *dflow;        // [E]
+  uint8_t *active;      // [E]
+  double *eo;           // [6][E] committed total strain (dissipation)
+  double *ecur;         // [6][E] last material strain (action path)
+  // shared nodes
+  int32_t NJ;
+  const int32_t *jnode; // [NJ]
+  const int32_t *jptr;  // [NJ+1]
+  const int32_t *jslot; // [jptr[NJ]]
+  int32_t nslot;
+  double *sbuf;         // [6][nslot]
+  // outputs
+  double *fout;         // [6][N] (action path)
+  double *fc;           // [3][N] contact force or null
+  double *rbuf;         // reaction contributions [n_react]
+  double *dbuf;         // [R] per-strut dissipation increment
+  int32_t *err;         // device error flag
+};
+
+struct StepParams {
+  double dt;
+  int32_t inelastic;
+  int32_t track_diss;
+};
+
+enum { MODE_STEP = 0, MODE_FORCE = 1 };
+
+// launchers (dl_kernels.hip)
+void launch_init_strain(const DevData &d, hipStream_t s);
+void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
+                     bool so3, bool plastic, bool contact, hipStream_t s);
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, bool contact,
+                  hipStream_t s);
+void launch_force_extras(const DevData &d, bool so3, bool gyro, bool contact, hipStream_t s);
+void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s);
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
+void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
+void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
+                     hipStream_t s);
+
+#endif

@@ -1,0 +1,559 @@
+// dl_kernels.hip -- gfx950 kernels of the explicit rod-lattice step (FP64).
+//
+// One explicit step is two launches (plus contact when enabled):
+//   k_strut  one thread per strut.  Walks the strut's element chain, computes
+//            every element's internal force (kinematics, strains, material
+//            return map, XI^T sigma), sums the two element contributions of
+//            each private (single-strut) node in registers and immediately
+//            applies the node update (mass, gyro, constraints, central
+//            difference, SO(3) composition).  Contributions to shared
+//            (joint) nodes go to a slot buffer.
+//   k_joint  one thread per shared node.  Sums its slots in a fixed order
+//            (CSR, strut order) and applies the same node update.
+// Assembly is therefore a deterministic gather: no floating-point atomics.
+// The reference evaluates the same quantities element by element and
+// scatters into the global vector (SpecialCosseratRodModel.cpp:899-949),
+// then updates every DOF (ExplicitModule.cpp:339-392, LeapFrogModule.cpp:44-84).
+#include <hip/hip_runtime.h>
+
+#include "dl_internal.h"
+
+#define DL_BLOCK 256
+
+// ------------------------------------------------------------------
+// node helpers
+// ------------------------------------------------------------------
+template <bool SO3>
+__device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4], Quat &q) {
+  const int64_t N = d.N;
+  if (SO3) {
+    q.w = d.rot[n]; q.x = d.rot[N + n]; q.y = d.rot[2 * N + n]; q.z = d.rot[3 * N + n];
+    r[0] = q.w; r[1] = q.x; r[2] = q.y; r[3] = q.z;
+  } else {
+    r[0] = d.rot[n]; r[1] = d.rot[N + n]; r[2] = d.rot[2 * N + n]; r[3] = 0.0;
+    q = qexp(r[0], r[1], r[2]);
+  }
+}
+
+// a = M^-1 (f_ext - f) with constraints, then v += dt a, u += dt v,
+// R <- exp(dt w) R (SO(3)) or theta += dt w (additive).
+// f: element (+ contact) internal force of the node at the step's start state.
+template <bool SO3, bool CONTACT>
+__device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
+                                            double f[6], const Quat q, const double r[4]) {
+  const int64_t N = d.N;
+  if (CONTACT) {
+    f[0] += d.fc[n];
+    f[1] += d.fc[N + n];
+    f[2] += d.fc[2 * N + n];
+  }
+  const int32_t b = d.bc[n];
+  unsigned mask = 0u;
+  double ac[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double fe[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (b >= 0) {
+    const int64_t nb = d.nbc;
+    mask = d.bc_mask[b];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      ac[c] = d.bc_a[c * nb + b];
+      fe[c] = d.bc_f[c * nb + b];
+    }
+    const int32_t rs = d.bc_rslot[b];
+    if (rs >= 0) d.rbuf[rs] = f[d.react_dof];
+  }
+  double v[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) v[c] = d.v[c * N + n];
+  const double mt = d.mt[n];
+  double J0[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  double R[9], J[9];
+  qmat(q, R);
+  spatial_inertia(R, J0, J);
+  if (SO3) {
+    // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
+    const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
+    const double jw1 = J[3] * v[3] + J[4] * v[4] + J[5] * v[5];
+    const double jw2 = J[6] * v[3] + J[7] * v[4] + J[8] * v[5];
+    const double gm = d.gm[n];
+    f[3] += gm * (v[4] * jw2 - v[5] * jw1);
+    f[4] += gm * (v[5] * jw0 - v[3] * jw2);
+    f[5] += gm * (v[3] * jw1 - v[4] * jw0);
+  }
+  double fr[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) fr[c] = fe[c] - f[c];
+  double a[6];
+  const double imt = 1.0 / mt;
+#pragma unroll
+  for (int c = 0; c < 3; c++) a[c] = (mask & (1u << c)) ? ac[c] : imt * fr[c];
+  if (SO3) {
+    solve3_masked(J, fr + 3, mask >> 3, ac + 3, a + 3);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) a[3 + c] = (mask & (8u << c)) ? ac[3 + c] : (1.0 / J[4 * c]) * fr[3 + c];
+  }
+  const double dt = sp.dt;
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    v[c] = v[c] + dt * a[c];
+    d.v[c * N + n] = v[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) d.u[c * N + n] = d.u[c * N + n] + dt * v[c];
+  if (SO3) {
+    const Quat dq = qexp(dt * v[3], dt * v[4], dt * v[5]);
+    const Quat qn = qnormalize(qmul(dq, q));
+    d.rot[n] = qn.w; d.rot[N + n] = qn.x; d.rot[2 * N + n] = qn.y; d.rot[3 * N + n] = qn.z;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) d.rot[c * N + n] = r[c] + dt * v[3 + c];
+  }
+}
+
+// ------------------------------------------------------------------
+// INIT: initial material strain at zero displacement (initStrain_,
+// SpecialCosseratRodModel.cpp:536-569)
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.E) return;
+  const int64_t E = d.E, N = d.N;
+  // element nodes from the chain: element e of strut s is chain position e + s
+  // -> found through a binary search of eptr
+  int32_t lo = 0, hi = d.R - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (d.eptr[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int64_t c = e + lo;
+  const int32_t na = d.cnode[c], nb = d.cnode[c + 1];
+  double Xa[3], Xb[3];
+  const double z[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < 3; i++) { Xa[i] = d.X0[i * N + na]; Xb[i] = d.X0[i * N + nb]; }
+  Quat qa, qb;
+  qa.w = d.qN[e]; qa.x = d.qN[E + e]; qa.y = d.qN[2 * E + e]; qa.z = d.qN[3 * E + e];
+  qb.w = d.qN[4 * E + e]; qb.x = d.qN[5 * E + e]; qb.y = d.qN[6 * E + e]; qb.z = d.qN[7 * E + e];
+  const Quat id = {1.0, 0.0, 0.0, 0.0};
+  qa = qmul(id, qa);
+  qb = qmul(id, qb);
+  ElemKin k;
+  elem_kinematics(Xa, z, Xb, z, qa, qb, k);
+  for (int i = 0; i < 6; i++) d.eps0[i * E + e] = k.eps[i];
+}
+
+// ------------------------------------------------------------------
+// strut kernel
+// ------------------------------------------------------------------
+template <int MODE, bool SO3, bool PLASTIC, bool CONTACT>
+__global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  double diss = 0.0;
+  if (s < d.R) {
+    const int64_t N = d.N, E = d.E, R = d.R;
+    const int64_t e0 = d.eptr[s], e1 = d.eptr[s + 1];
+    const int64_t c0 = e0 + s;
+    double C[6], pw[6], Hk[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) C[i] = d.C[i * R + s];
+    if (PLASTIC) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) { pw[i] = d.pw[i * R + s]; Hk[i] = d.Hk[i * R + s]; }
+    }
+    int32_t na = d.cnode[c0];
+    double Xa[3], ua[3], ra[4];
+    Quat qna;
+#pragma unroll
+    for (int i = 0; i < 3; i++) { Xa[i] = d.X0[i * N + na]; ua[i] = d.u[i * N + na]; }
+    load_rot<SO3>(d, na, ra, qna);
+    double facc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t e = e0; e < e1; ++e) {
+      const int64_t c = c0 + (e - e0);
+      const int32_t nb = d.cnode[c + 1];
+      double Xb[3], ub[3], rb[4];
+      Quat qnb;
+#pragma unroll
+      for (int i = 0; i < 3; i++) { Xb[i] = d.X0[i * N + nb]; ub[i] = d.u[i * N + nb]; }
+      load_rot<SO3>(d, nb, rb, qnb);
+      Quat qNa, qNb;
+      qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
+      qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+      const Quat qa = qmul(qna, qNa), qb = qmul(qnb, qNb);
+      ElemKin k;
+      elem_kinematics(Xa, ua, Xb, ub, qa, qb, k);
+      double strain[6], sm[6];
+#pragma unroll
+      for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - d.eps0[i * E + e];
+      if (!PLASTIC) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
+        if (MODE == MODE_FORCE) {
+#pragma unroll
+          for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
+        }
+      } else {
+        double ep[6], al[7], epo[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) { ep[i] = d.ep[i * E + e]; epo[i] = ep[i]; }
+#pragma unroll
+        for (int i = 0; i < 7; i++) al[i] = d.al[i * E + e];
+        double dfl;
+        const int it = return_map(pp, C, pw, Hk, strain, ep, al, sm, &dfl, sp.inelastic);
+        if (it < 0) *d.err = -3;
+        if (MODE == MODE_STEP) {
+          if (sp.track_diss) {
+            // applyDeform dissipation (ElastoPlasticRodMaterial.cpp:285-296)
+            if (fabs(dfl) > 2.220446049250313e-16) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < 6; i++) {
+                const double so = C[i] * (d.eo[i * E + e] - epo[i]);
+                t += ((so + sm[i]) / 2.0) * (ep[i] - epo[i]);
+              }
+              diss += k.w * t;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; i++) d.eo[i * E + e] = strain[i];
+          }
+#pragma unroll
+          for (int i = 0; i < 6; i++) d.ep[i * E + e] = ep[i];
+#pragma unroll
+          for (int i = 0; i < 7; i++) d.al[i * E + e] = al[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < 6; i++) { d.ep_cur[i * E + e] = ep[i]; d.ecur[i * E + e] = strain[i]; }
+#pragma unroll
+          for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
+          d.dflow[e] = dfl;
+        }
+        d.active[e] = (uint8_t)(it > 0);
+      }
+      double fa[6], fb[6];
+      elem_forces(k, sm, fa, fb);
+#pragma unroll
+      for (int i = 0; i < 6; i++) facc[i] += fa[i];
+      const int32_t slot = d.cslot[c];
+      if (slot >= 0) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = facc[i];
+      } else if (MODE == MODE_FORCE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
+      } else {
+        node_update<SO3, CONTACT>(d, sp, na, facc, qna, ra);
+      }
+#pragma unroll
+      for (int i = 0; i < 6; i++) facc[i] = fb[i];
+      na = nb;
+#pragma unroll
+      for (int i = 0; i < 3; i++) { Xa[i] = Xb[i]; ua[i] = ub[i]; }
+#pragma unroll
+      for (int i = 0; i < 4; i++) ra[i] = rb[i];
+      qna = qnb;
+    }
+    const int32_t slot = d.cslot[c0 + (e1 - e0)];
+    if (slot >= 0) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = facc[i];
+    } else if (MODE == MODE_FORCE) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
+    } else {
+      node_update<SO3, CONTACT>(d, sp, na, facc, qna, ra);
+    }
+  }
+  if (PLASTIC && MODE == MODE_STEP) {
+    if (sp.track_diss) {
+      // per-block partial of the dissipation increment, fixed tree order
+      __shared__ double red[DL_BLOCK];
+      red[threadIdx.x] = diss;
+      __syncthreads();
+      for (int w = DL_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) d.dbuf[blockIdx.x] = red[0];
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// joint kernel: shared nodes
+// ------------------------------------------------------------------
+template <int MODE, bool SO3, bool CONTACT>
+__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d.NJ) return;
+  const int32_t n = d.jnode[j];
+  const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
+  double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int32_t t = t0; t < t1; t++) {
+    const int64_t sl = d.jslot[t];
+#pragma unroll
+    for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + sl];
+  }
+  if (MODE == MODE_FORCE) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) d.fout[i * d.N + n] = f[i];
+    return;
+  }
+  double r[4];
+  Quat q;
+  load_rot<SO3>(d, n, r, q);
+  node_update<SO3, CONTACT>(d, sp, n, f, q, r);
+}
+
+// ------------------------------------------------------------------
+// action path: contact + gyro added to f_out (GET_INT_VECTOR)
+// ------------------------------------------------------------------
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro, int contact) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  if (contact)
+    for (int i = 0; i < 3; i++) d.fout[i * N + n] += d.fc[i * N + n];
+  if (gyro) {
+    double r[4];
+    Quat q;
+    load_rot<SO3>(d, (int32_t)n, r, q);
+    double R[9], J[9], J0[6], w[3];
+    for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+    for (int c = 0; c < 3; c++) w[c] = d.v[(3 + c) * N + n];
+    qmat(q, R);
+    spatial_inertia(R, J0, J);
+    const double jw0 = J[0] * w[0] + J[1] * w[1] + J[2] * w[2];
+    const double jw1 = J[3] * w[0] + J[4] * w[1] + J[5] * w[2];
+    const double jw2 = J[6] * w[0] + J[7] * w[1] + J[8] * w[2];
+    const double gm = d.gm[n];
+    d.fout[3 * N + n] += gm * (w[1] * jw2 - w[2] * jw1);
+    d.fout[4 * N + n] += gm * (w[2] * jw0 - w[0] * jw2);
+    d.fout[5 * N + n] += gm * (w[0] * jw1 - w[1] * jw0);
+  }
+}
+
+// ------------------------------------------------------------------
+// deterministic single-block sum
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_reduce_sum(const double *in, int64_t n, double *out,
+                                                     double *accum) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += in[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    if (accum) {
+      t = *accum + t;
+      *accum = t;
+    }
+    *out = t;
+  }
+}
+
+// ------------------------------------------------------------------
+// COMMIT (applyDeform): per-element potential energy and dissipation
+// increment, old <- curr
+// ------------------------------------------------------------------
+template <bool PLASTIC>
+__global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, double *ediss) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.E) return;
+  const int64_t E = d.E;
+  // strut of element e
+  int32_t lo = 0, hi = d.R - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (d.eptr[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int64_t R = d.R;
+  double C[6];
+  for (int i = 0; i < 6; i++) C[i] = d.C[i * R + lo];
+  if (!PLASTIC) {
+    double t = 0.0;
+    for (int i = 0; i < 6; i++) {
+      const double c = d.ecur[i * E + e];
+      t += c * (C[i] * c);
+    }
+    epot[e] = 0.5 * t;
+    ediss[e] = 0.0;
+    for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+    return;
+  }
+  double t = 0.0, ds = 0.0;
+  for (int i = 0; i < 6; i++) {
+    const double oel = d.eo[i * E + e] - d.ep[i * E + e];
+    const double cel = d.ecur[i * E + e] - d.ep_cur[i * E + e];
+    const double dep = d.ep_cur[i * E + e] - d.ep[i * E + e];
+    const double os = C[i] * oel, cs = C[i] * cel;
+    t += cel * cs;
+    ds += ((os + cs) / 2.0) * dep;
+  }
+  epot[e] = 0.5 * t;
+  ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
+  for (int i = 0; i < 6; i++) {
+    d.ep[i * E + e] = d.ep_cur[i * E + e];
+    d.eo[i * E + e] = d.ecur[i * E + e];
+  }
+  for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
+  d.dflow[e] = 0.0;
+}
+
+// ------------------------------------------------------------------
+// GET_MATRIX2: 6x6 block per node, row-major [N][36]
+// ------------------------------------------------------------------
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_mass_blocks(DevData d, int lumped, double *blk) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  double r[4];
+  Quat q;
+  load_rot<SO3>(d, (int32_t)n, r, q);
+  double R[9], J[9], J0[6];
+  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  qmat(q, R);
+  spatial_inertia(R, J0, J);
+  double *B = blk + 36 * n;
+  for (int i = 0; i < 36; i++) B[i] = 0.0;
+  for (int i = 0; i < 3; i++) B[7 * i] = d.mt[n];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (!lumped || i == j) B[6 * (3 + i) + 3 + j] = J[3 * i + j];
+}
+
+// ------------------------------------------------------------------
+// state conversion between SoA device state and DOF vectors [N][6]
+// ------------------------------------------------------------------
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_state_io(DevData d, int to_dof, double *disp,
+                                                       double *velo) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  if (to_dof) {
+    if (disp) {
+      for (int c = 0; c < 3; c++) disp[6 * n + c] = d.u[c * N + n];
+      if (SO3) {
+        Quat q;
+        q.w = d.rot[n]; q.x = d.rot[N + n]; q.y = d.rot[2 * N + n]; q.z = d.rot[3 * N + n];
+        double t[3];
+        qlog(q, t);
+        for (int c = 0; c < 3; c++) disp[6 * n + 3 + c] = t[c];
+      } else {
+        for (int c = 0; c < 3; c++) disp[6 * n + 3 + c] = d.rot[c * N + n];
+      }
+    }
+    if (velo)
+      for (int c = 0; c < 6; c++) velo[6 * n + c] = d.v[c * N + n];
+  } else {
+    if (disp) {
+      for (int c = 0; c < 3; c++) d.u[c * N + n] = disp[6 * n + c];
+      if (SO3) {
+        const Quat q = qexp(disp[6 * n + 3], disp[6 * n + 4], disp[6 * n + 5]);
+        d.rot[n] = q.w; d.rot[N + n] = q.x; d.rot[2 * N + n] = q.y; d.rot[3 * N + n] = q.z;
+      } else {
+        for (int c = 0; c < 3; c++) d.rot[c * N + n] = disp[6 * n + 3 + c];
+      }
+    }
+    if (velo)
+      for (int c = 0; c < 6; c++) d.v[c * N + n] = velo[6 * n + c];
+  }
+}
+
+// ------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + DL_BLOCK - 1) / DL_BLOCK); }
+
+void launch_init_strain(const DevData &d, hipStream_t s) {
+  if (d.E == 0) return;
+  hipLaunchKernelGGL(k_init_strain, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d);
+}
+
+template <int MODE, bool SO3, bool PLASTIC, bool CONTACT>
+static void ls(const DevData &d, const StepParams &sp, const PlasticParams &pp, hipStream_t s) {
+  hipLaunchKernelGGL((k_strut<MODE, SO3, PLASTIC, CONTACT>), dim3(nblk(d.R)), dim3(DL_BLOCK), 0,
+                     s, d, sp, pp);
+}
+
+void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
+                     bool so3, bool plastic, bool contact, hipStream_t s) {
+  if (d.R == 0) return;
+#define DL_LS(M, A, B, C)                                                  \
+  if (mode == M && so3 == A && plastic == B && contact == C) {            \
+    ls<M, A, B, C>(d, sp, pp, s);                                          \
+    return;                                                                \
+  }
+  DL_LS(MODE_STEP, true, false, false)
+  DL_LS(MODE_STEP, true, false, true)
+  DL_LS(MODE_STEP, true, true, false)
+  DL_LS(MODE_STEP, true, true, true)
+  DL_LS(MODE_STEP, false, false, false)
+  DL_LS(MODE_STEP, false, false, true)
+  DL_LS(MODE_STEP, false, true, false)
+  DL_LS(MODE_STEP, false, true, true)
+  DL_LS(MODE_FORCE, true, false, false)
+  DL_LS(MODE_FORCE, true, true, false)
+  DL_LS(MODE_FORCE, false, false, false)
+  DL_LS(MODE_FORCE, false, true, false)
+#undef DL_LS
+}
+
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, bool contact,
+                  hipStream_t s) {
+  if (d.NJ == 0) return;
+  const dim3 g(nblk(d.NJ)), b(DL_BLOCK);
+  if (mode == MODE_FORCE) {
+    if (so3) hipLaunchKernelGGL((k_joint<MODE_FORCE, true, false>), g, b, 0, s, d, sp);
+    else hipLaunchKernelGGL((k_joint<MODE_FORCE, false, false>), g, b, 0, s, d, sp);
+    return;
+  }
+  if (so3) {
+    if (contact) hipLaunchKernelGGL((k_joint<MODE_STEP, true, true>), g, b, 0, s, d, sp);
+    else hipLaunchKernelGGL((k_joint<MODE_STEP, true, false>), g, b, 0, s, d, sp);
+  } else {
+    if (contact) hipLaunchKernelGGL((k_joint<MODE_STEP, false, true>), g, b, 0, s, d, sp);
+    else hipLaunchKernelGGL((k_joint<MODE_STEP, false, false>), g, b, 0, s, d, sp);
+  }
+}
+
+void launch_force_extras(const DevData &d, bool so3, bool gyro, bool contact, hipStream_t s) {
+  if (!gyro && !contact) return;
+  if (so3) hipLaunchKernelGGL(k_force_extras<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                              (int)gyro, (int)contact);
+  else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                          (int)gyro, (int)contact);
+}
+
+void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, s, in, n, out, accum);
+}
+
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {
+  if (d.E == 0) return;
+  if (plastic) hipLaunchKernelGGL(k_commit<true>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
+  else hipLaunchKernelGGL(k_commit<false>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
+}
+
+void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_mass_blocks<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                              (int)lumped, blocks);
+  else hipLaunchKernelGGL(k_mass_blocks<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                          (int)lumped, blocks);
+}
+
+void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
+                     hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_state_io<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                              (int)to_dof, disp, velo);
+  else hipLaunchKernelGGL(k_state_io<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
+                          (int)to_dof, disp, velo);
+}

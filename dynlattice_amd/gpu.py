@@ -1,0 +1,314 @@
+"""ctypes binding of libdlgpu.so (the C ABI in include/dl_gpu.h).
+
+The HIP library is the only compute path: if ``libdlgpu.so`` is missing or
+cannot be loaded this module raises ImportError -- there is no CPU fallback.
+Build it with ``python -m dynlattice_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdlgpu.so")
+
+DL_OK = 0
+DL_ERR_INPUT = -1
+DL_ERR_ZERO_MASS = -2
+DL_ERR_PLASTIC_ITER = -3
+DL_ERR_HIP = -4
+DL_ERR_CAPACITY = -5
+DL_ERR_STATE = -6
+
+DL_INIT, DL_GET_INT_VECTOR, DL_GET_MATRIX2, DL_COMMIT, DL_CANCEL = range(5)
+DL_LOAD_CASE_DEFAULT, DL_LOAD_CASE_OUTPUT = 0, 1
+DL_TRACK_DISSIPATION = 1
+
+#: every symbol declared in include/dl_gpu.h
+EXPORTS = (
+    "dl_create", "dl_destroy", "dl_take_action", "dl_set_state", "dl_get_state",
+    "dl_set_bcs", "dl_set_reaction_group", "dl_explicit_step", "dl_get_history",
+    "dl_contact_pairs", "dl_contact_candidates", "dl_plastic_active", "dl_plastic_state",
+    "dl_phase_time", "dl_set_timing", "dl_sync", "dl_last_error",
+)
+
+dp = C.POINTER(C.c_double)
+ip32 = C.POINTER(C.c_int32)
+ip64 = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
+
+
+class DLError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dl error {code}: {msg}")
+        self.code = code
+
+
+class dl_mesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_elems", C.c_int64), ("n_struts", C.c_int32),
+                ("coords", dp), ("elem_nodes", ip32), ("elem_strut", ip32)]
+
+
+class dl_sections(C.Structure):
+    _fields_ = [("area", dp), ("I1", dp), ("I2", dp), ("J", dp), ("shear_factor", dp),
+                ("young", dp), ("shear_modulus", dp), ("density", dp),
+                ("inertia_correct", C.c_double)]
+
+
+class dl_material(C.Structure):
+    _fields_ = [("model", C.c_int32), ("weights", dp), ("yield_level", C.c_double),
+                ("kin_coupling", C.c_double * 6), ("iso_coupling", C.c_double),
+                ("has_iso", C.c_int32), ("iso_modulus", C.c_double), ("has_kin", C.c_int32),
+                ("kin_const", C.c_double * 6), ("kin_ratio", C.c_double),
+                ("precision", C.c_double), ("max_iter", C.c_int32),
+                ("exponent", C.c_double * 6)]
+
+
+class dl_contact(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("penalty_sts", C.c_double),
+                ("penalty_nts", C.c_double), ("radius", dp), ("max_pairs", C.c_int64)]
+
+
+class dl_options(C.Structure):
+    _fields_ = [("so3", C.c_int32), ("device", C.c_int32), ("stream", C.c_void_p),
+                ("flags", C.c_uint32)]
+
+
+class dl_params(C.Structure):
+    _fields_ = [("int_vector", dp), ("matrix2", dp), ("load_case", C.c_int32),
+                ("with_gyro", C.c_int32), ("lumped", C.c_int32), ("e_pot", C.c_double),
+                ("e_diss", C.c_double)]
+
+
+class dl_step_out(C.Structure):
+    _fields_ = [("reaction", C.c_double), ("e_diss", C.c_double), ("n_contacts", C.c_int64),
+                ("n_candidates", C.c_int64), ("steps_done", C.c_int64)]
+
+
+_lib = None
+
+
+def load():
+    """Load libdlgpu.so (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run `python -m dynlattice_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.dl_create.argtypes = [C.POINTER(dl_mesh), C.POINTER(dl_sections), C.POINTER(dl_material),
+                            C.POINTER(dl_contact), C.POINTER(dl_options), C.POINTER(C.c_void_p)]
+    L.dl_destroy.argtypes = [C.c_void_p]
+    L.dl_take_action.argtypes = [C.c_void_p, C.c_int32, C.POINTER(dl_params)]
+    L.dl_set_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+    L.dl_get_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+    L.dl_set_bcs.argtypes = [C.c_void_p, u8p, dp, dp, C.c_int64]
+    L.dl_set_reaction_group.argtypes = [C.c_void_p, ip32, C.c_int64, C.c_int32]
+    L.dl_explicit_step.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(dl_step_out)]
+    L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
+    L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
+    L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
+    L.dl_plastic_active.argtypes = [C.c_void_p, u8p, C.c_int64]
+    L.dl_plastic_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+    L.dl_phase_time.argtypes = [C.c_void_p, C.c_char_p, dp, ip64]
+    L.dl_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.dl_sync.argtypes = [C.c_void_p]
+    L.dl_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _p(a, t=dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class GpuModel:
+    """Device-resident rod lattice behind the C ABI (one dl_ctx)."""
+
+    def __init__(self, p, device=0, stream=None, flags=0, init=True, max_pairs=0):
+        self.L = load()
+        self.p = p
+        k = []
+        self._keep = k
+        co = _c(p.coords)
+        en = _c(p.elem_nodes, np.int32)
+        es = _c(p.elem_strut, np.int32)
+        k += [co, en, es]
+        mesh = dl_mesh(p.n_nodes, p.n_elems, p.n_struts, _p(co), _p(en, ip32), _p(es, ip32))
+        s = p.sections
+        arrs = [_c(getattr(s, f)) for f in ("area", "I1", "I2", "J", "shear_factor", "young",
+                                            "shear_modulus", "density")]
+        k += arrs
+        sec = dl_sections(*[_p(a) for a in arrs], s.inertia_correct)
+        mat = dl_material()
+        mat.model = 0
+        if p.plastic is not None:
+            q = p.plastic
+            w = _c(q.weights)
+            k.append(w)
+            mat.model = 1
+            mat.weights = _p(w)
+            mat.yield_level = q.yield_level
+            mat.kin_coupling[:] = list(q.kin_coupling)
+            mat.iso_coupling = q.iso_coupling
+            mat.has_iso = int(q.has_iso)
+            mat.iso_modulus = q.iso_modulus
+            mat.has_kin = int(q.has_kin)
+            mat.kin_const[:] = list(q.kin_const)
+            mat.kin_ratio = q.kin_ratio
+            mat.precision = q.precision
+            mat.max_iter = q.max_iter
+            mat.exponent[:] = list(q.exponent)
+        ct = dl_contact()
+        if p.contact is not None:
+            r = _c(p.contact.radius)
+            k.append(r)
+            ct.enabled = 1
+            ct.penalty_sts = p.contact.penalty_sts
+            ct.penalty_nts = p.contact.penalty_nts
+            ct.radius = _p(r)
+            ct.max_pairs = int(max_pairs)
+        opt = dl_options(int(p.so3), int(device), stream, int(flags))
+        h = C.c_void_p()
+        self._check(self.L.dl_create(C.byref(mesh), C.byref(sec), C.byref(mat), C.byref(ct),
+                                     C.byref(opt), C.byref(h)))
+        self.h = h
+        self.set_bcs(p.cmask, p.cval, p.fext)
+        self.set_state(p.disp0, p.velo0)
+        if init:
+            self.init()
+
+    def _check(self, rc):
+        if rc != DL_OK:
+            raise DLError(rc, self.L.dl_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- actions -------------------------------------------------------
+    def action(self, act, params=None):
+        prm = params if params is not None else dl_params()
+        self._check(self.L.dl_take_action(self.h, act, C.byref(prm)))
+        return prm
+
+    def init(self):
+        self.action(DL_INIT)
+
+    def int_vector(self, inelastic=True, gyro=False):
+        f = np.zeros(6 * self.p.n_nodes)
+        prm = dl_params()
+        prm.int_vector = _p(f)
+        prm.load_case = DL_LOAD_CASE_DEFAULT if inelastic else DL_LOAD_CASE_OUTPUT
+        prm.with_gyro = int(gyro)
+        self.action(DL_GET_INT_VECTOR, prm)
+        return f.reshape(-1, 6)
+
+    def mass_blocks(self, lumped=False):
+        b = np.zeros(36 * self.p.n_nodes)
+        prm = dl_params()
+        prm.matrix2 = _p(b)
+        prm.lumped = int(lumped)
+        self.action(DL_GET_MATRIX2, prm)
+        return b.reshape(-1, 6, 6)
+
+    def commit(self):
+        prm = self.action(DL_COMMIT)
+        return prm.e_pot, prm.e_diss
+
+    def cancel(self):
+        self.action(DL_CANCEL)
+
+    # --- state ---------------------------------------------------------
+    def set_state(self, disp=None, velo=None):
+        n = 6 * self.p.n_nodes
+        d = _c(disp).reshape(-1) if disp is not None else None
+        v = _c(velo).reshape(-1) if velo is not None else None
+        self._check(self.L.dl_set_state(self.h, _p(d), _p(v), n))
+
+    def get_state(self):
+        n = 6 * self.p.n_nodes
+        d = np.zeros(n)
+        v = np.zeros(n)
+        self._check(self.L.dl_get_state(self.h, _p(d), _p(v), n))
+        return d.reshape(-1, 6), v.reshape(-1, 6)
+
+    def set_bcs(self, cmask, cval, fext):
+        n = 6 * self.p.n_nodes
+        self._cm = _c(cmask, np.uint8).reshape(-1)
+        self._cv = _c(cval).reshape(-1)
+        self._fe = _c(fext).reshape(-1)
+        self._check(self.L.dl_set_bcs(self.h, _p(self._cm, u8p), _p(self._cv), _p(self._fe), n))
+
+    def set_reaction_group(self, nodes, dof=2):
+        g = _c(nodes, np.int32)
+        self._check(self.L.dl_set_reaction_group(self.h, _p(g, ip32), g.shape[0], int(dof)))
+
+    # --- stepping ------------------------------------------------------
+    def step(self, nsteps, dt=None, out=True):
+        dt = self.p.dt if dt is None else dt
+        o = dl_step_out()
+        self._check(self.L.dl_explicit_step(self.h, int(nsteps), float(dt),
+                                            C.byref(o) if out else None))
+        return o
+
+    def history(self, nsteps):
+        r = np.zeros(nsteps)
+        e = np.zeros(nsteps)
+        c = np.zeros(nsteps, np.int64)
+        n = C.c_int64()
+        self._check(self.L.dl_get_history(self.h, _p(r), _p(e), _p(c, ip64), nsteps, C.byref(n)))
+        return {"reaction": r[:n.value], "e_diss": e[:n.value], "n_contacts": c[:n.value]}
+
+    def sync(self):
+        self._check(self.L.dl_sync(self.h))
+
+    def set_timing(self, on=True):
+        self._check(self.L.dl_set_timing(self.h, int(on)))
+
+    def phase_time(self, phase):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(self.L.dl_phase_time(self.h, phase.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    # --- tables --------------------------------------------------------
+    def plastic_active(self):
+        m = np.zeros(self.p.n_elems, np.uint8)
+        self._check(self.L.dl_plastic_active(self.h, _p(m, u8p), self.p.n_elems))
+        return m
+
+    def plastic_state(self):
+        E = self.p.n_elems
+        ep = np.zeros(6 * E)
+        al = np.zeros(7 * E)
+        self._check(self.L.dl_plastic_state(self.h, _p(ep), _p(al), E))
+        return ep.reshape(E, 6), al.reshape(E, 7)
+
+    def _pairs(self, fn):
+        n = C.c_int64()
+        self._check(fn(self.h, None, None, 0, C.byref(n)))
+        k = n.value
+        a = np.zeros(max(k, 1), np.int32)
+        b = np.zeros(max(k, 1), np.int32)
+        self._check(fn(self.h, _p(a, ip32), _p(b, ip32), k, C.byref(n)))
+        return a[:k], b[:k]
+
+    def contact_pairs(self):
+        return self._pairs(self.L.dl_contact_pairs)
+
+    def contact_candidates(self):
+        return self._pairs(self.L.dl_contact_candidates)

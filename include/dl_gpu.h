@@ -1,0 +1,217 @@
+/*
+ * dl_gpu.h -- C ABI of the MI355X (gfx950) explicit-dynamics path for
+ * geometrically exact rod lattices (dynLattice hot path).
+ *
+ * The ABI mirrors the reference's JIVE model action surface
+ *   bool Model::takeAction(const String& action, const Properties& params,
+ *                          const Properties& globdat)
+ * (src/models/SpecialCosseratRodModel.h:164-166, src/models/RodContactModel.h:112-114,
+ *  dispatched by src/models/LatticeModel.cpp:115-120) and the explicit
+ * time-marching module's step (src/modules/ExplicitModule.cpp:259-459,
+ * src/modules/LeapFrogModule.cpp:44-84).
+ *
+ * Conventions
+ *  - Every entry point returns DL_OK (0) or a negative DL_ERR_* code; no C++
+ *    exception crosses the ABI.  dl_last_error() gives a message.
+ *  - Host arrays are caller-owned and copied in/out.  Device state (SoA,
+ *    FP64) is owned by the context and stays resident in HBM between calls.
+ *  - DOF vectors use the reference's layout: node-major, 6 DOFs per node in
+ *    the order dx, dy, dz, rx, ry, rz (SpecialCosseratRodModel.cpp:131-137).
+ *  - A context belongs to one HIP stream and is not thread-safe; separate
+ *    contexts are independent.
+ */
+#ifndef DL_GPU_H
+#define DL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DL_OK 0
+#define DL_ERR_INPUT -1        /* IllegalInputException / JEM_PRECHECK on input */
+#define DL_ERR_ZERO_MASS -2    /* ArithmeticException, ExplicitModule.cpp:452-454 */
+#define DL_ERR_PLASTIC_ITER -3 /* JEM_PRECHECK2(liter < maxIter_), ElastoPlasticRodMaterial.cpp:234 */
+#define DL_ERR_HIP -4          /* HIP runtime error */
+#define DL_ERR_CAPACITY -5     /* contact pair buffer too small */
+#define DL_ERR_STATE -6        /* call out of order (e.g. step before INIT) */
+
+/* material models: MaterialFactory "ElasticRod" / "ElastoPlasticRod"
+ * (ElasticRodMaterial.cpp:420-424, ElastoPlasticRodMaterial.cpp:388-392) */
+#define DL_ELASTIC 0
+#define DL_ELASTOPLASTIC 1
+
+/* actions: jive::model::Actions names handled by the rod/contact models */
+#define DL_INIT 0            /* Actions::INIT: frames, initial strain, contact blacklist */
+#define DL_GET_INT_VECTOR 1  /* Actions::GET_INT_VECTOR (+= into params.int_vector) */
+#define DL_GET_MATRIX2 2     /* Actions::GET_MATRIX2: block-lumped mass (6x6 per node) */
+#define DL_COMMIT 3          /* Actions::COMMIT: material applyDeform + energies */
+#define DL_CANCEL 4          /* Actions::CANCEL: material rejectDeform */
+
+/* load cases (globdat LOAD_CASE) */
+#define DL_LOAD_CASE_DEFAULT 0 /* inelastic material update */
+#define DL_LOAD_CASE_OUTPUT 1  /* "output": elastic predictor only (SpecialCosseratRodModel.cpp:762) */
+
+/* dl_options.flags */
+#define DL_TRACK_DISSIPATION 1u /* per-step plastic dissipation history */
+
+typedef struct dl_ctx dl_ctx;
+
+/* Mesh: nodes, 2-node line elements, and the element group ("beam_<s+1>")
+ * of every element.  The elements of one strut must be contiguous and form a
+ * chain in order (element k = (p_k, p_k+1)), which is what one GMSH curve
+ * meshed with Line2 elements is (GMSHInputModule.cpp:308-415). */
+typedef struct {
+  int64_t n_nodes;
+  int64_t n_elems;
+  int32_t n_struts;
+  const double *coords;      /* [3*n_nodes] */
+  const int32_t *elem_nodes; /* [2*n_elems] */
+  const int32_t *elem_strut; /* [n_elems] */
+} dl_mesh;
+
+/* Per-strut section and elastic constants, i.e. the ElasticRodMaterial of
+ * each rod model (ElasticRodMaterial.cpp:65-228). */
+typedef struct {
+  const double *area;          /* A */
+  const double *I1;            /* area moment, material x */
+  const double *I2;            /* area moment, material y */
+  const double *J;             /* polar moment */
+  const double *shear_factor;  /* kappa */
+  const double *young;         /* E */
+  const double *shear_modulus; /* G */
+  const double *density;       /* rho */
+  double inertia_correct;      /* ElasticRodMaterial.cpp:197-200; 1 = off */
+} dl_sections;
+
+/* Elastoplastic rod material (ElastoPlasticRodMaterial.cpp:55-147) with the
+ * yield condition restricted to the power-sum family
+ *   f = sum_i |w_i (s_i + ck_i q_i)|^p_i - Y (1 + ci q_0),  q = -H alpha,
+ *   H = blockdiag(H_iso, diag(kin_const + kin_ratio * C)),
+ * p_i = 1: the sum-abs conditions of tests/plastic/test1,2a,2b.pro;
+ * p_i ~ 2: the shape of tests/plastic/test3.pro. */
+typedef struct {
+  int32_t model;          /* DL_ELASTIC / DL_ELASTOPLASTIC */
+  const double *weights;  /* [6*n_struts] w_i */
+  double yield_level;     /* Y */
+  double kin_coupling[6]; /* ck_i */
+  double iso_coupling;    /* ci */
+  int32_t has_iso;        /* "isotropicCoefficient" given */
+  double iso_modulus;     /* H_iso */
+  int32_t has_kin;        /* "kinematicTensor" given (diagonal) */
+  double kin_const[6];
+  double kin_ratio;
+  double precision;       /* default 1e-5 */
+  int32_t max_iter;       /* default 20 */
+  double exponent[6];     /* p_i (>= 1) */
+} dl_material;
+
+/* Rod-rod penalty contact (RodContactModel.cpp:60-98).  radius is per strut;
+ * the pair threshold is r_A + r_B, equal to the reference's 2 r when all
+ * radii are equal. */
+typedef struct {
+  int32_t enabled;
+  double penalty_sts;
+  double penalty_nts;
+  const double *radius; /* [n_struts] */
+  int64_t max_pairs;    /* candidate-pair capacity; 0 = 32 * n_elems */
+} dl_contact;
+
+typedef struct {
+  int32_t so3;      /* dofs_SO3 given: consistent block mass, gyro, Lie-group update */
+  int32_t device;   /* HIP device ordinal */
+  void *stream;     /* hipStream_t to run on; NULL = the context creates its own */
+  uint32_t flags;   /* DL_TRACK_DISSIPATION */
+} dl_options;
+
+/* Action parameters (ActionParams::INT_VECTOR / MATRIX2, LOAD_CASE). */
+typedef struct {
+  double *int_vector;  /* [6*n_nodes] host, accumulated with += (GET_INT_VECTOR) */
+  double *matrix2;     /* [36*n_nodes] host 6x6 row-major blocks, += (GET_MATRIX2) */
+  int32_t load_case;   /* DL_LOAD_CASE_* */
+  int32_t with_gyro;   /* MATRIX2 passed with GET_INT_VECTOR: add gyroscopic term */
+  int32_t lumped;      /* GET_MATRIX2 into a diagonal (LUMPED) builder */
+  double e_pot;        /* COMMIT: potentialEnergy */
+  double e_diss;       /* COMMIT: dissipatedEnergy */
+} dl_params;
+
+/* Output of dl_explicit_step: values after the last step. */
+typedef struct {
+  double reaction;      /* reaction group sum at the last step's start state */
+  double e_diss;        /* accumulated plastic dissipation (DL_TRACK_DISSIPATION) */
+  int64_t n_contacts;   /* active contact pairs in the last step */
+  int64_t n_candidates; /* broadphase candidates in the last step */
+  int64_t steps_done;
+} dl_step_out;
+
+/* Replaces: model tree construction by factories (ModelFactory "Lattice" ->
+ * N x "specialCosseratRod" + "RodContact", LatticeModel.cpp:61-96;
+ * SpecialCosseratRodModel.cpp:100-186) and ExplicitModule::init mass set-up
+ * (ExplicitModule.cpp:58-196). */
+int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_material *mat,
+              const dl_contact *contact, const dl_options *opt, dl_ctx **out);
+int dl_destroy(dl_ctx *ctx);
+
+/* Replaces: Model::takeAction(action, params, globdat) for the rod lattice
+ * (SpecialCosseratRodModel.cpp:196-400; RodContactModel.cpp:107-240;
+ * LatticeModel.cpp:110-192). */
+int dl_take_action(dl_ctx *ctx, int32_t action, dl_params *params);
+
+/* Replaces: StateVector::get/store(disp|velo, STATE0|STATE1, dofs, globdat)
+ * (used at SpecialCosseratRodModel.cpp:349, LeapFrogModule.cpp:62-64). */
+int dl_set_state(dl_ctx *ctx, const double *disp, const double *velo, int64_t ndof);
+int dl_get_state(dl_ctx *ctx, double *disp, double *velo, int64_t ndof);
+
+/* Replaces: Constraints from GET_CONSTRAINTS (DirichletModel.cpp:326-366,
+ * ImpactModel.cpp:74-104) and f_ext from GET_EXT_VECTOR
+ * (NeumannModel.cpp:291-352): cmask[6N] 1 = constrained, cval[6N] the
+ * prescribed acceleration, fext[6N] constant external force. */
+int dl_set_bcs(dl_ctx *ctx, const uint8_t *cmask, const double *cval, const double *fext,
+               int64_t ndof);
+
+/* Replaces: GroupOutputModule's resp sum over a node group's DOF
+ * (GroupOutputModule.cpp:100-116,162), recorded once per step. */
+int dl_set_reaction_group(dl_ctx *ctx, const int32_t *nodes, int64_t n, int32_t dof);
+
+/* Replaces: ControlModule -> ExplicitModule::advance / LeapFrogModule::solve /
+ * ExplicitModule::commit, nsteps times with fixed dt (maxDtime = minDtime =
+ * deltaTime), with the intended store of u, v (LeapFrogModule.cpp:44-84,
+ * ExplicitModule.cpp:259-419).  Runs asynchronously on the context's stream;
+ * out (may be NULL) is filled after a stream synchronisation. */
+int dl_explicit_step(dl_ctx *ctx, int64_t nsteps, double dt, dl_step_out *out);
+
+/* Per-step history of the last dl_explicit_step call: reaction, dissipation,
+ * active contact pairs; n = number of steps recorded. */
+int dl_get_history(dl_ctx *ctx, double *reaction, double *e_diss, int64_t *n_contacts,
+                   int64_t cap, int64_t *n);
+
+/* Replaces: RodContactModel contactsA_/contactsB_ (RodContactModel.cpp:732-733):
+ * active pairs of the last force evaluation, sorted by (a, b). */
+int dl_contact_pairs(dl_ctx *ctx, int32_t *a, int32_t *b, int64_t cap, int64_t *n);
+/* Broadphase candidates of the last force evaluation, sorted by (a, b). */
+int dl_contact_candidates(dl_ctx *ctx, int32_t *a, int32_t *b, int64_t cap, int64_t *n);
+
+/* Replaces: the plastic-strain table of ElastoPlasticRodMaterial::getTable
+ * (ElastoPlasticRodMaterial.cpp:318-340): per element the plastic-correction
+ * flag of the last material update (mask[n_elems]), plastic strain [6E] and
+ * hardening parameters [7E] (iso, kin...) of the committed state. */
+int dl_plastic_active(dl_ctx *ctx, uint8_t *mask, int64_t n);
+int dl_plastic_state(dl_ctx *ctx, double *eps_p, double *alpha, int64_t n_elems);
+
+/* Device timing of the kernels of the last dl_explicit_step call, measured
+ * with HIP events on the context's stream: total milliseconds of the named
+ * phase ("step", "contact", "strut", "joint") and the number of steps. */
+int dl_phase_time(dl_ctx *ctx, const char *phase, double *ms, int64_t *count);
+int dl_set_timing(dl_ctx *ctx, int32_t enabled);
+
+/* Wait for the context's stream. */
+int dl_sync(dl_ctx *ctx);
+
+/* Message of the last failed call on this thread. */
+const char *dl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1443 @@
+/*
+ * ref_cpu.c -- the oracle: a CPU restatement of dynLattice's explicit
+ * dynamics path (rod element, rod materials, block-lumped mass, gyro,
+ * LeapFrog with store, rod-rod contact, commit energies).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, by the smoke check and by the
+ * CPU-baseline leg of bench.py.  The product library never links it.
+ *
+ * Operation order follows the reference line by line (matrix route: the
+ * Crisfield-Jelenic interpolation through logMat/expVec exactly as
+ * Line3D.cpp and helpers.cpp evaluate it, material calls as in
+ * ElasticRodMaterial.cpp / ElastoPlasticRodMaterial.cpp).  Repeated helper
+ * calls on identical inputs are evaluated once and reused: they are
+ * deterministic, so the values are the same bits.
+ *
+ * Third-party predicates the reference takes from JEM/JIVE (not vendored)
+ * are fixed here as named choices:
+ *   REF_IS_TINY           jem::isTiny(x)  -> |x| < DBL_MIN
+ *   ref_contains          StdLine::containsLocalPoint(u) -> -1 <= u <= 1
+ *   shape functions       StdLine2: N = (0.5(1-u), 0.5(1+u))
+ *   Gauss points          Gauss1: u=0, w=2;  Gauss2: u=-+1/sqrt(3), w=1
+ *   yield gradient        analytic, zeroed where |s_i| < DBL_EPSILON
+ *   consistent solve      exact per-node 6x6 solve of the reduced system
+ * Parity at these points is therefore "unpinned" against the real JEM/JIVE
+ * build, which cannot be built here.
+ */
+#include "ref_cpu.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TINY_H (DBL_EPSILON * 1e6) /* helpers.h:55 */
+#define REF_IS_TINY(x) (fabs(x) < DBL_MIN)
+
+/* ------------------------------------------------------------------ */
+/* small dense helpers (row-major 3x3)                                 */
+/* ------------------------------------------------------------------ */
+static void eye3(double R[9]) {
+  memset(R, 0, 9 * sizeof(double));
+  R[0] = R[4] = R[8] = 1.0;
+}
+static void skew3(const double v[3], double M[9]) { /* helpers.cpp:242-255 */
+  M[0] = 0.0;   M[1] = -1 * v[2]; M[2] = 1 * v[1];
+  M[3] = 1 * v[2]; M[4] = 0.0;   M[5] = -1 * v[0];
+  M[6] = -1 * v[1]; M[7] = 1 * v[0]; M[8] = 0.0;
+}
+static void unskew3(const double M[9], double v[3]) { /* helpers.cpp:257-281 */
+  v[0] = M[7] - M[5];
+  v[1] = M[2] - M[6];
+  v[2] = M[3] - M[1];
+  v[0] /= 2.; v[1] /= 2.; v[2] /= 2.;
+}
+static void mm3(const double A[9], const double B[9], double C[9]) {
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = A[3 * i] * B[j];
+      s += A[3 * i + 1] * B[3 + j];
+      s += A[3 * i + 2] * B[6 + j];
+      T[3 * i + j] = s;
+    }
+  memcpy(C, T, sizeof T);
+}
+static void mtm3(const double A[9], const double B[9], double C[9]) { /* A^T B */
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = A[i] * B[j];
+      s += A[3 + i] * B[3 + j];
+      s += A[6 + i] * B[6 + j];
+      T[3 * i + j] = s;
+    }
+  memcpy(C, T, sizeof T);
+}
+static void mmt3(const double A[9], const double B[9], double C[9]) { /* A B^T */
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = A[3 * i] * B[3 * j];
+      s += A[3 * i + 1] * B[3 * j + 1];
+      s += A[3 * i + 2] * B[3 * j + 2];
+      T[3 * i + j] = s;
+    }
+  memcpy(C, T, sizeof T);
+}
+static void mv3(const double A[9], const double x[3], double y[3]) {
+  double t[3];
+  for (int i = 0; i < 3; i++) {
+    double s = A[3 * i] * x[0];
+    s += A[3 * i + 1] * x[1];
+    s += A[3 * i + 2] * x[2];
+    t[i] = s;
+  }
+  y[0] = t[0]; y[1] = t[1]; y[2] = t[2];
+}
+static void mtv3(const double A[9], const double x[3], double y[3]) { /* A^T x */
+  double t[3];
+  for (int i = 0; i < 3; i++) {
+    double s = A[i] * x[0];
+    s += A[3 + i] * x[1];
+    s += A[6 + i] * x[2];
+    t[i] = s;
+  }
+  y[0] = t[0]; y[1] = t[1]; y[2] = t[2];
+}
+static double dot3(const double a[3], const double b[3]) {
+  double s = a[0] * b[0];
+  s += a[1] * b[1];
+  s += a[2] * b[2];
+  return s;
+}
+static double norm3(const double a[3]) { return sqrt(dot3(a, a)); }
+
+/* ------------------------------------------------------------------ */
+/* SO(3) maps: helpers.cpp                                             */
+/* ------------------------------------------------------------------ */
+void ref_expvec(const double psi[3], double R[9]) { /* helpers.cpp:139-157 */
+  eye3(R);
+  const double theta = norm3(psi);
+  if (REF_IS_TINY(theta)) {
+    double K[9];
+    skew3(psi, K);
+    for (int i = 0; i < 9; i++) R[i] += K[i];
+  } else {
+    double k[3] = {psi[0] / theta, psi[1] / theta, psi[2] / theta};
+    double K[9], KK[9];
+    skew3(k, K);
+    const double s = sin(theta);
+    for (int i = 0; i < 9; i++) R[i] += s * K[i];
+    mm3(K, K, KK);
+    const double c1 = 1 - cos(theta);
+    for (int i = 0; i < 9; i++) R[i] += c1 * KK[i];
+  }
+}
+
+void ref_expvecp(const double psi[3], const double psiP[3], double out[9]) { /* helpers.cpp:159-197 */
+  const double theta = norm3(psi);
+  if (theta < TINY_H) {
+    skew3(psiP, out);
+    return;
+  }
+  const double thetaP = dot3(psi, psiP) / theta;
+  double k[3], kP[3], K[9], KP[9], T[9];
+  for (int i = 0; i < 3; i++) {
+    k[i] = psi[i] / theta;
+    kP[i] = (psiP[i] * theta - psi[i] * thetaP) / theta / theta;
+  }
+  skew3(k, K);
+  skew3(kP, KP);
+  for (int i = 0; i < 9; i++) out[i] = 0;
+  const double ct = cos(theta), st = sin(theta);
+  for (int i = 0; i < 9; i++) out[i] += ct * thetaP * K[i];
+  for (int i = 0; i < 9; i++) out[i] += st * KP[i];
+  mm3(K, K, T);
+  for (int i = 0; i < 9; i++) out[i] += st * thetaP * T[i];
+  mm3(K, KP, T);
+  for (int i = 0; i < 9; i++) out[i] += (1 - ct) * T[i];
+  mm3(KP, K, T);
+  for (int i = 0; i < 9; i++) out[i] += (1 - ct) * T[i];
+}
+
+void ref_logmat(const double R[9], double rv[3]) { /* helpers.cpp:99-111 */
+  double tr = R[0];
+  tr += R[4];
+  tr += R[8];
+  const double tr_R = tr <= 3. ? tr : 3.;
+  const double theta = acos((tr_R - 1.) / 2.);
+  double M[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) M[3 * i + j] = R[3 * i + j] - R[3 * j + i];
+  unskew3(M, rv);
+  if (REF_IS_TINY(theta)) {
+    const double h = 1 / 2.;
+    rv[0] *= h; rv[1] *= h; rv[2] *= h;
+  } else {
+    const double f = theta / (2. * sin(theta));
+    rv[0] *= f; rv[1] *= f; rv[2] *= f;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* model                                                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int32_t n_el;
+  int32_t *elems;   /* global element indices in group order */
+  int32_t n_nd;
+  int32_t *nodes;   /* sorted unique global node indices (getNodeIndices) */
+  double *lambdaN;  /* [9*n_nd] LambdaN_ per rod node (SpecialCosseratRodModel.cpp:598) */
+  double box[6];    /* scratch: rod AABB */
+} strut_t;
+
+struct ref_model {
+  int64_t N, E;
+  int32_t R;
+  double *X0;        /* [3N] */
+  int32_t *en;       /* [2E] */
+  int32_t *es;       /* [E] */
+  strut_t *st;
+  int32_t *el_local; /* [E] index of element within its strut */
+  /* sections / materials per strut */
+  double *K;         /* [6R] diag material stiffness (calcMaterialStiff_) */
+  double *Mm;        /* [6R] diag material mass (calcMaterialMass_) */
+  double *area, *rho;
+  /* plastic */
+  ref_plastic pl;
+  double *pw;        /* [6R] yield weights */
+  double *Hk;        /* [6R] kinematic hardening diag */
+  /* material state per element (ipCount = 1) */
+  double *strain0;   /* [6E] matStrain0_ */
+  double *curr_strain, *old_strain; /* [6E] */
+  double *epot;      /* [E] energyPot_ */
+  double *curr_ep, *old_ep;         /* [6E] */
+  double *curr_al, *old_al;         /* [7E] iso, kin... */
+  double *dflow;     /* [E] currDeltaFlow_ */
+  double *ediss;     /* [E] energyDiss_ */
+  double *ehard;     /* [E] */
+  uint8_t *active;   /* [E] plastic correction taken in last call */
+  /* contact */
+  ref_contact_opts co;
+  double *crad;      /* [R] */
+  int64_t n_black;
+  int64_t *black;    /* sorted keys (min<<32 | max) of blacklisted pairs */
+  int64_t n_act, cap_act;
+  int32_t *actA, *actB;
+  int use_grid;
+  /* bcs */
+  uint8_t *cmask;
+  double *cval, *fext;
+  /* options */
+  int so3;
+  double inertia_correct;
+  int err;
+};
+
+int ref_last_error(const ref_model *m) { return m->err; }
+
+static int cmp_i32(const void *a, const void *b) {
+  int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+  return (x > y) - (x < y);
+}
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return (x > y) - (x < y);
+}
+
+static int rod_local_node(const strut_t *s, int32_t g) {
+  int lo = 0, hi = s->n_nd - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    if (s->nodes[mid] == g) return mid;
+    if (s->nodes[mid] < g) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+ref_model *ref_create(const ref_mesh *mesh, const ref_sections *sec,
+                      const ref_plastic *pl, const ref_contact_opts *co,
+                      const ref_options *opt, const double *disp0) {
+  (void)disp0;
+  ref_model *m = (ref_model *)calloc(1, sizeof(ref_model));
+  m->N = mesh->n_nodes;
+  m->E = mesh->n_elems;
+  m->R = mesh->n_struts;
+  const int64_t N = m->N, E = m->E;
+  const int32_t R = m->R;
+  m->X0 = (double *)malloc(sizeof(double) * 3 * N);
+  memcpy(m->X0, mesh->coords, sizeof(double) * 3 * N);
+  m->en = (int32_t *)malloc(sizeof(int32_t) * 2 * E);
+  memcpy(m->en, mesh->elem_nodes, sizeof(int32_t) * 2 * E);
+  m->es = (int32_t *)malloc(sizeof(int32_t) * E);
+  memcpy(m->es, mesh->elem_strut, sizeof(int32_t) * E);
+  m->el_local = (int32_t *)malloc(sizeof(int32_t) * E);
+
+  /* struts: element groups "beam_<s+1>" (LatticeModel.cpp:61-74) */
+  m->st = (strut_t *)calloc(R, sizeof(strut_t));
+  int32_t *cnt = (int32_t *)calloc(R, sizeof(int32_t));
+  for (int64_t e = 0; e < E; e++) {
+    if (m->es[e] < 0 || m->es[e] >= R) { m->err = REF_ERR_INPUT; free(cnt); return m; }
+    cnt[m->es[e]]++;
+  }
+  for (int32_t s = 0; s < R; s++) {
+    m->st[s].elems = (int32_t *)malloc(sizeof(int32_t) * (cnt[s] ? cnt[s] : 1));
+    m->st[s].n_el = 0;
+  }
+  for (int64_t e = 0; e < E; e++) {
+    strut_t *s = &m->st[m->es[e]];
+    m->el_local[e] = s->n_el;
+    s->elems[s->n_el++] = (int32_t)e;
+  }
+  for (int32_t s = 0; s < R; s++) {
+    strut_t *S = &m->st[s];
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * 2 * (S->n_el ? S->n_el : 1));
+    for (int i = 0; i < S->n_el; i++) {
+      tmp[2 * i] = m->en[2 * S->elems[i]];
+      tmp[2 * i + 1] = m->en[2 * S->elems[i] + 1];
+    }
+    qsort(tmp, 2 * S->n_el, sizeof(int32_t), cmp_i32);
+    int n = 0;
+    for (int i = 0; i < 2 * S->n_el; i++)
+      if (i == 0 || tmp[i] != tmp[i - 1]) tmp[n++] = tmp[i];
+    S->n_nd = n;
+    S->nodes = tmp;
+    S->lambdaN = (double *)calloc(9 * (n ? n : 1), sizeof(double));
+  }
+  free(cnt);
+
+  /* materials (ElasticRodMaterial.cpp:265-287) */
+  m->K = (double *)malloc(sizeof(double) * 6 * R);
+  m->Mm = (double *)malloc(sizeof(double) * 6 * R);
+  m->area = (double *)malloc(sizeof(double) * R);
+  m->rho = (double *)malloc(sizeof(double) * R);
+  m->inertia_correct = sec->inertia_correct;
+  for (int32_t s = 0; s < R; s++) {
+    const double G = sec->shear_modulus[s], Y = sec->young[s], k = sec->shear_factor[s];
+    const double A = sec->area[s], I1 = sec->I1[s], I2 = sec->I2[s], J = sec->J[s], rho = sec->density[s];
+    double *K = m->K + 6 * s, *M = m->Mm + 6 * s;
+    K[0] = G * k * A;
+    K[1] = G * k * A;
+    K[2] = Y * A;
+    K[3] = Y * I1;
+    K[4] = Y * I2;
+    K[5] = G * J;
+    M[0] = rho * A; M[1] = rho * A; M[2] = rho * A;
+    M[3] = rho * I1; M[4] = rho * I2; M[5] = rho * J;
+    if (sec->inertia_correct != 1.0)
+      for (int i = 3; i < 6; i++) M[i] *= sec->inertia_correct;
+    m->area[s] = A;
+    m->rho[s] = rho;
+  }
+
+  /* plastic */
+  memset(&m->pl, 0, sizeof m->pl);
+  m->pl.model = REF_ELASTIC;
+  if (pl) {
+    m->pl = *pl;
+    if (pl->model == REF_ELASTOPLASTIC) {
+      m->pw = (double *)malloc(sizeof(double) * 6 * R);
+      memcpy(m->pw, pl->weights, sizeof(double) * 6 * R);
+      m->Hk = (double *)malloc(sizeof(double) * 6 * R);
+      for (int32_t s = 0; s < R; s++)
+        for (int i = 0; i < 6; i++)
+          m->Hk[6 * s + i] = pl->kin_const[i] + pl->kin_ratio * m->K[6 * s + i];
+    }
+    m->pl.weights = NULL;
+  }
+
+  m->strain0 = (double *)calloc(6 * E, sizeof(double));
+  m->curr_strain = (double *)calloc(6 * E, sizeof(double));
+  m->old_strain = (double *)calloc(6 * E, sizeof(double));
+  m->epot = (double *)calloc(E, sizeof(double));
+  m->curr_ep = (double *)calloc(6 * E, sizeof(double));
+  m->old_ep = (double *)calloc(6 * E, sizeof(double));
+  m->curr_al = (double *)calloc(7 * E, sizeof(double));
+  m->old_al = (double *)calloc(7 * E, sizeof(double));
+  m->dflow = (double *)calloc(E, sizeof(double));
+  m->ediss = (double *)calloc(E, sizeof(double));
+  m->ehard = (double *)calloc(E, sizeof(double));
+  m->active = (uint8_t *)calloc(E, 1);
+
+  memset(&m->co, 0, sizeof m->co);
+  if (co && co->enabled) {
+    m->co = *co;
+    m->crad = (double *)malloc(sizeof(double) * R);
+    memcpy(m->crad, co->radius, sizeof(double) * R);
+    m->co.radius = NULL;
+  }
+  m->cap_act = 1024;
+  m->actA = (int32_t *)malloc(sizeof(int32_t) * m->cap_act);
+  m->actB = (int32_t *)malloc(sizeof(int32_t) * m->cap_act);
+
+  m->cmask = (uint8_t *)calloc(6 * N, 1);
+  m->cval = (double *)calloc(6 * N, sizeof(double));
+  m->fext = (double *)calloc(6 * N, sizeof(double));
+  m->so3 = opt ? opt->so3 : 1;
+  return m;
+}
+
+void ref_destroy(ref_model *m) {
+  if (!m) return;
+  for (int32_t s = 0; s < m->R; s++) {
+    free(m->st[s].elems);
+    free(m->st[s].nodes);
+    free(m->st[s].lambdaN);
+  }
+  free(m->st); free(m->X0); free(m->en); free(m->es); free(m->el_local);
+  free(m->K); free(m->Mm); free(m->area); free(m->rho); free(m->pw); free(m->Hk);
+  free(m->strain0); free(m->curr_strain); free(m->old_strain); free(m->epot);
+  free(m->curr_ep); free(m->old_ep); free(m->curr_al); free(m->old_al);
+  free(m->dflow); free(m->ediss); free(m->ehard); free(m->active);
+  free(m->crad); free(m->black); free(m->actA); free(m->actB);
+  free(m->cmask); free(m->cval); free(m->fext);
+  free(m);
+}
+
+int ref_set_bcs(ref_model *m, const uint8_t *cmask, const double *cval,
+                const double *fext) {
+  const int64_t n = 6 * m->N;
+  if (cmask) memcpy(m->cmask, cmask, n); else memset(m->cmask, 0, n);
+  if (cval) memcpy(m->cval, cval, n * sizeof(double)); else memset(m->cval, 0, n * sizeof(double));
+  if (fext) memcpy(m->fext, fext, n * sizeof(double)); else memset(m->fext, 0, n * sizeof(double));
+  return REF_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Line3D (2-node)                                                     */
+/* ------------------------------------------------------------------ */
+/* getShapeGradients (Line3D.cpp:102-115) with Gauss1: g[2], w */
+static void shape_grads_g1(const double c0[3], const double c1[3], double g[2], double *w) {
+  double d[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
+  const double L = norm3(d) + 0.;
+  /* ParametricLine: J = dN0/du * 0 + dN1/du * L  (local coords c1 = [0, L]) */
+  const double J = -0.5 * 0. + 0.5 * L;
+  g[0] = -0.5 / J;
+  g[1] = 0.5 / J;
+  *w = 2. * J;
+}
+
+/* getRefRot_ + getNodeRotVecs_ (Line3D.cpp:249-273) */
+static void node_rot_vecs(const double Rn0[9], const double Rn1[9], double Lr[9],
+                          double psi0[3], double psi1[3]) {
+  double T[9], phi[3], E[9];
+  mtm3(Rn0, Rn1, T);
+  ref_logmat(T, phi);
+  phi[0] = 0.5 * phi[0]; phi[1] = 0.5 * phi[1]; phi[2] = 0.5 * phi[2];
+  ref_expvec(phi, E);
+  mm3(Rn0, E, Lr);
+  mtm3(Lr, Rn0, T);
+  ref_logmat(T, psi0);
+  mtm3(Lr, Rn1, T);
+  ref_logmat(T, psi1);
+}
+
+/* ------------------------------------------------------------------ */
+/* materials                                                           */
+/* ------------------------------------------------------------------ */
+static void elastic_stress(const double *K, const double *strain, double *stress) {
+  for (int i = 0; i < 6; i++) stress[i] = K[i] * strain[i];
+}
+
+/* power-sum yield family: value and analytic gradient over (s, h0, hk[6])
+ *   f = sum_i |w_i a_i|^p_i - Y (1 + ci q0),  a_i = s_i + ck_i q_i
+ * p_i = 1 and p_i = 2 are evaluated without pow(). */
+static double pterm(double x, double p) { /* |x|^p */
+  const double ax = fabs(x);
+  if (p == 1.0) return ax;
+  if (p == 2.0) return ax * ax;
+  return pow(ax, p);
+}
+static double pderiv(double x, double p) { /* d|x|^p / dx without the sign */
+  const double ax = fabs(x);
+  if (p == 1.0) return 1.0;
+  if (p == 2.0) return 2.0 * ax;
+  return p * pow(ax, p - 1.0);
+}
+static double yield_value(const ref_model *m, const double *w, const double s[6],
+                          const double q[7]) {
+  double f = 0.;
+  for (int i = 0; i < 6; i++) {
+    const double a = s[i] + (m->pl.has_kin ? m->pl.kin_coupling[i] * q[1 + i] : 0.);
+    f += pterm(w[i] * a, m->pl.exponent[i]);
+  }
+  f -= m->pl.yield_level * (1. + (m->pl.has_iso ? m->pl.iso_coupling * q[0] : 0.));
+  return f;
+}
+static double sgn(double x) { return (x > 0) - (x < 0); }
+static void yield_grad(const ref_model *m, const double *w, const double s[6],
+                       const double q[7], double gs[6], double gq[7]) {
+  for (int i = 0; i < 6; i++) {
+    const double a = s[i] + (m->pl.has_kin ? m->pl.kin_coupling[i] * q[1 + i] : 0.);
+    const double d = pderiv(w[i] * a, m->pl.exponent[i]) * w[i] * sgn(a);
+    gs[i] = d;
+    gq[1 + i] = m->pl.has_kin ? d * m->pl.kin_coupling[i] : 0.;
+    /* numeric-derivative path: zero where |args_i| < eps (ElastoPlasticRodMaterial.cpp:249-253) */
+    if (fabs(s[i]) < DBL_EPSILON) gs[i] = 0.;
+  }
+  gq[0] = m->pl.has_iso ? -m->pl.yield_level * m->pl.iso_coupling : 0.;
+}
+
+/* Material::getStress(stress, strain, ielem, ip, inelastic) */
+static int material_stress(ref_model *m, int64_t e, const double strain[6],
+                           double stress[6], int inelastic) {
+  const int32_t s = m->es[e];
+  const double *K = m->K + 6 * s;
+  if (m->pl.model == REF_ELASTIC) { /* ElasticRodMaterial.cpp:363-367 */
+    memcpy(m->curr_strain + 6 * e, strain, 6 * sizeof(double));
+    elastic_stress(K, strain, stress);
+    return REF_OK;
+  }
+  /* ElastoPlasticRodMaterial.cpp:198-269 (cutting plane) */
+  const double *w = m->pw + 6 * s;
+  const double *Hk = m->Hk + 6 * s;
+  double ep[6], al[7], q[7], gs[6], gq[7], el[6];
+  memcpy(ep, m->old_ep + 6 * e, sizeof ep);
+  memcpy(al, m->old_al + 7 * e, sizeof al);
+  double dflow = 0.;
+  int liter = 0;
+  int took = 0;
+  for (;;) {
+    for (int i = 0; i < 6; i++) el[i] = strain[i] - ep[i];
+    elastic_stress(K, el, stress);
+    /* getHardVals: q = -H alpha, H = blockdiag(H_iso, diag(Hk)) */
+    q[0] = -1. * (m->pl.has_iso ? m->pl.iso_modulus * al[0] : 0.);
+    for (int i = 0; i < 6; i++) q[1 + i] = -1. * (m->pl.has_kin ? Hk[i] * al[1 + i] : 0.);
+    if (!inelastic) break;
+    const double f = yield_value(m, w, stress, q);
+    if (!(liter < m->pl.max_iter)) { m->err = REF_ERR_PLASTIC_ITER; return REF_ERR_PLASTIC_ITER; }
+    if (f < m->pl.precision) break;
+    yield_grad(m, w, stress, q, gs, gq);
+    double den = 0.;
+    {
+      double a = gs[0] * (K[0] * gs[0]);
+      for (int i = 1; i < 6; i++) a += gs[i] * (K[i] * gs[i]);
+      double b = 0.;
+      int first = 1;
+      if (m->pl.has_iso) { b = gq[0] * (m->pl.iso_modulus * gq[0]); first = 0; }
+      if (m->pl.has_kin)
+        for (int i = 0; i < 6; i++) {
+          const double t = gq[1 + i] * (Hk[i] * gq[1 + i]);
+          if (first) { b = t; first = 0; } else b += t;
+        }
+      den = a + b;
+    }
+    const double ddl = f / den;
+    for (int i = 0; i < 6; i++) ep[i] += ddl * gs[i];
+    if (m->pl.has_iso) al[0] += ddl * gq[0];
+    if (m->pl.has_kin)
+      for (int i = 0; i < 6; i++) al[1 + i] += ddl * gq[1 + i];
+    dflow += ddl;
+    liter++;
+    took = 1;
+  }
+  memcpy(m->curr_strain + 6 * e, strain, 6 * sizeof(double));
+  memcpy(m->curr_ep + 6 * e, ep, sizeof ep);
+  memcpy(m->curr_al + 7 * e, al, sizeof al);
+  m->dflow[e] = dflow;
+  m->active[e] = (uint8_t)took;
+  return REF_OK;
+}
+
+/* applyDeform (ElasticRodMaterial.cpp:377-389; ElastoPlasticRodMaterial.cpp:271-307) */
+static void apply_deform(ref_model *m) {
+  const int64_t E = m->E;
+  if (m->pl.model == REF_ELASTIC) {
+    for (int64_t e = 0; e < E; e++) {
+      const double *K = m->K + 6 * m->es[e];
+      const double *c = m->curr_strain + 6 * e;
+      double s[6];
+      elastic_stress(K, c, s);
+      double d = c[0] * s[0];
+      for (int i = 1; i < 6; i++) d += c[i] * s[i];
+      m->epot[e] = 0.5 * d;
+    }
+    memcpy(m->old_strain, m->curr_strain, 6 * E * sizeof(double));
+    return;
+  }
+  for (int64_t e = 0; e < E; e++) {
+    const int32_t s = m->es[e];
+    const double *K = m->K + 6 * s;
+    const double *Hk = m->Hk + 6 * s;
+    double oel[6], cel[6], dep[6], os[6], cs[6];
+    for (int i = 0; i < 6; i++) {
+      oel[i] = m->old_strain[6 * e + i] - m->old_ep[6 * e + i];
+      cel[i] = m->curr_strain[6 * e + i] - m->curr_ep[6 * e + i];
+      dep[i] = m->curr_ep[6 * e + i] - m->old_ep[6 * e + i];
+    }
+    elastic_stress(K, oel, os);
+    elastic_stress(K, cel, cs);
+    {
+      const double *a = m->curr_al + 7 * e;
+      double h = 0.;
+      int first = 1;
+      if (m->pl.has_iso) { h = a[0] * (m->pl.iso_modulus * a[0]); first = 0; }
+      if (m->pl.has_kin)
+        for (int i = 0; i < 6; i++) {
+          const double t = a[1 + i] * (Hk[i] * a[1 + i]);
+          if (first) { h = t; first = 0; } else h += t;
+        }
+      m->ehard[e] = 0.5 * h;
+    }
+    double d = cel[0] * cs[0];
+    for (int i = 1; i < 6; i++) d += cel[i] * cs[i];
+    m->epot[e] = 0.5 * d;
+    if (fabs(m->dflow[e]) > DBL_EPSILON) {
+      double t = ((os[0] + cs[0]) / 2.) * dep[0];
+      for (int i = 1; i < 6; i++) t += ((os[i] + cs[i]) / 2.) * dep[i];
+      m->ediss[e] += t;
+    }
+  }
+  memcpy(m->old_al, m->curr_al, 7 * E * sizeof(double));
+  memcpy(m->old_ep, m->curr_ep, 6 * E * sizeof(double));
+  memcpy(m->old_strain, m->curr_strain, 6 * E * sizeof(double));
+  memset(m->dflow, 0, E * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* rod element                                                         */
+/* ------------------------------------------------------------------ */
+/* getDisplacments_ (SpecialCosseratRodModel.cpp:773-798): X0, u, Lambda */
+static void elem_gather(const ref_model *m, int64_t e, const double *disp,
+                        double c[2][3], double u[2][3], double L[2][9]) {
+  const strut_t *S = &m->st[m->es[e]];
+  for (int k = 0; k < 2; k++) {
+    const int32_t n = m->en[2 * e + k];
+    for (int i = 0; i < 3; i++) {
+      c[k][i] = m->X0[3 * n + i];
+      u[k][i] = disp ? disp[6 * n + i] : 0.;
+    }
+    double th[3] = {disp ? disp[6 * n + 3] : 0., disp ? disp[6 * n + 4] : 0.,
+                    disp ? disp[6 * n + 5] : 0.};
+    double Ex[9];
+    ref_expvec(th, Ex);
+    const int ln = rod_local_node(S, n);
+    mm3(Ex, S->lambdaN + 9 * ln, L[k]);
+  }
+}
+
+/* getStrains_ (SpecialCosseratRodModel.cpp:694-744), material frame, plus the
+ * interpolated rotation Lambda_ip and phi' used by getXi / getPi. */
+static void elem_strains(const double c[2][3], const double u[2][3],
+                         const double L[2][9], const double *strain0,
+                         double strain[6], double Lip[9], double phiP[3],
+                         double g[2], double *w) {
+  shape_grads_g1(c[0], c[1], g, w);
+  for (int i = 0; i < 3; i++) {
+    double s = (c[0][i] + u[0][i]) * g[0];
+    s += (c[1][i] + u[1][i]) * g[1];
+    phiP[i] = s;
+  }
+  double Lr[9], psi0[3], psi1[3];
+  node_rot_vecs(L[0], L[1], Lr, psi0, psi1);
+  /* getRotations: ip_psi = node_psi * N,  N = (0.5, 0.5) at Gauss1 */
+  double ippsi[3], ippsiP[3], Ex[9], LP[9], T[9];
+  for (int i = 0; i < 3; i++) {
+    double s = psi0[i] * 0.5;
+    s += psi1[i] * 0.5;
+    ippsi[i] = s;
+    double t = psi0[i] * g[0];
+    t += psi1[i] * g[1];
+    ippsiP[i] = t;
+  }
+  ref_expvec(ippsi, Ex);
+  mm3(Lr, Ex, Lip);
+  /* getRotationGradients */
+  ref_expvecp(ippsi, ippsiP, T);
+  mm3(Lr, T, LP);
+  mtv3(Lip, phiP, strain);
+  mtm3(Lip, LP, T);
+  unskew3(T, strain + 3);
+  if (strain0)
+    for (int i = 0; i < 6; i++) strain[i] -= strain0[i];
+}
+
+int ref_elem_fint(ref_model *m, int64_t e, const double *disp, double *f, int inelastic) {
+  double c[2][3], u[2][3], L[2][9], strain[6], Lip[9], phiP[3], g[2], w, sm[6], ss[6];
+  elem_gather(m, e, disp, c, u, L);
+  elem_strains(c, u, L, m->strain0 + 6 * e, strain, Lip, phiP, g, &w);
+  int rc = material_stress(m, e, strain, sm, inelastic);
+  if (rc) return rc;
+  /* spatial: PI * stress (SpecialCosseratRodModel.cpp:764-770) */
+  mv3(Lip, sm, ss);
+  mv3(Lip, sm + 3, ss + 3);
+  /* XI (Line3D.cpp:141-162) and fint[Idofs] += w * XI * stress (:945) */
+  double SK[9];
+  skew3(phiP, SK);
+  for (int k = 0; k < 2; k++) {
+    const double N = 0.5;
+    double r[6];
+    for (int i = 0; i < 3; i++) r[i] = g[k] * ss[i];
+    for (int i = 0; i < 3; i++) {
+      double t = (-1. * N * SK[3 * i + 0]) * ss[0];
+      t += (-1. * N * SK[3 * i + 1]) * ss[1];
+      t += (-1. * N * SK[3 * i + 2]) * ss[2];
+      t += g[k] * ss[3 + i];
+      r[3 + i] = t;
+    }
+    for (int i = 0; i < 6; i++) f[6 * k + i] = w * r[i];
+  }
+  return REF_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* INIT: initRotation_ + initStrain_ (SpecialCosseratRodModel.cpp:536-658) */
+/* ------------------------------------------------------------------ */
+static void init_rotation(ref_model *m, strut_t *S) {
+  const int n = S->n_nd;
+  double *dirs = (double *)calloc(3 * (n ? n : 1), sizeof(double));
+  for (int i = 0; i < S->n_el; i++) {
+    const int64_t e = S->elems[i];
+    const int32_t a = m->en[2 * e], b = m->en[2 * e + 1];
+    const int la = rod_local_node(S, a), lb = rod_local_node(S, b);
+    for (int d = 0; d < 3; d++) {
+      const double t = m->X0[3 * b + d] - m->X0[3 * a + d];
+      dirs[3 * la + d] += t;
+      dirs[3 * lb + d] += t;
+    }
+  }
+  for (int k = 0; k < n; k++) {
+    double *d = dirs + 3 * k;
+    const double nn = norm3(d);
+    d[0] = d[0] / nn; d[1] = d[1] / nn; d[2] = d[2] / nn;
+    /* v = e3^T skew(d) = e3 x d; c = d . e3 */
+    double v[3] = {-1 * d[1], 1 * d[0], 0.0};
+    const double cc = d[0] * 0. + d[1] * 0. + d[2] * 1.;
+    double *Rm = S->lambdaN + 9 * k;
+    eye3(Rm);
+    if (fabs(cc + 1.0) > DBL_EPSILON) {
+      double Sv[9], SS[9];
+      skew3(v, Sv);
+      mm3(Sv, Sv, SS);
+      const double f = 1 / (1 + cc);
+      for (int i = 0; i < 9; i++) Rm[i] += Sv[i] + f * SS[i];
+    } else {
+      for (int i = 0; i < 9; i++) Rm[i] *= -1.;
+    }
+  }
+  free(dirs);
+}
+
+/* ------------------------------------------------------------------ */
+/* contact                                                             */
+/* ------------------------------------------------------------------ */
+static int ref_contains(double u) { return u >= -1.0 && u <= 1.0; }
+
+static void global_point(double u, const double p0[3], const double p1[3], double x[3]) {
+  const double N0 = 0.5 * (1. - u), N1 = 0.5 * (1. + u);
+  for (int i = 0; i < 3; i++) {
+    double s = p0[i] * N0;
+    s += p1[i] * N1;
+    x[i] = s;
+  }
+}
+
+static void closest_points(const double pA[2][3], const double pB[2][3], double *uA, double *uB) {
+  double bA[3], bB[3], tA[3], tB[3], d[3], vA[3], vB[3];
+  for (int i = 0; i < 3; i++) {
+    bA[i] = pA[1][i] + pA[0][i];
+    bB[i] = pB[1][i] + pB[0][i];
+    tA[i] = pA[1][i] - pA[0][i];
+    tB[i] = pB[1][i] - pB[0][i];
+    d[i] = bB[i] - bA[i];
+  }
+  const double tBtA = dot3(tB, tA), tBtB = dot3(tB, tB), tAtA = dot3(tA, tA);
+  for (int i = 0; i < 3; i++) {
+    vA[i] = tB[i] * tBtA - tA[i] * tBtB;
+    vB[i] = tA[i] * tBtA - tB[i] * tAtA;
+  }
+  const double den = tBtB * tAtA - tBtA * tBtA;
+  *uA = -1. * dot3(d, vA) / den;
+  *uB = +1. * dot3(d, vB) / den;
+}
+
+static double closest_point(const double s[3], const double pM[2][3]) {
+  double bM[3], tM[3];
+  for (int i = 0; i < 3; i++) {
+    bM[i] = pM[1][i] + pM[0][i];
+    tM[i] = pM[1][i] - pM[0][i];
+  }
+  return (2 * dot3(s, tM) - dot3(tM, bM)) / dot3(tM, tM);
+}
+
+/* computeSTS_ force part (RodContactModel.cpp:1031-1152); f may be NULL */
+static int contact_sts(const double pA[2][3], const double pB[2][3], double uA, double uB,
+                       double R, double kap, double f[12]) {
+  double xA[3], xB[3], d[3];
+  global_point(uA, pA[0], pA[1], xA);
+  global_point(uB, pB[0], pB[1], xB);
+  for (int i = 0; i < 3; i++) d[i] = xB[i] - xA[i];
+  const double dist = norm3(d);
+  if (REF_IS_TINY(dist)) return 0;
+  if (dist > R) return 0;
+  if (!f) return 1;
+  double n[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
+  const double NA[2] = {0.5 * (1. - uA), 0.5 * (1. + uA)};
+  const double NB[2] = {0.5 * (1. - uB), 0.5 * (1. + uB)};
+  const double sc = kap * (dist - R);
+  for (int i = 0; i < 3; i++) {
+    f[0 + i] += sc * ((-1. * NA[0]) * n[i]);
+    f[3 + i] += sc * ((-1. * NA[1]) * n[i]);
+    f[6 + i] += sc * (NB[0] * n[i]);
+    f[9 + i] += sc * (NB[1] * n[i]);
+  }
+  return 1;
+}
+
+/* computeNTS_ force part (RodContactModel.cpp:1157-1208); f[9] may be NULL */
+static int contact_nts(const double s[3], const double pM[2][3], double uM, double R,
+                       double kap, double f[9]) {
+  double xM[3], d[3];
+  global_point(uM, pM[0], pM[1], xM);
+  for (int i = 0; i < 3; i++) d[i] = xM[i] - s[i];
+  const double dist = norm3(d);
+  if (REF_IS_TINY(dist)) return 0;
+  if (dist > R) return 0;
+  if (!f) return 1;
+  double n[3];
+  for (int i = 0; i < 3; i++) n[i] = (s[i] - xM[i]) / dist;
+  const double sc = kap * (dist - R);
+  const double a = -.5 * (1. - uM), b = -.5 * (1. + uM);
+  for (int i = 0; i < 3; i++) {
+    f[i] += sc * n[i];
+    f[3 + i] += sc * (a * n[i]);
+    f[6 + i] += sc * (b * n[i]);
+  }
+  return 1;
+}
+
+static void node_pos(const ref_model *m, int32_t n, const double *disp, double x[3]) {
+  for (int i = 0; i < 3; i++) x[i] = m->X0[3 * n + i] + (disp ? disp[6 * n + i] : 0.);
+}
+
+static int strut_has_node(const ref_model *m, int32_t s, int32_t n) {
+  return rod_local_node(&m->st[s], n) >= 0;
+}
+
+/* element AABB inflated by the strut's contact radius */
+static void elem_box(const ref_model *m, int64_t e, const double *disp, double box[6]) {
+  double a[3], b[3];
+  node_pos(m, m->en[2 * e], disp, a);
+  node_pos(m, m->en[2 * e + 1], disp, b);
+  const double r = m->crad[m->es[e]];
+  for (int i = 0; i < 3; i++) {
+    box[i] = (a[i] < b[i] ? a[i] : b[i]) - r;
+    box[3 + i] = (a[i] > b[i] ? a[i] : b[i]) + r;
+  }
+}
+static int box_overlap(const double A[6], const double B[6]) {
+  int ok = 1;
+  for (int i = 0; i < 3; i++) ok &= A[3 + i] >= B[i] && B[3 + i] >= A[i];
+  return ok;
+}
+/* element A may meet strut B: neither of its nodes belongs to strut B (RodContactModel.cpp:393,413) */
+static int elem_free_of(const ref_model *m, int64_t e, int32_t s) {
+  return !(strut_has_node(m, s, m->en[2 * e]) || strut_has_node(m, s, m->en[2 * e + 1]));
+}
+
+typedef struct {
+  int64_t n, cap;
+  int32_t *a, *b;
+} pairbuf;
+static void pb_push(pairbuf *p, int32_t a, int32_t b) {
+  if (p->n == p->cap) {
+    p->cap = p->cap ? 2 * p->cap : 1024;
+    p->a = (int32_t *)realloc(p->a, sizeof(int32_t) * p->cap);
+    p->b = (int32_t *)realloc(p->b, sizeof(int32_t) * p->cap);
+  }
+  p->a[p->n] = a;
+  p->b[p->n] = b;
+  p->n++;
+}
+
+/* findPossibleBeams_ + findPossibleElements_ (RodContactModel.cpp:245-446), O(R^2) */
+static void broadphase_brute(ref_model *m, const double *disp, pairbuf *out) {
+  const int32_t R = m->R;
+  for (int32_t s = 0; s < R; s++) {
+    strut_t *S = &m->st[s];
+    const double r = m->crad[s];
+    for (int i = 0; i < 3; i++) { S->box[i] = INFINITY; S->box[3 + i] = -INFINITY; }
+    for (int k = 0; k < S->n_nd; k++) {
+      double x[3];
+      node_pos(m, S->nodes[k], disp, x);
+      for (int i = 0; i < 3; i++) {
+        if (x[i] < S->box[i]) S->box[i] = x[i];
+        if (x[i] > S->box[3 + i]) S->box[3 + i] = x[i];
+      }
+    }
+    for (int i = 0; i < 3; i++) { S->box[i] -= r; S->box[3 + i] += r; }
+  }
+  for (int32_t i = 0; i < R; i++)
+    for (int32_t j = R - 1; j > i; j--) {
+      if (!box_overlap(m->st[i].box, m->st[j].box)) continue;
+      const strut_t *A = &m->st[i], *B = &m->st[j];
+      for (int ka = 0; ka < A->n_el; ka++) {
+        const int64_t ea = A->elems[ka];
+        if (!elem_free_of(m, ea, j)) continue;
+        double ba[6];
+        elem_box(m, ea, disp, ba);
+        for (int kb = 0; kb < B->n_el; kb++) {
+          const int64_t eb = B->elems[kb];
+          if (!elem_free_of(m, eb, i)) continue;
+          double bb[6];
+          elem_box(m, eb, disp, bb);
+          if (box_overlap(ba, bb)) pb_push(out, (int32_t)ea, (int32_t)eb);
+        }
+      }
+    }
+}
+
+/* Uniform-grid broadphase producing the same candidate set (order differs):
+ * cell = element AABB min corner / h, h >= largest AABB extent. */
+static void broadphase_grid(ref_model *m, const double *disp, pairbuf *out) {
+  const int64_t E = m->E;
+  double *boxes = (double *)malloc(sizeof(double) * 6 * E);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, ext = 0.;
+  for (int64_t e = 0; e < E; e++) {
+    double *b = boxes + 6 * e;
+    elem_box(m, e, disp, b);
+    for (int i = 0; i < 3; i++) {
+      if (b[i] < lo[i]) lo[i] = b[i];
+      if (b[3 + i] - b[i] > ext) ext = b[3 + i] - b[i];
+    }
+  }
+  const double h = ext * 1.0001 + 1e-300;
+  double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t e = 0; e < E; e++)
+    for (int i = 0; i < 3; i++)
+      if (boxes[6 * e + i] > hi[i]) hi[i] = boxes[6 * e + i];
+  int64_t dim[3];
+  for (int i = 0; i < 3; i++) dim[i] = (int64_t)floor((hi[i] - lo[i]) / h) + 1;
+  const int64_t ncell = dim[0] * dim[1] * dim[2];
+  int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * E);
+  int64_t *start = (int64_t *)calloc(ncell + 1, sizeof(int64_t));
+  int32_t *sorted = (int32_t *)malloc(sizeof(int32_t) * E);
+  for (int64_t e = 0; e < E; e++) {
+    int64_t c[3];
+    for (int i = 0; i < 3; i++) {
+      c[i] = (int64_t)floor((boxes[6 * e + i] - lo[i]) / h);
+      if (c[i] < 0) c[i] = 0;
+      if (c[i] >= dim[i]) c[i] = dim[i] - 1;
+    }
+    cell[e] = (c[2] * dim[1] + c[1]) * dim[0] + c[0];
+    start[cell[e] + 1]++;
+  }
+  for (int64_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * ncell);
+  memcpy(fill, start, sizeof(int64_t) * ncell);
+  for (int64_t e = 0; e < E; e++) sorted[fill[cell[e]]++] = (int32_t)e;
+  for (int64_t a = 0; a < E; a++) {
+    const int64_t ca = cell[a];
+    const int64_t cx = ca % dim[0], cy = (ca / dim[0]) % dim[1], cz = ca / (dim[0] * dim[1]);
+    const int32_t sa = m->es[a];
+    for (int64_t dz = -1; dz <= 1; dz++)
+      for (int64_t dy = -1; dy <= 1; dy++)
+        for (int64_t dx = -1; dx <= 1; dx++) {
+          const int64_t x = cx + dx, y = cy + dy, z = cz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= dim[0] || y >= dim[1] || z >= dim[2]) continue;
+          const int64_t c = (z * dim[1] + y) * dim[0] + x;
+          for (int64_t k = start[c]; k < start[c + 1]; k++) {
+            const int32_t b = sorted[k];
+            const int32_t sb = m->es[b];
+            if (sb <= sa) continue; /* A belongs to the lower strut index */
+            if (!elem_free_of(m, a, sb) || !elem_free_of(m, b, sa)) continue;
+            if (box_overlap(boxes + 6 * a, boxes + 6 * b)) pb_push(out, (int32_t)a, b);
+          }
+        }
+  }
+  free(boxes); free(cell); free(start); free(sorted); free(fill);
+}
+
+static void broadphase(ref_model *m, const double *disp, pairbuf *out) {
+  out->n = 0;
+  if (m->use_grid) broadphase_grid(m, disp, out);
+  else broadphase_brute(m, disp, out);
+}
+
+static int is_blacklisted(const ref_model *m, int32_t a, int32_t b) {
+  const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+  const int64_t key = (lo << 32) | hi;
+  int64_t l = 0, h = m->n_black - 1;
+  while (l <= h) {
+    const int64_t mid = (l + h) >> 1;
+    if (m->black[mid] == key) return 1;
+    if (m->black[mid] < key) l = mid + 1; else h = mid - 1;
+  }
+  return 0;
+}
+
+/* per-candidate narrowphase shared by computeContacts_ and computeBlacklist_.
+ * Returns 1 if the contact is closed; when f != NULL fills f[12] (A0,A1,B0,B1
+ * translational) with the contribution laid out as in computeContacts_. */
+static int narrowphase(const ref_model *m, int32_t ea, int32_t eb, const double *disp,
+                       double f[12]) {
+  const int32_t nA[2] = {m->en[2 * ea], m->en[2 * ea + 1]};
+  const int32_t nB[2] = {m->en[2 * eb], m->en[2 * eb + 1]};
+  if (nA[0] == nB[0] || nA[0] == nB[1] || nA[1] == nB[0] || nA[1] == nB[1]) return 0;
+  double pA[2][3], pB[2][3];
+  for (int k = 0; k < 2; k++) {
+    node_pos(m, nA[k], disp, pA[k]);
+    node_pos(m, nB[k], disp, pB[k]);
+  }
+  const double R = m->crad[m->es[ea]] + m->crad[m->es[eb]];
+  const double kS = m->co.penalty_sts, kN = m->co.penalty_nts;
+  double uA, uB;
+  closest_points(pA, pB, &uA, &uB);
+  if (f) memset(f, 0, 12 * sizeof(double));
+  if (ref_contains(uA) && ref_contains(uB)) {
+    return contact_sts(pA, pB, uA, uB, R, kS, f);
+  } else if (!ref_contains(uA) && ref_contains(uB)) {
+    const int iA = (uA < -1.) ? 0 : 1;
+    uB = closest_point(pB[iA], pA); /* RodContactModel.cpp:575: node of B projected on A */
+    if (!ref_contains(uB)) return 0;
+    double g[9] = {0};
+    if (!contact_nts(pA[iA], pB, uB, R, kN, f ? g : NULL)) return 0;
+    if (f) {
+      for (int i = 0; i < 3; i++) {
+        f[3 * iA + i] = g[i];
+        f[6 + i] = g[3 + i];
+        f[9 + i] = g[6 + i];
+      }
+    }
+    return 1;
+  } else if (!ref_contains(uB) && ref_contains(uA)) {
+    const int iB = (uB < -1.) ? 0 : 1;
+    uA = closest_point(pB[iB], pA);
+    if (!ref_contains(uA)) return 0;
+    double g[9] = {0};
+    if (!contact_nts(pB[iB], pA, uA, R, kN, f ? g : NULL)) return 0;
+    if (f) {
+      for (int i = 0; i < 3; i++) {
+        f[6 + 3 * iB + i] = g[i];
+        f[0 + i] = g[3 + i];
+        f[3 + i] = g[6 + i];
+      }
+    }
+    return 1;
+  } else {
+    int closed = 0;
+    for (int iA = 0; iA < 2; iA++) {
+      const double u = closest_point(pA[iA], pB);
+      if (!ref_contains(u)) continue;
+      double g[9] = {0};
+      if (contact_nts(pA[iA], pB, u, R, kN, f ? g : NULL)) {
+        if (f)
+          for (int i = 0; i < 3; i++) {
+            f[3 * iA + i] += g[i];
+            f[6 + i] += g[3 + i];
+            f[9 + i] += g[6 + i];
+          }
+        closed = 1;
+      }
+    }
+    for (int iB = 0; iB < 2; iB++) {
+      const double u = closest_point(pB[iB], pA);
+      if (!ref_contains(u)) continue;
+      double g[9] = {0};
+      if (contact_nts(pB[iB], pA, u, R, kN, f ? g : NULL)) {
+        if (f)
+          for (int i = 0; i < 3; i++) {
+            f[0 + i] += g[3 + i];
+            f[3 + i] += g[6 + i];
+            f[6 + 3 * iB + i] += g[i];
+          }
+        closed = 1;
+      }
+    }
+    return closed;
+  }
+}
+
+static void compute_blacklist(ref_model *m, const double *disp) { /* RodContactModel.cpp:743-936 */
+  pairbuf pb = {0};
+  broadphase(m, disp, &pb);
+  free(m->black);
+  m->black = (int64_t *)malloc(sizeof(int64_t) * (pb.n ? pb.n : 1));
+  m->n_black = 0;
+  for (int64_t i = 0; i < pb.n; i++)
+    if (narrowphase(m, pb.a[i], pb.b[i], disp, NULL)) {
+      const int64_t a = pb.a[i], b = pb.b[i];
+      const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+      m->black[m->n_black++] = (lo << 32) | hi;
+    }
+  qsort(m->black, m->n_black, sizeof(int64_t), cmp_i64);
+  free(pb.a); free(pb.b);
+}
+
+static void act_push(ref_model *m, int32_t a, int32_t b) {
+  if (m->n_act == m->cap_act) {
+    m->cap_act *= 2;
+    m->actA = (int32_t *)realloc(m->actA, sizeof(int32_t) * m->cap_act);
+    m->actB = (int32_t *)realloc(m->actB, sizeof(int32_t) * m->cap_act);
+  }
+  m->actA[m->n_act] = a;
+  m->actB[m->n_act] = b;
+  m->n_act++;
+}
+
+int64_t ref_contact_force(ref_model *m, const double *disp, double *fint) {
+  m->n_act = 0;
+  if (!m->co.enabled) return 0;
+  pairbuf pb = {0};
+  broadphase(m, disp, &pb);
+  for (int64_t i = 0; i < pb.n; i++) {
+    const int32_t ea = pb.a[i], eb = pb.b[i];
+    if (is_blacklisted(m, ea, eb)) continue;
+    double f[12];
+    if (!narrowphase(m, ea, eb, disp, f)) continue;
+    const int32_t nd[4] = {m->en[2 * ea], m->en[2 * ea + 1], m->en[2 * eb], m->en[2 * eb + 1]};
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) fint[6 * nd[k] + d] += f[3 * k + d];
+    act_push(m, ea, eb);
+  }
+  free(pb.a); free(pb.b);
+  return m->n_act;
+}
+
+int64_t ref_contact_candidates(ref_model *m, const double *disp, int32_t *a, int32_t *b, int64_t cap) {
+  if (!m->co.enabled) return 0;
+  pairbuf pb = {0};
+  broadphase(m, disp, &pb);
+  for (int64_t i = 0; i < pb.n && i < cap; i++) { a[i] = pb.a[i]; b[i] = pb.b[i]; }
+  const int64_t n = pb.n;
+  free(pb.a); free(pb.b);
+  return n;
+}
+
+int64_t ref_contact_active(ref_model *m, int32_t *a, int32_t *b, int64_t cap) {
+  for (int64_t i = 0; i < m->n_act && i < cap; i++) { a[i] = m->actA[i]; b[i] = m->actB[i]; }
+  return m->n_act;
+}
+
+int64_t ref_blacklist(ref_model *m, int32_t *a, int32_t *b, int64_t cap) {
+  for (int64_t i = 0; i < m->n_black && i < cap; i++) {
+    a[i] = (int32_t)(m->black[i] >> 32);
+    b[i] = (int32_t)(m->black[i] & 0xffffffff);
+  }
+  return m->n_black;
+}
+
+void ref_set_grid(ref_model *m, int use_grid) { m->use_grid = use_grid; }
+
+int ref_init(ref_model *m, const double *disp) {
+  for (int32_t s = 0; s < m->R; s++) init_rotation(m, &m->st[s]);
+  /* initStrain_: strains at null displacement, Lambda = LambdaN */
+  for (int64_t e = 0; e < m->E; e++) {
+    double c[2][3], u[2][3], L[2][9], Lip[9], phiP[3], g[2], w;
+    elem_gather(m, e, NULL, c, u, L);
+    elem_strains(c, u, L, NULL, m->strain0 + 6 * e, Lip, phiP, g, &w);
+  }
+  if (m->co.enabled) compute_blacklist(m, disp);
+  return m->err;
+}
+
+int ref_frames(ref_model *m, double *lambda_n, double *strain0) {
+  for (int64_t e = 0; e < m->E; e++) {
+    const strut_t *S = &m->st[m->es[e]];
+    for (int k = 0; k < 2; k++) {
+      const int ln = rod_local_node(S, m->en[2 * e + k]);
+      if (lambda_n) memcpy(lambda_n + 18 * e + 9 * k, S->lambdaN + 9 * ln, 9 * sizeof(double));
+    }
+  }
+  if (strain0) memcpy(strain0, m->strain0, 6 * m->E * sizeof(double));
+  return REF_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* mass (assembleM_, SpecialCosseratRodModel.cpp:969-1031)             */
+/* ------------------------------------------------------------------ */
+int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped) {
+  const int64_t N = m->N;
+  memset(blocks, 0, sizeof(double) * 36 * N);
+  const double xi[2] = {-1. / sqrt(3.), 1. / sqrt(3.)};
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    const double *Mm = m->Mm + 6 * s;
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      double c[2][3], u[2][3], L[2][9];
+      elem_gather(m, e, disp, c, u, L);
+      /* Gauss2 rotations (getRotations with massShape) */
+      double Lr[9], psi0[3], psi1[3], Lip[2][9];
+      node_rot_vecs(L[0], L[1], Lr, psi0, psi1);
+      double sh[2][2]; /* sh[node][ip] */
+      for (int ip = 0; ip < 2; ip++) {
+        sh[0][ip] = 0.5 * (1. - xi[ip]);
+        sh[1][ip] = 0.5 * (1. + xi[ip]);
+        double ps[3], Ex[9];
+        for (int i = 0; i < 3; i++) {
+          double t = psi0[i] * sh[0][ip];
+          t += psi1[i] * sh[1][ip];
+          ps[i] = t;
+        }
+        ref_expvec(ps, Ex);
+        mm3(Lr, Ex, Lip[ip]);
+      }
+      /* getIntegrationWeights: w_ip = 1 * |J|, J = 0.5 L */
+      double d[3] = {c[1][0] - c[0][0], c[1][1] - c[0][1], c[1][2] - c[0][2]};
+      const double Lc = norm3(d) + 0.;
+      const double Jc = -0.5 * 0. + 0.5 * Lc;
+      const double wts[2] = {1. * Jc, 1. * Jc};
+      const double l = (wts[0] + wts[1]) / (2 - 1);
+      /* material masses */
+      const double Mt[9] = {Mm[0], 0, 0, 0, Mm[1], 0, 0, 0, Mm[2]};
+      double Mr[9] = {Mm[3] * l, 0, 0, 0, Mm[4] * l, 0, 0, 0, Mm[5] * l};
+      const double add = m->area[s] * m->rho[s] * pow(l, 3) / 12.;
+      Mr[0] += add;
+      Mr[4] += add;
+      for (int in = 0; in < 2; in++) {
+        const int32_t gn = m->en[2 * e + in];
+        for (int jn = 0; jn < 2; jn++) {
+          double SI[36];
+          memset(SI, 0, sizeof SI);
+          for (int ip = 0; ip < 2; ip++) {
+            double T[9], T2[9];
+            mm3(Lip[ip], Mt, T);
+            mmt3(T, Lip[ip], T2);
+            const double sc = wts[ip] * sh[in][ip] * sh[jn][ip];
+            for (int i = 0; i < 3; i++)
+              for (int j = 0; j < 3; j++) SI[6 * i + j] += sc * T2[3 * i + j];
+          }
+          if (in == jn) {
+            double T[9], T2[9];
+            mm3(L[in], Mr, T);
+            mmt3(T, L[in], T2);
+            for (int i = 0; i < 3; i++)
+              for (int j = 0; j < 3; j++) SI[6 * (3 + i) + 3 + j] += T2[3 * i + j];
+            if (in == 0 || in == 1)
+              for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) SI[6 * (3 + i) + 3 + j] /= 2.;
+          }
+          double *B = blocks + 36 * gn;
+          if (lumped) {
+            for (int i = 0; i < 6; i++) B[7 * i] += SI[7 * i];
+          } else {
+            for (int i = 0; i < 36; i++) B[i] += SI[i];
+          }
+        }
+      }
+    }
+  }
+  return REF_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* GET_INT_VECTOR                                                      */
+/* ------------------------------------------------------------------ */
+static int int_vector(ref_model *m, const double *disp, const double *velo, double *fint,
+                      double *fgyro, const double *blocks, int inelastic) {
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      double f[12];
+      const int rc = ref_elem_fint(m, e, disp, f, inelastic);
+      if (rc) return rc;
+      for (int k = 0; k < 2; k++) {
+        const int32_t n = m->en[2 * e + k];
+        for (int i = 0; i < 6; i++) fint[6 * n + i] += f[6 * k + i];
+      }
+    }
+    if (fgyro) { /* assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): per rod, its nodes */
+      for (int k = 0; k < S->n_nd; k++) {
+        const int32_t n = S->nodes[k];
+        const double *B = blocks + 36 * n;
+        const double *v = velo + 6 * n;
+        double t[3], Sw[9], g[3];
+        for (int i = 0; i < 3; i++) {
+          double a = B[6 * (3 + i)] * v[0];
+          for (int j = 1; j < 6; j++) a += B[6 * (3 + i) + j] * v[j];
+          t[i] = a;
+        }
+        skew3(v + 3, Sw);
+        mv3(Sw, t, g);
+        for (int i = 0; i < 3; i++) fgyro[6 * n + 3 + i] += g[i];
+      }
+    }
+  }
+  return REF_OK;
+}
+
+int ref_get_int_vector(ref_model *m, const double *disp, const double *velo, double *fint,
+                       int inelastic, int with_gyro, int with_contact) {
+  double *blocks = NULL, *fg = NULL;
+  if (with_gyro) {
+    blocks = (double *)malloc(sizeof(double) * 36 * m->N);
+    ref_mass_blocks(m, disp, blocks, 0);
+    fg = (double *)calloc(6 * m->N, sizeof(double));
+  }
+  int rc = int_vector(m, disp, velo, fint, fg, blocks, inelastic);
+  if (!rc && with_contact) ref_contact_force(m, disp, fint);
+  if (fg)
+    for (int64_t i = 0; i < 6 * m->N; i++) fint[i] += fg[i];
+  free(blocks);
+  free(fg);
+  return rc;
+}
+
+/* solve the 6x6 block system with constrained dofs (reduced system, Gauss
+ * elimination with partial pivoting) */
+static void solve_block(const double *B, const double *f, const uint8_t *cm, const double *cv,
+                        double *a) {
+  int fr[6], nf = 0;
+  for (int i = 0; i < 6; i++) {
+    if (cm[i]) a[i] = cv[i];
+    else fr[nf++] = i;
+  }
+  if (!nf) return;
+  double A[6][7];
+  for (int r = 0; r < nf; r++) {
+    double rhs = f[fr[r]];
+    for (int j = 0; j < 6; j++)
+      if (cm[j]) rhs -= B[6 * fr[r] + j] * cv[j];
+    for (int c = 0; c < nf; c++) A[r][c] = B[6 * fr[r] + fr[c]];
+    A[r][nf] = rhs;
+  }
+  for (int k = 0; k < nf; k++) {
+    int p = k;
+    for (int r = k + 1; r < nf; r++)
+      if (fabs(A[r][k]) > fabs(A[p][k])) p = r;
+    if (p != k)
+      for (int c = 0; c <= nf; c++) { double t = A[k][c]; A[k][c] = A[p][c]; A[p][c] = t; }
+    for (int r = k + 1; r < nf; r++) {
+      const double l = A[r][k] / A[k][k];
+      for (int c = k; c <= nf; c++) A[r][c] -= l * A[k][c];
+    }
+  }
+  double x[6];
+  for (int k = nf - 1; k >= 0; k--) {
+    double s = A[k][nf];
+    for (int c = k + 1; c < nf; c++) s -= A[k][c] * x[c];
+    x[k] = s / A[k][k];
+  }
+  for (int r = 0; r < nf; r++) a[fr[r]] = x[r];
+}
+
+int ref_commit(ref_model *m, const double *disp, double *e_pot, double *e_diss) {
+  (void)disp;
+  apply_deform(m);
+  /* getPotentialEnergy_ / getDissipatedEnergy_: sum_e sum_I w N_I energy_e */
+  double ep = 0., ed = 0.;
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    double sp = 0., sd = 0.;
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      const int32_t a = m->en[2 * e], b = m->en[2 * e + 1];
+      double g[2], w;
+      shape_grads_g1(m->X0 + 3 * a, m->X0 + 3 * b, g, &w);
+      for (int k = 0; k < 2; k++) {
+        sp += w * 0.5 * m->epot[e];
+        sd += w * 0.5 * (m->pl.model == REF_ELASTOPLASTIC ? m->ediss[e] : 0.);
+      }
+    }
+    ed += sd;
+    ep += sp;
+  }
+  if (e_pot) *e_pot = ep;
+  if (e_diss) *e_diss = ed;
+  return REF_OK;
+}
+
+/* ExplicitModule::advance/commit + LeapFrogModule::solve with the intended
+ * store (LeapFrogModule.cpp:44-84; ExplicitModule.cpp:259-419). */
+int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t nsteps,
+                 const int32_t *group, int64_t group_n, int32_t group_dof, double *react,
+                 int commit_energy, double *energies, double *unused, int64_t *n_contacts) {
+  (void)unused;
+  const int64_t N = m->N, nd = 6 * N;
+  double *blocks = (double *)malloc(sizeof(double) * 36 * N);
+  double *fint = (double *)malloc(sizeof(double) * nd);
+  double *fg = (double *)malloc(sizeof(double) * nd);
+  double *acc = (double *)malloc(sizeof(double) * nd);
+  int rc = REF_OK;
+  for (int64_t st = 0; st < nsteps && !rc; st++) {
+    /* advance: updCond default true -> updateMass_ at the current state */
+    ref_mass_blocks(m, disp, blocks, !m->so3);
+    if (!m->so3) {
+      for (int64_t n = 0; n < N; n++)
+        for (int i = 0; i < 6; i++)
+          if (!(blocks[36 * n + 7 * i] > DBL_MIN)) { rc = REF_ERR_ZERO_MASS; m->err = rc; }
+      if (rc) break;
+    }
+    /* getForce */
+    memset(fint, 0, sizeof(double) * nd);
+    memset(fg, 0, sizeof(double) * nd);
+    rc = int_vector(m, disp, velo, fint, m->so3 ? fg : NULL, blocks, 1);
+    if (rc) break;
+    int64_t nc = 0;
+    if (m->co.enabled) nc = ref_contact_force(m, disp, fint);
+    if (n_contacts) n_contacts[st] = nc;
+    if (react) {
+      double r = 0.;
+      for (int64_t k = 0; k < group_n; k++) r += fint[6 * (int64_t)group[k] + group_dof];
+      react[st] = r;
+    }
+    /* fres = fext - fint; getAcce */
+    for (int64_t n = 0; n < N; n++) {
+      double fr[6];
+      for (int i = 0; i < 6; i++) fr[i] = m->fext[6 * n + i] - (fint[6 * n + i] + fg[6 * n + i]);
+      if (m->so3) {
+        solve_block(blocks + 36 * n, fr, m->cmask + 6 * n, m->cval + 6 * n, acc + 6 * n);
+      } else {
+        for (int i = 0; i < 6; i++) {
+          acc[6 * n + i] = (1 / blocks[36 * n + 7 * i]) * fr[i];
+          if (m->cmask[6 * n + i]) acc[6 * n + i] = m->cval[6 * n + i];
+        }
+      }
+    }
+    /* v_new = v + dt a;  u_new = u + dt v_new (+ SO(3) composition) */
+    for (int64_t i = 0; i < nd; i++) velo[i] = velo[i] + dt * acc[i];
+    for (int64_t n = 0; n < N; n++) {
+      double *u = disp + 6 * n, *v = velo + 6 * n;
+      double th_old[3] = {u[3], u[4], u[5]};
+      for (int i = 0; i < 6; i++) u[i] = u[i] + dt * v[i];
+      if (m->so3) {
+        double dr[3] = {dt * v[3], dt * v[4], dt * v[5]};
+        double Ro[9], Vu[9], Rn[9];
+        ref_expvec(th_old, Ro);
+        ref_expvec(dr, Vu);
+        mm3(Vu, Ro, Rn);
+        ref_logmat(Rn, u + 3);
+      }
+    }
+    /* commit (error = 0 -> accepted; dt pinned) */
+    if (commit_energy && energies) {
+      double ep, ed;
+      ref_commit(m, disp, &ep, &ed);
+      double ek = 0.;
+      for (int64_t n = 0; n < N; n++) {
+        const double *B = blocks + 36 * n, *v = velo + 6 * n;
+        double t = 0.;
+        for (int i = 0; i < 6; i++) {
+          double a = B[6 * i] * v[0];
+          for (int j = 1; j < 6; j++) a += B[6 * i + j] * v[j];
+          t += v[i] * a;
+        }
+        ek += 0.5 * t;
+      }
+      energies[3 * st] = ep;
+      energies[3 * st + 1] = ed;
+      energies[3 * st + 2] = ek;
+    } else {
+      apply_deform(m);
+    }
+  }
+  free(blocks); free(fint); free(fg); free(acc);
+  return rc;
+}
+
+int ref_plastic_state(ref_model *m, double *eps_p, double *alpha, double *dflow, uint8_t *active) {
+  const int64_t E = m->E;
+  if (eps_p) memcpy(eps_p, m->curr_ep, 6 * E * sizeof(double));
+  if (alpha) memcpy(alpha, m->curr_al, 7 * E * sizeof(double));
+  if (dflow) memcpy(dflow, m->dflow, E * sizeof(double));
+  if (active) memcpy(active, m->active, E);
+  return REF_OK;
+}
+
+int ref_curr_strains(ref_model *m, double *strain) {
+  memcpy(strain, m->curr_strain, 6 * m->E * sizeof(double));
+  return REF_OK;
+}

@@ -1,0 +1,195 @@
+"""GPU parity of the element force, mass, gyro and LeapFrog step against the oracle.
+
+Tolerances: nodal internal forces within 1e-11
+relative (infinity norm over the vector) per evaluation; nodal displacements
+within 1e-8 relative after the stated horizon; plastic active sets
+bit-exact except points within 1e-12 of the yield surface.
+"""
+import numpy as np
+import pytest
+
+from dynlattice_amd import lattice as L
+from oracle.ref_cpu import RefModel
+from tests.conftest import rel_inf
+from tests.helpers import random_state, small_bcc
+
+pytestmark = pytest.mark.gpu
+
+F_TOL = 1e-11
+U_TOL = 1e-8
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from dynlattice_amd import gpu as G
+    G.load()
+    return G
+
+
+def _pair(p, **kw):
+    from dynlattice_amd.gpu import GpuModel
+    return RefModel(p), GpuModel(p, **kw)
+
+
+@pytest.mark.parametrize("case", ["cantilever", "bcc"])
+def test_rest_state_zero_force(gpu, case):
+    p = L.cantilever_c1() if case == "cantilever" else small_bcc()
+    _, g = _pair(p)
+    f = g.int_vector()
+    assert np.all(f == 0.0)
+
+
+@pytest.mark.parametrize("case", ["cantilever", "bcc", "bcc_plastic"])
+def test_int_vector_matches_oracle(gpu, case):
+    if case == "cantilever":
+        p = L.cantilever_c1()
+    else:
+        p = small_bcc(plastic=(case == "bcc_plastic"))
+    r, g = _pair(p)
+    d, v = random_state(p, seed=3, du=2e-5 if case != "cantilever" else 1e-3, dth=0.05)
+    g.set_state(d, v)
+    fr = r.int_vector(d, v)
+    fg = g.int_vector()
+    print(case, "rel", rel_inf(fg, fr), "max", np.abs(fr).max())
+    assert rel_inf(fg, fr) <= F_TOL
+
+
+def test_int_vector_output_load_case_plastic(gpu):
+    p = small_bcc(plastic=True)
+    r, g = _pair(p)
+    d, v = random_state(p, seed=4, du=4e-5, dth=0.08)
+    g.set_state(d, v)
+    fr = r.int_vector(d, v, inelastic=False)
+    fg = g.int_vector(inelastic=False)
+    assert rel_inf(fg, fr) <= F_TOL
+
+
+def test_gyro_matches_oracle(gpu):
+    p = small_bcc()
+    r, g = _pair(p)
+    d, v = random_state(p, seed=5, du=1e-5, dth=0.3, dv=30.0)
+    g.set_state(d, v)
+    fr = r.int_vector(d, v, gyro=True)
+    fg = g.int_vector(gyro=True)
+    assert rel_inf(fg, fr) <= F_TOL
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_mass_blocks_match_oracle(gpu, lumped):
+    p = small_bcc()
+    r, g = _pair(p)
+    d, _ = random_state(p, seed=6, dth=0.4)
+    g.set_state(d)
+    br = r.mass_blocks(d, lumped=lumped)
+    bg = g.mass_blocks(lumped=lumped)
+    assert rel_inf(bg, br) <= 1e-12
+
+
+def test_state_roundtrip(gpu):
+    p = small_bcc()
+    _, g = _pair(p)
+    d, v = random_state(p, seed=8, dth=0.7, dv=1.0)
+    g.set_state(d, v)
+    d2, v2 = g.get_state()
+    assert np.array_equal(v2, v)
+    assert np.array_equal(d2[:, :3], d[:, :3])
+    assert np.abs(d2[:, 3:] - d[:, 3:]).max() < 1e-14
+
+
+def test_leapfrog_cantilever(gpu):
+    """Plain cantilever (32 elements, tip force and torque), SO(3) mode."""
+    p = L.cantilever_c1()
+    r, g = _pair(p)
+    n = 2000
+    r.leapfrog(n)
+    g.step(n)
+    d, v = g.get_state()
+    dr = r.disp.reshape(-1, 6)
+    print("tip", d[-1], dr[-1])
+    assert rel_inf(d[:, :3], dr[:, :3]) <= U_TOL
+    assert rel_inf(d[:, 3:], dr[:, 3:]) <= U_TOL
+
+
+@pytest.mark.parametrize("so3", [True, False])
+def test_leapfrog_bcc_reaction(gpu, so3):
+    p = small_bcc()
+    p.so3 = so3
+    r, g = _pair(p)
+    g.set_reaction_group(p.groups["bottom"], 2)
+    n = 400
+    out = r.leapfrog(n, group=p.groups["bottom"], group_dof=2)
+    g.step(n)
+    h = g.history(n)
+    d, v = g.get_state()
+    dr = r.disp.reshape(-1, 6)
+    assert rel_inf(d, dr) <= U_TOL
+    assert rel_inf(v, r.velo.reshape(-1, 6)) <= 1e-7
+    assert rel_inf(h["reaction"], out["reaction"]) <= 1e-9
+
+
+def test_plastic_step_active_set(gpu):
+    p = small_bcc(plastic=True)
+    p.velo0[p.groups["top"], 2] = -40.0
+    r, g = _pair(p, flags=1)
+    g.set_reaction_group(p.groups["bottom"], 2)
+    n = 300
+    out = r.leapfrog(n, group=p.groups["bottom"], group_dof=2, energy=True)
+    o = g.step(n)
+    h = g.history(n)
+    d, _ = g.get_state()
+    assert rel_inf(d, r.disp.reshape(-1, 6)) <= U_TOL
+    ep_r, al_r, _, act_r = r.plastic_state()
+    ep_g, al_g = g.plastic_state()
+    act_g = g.plastic_active()
+    assert act_r.sum() > 0, "case must yield"
+    assert np.array_equal(act_g, act_r)
+    assert rel_inf(ep_g, ep_r) <= 1e-8
+    ediss_r = out["energies"][:, 1]
+    assert ediss_r[-1] > 0
+    assert abs(o.e_diss - ediss_r[-1]) <= 1e-8 * ediss_r[-1]
+
+
+def test_step_deterministic(gpu):
+    from dynlattice_amd.gpu import GpuModel
+    p = small_bcc(plastic=True)
+    p.velo0[p.groups["top"], 2] = -40.0
+    res = []
+    for _ in range(2):
+        g = GpuModel(p)
+        g.step(200)
+        res.append(g.get_state()[0])
+    assert np.array_equal(res[0], res[1])
+
+
+def test_commit_energy_matches_oracle(gpu):
+    p = small_bcc(plastic=True)
+    r, g = _pair(p)
+    d, v = random_state(p, seed=9, du=4e-5, dth=0.08)
+    g.set_state(d, v)
+    r.int_vector(d, v)
+    g.int_vector()
+    ep_r, ed_r = r.commit()
+    ep_g, ed_g = g.commit()
+    assert abs(ep_g - ep_r) <= 1e-10 * abs(ep_r)
+    assert abs(ed_g - ed_r) <= 1e-10 * max(abs(ed_r), 1e-30)
+
+
+def test_zero_mass_is_an_error(gpu):
+    from dynlattice_amd.gpu import DLError, GpuModel, DL_ERR_ZERO_MASS
+    p = L.cantilever_c1()
+    p.sections.density[:] = 0.0
+    with pytest.raises(DLError) as ei:
+        GpuModel(p)
+    assert ei.value.code == DL_ERR_ZERO_MASS
+
+
+def test_plastic_iteration_cap_is_an_error(gpu):
+    from dynlattice_amd.gpu import DLError, DL_ERR_PLASTIC_ITER
+    p = small_bcc(plastic=True)
+    p.plastic.max_iter = 0
+    _, g = _pair(p)
+    d, v = random_state(p, seed=3, du=4e-5, dth=0.08)
+    g.set_state(d, v)
+    with pytest.raises(DLError) as ei:
+        g.int_vector()
+    assert ei.value.code == DL_ERR_PLASTIC_ITER
