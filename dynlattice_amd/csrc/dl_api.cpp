@@ -488,13 +488,13 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   // contact
   if (contact && contact->enabled) {
     c->contact = true;
-    TRY(contact_create(c->cs, d, mesh, contact, [&](void **p, size_t bytes) {
+    // the allocator outlives dl_create (pair lists grow later): capture the context by value
+    TRY(contact_create(c->cs, d, mesh, contact, [c](void **p, size_t bytes) {
       hipError_t he = hipMalloc(p, bytes ? bytes : 1);
       if (he != hipSuccess) return fail(DL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(he));
       c->allocs.push_back(*p);
       return (int)DL_OK;
     }));
-    d.fc = c->cs.fc;
   }
 #undef TRY
   *out = c;
@@ -685,9 +685,9 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
       int rc = contact_force(c->cs, c->d, c->stream);
       if (rc) return rc;
     }
-    launch_strut_pp(c->d, sp, c->pp, MODE_FORCE, c->so3, c->plastic, false, c->stream);
-    launch_joint(c->d, sp, MODE_FORCE, c->so3, false, c->stream);
-    launch_force_extras(c->d, c->so3, p->with_gyro && c->so3, c->contact, c->stream);
+    launch_strut_pp(c->d, sp, c->pp, MODE_FORCE, c->so3, c->plastic, c->contact, c->stream);
+    launch_joint(c->d, sp, MODE_FORCE, c->so3, c->stream);
+    launch_force_extras(c->d, c->so3, p->with_gyro != 0, c->stream);
     HIPCHK(hipGetLastError());
     std::vector<double> h(6 * N);
     HIPCHK(hipMemcpyAsync(h.data(), c->d.fout, 6 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -810,7 +810,7 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 1], c->stream));
     launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->plastic, c->contact, c->stream);
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 2], c->stream));
-    launch_joint(c->d, sp, MODE_STEP, c->so3, c->contact, c->stream);
+    launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 3], c->stream));
     if (c->n_react > 0) launch_reduce_sum(c->d.rbuf, c->n_react, c->hist_react + st, nullptr, c->stream);
     if (c->track_diss) launch_reduce_sum(c->d.dbuf, c->strut_blocks, c->hist_diss + st, c->diss_total, c->stream);

@@ -1,15 +1,61 @@
-// dl_contact.h -- rod-rod penalty contact on the device (uniform-grid
-// broadphase + closest-point narrowphase), RodContactModel.cpp:245-1208.
+// dl_contact.h -- rod-rod penalty contact on the device: uniform-grid
+// broadphase (integer cell keys, counting sort, per-cell id order) and the
+// closest-point narrowphase of RodContactModel.cpp:245-1208.
 #ifndef DL_CONTACT_H
 #define DL_CONTACT_H
 
 #include <functional>
+#include <vector>
 
 #include "../../include/dl_gpu.h"
 #include "dl_internal.h"
 
+// Device-side contact data (passed by value to the contact kernels).
+struct ContactDev {
+  int64_t E, N;
+  int32_t R;
+  const int32_t *es;    // [E] strut of element
+  const int32_t *en0;   // [E] element node a
+  const int32_t *en1;   // [E] element node b
+  const double *crad;   // [R] contact radius
+  const int32_t *nsptr; // [N+1] node -> struts CSR
+  const int32_t *nslist;
+  double kS, kN;
+  // grid
+  double org[3];
+  int32_t dim[3];
+  int64_t ncell;
+  double h0;
+  double *hbuf;         // [1] cell size of the current evaluation
+  double *box;          // [6][E]
+  double *ext_part;     // [nblk]
+  int32_t *cell;        // [E]
+  int32_t *rank;        // [E]
+  int32_t *ccount;      // [ncell]
+  int32_t *cstart;      // [ncell+1]
+  int32_t *bsum;        // scan block sums
+  int32_t *sorted;      // [E]
+  // blacklist (sorted keys lo<<32|hi)
+  const unsigned long long *black;
+  int64_t nblack;
+  // per-element contact forces
+  int32_t *cidx;        // [E] slot or -1
+  double *fcs;          // [6][E]
+  int32_t *slot_ctr;    // [1]
+  // listing
+  int32_t *lcount;      // [E]
+  int32_t *loff;        // [E+1]
+  int32_t *la, *lb;     // [lcap]
+  int64_t lcap;
+};
+
 struct ContactState {
-  double *fc = nullptr; // [3][N] per-node contact force of the last evaluation
+  ContactDev c{};
+  int32_t nblk_e = 0;
+  int64_t nscan_blk = 0;
+  std::vector<int32_t> last_a, last_b;     // active pairs of the last listed evaluation
+  std::vector<int32_t> cand_a, cand_b;     // candidates of the last listed evaluation
+  std::function<int(void **, size_t)> alloc;
 };
 
 using DevAlloc = std::function<int(void **, size_t)>;
@@ -17,7 +63,9 @@ using DevAlloc = std::function<int(void **, size_t)>;
 int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_contact *co,
                    const DevAlloc &alloc);
 int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s);
+// evaluation at the current state for the action path (also lists pairs)
 int contact_force(ContactState &cs, const DevData &d, hipStream_t s);
+// evaluation inside the explicit step: per-element forces + counts
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
                  int64_t *n_cand);
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,

@@ -1,13 +1,571 @@
-// dl_contact.hip -- rod-rod contact (placeholder until the grid broadphase lands).
-#include "dl_contact.h"
+// dl_contact.hip -- rod-rod penalty contact on gfx950.
+//
+// Per force evaluation:
+//   kc_boxes     element AABBs (nodes +- own contact radius), block max extent
+//   kc_grid      cell size h = max(h0, 1.001 * largest extent)
+//   kc_cells     integer cell of each AABB's min corner (clamped to the grid),
+//                per-cell counts with integer atomics
+//   scan         exclusive scan of the counts (3 launches)
+//   kc_scatter   elements into cell order; kc_sortcells sorts each cell by id
+//   kc_narrow    per element X: every Y in the 27 neighbour cells that the
+//                reference would pair with X (different strut, no node of X in
+//                Y's strut and vice versa, AABB overlap) -> closest points ->
+//                STS / NTS branch -> penalty force on X's own two nodes.
+// Two AABBs that overlap have min corners within h of each other, so their
+// cells differ by at most one in every direction (clamping is 1-Lipschitz in
+// the cell index, so it keeps that property).  The candidate set therefore
+// equals the reference's all-pairs set (RodContactModel.cpp:316-350,388-440).
+// Each pair is evaluated once from each side; each side accumulates only its
+// own nodes' forces, in a fixed order, so no floating-point atomics are used
+// and the result is bitwise deterministic.  The strut kernel adds the
+// per-element forces to its nodal sums.
+#include <hip/hip_runtime.h>
 
-int contact_create(ContactState &, DevData &, const dl_mesh *, const dl_contact *, const DevAlloc &) {
-  return dl_set_error(DL_ERR_INPUT, "contact is not available in this build");
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "dl_contact.h"
+#include "dl_contact_math.h"
+
+#define CB 256
+
+enum { LIST_CAND = 0, LIST_ACTIVE = 1, LIST_CLOSED = 2 };
+
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d) {
+  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
+  double ext = 0.0;
+  if (e < c.E) {
+    const int64_t N = c.N, E = c.E;
+    const int32_t a = c.en0[e], b = c.en1[e];
+    const double r = c.crad[c.es[e]];
+    for (int i = 0; i < 3; i++) {
+      const double xa = d.X0[i * N + a] + d.u[i * N + a];
+      const double xb = d.X0[i * N + b] + d.u[i * N + b];
+      const double lo = (xa < xb ? xa : xb) - r;
+      const double hi = (xa > xb ? xa : xb) + r;
+      c.box[i * E + e] = lo;
+      c.box[(3 + i) * E + e] = hi;
+      ext = fmax(ext, hi - lo);
+    }
+  }
+  __shared__ double red[CB];
+  red[threadIdx.x] = ext;
+  __syncthreads();
+  for (int w = CB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c.ext_part[blockIdx.x] = red[0];
 }
-int contact_init_blacklist(ContactState &, const DevData &, hipStream_t) { return DL_OK; }
-int contact_force(ContactState &, const DevData &, hipStream_t) { return DL_OK; }
-int contact_step(ContactState &, const DevData &, hipStream_t, int64_t *, int64_t *) { return DL_OK; }
-int contact_read_pairs(ContactState &, hipStream_t, bool, int32_t *, int32_t *, int64_t, int64_t *n) {
-  if (n) *n = 0;
+
+__global__ __launch_bounds__(1024) void kc_grid(ContactDev c, int32_t nblk) {
+  __shared__ double red[1024];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 1024) m = fmax(m, c.ext_part[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    c.hbuf[0] = fmax(c.h0, 1.001 * red[0]);
+    c.slot_ctr[0] = 0;
+  }
+}
+
+__global__ __launch_bounds__(CB) void kc_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int32_t cell_coord(double x, double o, double h, int32_t dim) {
+  double v = floor((x - o) / h);
+  v = fmin(fmax(v, 0.0), (double)(dim - 1));
+  return (int32_t)v;
+}
+
+__global__ __launch_bounds__(CB) void kc_cells(ContactDev c) {
+  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (e >= c.E) return;
+  const double h = c.hbuf[0];
+  const int64_t E = c.E;
+  const int32_t cx = cell_coord(c.box[e], c.org[0], h, c.dim[0]);
+  const int32_t cy = cell_coord(c.box[E + e], c.org[1], h, c.dim[1]);
+  const int32_t cz = cell_coord(c.box[2 * E + e], c.org[2], h, c.dim[2]);
+  const int32_t lin = (cz * c.dim[1] + cy) * c.dim[0] + cx;
+  c.cell[e] = lin;
+  c.rank[e] = atomicAdd(&c.ccount[lin], 1);
+}
+
+// exclusive scan of n int32 values (chunks of 1024 per block)
+__global__ __launch_bounds__(CB) void kc_scan1(const int32_t *in, int64_t n, int32_t *out,
+                                               int32_t *bsum) {
+  __shared__ int32_t s[CB];
+  const int64_t base = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
+  int32_t v[4];
+  int32_t t = 0;
+  for (int k = 0; k < 4; k++) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    t += v[k];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < CB; off <<= 1) {
+    const int32_t x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  int32_t run = s[threadIdx.x] - t;
+  for (int k = 0; k < 4; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == CB - 1) bsum[blockIdx.x] = s[CB - 1];
+}
+
+__global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int32_t *total) {
+  __shared__ int32_t s[1024];
+  int32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += 1024) {
+    const int64_t i = b0 + threadIdx.x;
+    const int32_t v = i < nb ? bsum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int32_t x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+    const int32_t tot = s[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const int32_t *bsum) {
+  const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (i < n) out[i] += bsum[i / 1024];
+}
+
+__global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
+  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (e >= c.E) return;
+  c.sorted[c.cstart[c.cell[e]] + c.rank[e]] = (int32_t)e;
+}
+
+__global__ __launch_bounds__(CB) void kc_sortcells(ContactDev c) {
+  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (k >= c.ncell) return;
+  const int32_t b = c.cstart[k], e = c.cstart[k + 1];
+  for (int32_t i = b + 1; i < e; i++) {
+    const int32_t v = c.sorted[i];
+    int32_t j = i - 1;
+    while (j >= b && c.sorted[j] > v) {
+      c.sorted[j + 1] = c.sorted[j];
+      j--;
+    }
+    c.sorted[j + 1] = v;
+  }
+}
+
+__device__ __forceinline__ bool node_in_strut(const ContactDev &c, int32_t n, int32_t s) {
+  for (int32_t k = c.nsptr[n]; k < c.nsptr[n + 1]; k++)
+    if (c.nslist[k] == s) return true;
+  return false;
+}
+
+__device__ __forceinline__ bool blacklisted(const ContactDev &c, int32_t a, int32_t b) {
+  if (c.nblack == 0) return false;
+  const unsigned long long lo = (unsigned long long)(a < b ? a : b);
+  const unsigned long long hi = (unsigned long long)(a < b ? b : a);
+  const unsigned long long key = (lo << 32) | hi;
+  int64_t l = 0, r = c.nblack - 1;
+  while (l <= r) {
+    const int64_t m = (l + r) >> 1;
+    const unsigned long long v = c.black[m];
+    if (v == key) return true;
+    if (v < key) l = m + 1; else r = m - 1;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void elem_pos(const DevData &d, int32_t n0, int32_t n1, double p[2][3]) {
+  const int64_t N = d.N;
+  for (int i = 0; i < 3; i++) {
+    p[0][i] = d.X0[i * N + n0] + d.u[i * N + n0];
+    p[1][i] = d.X0[i * N + n1] + d.u[i * N + n1];
+  }
+}
+
+// MODE 0: forces (+ counts into cnt[0] active, cnt[1] candidates)
+// MODE 1: count the listed pairs of kind `kind` per element (A side)
+// MODE 2: emit them at loff[X]
+template <int MODE>
+__global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kind,
+                                                unsigned long long *cnt_act,
+                                                unsigned long long *cnt_cand) {
+  const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  unsigned long long nact = 0, ncand = 0;
+  if (X < c.E) {
+    const int64_t E = c.E;
+    const int32_t sX = c.es[X];
+    const int32_t xa = c.en0[X], xb = c.en1[X];
+    double bx[6];
+    for (int i = 0; i < 6; i++) bx[i] = c.box[i * E + X];
+    double pX[2][3];
+    elem_pos(d, xa, xb, pX);
+    const int32_t lin = c.cell[X];
+    const int32_t cx = lin % c.dim[0];
+    const int32_t cy = (lin / c.dim[0]) % c.dim[1];
+    const int32_t cz = lin / (c.dim[0] * c.dim[1]);
+    double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
+    bool any = false;
+    int32_t nlist = 0;
+    const int32_t off = (MODE == 2) ? c.loff[X] : 0;
+    for (int dz = -1; dz <= 1; dz++) {
+      const int32_t z = cz + dz;
+      if (z < 0 || z >= c.dim[2]) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int32_t y = cy + dy;
+        if (y < 0 || y >= c.dim[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int32_t x = cx + dx;
+          if (x < 0 || x >= c.dim[0]) continue;
+          const int32_t cc = (z * c.dim[1] + y) * c.dim[0] + x;
+          const int32_t k1 = c.cstart[cc + 1];
+          for (int32_t k = c.cstart[cc]; k < k1; k++) {
+            const int32_t Y = c.sorted[k];
+            const int32_t sY = c.es[Y];
+            if (sY == sX) continue;
+            bool ov = true;
+            for (int i = 0; i < 3; i++)
+              ov = ov && (bx[3 + i] >= c.box[i * E + Y]) && (c.box[(3 + i) * E + Y] >= bx[i]);
+            if (!ov) continue;
+            const int32_t ya = c.en0[Y], yb = c.en1[Y];
+            if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
+                node_in_strut(c, yb, sX))
+              continue;
+            const bool XisA = sX < sY;
+            if (XisA) ncand++;
+            if (MODE != 0 && kind == LIST_CAND) {
+              if (XisA) {
+                if (MODE == 2) { c.la[off + nlist] = (int32_t)X; c.lb[off + nlist] = Y; }
+                nlist++;
+              }
+              continue;
+            }
+            if (MODE != 0 && !XisA) continue;
+            const int32_t eA = XisA ? (int32_t)X : Y, eB = XisA ? Y : (int32_t)X;
+            if (kind != LIST_CLOSED && blacklisted(c, eA, eB)) continue;
+            double pY[2][3];
+            elem_pos(d, ya, yb, pY);
+            const double R = c.crad[sX] + c.crad[sY];
+            if (MODE == 0) {
+              double f[12];
+              const bool closed = XisA ? cm_narrowphase(pX, pY, R, c.kS, c.kN, f)
+                                       : cm_narrowphase(pY, pX, R, c.kS, c.kN, f);
+              if (closed) {
+                any = true;
+                const int o = XisA ? 0 : 6;
+                for (int i = 0; i < 3; i++) {
+                  fa[i] += f[o + i];
+                  fb[i] += f[o + 3 + i];
+                }
+                if (XisA) nact++;
+              }
+            } else {
+              const bool closed = cm_narrowphase(pX, pY, R, c.kS, c.kN, nullptr);
+              if (closed) {
+                if (MODE == 2) { c.la[off + nlist] = (int32_t)X; c.lb[off + nlist] = Y; }
+                nlist++;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (MODE == 0) {
+      if (any) {
+        const int32_t slot = atomicAdd(c.slot_ctr, 1);
+        c.cidx[X] = slot;
+        for (int i = 0; i < 3; i++) {
+          c.fcs[i * E + slot] = fa[i];
+          c.fcs[(3 + i) * E + slot] = fb[i];
+        }
+      } else {
+        c.cidx[X] = -1;
+      }
+    }
+    if (MODE == 1) c.lcount[X] = nlist;
+  }
+  if (MODE == 0) {
+    __shared__ unsigned long long ra[CB], rc[CB];
+    ra[threadIdx.x] = nact;
+    rc[threadIdx.x] = ncand;
+    __syncthreads();
+    for (int w = CB / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        ra[threadIdx.x] += ra[threadIdx.x + w];
+        rc[threadIdx.x] += rc[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (cnt_act && ra[0]) atomicAdd(cnt_act, ra[0]);
+      if (cnt_cand && rc[0]) atomicAdd(cnt_cand, rc[0]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------
+static unsigned nb(int64_t n, int b = CB) { return (unsigned)((n + b - 1) / b); }
+
+#define CHK(x)                                                                          \
+  do {                                                                                  \
+    hipError_t _e = (x);                                                                \
+    if (_e != hipSuccess)                                                               \
+      return dl_set_error(DL_ERR_HIP, (std::string("HIP: ") + hipGetErrorString(_e)).c_str()); \
+  } while (0)
+
+template <class T>
+static int calloc_dev(ContactState &cs, T **p, size_t n) {
+  return cs.alloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+template <class T>
+static int up(ContactState &cs, T **p, const std::vector<T> &h) {
+  int rc = calloc_dev(cs, p, h.size());
+  if (rc) return rc;
+  if (!h.empty()) CHK(hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return DL_OK;
+}
+
+int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_contact *co,
+                   const DevAlloc &alloc) {
+  cs.alloc = alloc;
+  ContactDev &c = cs.c;
+  const int64_t E = mesh->n_elems, N = mesh->n_nodes;
+  const int32_t R = mesh->n_struts;
+  c.E = E;
+  c.N = N;
+  c.R = R;
+  if (!co->radius) return dl_set_error(DL_ERR_INPUT, "contact radius array is required");
+  if (!(co->penalty_sts >= 0.0) || !(co->penalty_nts >= 0.0))
+    return dl_set_error(DL_ERR_INPUT, "contact penalties must be >= 0");
+  std::vector<double> crad(co->radius, co->radius + R);
+  for (double r : crad)
+    if (!(r > 0.0)) return dl_set_error(DL_ERR_INPUT, "contact radius must be > 0");
+  c.kS = co->penalty_sts;
+  c.kN = co->penalty_nts;
+  const int32_t *en = mesh->elem_nodes, *esp = mesh->elem_strut;
+  std::vector<int32_t> es(esp, esp + E), e0(E), e1(E);
+  for (int64_t e = 0; e < E; e++) {
+    e0[e] = en[2 * e];
+    e1[e] = en[2 * e + 1];
+  }
+  // node -> struts (sorted, unique)
+  std::vector<int64_t> key(2 * E);
+  for (int64_t e = 0; e < E; e++) {
+    key[2 * e] = ((int64_t)e0[e] << 32) | (uint32_t)es[e];
+    key[2 * e + 1] = ((int64_t)e1[e] << 32) | (uint32_t)es[e];
+  }
+  std::sort(key.begin(), key.end());
+  key.erase(std::unique(key.begin(), key.end()), key.end());
+  std::vector<int32_t> nsptr(N + 1, 0), nslist(key.size());
+  for (size_t k = 0; k < key.size(); k++) {
+    nsptr[(key[k] >> 32) + 1]++;
+    nslist[k] = (int32_t)(key[k] & 0xffffffff);
+  }
+  for (int64_t n = 0; n < N; n++) nsptr[n + 1] += nsptr[n];
+  // grid from the rest configuration with a margin
+  const double *X = mesh->coords;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t n = 0; n < N; n++)
+    for (int i = 0; i < 3; i++) {
+      lo[i] = std::min(lo[i], X[3 * n + i]);
+      hi[i] = std::max(hi[i], X[3 * n + i]);
+    }
+  double ext = 0.0, rmax = 0.0;
+  for (int64_t e = 0; e < E; e++) {
+    const double r = crad[es[e]];
+    rmax = std::max(rmax, r);
+    for (int i = 0; i < 3; i++)
+      ext = std::max(ext, std::fabs(X[3 * e1[e] + i] - X[3 * e0[e] + i]) + 2 * r);
+  }
+  double h0 = 1.001 * ext;
+  double span = 0.0;
+  for (int i = 0; i < 3; i++) span = std::max(span, hi[i] - lo[i]);
+  const double margin = 0.1 * span + 2 * h0 + 2 * rmax;
+  int64_t dims[3], ncell;
+  for (;;) {
+    ncell = 1;
+    for (int i = 0; i < 3; i++) {
+      dims[i] = (int64_t)std::ceil((hi[i] - lo[i] + 2 * margin) / h0) + 1;
+      ncell *= dims[i];
+    }
+    if (ncell <= 4 * E + 4096) break;
+    h0 *= 1.25;
+  }
+  for (int i = 0; i < 3; i++) {
+    c.org[i] = lo[i] - margin;
+    c.dim[i] = (int32_t)dims[i];
+  }
+  c.ncell = ncell;
+  c.h0 = h0;
+  int rc;
+  int32_t *pi;
+  double *pd;
+  if ((rc = up(cs, &pi, es))) return rc; c.es = pi;
+  if ((rc = up(cs, &pi, e0))) return rc; c.en0 = pi;
+  if ((rc = up(cs, &pi, e1))) return rc; c.en1 = pi;
+  if ((rc = up(cs, &pd, crad))) return rc; c.crad = pd;
+  if ((rc = up(cs, &pi, nsptr))) return rc; c.nsptr = pi;
+  if ((rc = up(cs, &pi, nslist))) return rc; c.nslist = pi;
+  cs.nblk_e = (int32_t)nb(E);
+  cs.nscan_blk = (std::max<int64_t>(ncell, E) + 1023) / 1024 + 1;
+  if ((rc = calloc_dev(cs, &c.hbuf, 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.box, 6 * E))) return rc;
+  if ((rc = calloc_dev(cs, &c.ext_part, cs.nblk_e))) return rc;
+  if ((rc = calloc_dev(cs, &c.cell, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.rank, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.ccount, ncell))) return rc;
+  if ((rc = calloc_dev(cs, &c.cstart, ncell + 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.bsum, cs.nscan_blk + 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.sorted, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.cidx, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.fcs, 6 * E))) return rc;
+  if ((rc = calloc_dev(cs, &c.slot_ctr, 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.lcount, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.loff, E + 1))) return rc;
+  c.black = nullptr;
+  c.nblack = 0;
+  c.la = c.lb = nullptr;
+  c.lcap = 0;
+  CHK(hipMemset(c.cidx, 0xff, sizeof(int32_t) * E)); // -1: no contact before the first evaluation
+  d.cidx = c.cidx;
+  d.fcs = c.fcs;
+  return DL_OK;
+}
+
+static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *out, hipStream_t s) {
+  const int64_t nblk = (n + 1023) / 1024;
+  ContactDev &c = cs.c;
+  hipLaunchKernelGGL(kc_scan1, dim3((unsigned)nblk), dim3(CB), 0, s, in, n, out, c.bsum);
+  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.bsum, nblk, out + n);
+  hipLaunchKernelGGL(kc_scan3, dim3(nb(n)), dim3(CB), 0, s, out, n, (const int32_t *)c.bsum);
+}
+
+// broadphase up to sorted cell lists
+static void broadphase(ContactState &cs, const DevData &d, hipStream_t s) {
+  ContactDev &c = cs.c;
+  hipLaunchKernelGGL(kc_boxes, dim3(cs.nblk_e), dim3(CB), 0, s, c, d);
+  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, cs.nblk_e);
+  hipLaunchKernelGGL(kc_fill_i32, dim3(nb(c.ncell)), dim3(CB), 0, s, c.ccount, c.ncell, 0);
+  hipLaunchKernelGGL(kc_cells, dim3(cs.nblk_e), dim3(CB), 0, s, c);
+  scan_i32(cs, c.ccount, c.ncell, c.cstart, s);
+  hipLaunchKernelGGL(kc_scatter, dim3(cs.nblk_e), dim3(CB), 0, s, c);
+  hipLaunchKernelGGL(kc_sortcells, dim3(nb(c.ncell)), dim3(CB), 0, s, c);
+}
+
+static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,
+                      std::vector<int32_t> &A, std::vector<int32_t> &B) {
+  ContactDev &c = cs.c;
+  hipLaunchKernelGGL(kc_narrow<1>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr);
+  scan_i32(cs, c.lcount, c.E, c.loff, s);
+  int32_t total = 0;
+  CHK(hipMemcpyAsync(&total, c.loff + c.E, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CHK(hipStreamSynchronize(s));
+  if (total > c.lcap) {
+    int rc;
+    const int64_t cap = std::max<int64_t>(total, 1024);
+    if ((rc = calloc_dev(cs, &c.la, cap))) return rc;
+    if ((rc = calloc_dev(cs, &c.lb, cap))) return rc;
+    c.lcap = cap;
+  }
+  A.assign(total, 0);
+  B.assign(total, 0);
+  if (total > 0) {
+    hipLaunchKernelGGL(kc_narrow<2>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr);
+    CHK(hipGetLastError());
+    CHK(hipMemcpyAsync(A.data(), c.la, sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
+    CHK(hipMemcpyAsync(B.data(), c.lb, sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+  }
+  // canonical order: sorted by (a, b)
+  std::vector<int64_t> k(total);
+  for (int32_t i = 0; i < total; i++) k[i] = ((int64_t)A[i] << 32) | (uint32_t)B[i];
+  std::sort(k.begin(), k.end());
+  for (int32_t i = 0; i < total; i++) {
+    A[i] = (int32_t)(k[i] >> 32);
+    B[i] = (int32_t)(k[i] & 0xffffffff);
+  }
+  return DL_OK;
+}
+
+int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s) {
+  ContactDev &c = cs.c;
+  c.nblack = 0;
+  broadphase(cs, d, s);
+  std::vector<int32_t> A, B;
+  int rc = list_pairs(cs, d, s, LIST_CLOSED, A, B);
+  if (rc) return rc;
+  std::vector<unsigned long long> keys(A.size());
+  for (size_t i = 0; i < A.size(); i++) {
+    const unsigned long long lo = (unsigned long long)std::min(A[i], B[i]);
+    const unsigned long long hi = (unsigned long long)std::max(A[i], B[i]);
+    keys[i] = (lo << 32) | hi;
+  }
+  std::sort(keys.begin(), keys.end());
+  unsigned long long *pk;
+  if ((rc = up(cs, &pk, keys))) return rc;
+  c.black = pk;
+  c.nblack = (int64_t)keys.size();
+  cs.last_a.clear();
+  cs.last_b.clear();
+  return DL_OK;
+}
+
+int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
+  ContactDev &c = cs.c;
+  broadphase(cs, d, s);
+  hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
+                     nullptr, nullptr);
+  CHK(hipGetLastError());
+  int rc = list_pairs(cs, d, s, LIST_ACTIVE, cs.last_a, cs.last_b);
+  if (rc) return rc;
+  return list_pairs(cs, d, s, LIST_CAND, cs.cand_a, cs.cand_b);
+}
+
+int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
+                 int64_t *n_cand) {
+  ContactDev &c = cs.c;
+  broadphase(cs, d, s);
+  hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
+                     (unsigned long long *)n_active, (unsigned long long *)n_cand);
+  CHK(hipGetLastError());
+  return DL_OK;
+}
+
+int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,
+                       int64_t cap, int64_t *n) {
+  CHK(hipStreamSynchronize(s));
+  const std::vector<int32_t> &A = active ? cs.last_a : cs.cand_a;
+  const std::vector<int32_t> &B = active ? cs.last_b : cs.cand_b;
+  if (n) *n = (int64_t)A.size();
+  const int64_t k = std::min<int64_t>(cap, (int64_t)A.size());
+  for (int64_t i = 0; i < k; i++) {
+    if (a) a[i] = A[i];
+    if (b) b[i] = B[i];
+  }
   return DL_OK;
 }

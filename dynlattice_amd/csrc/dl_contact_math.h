@@ -1,0 +1,191 @@
+// dl_contact_math.h -- closest-point narrowphase of the rod-rod penalty
+// contact, shared by the device kernels (dl_contact.hip).
+//
+// Restates RodContactModel::computeContacts_ for one candidate pair of
+// 2-node elements (RodContactModel.cpp:451-738) with findClosestPoints_
+// (:953-991), getClosestPoint_ (:996-1026), computeSTS_ (:1031-1152, force
+// part) and computeNTS_ (:1157-1208, force part), in the same operation
+// order (no FMA contraction) so that the closed/open decision of every pair
+// is the same bit pattern as the oracle's.
+#ifndef DL_CONTACT_MATH_H
+#define DL_CONTACT_MATH_H
+
+#include "dl_device.h"
+
+// StdLine::containsLocalPoint(u): -1 <= u <= 1 (NaN is outside)
+DL_HD bool cm_contains(double u) { return u >= -1.0 && u <= 1.0; }
+
+DL_HD double cm_dot(const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+  double s = a[0] * b[0];
+  s += a[1] * b[1];
+  s += a[2] * b[2];
+  return s;
+}
+
+DL_HD void cm_point(double u, const double p0[3], const double p1[3], double x[3]) {
+#pragma clang fp contract(off)
+  const double N0 = 0.5 * (1. - u), N1 = 0.5 * (1. + u);
+  for (int i = 0; i < 3; i++) {
+    double s = p0[i] * N0;
+    s += p1[i] * N1;
+    x[i] = s;
+  }
+}
+
+DL_HD void cm_closest_points(const double pA[2][3], const double pB[2][3], double *uA,
+                             double *uB) {
+#pragma clang fp contract(off)
+  double bA[3], bB[3], tA[3], tB[3], d[3], vA[3], vB[3];
+  for (int i = 0; i < 3; i++) {
+    bA[i] = pA[1][i] + pA[0][i];
+    bB[i] = pB[1][i] + pB[0][i];
+    tA[i] = pA[1][i] - pA[0][i];
+    tB[i] = pB[1][i] - pB[0][i];
+    d[i] = bB[i] - bA[i];
+  }
+  const double tBtA = cm_dot(tB, tA), tBtB = cm_dot(tB, tB), tAtA = cm_dot(tA, tA);
+  for (int i = 0; i < 3; i++) {
+    vA[i] = tB[i] * tBtA - tA[i] * tBtB;
+    vB[i] = tA[i] * tBtA - tB[i] * tAtA;
+  }
+  const double den = tBtB * tAtA - tBtA * tBtA;
+  *uA = -1. * cm_dot(d, vA) / den;
+  *uB = +1. * cm_dot(d, vB) / den;
+}
+
+DL_HD double cm_closest_point(const double s[3], const double pM[2][3]) {
+#pragma clang fp contract(off)
+  double bM[3], tM[3];
+  for (int i = 0; i < 3; i++) {
+    bM[i] = pM[1][i] + pM[0][i];
+    tM[i] = pM[1][i] - pM[0][i];
+  }
+  return (2 * cm_dot(s, tM) - cm_dot(tM, bM)) / cm_dot(tM, tM);
+}
+
+// jem::isTiny(d): |d| < DBL_MIN
+DL_HD bool cm_tiny(double d) { return fabs(d) < 2.2250738585072014e-308; }
+
+// segment-to-segment; f[12] += (A0, A1, B0, B1) when f != nullptr
+DL_HD bool cm_sts(const double pA[2][3], const double pB[2][3], double uA, double uB, double R,
+                  double kap, double *f) {
+#pragma clang fp contract(off)
+  double xA[3], xB[3], d[3];
+  cm_point(uA, pA[0], pA[1], xA);
+  cm_point(uB, pB[0], pB[1], xB);
+  for (int i = 0; i < 3; i++) d[i] = xB[i] - xA[i];
+  const double dist = sqrt(cm_dot(d, d));
+  if (cm_tiny(dist)) return false;
+  if (dist > R) return false;
+  if (!f) return true;
+  double n[3];
+  for (int i = 0; i < 3; i++) n[i] = d[i] / dist;
+  const double NA0 = 0.5 * (1. - uA), NA1 = 0.5 * (1. + uA);
+  const double NB0 = 0.5 * (1. - uB), NB1 = 0.5 * (1. + uB);
+  const double sc = kap * (dist - R);
+  for (int i = 0; i < 3; i++) {
+    f[0 + i] += sc * ((-1. * NA0) * n[i]);
+    f[3 + i] += sc * ((-1. * NA1) * n[i]);
+    f[6 + i] += sc * (NB0 * n[i]);
+    f[9 + i] += sc * (NB1 * n[i]);
+  }
+  return true;
+}
+
+// node-to-segment; f[9] += (slave, M0, M1) when f != nullptr
+DL_HD bool cm_nts(const double s[3], const double pM[2][3], double uM, double R, double kap,
+                  double *f) {
+#pragma clang fp contract(off)
+  double xM[3], d[3];
+  cm_point(uM, pM[0], pM[1], xM);
+  for (int i = 0; i < 3; i++) d[i] = xM[i] - s[i];
+  const double dist = sqrt(cm_dot(d, d));
+  if (cm_tiny(dist)) return false;
+  if (dist > R) return false;
+  if (!f) return true;
+  double n[3];
+  for (int i = 0; i < 3; i++) n[i] = (s[i] - xM[i]) / dist;
+  const double sc = kap * (dist - R);
+  const double a = -.5 * (1. - uM), b = -.5 * (1. + uM);
+  for (int i = 0; i < 3; i++) {
+    f[i] += sc * n[i];
+    f[3 + i] += sc * (a * n[i]);
+    f[6 + i] += sc * (b * n[i]);
+  }
+  return true;
+}
+
+// computeContacts_ per candidate pair (A, B) after the shared-node and
+// blacklist filters.  Returns whether the contact is closed; f[12] (may be
+// nullptr) receives the translational contributions at (A0, A1, B0, B1).
+DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R, double kS,
+                          double kN, double *f) {
+  double uA, uB;
+  cm_closest_points(pA, pB, &uA, &uB);
+  if (f)
+    for (int i = 0; i < 12; i++) f[i] = 0.0;
+  const bool inA = cm_contains(uA), inB = cm_contains(uB);
+  if (inA && inB) return cm_sts(pA, pB, uA, uB, R, kS, f);
+  if (!inA && inB) {
+    const int iA = (uA < -1.) ? 0 : 1;
+    // RodContactModel.cpp:575 projects node iA of B (not of A) onto A
+    const double u = cm_closest_point(pB[iA], pA);
+    if (!cm_contains(u)) return false;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!cm_nts(pA[iA], pB, u, R, kN, f ? g : nullptr)) return false;
+    if (f)
+      for (int i = 0; i < 3; i++) {
+        f[3 * iA + i] = g[i];
+        f[6 + i] = g[3 + i];
+        f[9 + i] = g[6 + i];
+      }
+    return true;
+  }
+  if (!inB && inA) {
+    const int iB = (uB < -1.) ? 0 : 1;
+    const double u = cm_closest_point(pB[iB], pA);
+    if (!cm_contains(u)) return false;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!cm_nts(pB[iB], pA, u, R, kN, f ? g : nullptr)) return false;
+    if (f)
+      for (int i = 0; i < 3; i++) {
+        f[6 + 3 * iB + i] = g[i];
+        f[0 + i] = g[3 + i];
+        f[3 + i] = g[6 + i];
+      }
+    return true;
+  }
+  bool closed = false;
+  for (int iA = 0; iA < 2; iA++) {
+    const double u = cm_closest_point(pA[iA], pB);
+    if (!cm_contains(u)) continue;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (cm_nts(pA[iA], pB, u, R, kN, f ? g : nullptr)) {
+      if (f)
+        for (int i = 0; i < 3; i++) {
+          f[3 * iA + i] += g[i];
+          f[6 + i] += g[3 + i];
+          f[9 + i] += g[6 + i];
+        }
+      closed = true;
+    }
+  }
+  for (int iB = 0; iB < 2; iB++) {
+    const double u = cm_closest_point(pB[iB], pA);
+    if (!cm_contains(u)) continue;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (cm_nts(pB[iB], pA, u, R, kN, f ? g : nullptr)) {
+      if (f)
+        for (int i = 0; i < 3; i++) {
+          f[0 + i] += g[3 + i];
+          f[3 + i] += g[6 + i];
+          f[6 + 3 * iB + i] += g[i];
+        }
+      closed = true;
+    }
+  }
+  return closed;
+}
+
+#endif

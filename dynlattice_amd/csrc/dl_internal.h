@@ -59,7 +59,9 @@ struct DevData {
   double *sbuf;         // [6][nslot]
   // outputs
   double *fout;         // [6][N] (action path)
-  double *fc;           // [3][N] contact force or null
+  // contact: per-element translational contact force on (node a, node b)
+  const int32_t *cidx;  // [E] slot into fcs or -1
+  const double *fcs;    // [6][E] (a: 0..2, b: 3..5)
   double *rbuf;         // reaction contributions [n_react]
   double *dbuf;         // [R] per-strut dissipation increment
   int32_t *err;         // device error flag
@@ -77,9 +79,8 @@ enum { MODE_STEP = 0, MODE_FORCE = 1 };
 void launch_init_strain(const DevData &d, hipStream_t s);
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
                      bool so3, bool plastic, bool contact, hipStream_t s);
-void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, bool contact,
-                  hipStream_t s);
-void launch_force_extras(const DevData &d, bool so3, bool gyro, bool contact, hipStream_t s);
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s);
+void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s);
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);

@@ -38,15 +38,10 @@ __device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4
 // a = M^-1 (f_ext - f) with constraints, then v += dt a, u += dt v,
 // R <- exp(dt w) R (SO(3)) or theta += dt w (additive).
 // f: element (+ contact) internal force of the node at the step's start state.
-template <bool SO3, bool CONTACT>
+template <bool SO3>
 __device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
                                             double f[6], const Quat q, const double r[4]) {
   const int64_t N = d.N;
-  if (CONTACT) {
-    f[0] += d.fc[n];
-    f[1] += d.fc[N + n];
-    f[2] += d.fc[2 * N + n];
-  }
   const int32_t b = d.bc[n];
   unsigned mask = 0u;
   double ac[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -232,6 +227,17 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
       }
       double fa[6], fb[6];
       elem_forces(k, sm, fa, fb);
+      if (CONTACT) {
+        // rod-rod contact force on this element's nodes (RodContactModel.cpp:729)
+        const int32_t ci = d.cidx[e];
+        if (ci >= 0) {
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            fa[i] += d.fcs[i * E + ci];
+            fb[i] += d.fcs[(3 + i) * E + ci];
+          }
+        }
+      }
 #pragma unroll
       for (int i = 0; i < 6; i++) facc[i] += fa[i];
       const int32_t slot = d.cslot[c];
@@ -242,7 +248,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 #pragma unroll
         for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
       } else {
-        node_update<SO3, CONTACT>(d, sp, na, facc, qna, ra);
+        node_update<SO3>(d, sp, na, facc, qna, ra);
       }
 #pragma unroll
       for (int i = 0; i < 6; i++) facc[i] = fb[i];
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 #pragma unroll
       for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
     } else {
-      node_update<SO3, CONTACT>(d, sp, na, facc, qna, ra);
+      node_update<SO3>(d, sp, na, facc, qna, ra);
     }
   }
   if (PLASTIC && MODE == MODE_STEP) {
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 // ------------------------------------------------------------------
 // joint kernel: shared nodes
 // ------------------------------------------------------------------
-template <int MODE, bool SO3, bool CONTACT>
+template <int MODE, bool SO3>
 __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.NJ) return;
@@ -302,19 +308,17 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
   double r[4];
   Quat q;
   load_rot<SO3>(d, n, r, q);
-  node_update<SO3, CONTACT>(d, sp, n, f, q, r);
+  node_update<SO3>(d, sp, n, f, q, r);
 }
 
 // ------------------------------------------------------------------
-// action path: contact + gyro added to f_out (GET_INT_VECTOR)
+// action path: gyroscopic term added to f_out (GET_INT_VECTOR with MATRIX2)
 // ------------------------------------------------------------------
 template <bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro, int contact) {
+__global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= d.N) return;
   const int64_t N = d.N;
-  if (contact)
-    for (int i = 0; i < 3; i++) d.fout[i * N + n] += d.fc[i * N + n];
   if (gyro) {
     double r[4];
     Quat q;
@@ -501,36 +505,32 @@ void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams
   DL_LS(MODE_STEP, false, true, false)
   DL_LS(MODE_STEP, false, true, true)
   DL_LS(MODE_FORCE, true, false, false)
+  DL_LS(MODE_FORCE, true, false, true)
   DL_LS(MODE_FORCE, true, true, false)
+  DL_LS(MODE_FORCE, true, true, true)
   DL_LS(MODE_FORCE, false, false, false)
+  DL_LS(MODE_FORCE, false, false, true)
   DL_LS(MODE_FORCE, false, true, false)
+  DL_LS(MODE_FORCE, false, true, true)
 #undef DL_LS
 }
 
-void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, bool contact,
-                  hipStream_t s) {
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s) {
   if (d.NJ == 0) return;
   const dim3 g(nblk(d.NJ)), b(DL_BLOCK);
   if (mode == MODE_FORCE) {
-    if (so3) hipLaunchKernelGGL((k_joint<MODE_FORCE, true, false>), g, b, 0, s, d, sp);
-    else hipLaunchKernelGGL((k_joint<MODE_FORCE, false, false>), g, b, 0, s, d, sp);
+    if (so3) hipLaunchKernelGGL((k_joint<MODE_FORCE, true>), g, b, 0, s, d, sp);
+    else hipLaunchKernelGGL((k_joint<MODE_FORCE, false>), g, b, 0, s, d, sp);
     return;
   }
-  if (so3) {
-    if (contact) hipLaunchKernelGGL((k_joint<MODE_STEP, true, true>), g, b, 0, s, d, sp);
-    else hipLaunchKernelGGL((k_joint<MODE_STEP, true, false>), g, b, 0, s, d, sp);
-  } else {
-    if (contact) hipLaunchKernelGGL((k_joint<MODE_STEP, false, true>), g, b, 0, s, d, sp);
-    else hipLaunchKernelGGL((k_joint<MODE_STEP, false, false>), g, b, 0, s, d, sp);
-  }
+  if (so3) hipLaunchKernelGGL((k_joint<MODE_STEP, true>), g, b, 0, s, d, sp);
+  else hipLaunchKernelGGL((k_joint<MODE_STEP, false>), g, b, 0, s, d, sp);
 }
 
-void launch_force_extras(const DevData &d, bool so3, bool gyro, bool contact, hipStream_t s) {
-  if (!gyro && !contact) return;
-  if (so3) hipLaunchKernelGGL(k_force_extras<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
-                              (int)gyro, (int)contact);
-  else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d,
-                          (int)gyro, (int)contact);
+void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
+  if (!gyro) return;
+  if (so3) hipLaunchKernelGGL(k_force_extras<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, 1);
+  else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, 1);
 }
 
 void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s) {
