@@ -105,9 +105,10 @@ class Plastic:
     exponent: tuple = (1.0,) * 6
 
     @staticmethod
-    def resultant_capacities(radius, sigma_y, kin_ratio=0.01):
+    def resultant_capacities(radius, sigma_y, kin_ratio=0.01, exponent=1.0):
         """Plastic capacities of a solid circle: N_p = s_y A, V_p = s_y A/sqrt3,
-        M_p = s_y 4 r^3/3, T_p = s_y 2 pi r^3/(3 sqrt3); H_kin = kin_ratio * C."""
+        M_p = s_y 4 r^3/3, T_p = s_y 2 pi r^3/(3 sqrt3); H_kin = kin_ratio * C.
+        exponent 1: sum-abs interaction; 2: quadratic interaction."""
         r = np.asarray(radius, np.float64)
         sy = np.broadcast_to(np.asarray(sigma_y, np.float64), r.shape)
         A = math.pi * r ** 2
@@ -116,7 +117,8 @@ class Plastic:
         Mp = sy * 4.0 * r ** 3 / 3.0
         Tp = sy * 2.0 * math.pi * r ** 3 / (3.0 * math.sqrt(3.0))
         w = np.stack([1 / Vp, 1 / Vp, 1 / Np, 1 / Mp, 1 / Mp, 1 / Tp], axis=1)
-        return Plastic(weights=np.ascontiguousarray(w), kin_ratio=kin_ratio)
+        return Plastic(weights=np.ascontiguousarray(w), kin_ratio=kin_ratio,
+                       exponent=(float(exponent),) * 6)
 
 
 @dataclass
@@ -295,13 +297,14 @@ def bcc_c3(n: int = 40, contact: bool = True) -> Problem:
     """Elastoplastic BCC crush with contact: n^3 cells (n = 40: 2,048,000
     elements), 4 elements per strut, r = 0.4 mm exp(0.1 N(0,1) - 0.005)
     (lognormal, mean 0.4 mm), seed 3, E = 70 GPa, nu = 0.3, rho = 2700,
-    yield 250 MPa (sum-abs resultant family), kinematic hardening H = C/100,
+    yield 250 MPa (quadratic interaction of the resultants, p = 2), kinematic
+    hardening H = C/100,
     penalty contact 1e6 N/m; top plate at -20 m/s."""
     m, cell = 4, 10e-3
     R = 8 * n ** 3
     r = 0.4e-3 * np.exp(0.1 * normal(3, R) - 0.005)
     p = bcc_lattice(n, m, cell, r, 70e9, 0.3, 2700.0)
-    p.plastic = Plastic.resultant_capacities(r, 250e6, kin_ratio=0.01)
+    p.plastic = Plastic.resultant_capacities(r, 250e6, kin_ratio=0.01, exponent=2.0)
     if contact:
         p.contact = Contact(radius=r.copy(), penalty_sts=1e6, penalty_nts=1e6)
     crush_bcs(p, -20.0)
@@ -335,7 +338,7 @@ def c4_instance(i: int) -> Problem:
     sy = 150e6 + 200e6 * u[3]
     R = 8 * n ** 3
     p = bcc_lattice(n, m, 10e-3, np.full(R, r), 70e9, 0.3, 2700.0)
-    p.plastic = Plastic.resultant_capacities(np.full(R, r), sy, kin_ratio=0.01)
+    p.plastic = Plastic.resultant_capacities(np.full(R, r), sy, kin_ratio=0.01, exponent=2.0)
     crush_bcs(p, -20.0)
     p.dt = stable_dt(p, 0.5)
     p.name = f"sweep-{i}-n{n}-m{m}"
