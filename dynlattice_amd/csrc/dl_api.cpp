@@ -51,6 +51,7 @@ struct dl_ctx {
   bool own_stream = false;
   bool so3 = true;
   bool plastic = false;
+  int mat_kind = 0; // 0 elastic; 1/2/3 plastic with yield exponents all 1 / all 2 / general
   bool contact = false;
   bool initialized = false;
   bool track_diss = false;
@@ -239,37 +240,70 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     cnode[c0] = en[2 * eptr[s]];
     for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) cnode[c0 + (e - eptr[s]) + 1] = en[2 * e + 1];
   }
-  // node occurrences and gyro multiplicity (distinct struts per node)
-  std::vector<int32_t> occ(N, 0), nstr(N, 0), last(N, -1);
+  // gyro multiplicity: number of distinct struts holding the node
+  std::vector<int32_t> nstr(N, 0), last(N, -1);
   for (int32_t s = 0; s < R; s++)
     for (int64_t p = eptr[s] + s; p <= eptr[s + 1] + s; p++) {
       const int32_t n = cnode[p];
-      occ[n]++;
       if (last[n] != s) { nstr[n]++; last[n] = s; }
     }
   for (int64_t n = 0; n < N; n++)
-    if (occ[n] == 0) {
+    if (nstr[n] == 0) {
       dl_destroy(c);
       return fail(DL_ERR_ZERO_MASS, "node " + std::to_string(n) + " belongs to no element (zero mass)");
     }
-  // slots for shared nodes, in (strut, chain position) order
-  std::vector<int32_t> cslot(E + R, -1);
+  // chains: struts cut into pieces of at most 64 elements (one wavefront);
+  // a cut node then occurs twice and becomes a shared node
+  std::vector<int64_t> gptr;
+  for (int32_t s = 0; s < R; s++)
+    for (int64_t b = eptr[s]; b < eptr[s + 1]; b += 64) gptr.push_back(b);
+  gptr.push_back(E);
+  const int64_t NG = (int64_t)gptr.size() - 1;
+  auto gnode = [&](int64_t g, int64_t k) -> int32_t { // chain position k of chain g
+    return k == 0 ? en[2 * gptr[g]] : en[2 * (gptr[g] + k - 1) + 1];
+  };
+  std::vector<int32_t> occ(N, 0);
+  for (int64_t g = 0; g < NG; g++)
+    for (int64_t k = 0; k <= gptr[g + 1] - gptr[g]; k++) occ[gnode(g, k)]++;
+  // slots for shared nodes in (chain, position) order, i.e. strut order
+  std::vector<int32_t> sa(E, -1), sb(E, -1), jidx(N, -1), jnode;
+  std::vector<uint8_t> eflag(E, 0);
+  std::vector<std::pair<int32_t, int32_t>> slot_node; // (slot, node)
   int32_t nslot = 0;
-  for (int64_t p = 0; p < E + R; p++)
-    if (occ[cnode[p]] > 1) cslot[p] = nslot++;
-  std::vector<int32_t> jidx(N, -1), jnode;
+  for (int64_t g = 0; g < NG; g++) {
+    const int64_t len = gptr[g + 1] - gptr[g];
+    for (int64_t k = 0; k <= len; k++) {
+      const int32_t n = gnode(g, k);
+      int32_t sl = -1;
+      if (occ[n] > 1) {
+        sl = nslot++;
+        slot_node.push_back({sl, n});
+      }
+      if (k < len) sa[gptr[g] + k] = sl;
+      else sb[gptr[g] + len - 1] = sl;
+    }
+    eflag[gptr[g]] |= 1u;
+    eflag[gptr[g] + len - 1] |= 2u;
+  }
   for (int64_t n = 0; n < N; n++)
     if (occ[n] > 1) { jidx[n] = (int32_t)jnode.size(); jnode.push_back((int32_t)n); }
   const int32_t NJ = (int32_t)jnode.size();
   std::vector<int32_t> jptr(NJ + 1, 0), jslot(nslot);
-  for (int64_t p = 0; p < E + R; p++)
-    if (cslot[p] >= 0) jptr[jidx[cnode[p]] + 1]++;
+  for (auto &sn : slot_node) jptr[jidx[sn.second] + 1]++;
   for (int32_t j = 0; j < NJ; j++) jptr[j + 1] += jptr[j];
   {
     std::vector<int32_t> fill(jptr.begin(), jptr.end() - 1);
-    for (int64_t p = 0; p < E + R; p++)
-      if (cslot[p] >= 0) jslot[fill[jidx[cnode[p]]]++] = cslot[p];
+    for (auto &sn : slot_node) jslot[fill[jidx[sn.second]]++] = sn.first;
   }
+  // wavefront packing: a chain never straddles two 64-lane wavefronts
+  std::vector<int32_t> lane_e;
+  for (int64_t g = 0; g < NG; g++) {
+    const int64_t len = gptr[g + 1] - gptr[g];
+    if ((int64_t)(lane_e.size() % 64) + len > 64)
+      while (lane_e.size() % 64) lane_e.push_back(-1);
+    for (int64_t k = 0; k < len; k++) lane_e.push_back((int32_t)(gptr[g] + k));
+  }
+  while (lane_e.size() % 256) lane_e.push_back(-1);
 
   // frames per (strut, node) -> per (element, end) quaternions; masses
   const double *X = mesh->coords;
@@ -438,11 +472,21 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     d.bc_mask = nullptr; d.bc_a = nullptr; d.bc_f = nullptr; d.bc_rslot = nullptr;
   }
   {
-    int64_t *pl;
-    TRY(upload(c, &pl, eptr)); d.eptr = pl;
     int32_t *pi;
-    TRY(upload(c, &pi, cnode)); d.cnode = pi;
-    TRY(upload(c, &pi, cslot)); d.cslot = pi;
+    std::vector<int32_t> e0(E), e1(E), esv(es, es + E);
+    for (int64_t e = 0; e < E; e++) {
+      e0[e] = en[2 * e];
+      e1[e] = en[2 * e + 1];
+    }
+    TRY(upload(c, &pi, e0)); d.en0 = pi;
+    TRY(upload(c, &pi, e1)); d.en1 = pi;
+    TRY(upload(c, &pi, esv)); d.es = pi;
+    TRY(upload(c, &pi, sa)); d.sa = pi;
+    TRY(upload(c, &pi, sb)); d.sb = pi;
+    TRY(upload(c, &pi, lane_e)); d.lane_e = pi;
+    d.NL = (int64_t)lane_e.size();
+    uint8_t *pu;
+    TRY(upload(c, &pu, eflag)); d.eflag = pu;
     d.NJ = NJ;
     d.nslot = nslot;
     TRY(upload(c, &pi, jnode)); d.jnode = pi;
@@ -474,12 +518,18 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       P.has_kin = mat->has_kin;
       P.max_iter = mat->max_iter;
       for (int i = 0; i < 6; i++) P.exponent[i] = mat->exponent[i];
+      bool all1 = true, all2 = true;
+      for (int i = 0; i < 6; i++) {
+        all1 = all1 && mat->exponent[i] == 1.0;
+        all2 = all2 && mat->exponent[i] == 2.0;
+      }
+      c->mat_kind = all1 ? 1 : (all2 ? 2 : 3);
     }
   }
   TRY(dalloc(c, &d.fout, 6 * N));
   TRY(dalloc(c, &d.err, 1));
   HIPCHK(hipMemset(d.err, 0, sizeof(int32_t)));
-  c->strut_blocks = (R + 255) / 256;
+  c->strut_blocks = (int32_t)(d.NL / 256);
   TRY(dalloc(c, &d.dbuf, (size_t)c->strut_blocks));
   TRY(dalloc(c, &c->diss_total, 1));
   HIPCHK(hipMemset(c->diss_total, 0, sizeof(double)));
@@ -685,7 +735,7 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
       int rc = contact_force(c->cs, c->d, c->stream);
       if (rc) return rc;
     }
-    launch_strut_pp(c->d, sp, c->pp, MODE_FORCE, c->so3, c->plastic, c->contact, c->stream);
+    launch_strut_pp(c->d, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, c->stream);
     launch_joint(c->d, sp, MODE_FORCE, c->so3, c->stream);
     launch_force_extras(c->d, c->so3, p->with_gyro != 0, c->stream);
     HIPCHK(hipGetLastError());
@@ -808,7 +858,7 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
       if (rc) return rc;
     }
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 1], c->stream));
-    launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->plastic, c->contact, c->stream);
+    launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 2], c->stream));
     launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
     if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 3], c->stream));

@@ -34,7 +34,7 @@ struct ContactDev {
   int32_t *ccount;      // [ncell]
   int32_t *cstart;      // [ncell+1]
   int32_t *bsum;        // scan block sums
-  int32_t *sorted;      // [E]
+  void *rec;            // [E] cell-ordered 64-byte records (box, element, strut, cell)
   // blacklist (sorted keys lo<<32|hi)
   const unsigned long long *black;
   int64_t nblack;

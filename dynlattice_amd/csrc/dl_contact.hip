@@ -155,24 +155,43 @@ __global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const in
   if (i < n) out[i] += bsum[i / 1024];
 }
 
+// cell-ordered element record: AABB, element, strut, cell (one 64-byte line)
+struct __align__(64) CRec {
+  double lo[3], hi[3];
+  int32_t e, s, cell, pad;
+};
+
 __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (e >= c.E) return;
-  c.sorted[c.cstart[c.cell[e]] + c.rank[e]] = (int32_t)e;
+  const int64_t E = c.E;
+  CRec r;
+  for (int i = 0; i < 3; i++) {
+    r.lo[i] = c.box[i * E + e];
+    r.hi[i] = c.box[(3 + i) * E + e];
+  }
+  r.e = (int32_t)e;
+  r.s = c.es[e];
+  r.cell = c.cell[e];
+  r.pad = 0;
+  ((CRec *)c.rec)[c.cstart[c.cell[e]] + c.rank[e]] = r;
 }
 
+// each cell's records in element order (deterministic traversal)
 __global__ __launch_bounds__(CB) void kc_sortcells(ContactDev c) {
   const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (k >= c.ncell) return;
   const int32_t b = c.cstart[k], e = c.cstart[k + 1];
+  if (e - b < 2) return;
+  CRec *r = (CRec *)c.rec;
   for (int32_t i = b + 1; i < e; i++) {
-    const int32_t v = c.sorted[i];
+    const CRec v = r[i];
     int32_t j = i - 1;
-    while (j >= b && c.sorted[j] > v) {
-      c.sorted[j + 1] = c.sorted[j];
+    while (j >= b && r[j].e > v.e) {
+      r[j + 1] = r[j];
       j--;
     }
-    c.sorted[j + 1] = v;
+    r[j + 1] = v;
   }
 }
 
@@ -205,88 +224,85 @@ __device__ __forceinline__ void elem_pos(const DevData &d, int32_t n0, int32_t n
   }
 }
 
-// MODE 0: forces (+ counts into cnt[0] active, cnt[1] candidates)
-// MODE 1: count the listed pairs of kind `kind` per element (A side)
-// MODE 2: emit them at loff[X]
+// One thread per element, in cell order (thread k owns the element of
+// record k).  MODE 0: contact forces on the element's own nodes (+ counts of
+// active pairs and candidates, each pair counted on its A side).  MODE 1:
+// count the listed pairs of `kind` (A side) into lcount[k].  MODE 2: emit
+// them at loff[k].
 template <int MODE>
 __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kind,
                                                 unsigned long long *cnt_act,
                                                 unsigned long long *cnt_cand) {
-  const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
   unsigned long long nact = 0, ncand = 0;
-  if (X < c.E) {
-    const int64_t E = c.E;
-    const int32_t sX = c.es[X];
+  const CRec *rec = (const CRec *)c.rec;
+  if (k < c.E) {
+    const CRec x = rec[k];
+    const int32_t X = x.e, sX = x.s;
     const int32_t xa = c.en0[X], xb = c.en1[X];
-    double bx[6];
-    for (int i = 0; i < 6; i++) bx[i] = c.box[i * E + X];
     double pX[2][3];
     elem_pos(d, xa, xb, pX);
-    const int32_t lin = c.cell[X];
-    const int32_t cx = lin % c.dim[0];
-    const int32_t cy = (lin / c.dim[0]) % c.dim[1];
-    const int32_t cz = lin / (c.dim[0] * c.dim[1]);
+    const int32_t cx = x.cell % c.dim[0];
+    const int32_t cy = (x.cell / c.dim[0]) % c.dim[1];
+    const int32_t cz = x.cell / (c.dim[0] * c.dim[1]);
     double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
     bool any = false;
     int32_t nlist = 0;
-    const int32_t off = (MODE == 2) ? c.loff[X] : 0;
+    const int32_t off = (MODE == 2) ? c.loff[k] : 0;
     for (int dz = -1; dz <= 1; dz++) {
       const int32_t z = cz + dz;
       if (z < 0 || z >= c.dim[2]) continue;
       for (int dy = -1; dy <= 1; dy++) {
         const int32_t y = cy + dy;
         if (y < 0 || y >= c.dim[1]) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int32_t x = cx + dx;
-          if (x < 0 || x >= c.dim[0]) continue;
-          const int32_t cc = (z * c.dim[1] + y) * c.dim[0] + x;
-          const int32_t k1 = c.cstart[cc + 1];
-          for (int32_t k = c.cstart[cc]; k < k1; k++) {
-            const int32_t Y = c.sorted[k];
-            const int32_t sY = c.es[Y];
-            if (sY == sX) continue;
-            bool ov = true;
-            for (int i = 0; i < 3; i++)
-              ov = ov && (bx[3 + i] >= c.box[i * E + Y]) && (c.box[(3 + i) * E + Y] >= bx[i]);
-            if (!ov) continue;
-            const int32_t ya = c.en0[Y], yb = c.en1[Y];
-            if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
-                node_in_strut(c, yb, sX))
-              continue;
-            const bool XisA = sX < sY;
-            if (XisA) ncand++;
-            if (MODE != 0 && kind == LIST_CAND) {
-              if (XisA) {
-                if (MODE == 2) { c.la[off + nlist] = (int32_t)X; c.lb[off + nlist] = Y; }
-                nlist++;
-              }
-              continue;
+        // the three cells x-1..x+1 of a row are contiguous in cell order
+        const int32_t x0 = cx > 0 ? cx - 1 : 0;
+        const int32_t x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
+        const int32_t row = (z * c.dim[1] + y) * c.dim[0];
+        const int32_t j1 = c.cstart[row + x1 + 1];
+        for (int32_t j = c.cstart[row + x0]; j < j1; j++) {
+          const CRec yr = rec[j];
+          if (yr.s == sX) continue;
+          bool ov = true;
+          for (int i = 0; i < 3; i++) ov = ov && (x.hi[i] >= yr.lo[i]) && (yr.hi[i] >= x.lo[i]);
+          if (!ov) continue;
+          const int32_t Y = yr.e, sY = yr.s;
+          const int32_t ya = c.en0[Y], yb = c.en1[Y];
+          if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
+              node_in_strut(c, yb, sX))
+            continue;
+          const bool XisA = sX < sY;
+          if (XisA) ncand++;
+          if (MODE != 0 && kind == LIST_CAND) {
+            if (XisA) {
+              if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
+              nlist++;
             }
-            if (MODE != 0 && !XisA) continue;
-            const int32_t eA = XisA ? (int32_t)X : Y, eB = XisA ? Y : (int32_t)X;
-            if (kind != LIST_CLOSED && blacklisted(c, eA, eB)) continue;
-            double pY[2][3];
-            elem_pos(d, ya, yb, pY);
-            const double R = c.crad[sX] + c.crad[sY];
-            if (MODE == 0) {
-              double f[12];
-              const bool closed = XisA ? cm_narrowphase(pX, pY, R, c.kS, c.kN, f)
-                                       : cm_narrowphase(pY, pX, R, c.kS, c.kN, f);
-              if (closed) {
-                any = true;
-                const int o = XisA ? 0 : 6;
-                for (int i = 0; i < 3; i++) {
-                  fa[i] += f[o + i];
-                  fb[i] += f[o + 3 + i];
-                }
-                if (XisA) nact++;
+            continue;
+          }
+          if (MODE != 0 && !XisA) continue;
+          if (kind != LIST_CLOSED && blacklisted(c, XisA ? X : Y, XisA ? Y : X)) continue;
+          double pY[2][3];
+          elem_pos(d, ya, yb, pY);
+          const double R = c.crad[sX] + c.crad[sY];
+          if (MODE == 0) {
+            // this pair's contribution summed from zero, then added
+            double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+            const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                                     : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+            if (closed) {
+              any = true;
+              for (int i = 0; i < 3; i++) {
+                fa[i] += g0[i];
+                fb[i] += g1[i];
               }
-            } else {
-              const bool closed = cm_narrowphase(pX, pY, R, c.kS, c.kN, nullptr);
-              if (closed) {
-                if (MODE == 2) { c.la[off + nlist] = (int32_t)X; c.lb[off + nlist] = Y; }
-                nlist++;
-              }
+              if (XisA) nact++;
+            }
+          } else {
+            double g0[3], g1[3];
+            if (cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)) {
+              if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
+              nlist++;
             }
           }
         }
@@ -296,6 +312,7 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
       if (any) {
         const int32_t slot = atomicAdd(c.slot_ctr, 1);
         c.cidx[X] = slot;
+        const int64_t E = c.E;
         for (int i = 0; i < 3; i++) {
           c.fcs[i * E + slot] = fa[i];
           c.fcs[(3 + i) * E + slot] = fb[i];
@@ -304,7 +321,7 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
         c.cidx[X] = -1;
       }
     }
-    if (MODE == 1) c.lcount[X] = nlist;
+    if (MODE == 1) c.lcount[k] = nlist;
   }
   if (MODE == 0) {
     __shared__ unsigned long long ra[CB], rc[CB];
@@ -441,7 +458,7 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = calloc_dev(cs, &c.ccount, ncell))) return rc;
   if ((rc = calloc_dev(cs, &c.cstart, ncell + 1))) return rc;
   if ((rc = calloc_dev(cs, &c.bsum, cs.nscan_blk + 1))) return rc;
-  if ((rc = calloc_dev(cs, &c.sorted, E))) return rc;
+  if ((rc = calloc_dev(cs, (double **)&c.rec, 8 * (size_t)E))) return rc;
   if ((rc = calloc_dev(cs, &c.cidx, E))) return rc;
   if ((rc = calloc_dev(cs, &c.fcs, 6 * E))) return rc;
   if ((rc = calloc_dev(cs, &c.slot_ctr, 1))) return rc;

@@ -117,70 +117,102 @@ DL_HD bool cm_nts(const double s[3], const double pM[2][3], double uM, double R,
 }
 
 // computeContacts_ per candidate pair (A, B) after the shared-node and
-// blacklist filters.  Returns whether the contact is closed; f[12] (may be
-// nullptr) receives the translational contributions at (A0, A1, B0, B1).
+// blacklist filters.  Returns whether the contact is closed.  When F is true
+// the translational contributions are added: SIDE 0 -> (A0, A1) into (f0, f1),
+// SIDE 1 -> (B0, B1) into (f0, f1).  All indices are static so nothing spills.
+template <bool F, int SIDE>
 DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R, double kS,
-                          double kN, double *f) {
+                          double kN, double f0[3], double f1[3]) {
   double uA, uB;
   cm_closest_points(pA, pB, &uA, &uB);
-  if (f)
-    for (int i = 0; i < 12; i++) f[i] = 0.0;
   const bool inA = cm_contains(uA), inB = cm_contains(uB);
-  if (inA && inB) return cm_sts(pA, pB, uA, uB, R, kS, f);
+  if (inA && inB) {
+    double g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!cm_sts(pA, pB, uA, uB, R, kS, F ? g : nullptr)) return false;
+    if (F)
+      for (int i = 0; i < 3; i++) {
+        f0[i] += g[6 * SIDE + i];
+        f1[i] += g[6 * SIDE + 3 + i];
+      }
+    return true;
+  }
   if (!inA && inB) {
-    const int iA = (uA < -1.) ? 0 : 1;
+    const bool i1 = !(uA < -1.);
     // RodContactModel.cpp:575 projects node iA of B (not of A) onto A
-    const double u = cm_closest_point(pB[iA], pA);
+    const double sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
+                          i1 ? pB[1][2] : pB[0][2]};
+    const double sA[3] = {i1 ? pA[1][0] : pA[0][0], i1 ? pA[1][1] : pA[0][1],
+                          i1 ? pA[1][2] : pA[0][2]};
+    const double u = cm_closest_point(sB, pA);
     if (!cm_contains(u)) return false;
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (!cm_nts(pA[iA], pB, u, R, kN, f ? g : nullptr)) return false;
-    if (f)
+    if (!cm_nts(sA, pB, u, R, kN, F ? g : nullptr)) return false;
+    if (F)
       for (int i = 0; i < 3; i++) {
-        f[3 * iA + i] = g[i];
-        f[6 + i] = g[3 + i];
-        f[9 + i] = g[6 + i];
+        if (SIDE == 0) {
+          f0[i] += i1 ? 0.0 : g[i];
+          f1[i] += i1 ? g[i] : 0.0;
+        } else {
+          f0[i] += g[3 + i];
+          f1[i] += g[6 + i];
+        }
       }
     return true;
   }
   if (!inB && inA) {
-    const int iB = (uB < -1.) ? 0 : 1;
-    const double u = cm_closest_point(pB[iB], pA);
+    const bool i1 = !(uB < -1.);
+    const double sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
+                          i1 ? pB[1][2] : pB[0][2]};
+    const double u = cm_closest_point(sB, pA);
     if (!cm_contains(u)) return false;
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (!cm_nts(pB[iB], pA, u, R, kN, f ? g : nullptr)) return false;
-    if (f)
+    if (!cm_nts(sB, pA, u, R, kN, F ? g : nullptr)) return false;
+    if (F)
       for (int i = 0; i < 3; i++) {
-        f[6 + 3 * iB + i] = g[i];
-        f[0 + i] = g[3 + i];
-        f[3 + i] = g[6 + i];
+        if (SIDE == 1) {
+          f0[i] += i1 ? 0.0 : g[i];
+          f1[i] += i1 ? g[i] : 0.0;
+        } else {
+          f0[i] += g[3 + i];
+          f1[i] += g[6 + i];
+        }
       }
     return true;
   }
+  // speculative NTS from all four end nodes
   bool closed = false;
+#pragma unroll
   for (int iA = 0; iA < 2; iA++) {
     const double u = cm_closest_point(pA[iA], pB);
     if (!cm_contains(u)) continue;
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (cm_nts(pA[iA], pB, u, R, kN, f ? g : nullptr)) {
-      if (f)
+    if (cm_nts(pA[iA], pB, u, R, kN, F ? g : nullptr)) {
+      if (F)
         for (int i = 0; i < 3; i++) {
-          f[3 * iA + i] += g[i];
-          f[6 + i] += g[3 + i];
-          f[9 + i] += g[6 + i];
+          if (SIDE == 0) {
+            if (iA == 0) f0[i] += g[i]; else f1[i] += g[i];
+          } else {
+            f0[i] += g[3 + i];
+            f1[i] += g[6 + i];
+          }
         }
       closed = true;
     }
   }
+#pragma unroll
   for (int iB = 0; iB < 2; iB++) {
     const double u = cm_closest_point(pB[iB], pA);
     if (!cm_contains(u)) continue;
     double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (cm_nts(pB[iB], pA, u, R, kN, f ? g : nullptr)) {
-      if (f)
+    if (cm_nts(pB[iB], pA, u, R, kN, F ? g : nullptr)) {
+      if (F)
         for (int i = 0; i < 3; i++) {
-          f[0 + i] += g[3 + i];
-          f[3 + i] += g[6 + i];
-          f[6 + 3 * iB + i] += g[i];
+          if (SIDE == 0) {
+            f0[i] += g[3 + i];
+            f1[i] += g[6 + i];
+          } else {
+            if (iB == 0) f0[i] += g[i]; else f1[i] += g[i];
+          }
         }
       closed = true;
     }

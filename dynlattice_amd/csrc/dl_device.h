@@ -182,15 +182,23 @@ struct PlasticParams {
 };
 
 DL_HD double dl_sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
-// |x|^p and d|x|^p/d|x| of the power-sum yield family (p = 1: sum-abs)
+// |x|^p and d|x|^p/d|x| of the power-sum yield family.  EXPK selects the
+// exponent kind uniformly for all six components: 1 (sum-abs), 2 (quadratic)
+// or 0 (general p_i through pow).
+template <int EXPK>
 DL_HD double yield_pterm(double x, double p) {
   const double ax = fabs(x);
+  if (EXPK == 1) return ax;
+  if (EXPK == 2) return ax * ax;
   if (p == 1.0) return ax;
   if (p == 2.0) return ax * ax;
   return pow(ax, p);
 }
+template <int EXPK>
 DL_HD double yield_pderiv(double x, double p) {
   const double ax = fabs(x);
+  if (EXPK == 1) return 1.0;
+  if (EXPK == 2) return 2.0 * ax;
   if (p == 1.0) return 1.0;
   if (p == 2.0) return 2.0 * ax;
   return p * pow(ax, p - 1.0);
@@ -198,6 +206,7 @@ DL_HD double yield_pderiv(double x, double p) {
 
 // Returns the number of cutting-plane corrections taken, or -1 when the
 // iteration cap is hit.  ep[6], al[7] (iso, kin x6) in/out; stress out.
+template <int EXPK>
 DL_HD int return_map(const PlasticParams &P, const double C[6], const double w[6],
                      const double Hk[6], const double strain[6], double ep[6], double al[7],
                      double stress[6], double *dflow_out, int inelastic) {
@@ -213,14 +222,14 @@ DL_HD int return_map(const PlasticParams &P, const double C[6], const double w[6
     double f = 0.0;
     for (int i = 0; i < 6; i++) {
       a[i] = stress[i] + (P.has_kin ? P.kin_coupling[i] * q[1 + i] : 0.0);
-      f += yield_pterm(w[i] * a[i], P.exponent[i]);
+      f += yield_pterm<EXPK>(w[i] * a[i], P.exponent[i]);
     }
     f -= P.yield_level * (1.0 + (P.has_iso ? P.iso_coupling * q[0] : 0.0));
     if (!(liter < P.max_iter)) { *dflow_out = dflow; return -1; }
     if (f < P.precision) break;
     double gs[6], gq[7];
     for (int i = 0; i < 6; i++) {
-      const double dd = yield_pderiv(w[i] * a[i], P.exponent[i]) * w[i] * dl_sgn(a[i]);
+      const double dd = yield_pderiv<EXPK>(w[i] * a[i], P.exponent[i]) * w[i] * dl_sgn(a[i]);
       gs[i] = dd;
       gq[1 + i] = P.has_kin ? dd * P.kin_coupling[i] : 0.0;
       if (fabs(stress[i]) < 2.220446049250313e-16) gs[i] = 0.0;

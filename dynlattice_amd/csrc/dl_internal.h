@@ -32,10 +32,16 @@ struct DevData {
   int32_t nbc;
   int32_t react_dof;
   // struts
-  const int64_t *eptr;  // [R+1]
-  const int32_t *cnode; // [E+R] chain nodes
-  const int32_t *cslot; // [E+R] shared slot or -1
   const double *C;      // [6][R] material stiffness diag
+  // element -> lane packing: chains of <= 64 elements in consecutive lanes
+  int64_t NL;             // lanes (multiple of the block size)
+  const int32_t *lane_e;  // [NL] element of the lane or -1
+  const int32_t *en0;     // [E] node a
+  const int32_t *en1;     // [E] node b
+  const int32_t *es;      // [E] strut
+  const uint8_t *eflag;   // [E] bit 0: first element of its chain, bit 1: last
+  const int32_t *sa;      // [E] slot of node a if shared, else -1
+  const int32_t *sb;      // [E] slot of node b (chain end) if shared, else -1
   // elements ([c][E])
   const double *qN;     // 8: LambdaN at end a (4), end b (4)
   double *eps0;         // 6
@@ -78,7 +84,7 @@ enum { MODE_STEP = 0, MODE_FORCE = 1 };
 // launchers (dl_kernels.hip)
 void launch_init_strain(const DevData &d, hipStream_t s);
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
-                     bool so3, bool plastic, bool contact, hipStream_t s);
+                     bool so3, int mat, bool contact, hipStream_t s);
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s);
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s);

@@ -116,15 +116,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.E) return;
   const int64_t E = d.E, N = d.N;
-  // element nodes from the chain: element e of strut s is chain position e + s
-  // -> found through a binary search of eptr
-  int32_t lo = 0, hi = d.R - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (d.eptr[mid] <= e) lo = mid; else hi = mid - 1;
-  }
-  const int64_t c = e + lo;
-  const int32_t na = d.cnode[c], nb = d.cnode[c + 1];
+  const int32_t na = d.en0[e], nb = d.en1[e];
   double Xa[3], Xb[3];
   const double z[3] = {0.0, 0.0, 0.0};
   for (int i = 0; i < 3; i++) { Xa[i] = d.X0[i * N + na]; Xb[i] = d.X0[i * N + nb]; }
@@ -140,134 +132,149 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
 }
 
 // ------------------------------------------------------------------
-// strut kernel
+// element-chain kernel: one thread (lane) per element
+//
+// The elements of a chain (a strut, or a <= 64-element piece of a longer
+// strut) sit in consecutive lanes of one 64-wide wavefront.  Lane k owns the
+// chain node p_k (its element's node a): the node's internal force is
+// fb(k-1) + fa(k), with fb(k-1) taken from the left neighbour lane by a
+// shuffle.  The last lane of a chain also owns p_m.  A private node is
+// updated in place right away; a shared node's contribution goes to its
+// slot and the joint kernel finishes it.
 // ------------------------------------------------------------------
-template <int MODE, bool SO3, bool PLASTIC, bool CONTACT>
+// MAT: 0 elastic; 1, 2, 3 elastoplastic with yield exponents all 1, all 2, general
+template <int MODE, bool SO3, int MAT, bool CONTACT>
 __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr bool PLASTIC = MAT != 0;
+  const int64_t L = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  const int32_t e = (L < d.NL) ? d.lane_e[L] : -1;
+  const int64_t N = d.N, E = d.E, R = d.R;
+  double fa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double ra[4], rb[4];
+  Quat qna, qnb;
+  int32_t na = 0, nb = 0;
+  uint32_t fl = 0;
   double diss = 0.0;
-  if (s < d.R) {
-    const int64_t N = d.N, E = d.E, R = d.R;
-    const int64_t e0 = d.eptr[s], e1 = d.eptr[s + 1];
-    const int64_t c0 = e0 + s;
-    double C[6], pw[6], Hk[6];
+  if (e >= 0) {
+    na = d.en0[e];
+    nb = d.en1[e];
+    fl = d.eflag[e];
+    const int32_t s = d.es[e];
+    double C[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) C[i] = d.C[i * R + s];
-    if (PLASTIC) {
+    double Xa[3], ua[3], Xb[3], ub[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      Xa[i] = d.X0[i * N + na]; ua[i] = d.u[i * N + na];
+      Xb[i] = d.X0[i * N + nb]; ub[i] = d.u[i * N + nb];
+    }
+    load_rot<SO3>(d, na, ra, qna);
+    load_rot<SO3>(d, nb, rb, qnb);
+    Quat qNa, qNb;
+    qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
+    qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+    ElemKin k;
+    elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), k);
+    double strain[6], sm[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - d.eps0[i * E + e];
+    if (!PLASTIC) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
+      if (MODE == MODE_FORCE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
+      }
+    } else {
+      double pw[6], Hk[6];
 #pragma unroll
       for (int i = 0; i < 6; i++) { pw[i] = d.pw[i * R + s]; Hk[i] = d.Hk[i * R + s]; }
-    }
-    int32_t na = d.cnode[c0];
-    double Xa[3], ua[3], ra[4];
-    Quat qna;
+      double ep[6], al[7], epo[6];
 #pragma unroll
-    for (int i = 0; i < 3; i++) { Xa[i] = d.X0[i * N + na]; ua[i] = d.u[i * N + na]; }
-    load_rot<SO3>(d, na, ra, qna);
-    double facc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int64_t e = e0; e < e1; ++e) {
-      const int64_t c = c0 + (e - e0);
-      const int32_t nb = d.cnode[c + 1];
-      double Xb[3], ub[3], rb[4];
-      Quat qnb;
+      for (int i = 0; i < 6; i++) { ep[i] = d.ep[i * E + e]; epo[i] = ep[i]; }
 #pragma unroll
-      for (int i = 0; i < 3; i++) { Xb[i] = d.X0[i * N + nb]; ub[i] = d.u[i * N + nb]; }
-      load_rot<SO3>(d, nb, rb, qnb);
-      Quat qNa, qNb;
-      qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
-      qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
-      const Quat qa = qmul(qna, qNa), qb = qmul(qnb, qNb);
-      ElemKin k;
-      elem_kinematics(Xa, ua, Xb, ub, qa, qb, k);
-      double strain[6], sm[6];
+      for (int i = 0; i < 7; i++) al[i] = d.al[i * E + e];
+      double dfl;
+      const int it = return_map<MAT == 3 ? 0 : MAT>(pp, C, pw, Hk, strain, ep, al, sm, &dfl,
+                                                     sp.inelastic);
+      if (it < 0) *d.err = -3;
+      if (MODE == MODE_STEP) {
+        if (sp.track_diss) {
+          // applyDeform dissipation (ElastoPlasticRodMaterial.cpp:285-296)
+          if (fabs(dfl) > 2.220446049250313e-16) {
+            double t = 0.0;
 #pragma unroll
-      for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - d.eps0[i * E + e];
-      if (!PLASTIC) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
-        if (MODE == MODE_FORCE) {
-#pragma unroll
-          for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
-        }
-      } else {
-        double ep[6], al[7], epo[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) { ep[i] = d.ep[i * E + e]; epo[i] = ep[i]; }
-#pragma unroll
-        for (int i = 0; i < 7; i++) al[i] = d.al[i * E + e];
-        double dfl;
-        const int it = return_map(pp, C, pw, Hk, strain, ep, al, sm, &dfl, sp.inelastic);
-        if (it < 0) *d.err = -3;
-        if (MODE == MODE_STEP) {
-          if (sp.track_diss) {
-            // applyDeform dissipation (ElastoPlasticRodMaterial.cpp:285-296)
-            if (fabs(dfl) > 2.220446049250313e-16) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < 6; i++) {
-                const double so = C[i] * (d.eo[i * E + e] - epo[i]);
-                t += ((so + sm[i]) / 2.0) * (ep[i] - epo[i]);
-              }
-              diss += k.w * t;
+            for (int i = 0; i < 6; i++) {
+              const double so = C[i] * (d.eo[i * E + e] - epo[i]);
+              t += ((so + sm[i]) / 2.0) * (ep[i] - epo[i]);
             }
-#pragma unroll
-            for (int i = 0; i < 6; i++) d.eo[i * E + e] = strain[i];
+            diss = k.w * t;
           }
+#pragma unroll
+          for (int i = 0; i < 6; i++) d.eo[i * E + e] = strain[i];
+        }
+        if (it > 0) {
 #pragma unroll
           for (int i = 0; i < 6; i++) d.ep[i * E + e] = ep[i];
 #pragma unroll
           for (int i = 0; i < 7; i++) d.al[i * E + e] = al[i];
-        } else {
-#pragma unroll
-          for (int i = 0; i < 6; i++) { d.ep_cur[i * E + e] = ep[i]; d.ecur[i * E + e] = strain[i]; }
-#pragma unroll
-          for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
-          d.dflow[e] = dfl;
         }
-        d.active[e] = (uint8_t)(it > 0);
-      }
-      double fa[6], fb[6];
-      elem_forces(k, sm, fa, fb);
-      if (CONTACT) {
-        // rod-rod contact force on this element's nodes (RodContactModel.cpp:729)
-        const int32_t ci = d.cidx[e];
-        if (ci >= 0) {
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            fa[i] += d.fcs[i * E + ci];
-            fb[i] += d.fcs[(3 + i) * E + ci];
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 6; i++) facc[i] += fa[i];
-      const int32_t slot = d.cslot[c];
-      if (slot >= 0) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = facc[i];
-      } else if (MODE == MODE_FORCE) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
       } else {
-        node_update<SO3>(d, sp, na, facc, qna, ra);
+#pragma unroll
+        for (int i = 0; i < 6; i++) { d.ep_cur[i * E + e] = ep[i]; d.ecur[i * E + e] = strain[i]; }
+#pragma unroll
+        for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
+        d.dflow[e] = dfl;
       }
-#pragma unroll
-      for (int i = 0; i < 6; i++) facc[i] = fb[i];
-      na = nb;
-#pragma unroll
-      for (int i = 0; i < 3; i++) { Xa[i] = Xb[i]; ua[i] = ub[i]; }
-#pragma unroll
-      for (int i = 0; i < 4; i++) ra[i] = rb[i];
-      qna = qnb;
+      d.active[e] = (uint8_t)(it > 0);
     }
-    const int32_t slot = d.cslot[c0 + (e1 - e0)];
+    elem_forces(k, sm, fa, fb);
+    if (CONTACT) {
+      // rod-rod contact force on this element's nodes (RodContactModel.cpp:729)
+      const int32_t ci = d.cidx[e];
+      if (ci >= 0) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          fa[i] += d.fcs[i * E + ci];
+          fb[i] += d.fcs[(3 + i) * E + ci];
+        }
+      }
+    }
+  }
+  // node a of lane k receives fb of lane k-1 (same chain, same wavefront)
+  double f[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) f[i] = __shfl_up(fb[i], 1, 64);
+  if (e >= 0) {
+    if (fl & 1u) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) f[i] = fa[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; i++) f[i] += fa[i];
+    }
+    const int32_t slot = d.sa[e];
     if (slot >= 0) {
 #pragma unroll
-      for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = facc[i];
+      for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = f[i];
     } else if (MODE == MODE_FORCE) {
 #pragma unroll
-      for (int i = 0; i < 6; i++) d.fout[i * N + na] = facc[i];
+      for (int i = 0; i < 6; i++) d.fout[i * N + na] = f[i];
     } else {
-      node_update<SO3>(d, sp, na, facc, qna, ra);
+      node_update<SO3>(d, sp, na, f, qna, ra);
+    }
+    if (fl & 2u) { // last element of the chain: also node b
+      const int32_t sl = d.sb[e];
+      if (sl >= 0) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + sl] = fb[i];
+      } else if (MODE == MODE_FORCE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.fout[i * N + nb] = fb[i];
+      } else {
+        node_update<SO3>(d, sp, nb, fb, qnb, rb);
+      }
     }
   }
   if (PLASTIC && MODE == MODE_STEP) {
@@ -371,12 +378,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.E) return;
   const int64_t E = d.E;
-  // strut of element e
-  int32_t lo = 0, hi = d.R - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (d.eptr[mid] <= e) lo = mid; else hi = mid - 1;
-  }
+  const int32_t lo = d.es[e];
   const int64_t R = d.R;
   double C[6];
   for (int i = 0; i < 6; i++) C[i] = d.C[i * R + lo];
@@ -482,37 +484,41 @@ void launch_init_strain(const DevData &d, hipStream_t s) {
   hipLaunchKernelGGL(k_init_strain, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d);
 }
 
-template <int MODE, bool SO3, bool PLASTIC, bool CONTACT>
+template <int MODE, bool SO3, int MAT, bool CONTACT>
 static void ls(const DevData &d, const StepParams &sp, const PlasticParams &pp, hipStream_t s) {
-  hipLaunchKernelGGL((k_strut<MODE, SO3, PLASTIC, CONTACT>), dim3(nblk(d.R)), dim3(DL_BLOCK), 0,
-                     s, d, sp, pp);
+  hipLaunchKernelGGL((k_strut<MODE, SO3, MAT, CONTACT>), dim3(nblk(d.NL)), dim3(DL_BLOCK), 0, s,
+                     d, sp, pp);
 }
 
-void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
-                     bool so3, bool plastic, bool contact, hipStream_t s) {
-  if (d.R == 0) return;
-#define DL_LS(M, A, B, C)                                                  \
-  if (mode == M && so3 == A && plastic == B && contact == C) {            \
-    ls<M, A, B, C>(d, sp, pp, s);                                          \
-    return;                                                                \
+template <int MODE, bool SO3, int MAT>
+static void ls_c(const DevData &d, const StepParams &sp, const PlasticParams &pp, bool contact,
+                 hipStream_t s) {
+  if (contact) ls<MODE, SO3, MAT, true>(d, sp, pp, s);
+  else ls<MODE, SO3, MAT, false>(d, sp, pp, s);
+}
+
+template <int MODE, bool SO3>
+static void ls_m(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mat,
+                 bool contact, hipStream_t s) {
+  switch (mat) {
+  case 0: ls_c<MODE, SO3, 0>(d, sp, pp, contact, s); break;
+  case 1: ls_c<MODE, SO3, 1>(d, sp, pp, contact, s); break;
+  case 2: ls_c<MODE, SO3, 2>(d, sp, pp, contact, s); break;
+  default: ls_c<MODE, SO3, 3>(d, sp, pp, contact, s); break;
   }
-  DL_LS(MODE_STEP, true, false, false)
-  DL_LS(MODE_STEP, true, false, true)
-  DL_LS(MODE_STEP, true, true, false)
-  DL_LS(MODE_STEP, true, true, true)
-  DL_LS(MODE_STEP, false, false, false)
-  DL_LS(MODE_STEP, false, false, true)
-  DL_LS(MODE_STEP, false, true, false)
-  DL_LS(MODE_STEP, false, true, true)
-  DL_LS(MODE_FORCE, true, false, false)
-  DL_LS(MODE_FORCE, true, false, true)
-  DL_LS(MODE_FORCE, true, true, false)
-  DL_LS(MODE_FORCE, true, true, true)
-  DL_LS(MODE_FORCE, false, false, false)
-  DL_LS(MODE_FORCE, false, false, true)
-  DL_LS(MODE_FORCE, false, true, false)
-  DL_LS(MODE_FORCE, false, true, true)
-#undef DL_LS
+}
+
+// mat: 0 elastic, 1/2/3 elastoplastic with yield exponents all 1 / all 2 / general
+void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
+                     bool so3, int mat, bool contact, hipStream_t s) {
+  if (d.NL == 0) return;
+  if (mode == MODE_STEP) {
+    if (so3) ls_m<MODE_STEP, true>(d, sp, pp, mat, contact, s);
+    else ls_m<MODE_STEP, false>(d, sp, pp, mat, contact, s);
+  } else {
+    if (so3) ls_m<MODE_FORCE, true>(d, sp, pp, mat, contact, s);
+    else ls_m<MODE_FORCE, false>(d, sp, pp, mat, contact, s);
+  }
 }
 
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s) {
