@@ -84,8 +84,28 @@ def cpu_baseline(p, budget_s, steps_gpu):
                       f"{dt:.1f} s"}
 
 
+def calibrate_copy(nbytes=1 << 30):
+    """A device-to-device copy of known size (the runtime's copy kernel), used
+    to scale the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # DeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -95,6 +115,9 @@ def main():
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing pass")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)

@@ -288,12 +288,20 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   for (int64_t n = 0; n < N; n++)
     if (occ[n] > 1) { jidx[n] = (int32_t)jnode.size(); jnode.push_back((int32_t)n); }
   const int32_t NJ = (int32_t)jnode.size();
+  // renumber the slots so that the contributions to one shared node are
+  // contiguous (in chain order): the joint kernel then reads
+  // sbuf[jptr[j] .. jptr[j+1]) without indirection
   std::vector<int32_t> jptr(NJ + 1, 0), jslot(nslot);
   for (auto &sn : slot_node) jptr[jidx[sn.second] + 1]++;
   for (int32_t j = 0; j < NJ; j++) jptr[j + 1] += jptr[j];
   {
-    std::vector<int32_t> fill(jptr.begin(), jptr.end() - 1);
-    for (auto &sn : slot_node) jslot[fill[jidx[sn.second]]++] = sn.first;
+    std::vector<int32_t> fill(jptr.begin(), jptr.end() - 1), remap(nslot);
+    for (auto &sn : slot_node) remap[sn.first] = fill[jidx[sn.second]]++;
+    for (int64_t e = 0; e < E; e++) {
+      if (sa[e] >= 0) sa[e] = remap[sa[e]];
+      if (sb[e] >= 0) sb[e] = remap[sb[e]];
+    }
+    for (int32_t t = 0; t < nslot; t++) jslot[t] = t;
   }
   // wavefront packing: a chain never straddles two 64-lane wavefronts
   std::vector<int32_t> lane_e;

@@ -35,6 +35,8 @@ struct ContactDev {
   int32_t *cstart;      // [ncell+1]
   int32_t *bsum;        // scan block sums
   void *rec;            // [E] cell-ordered 64-byte records (box, element, strut, cell)
+  int32_t *tmp;         // [E] element ids in cell order, unsorted within a cell
+  void *frec;           // [E] 32-byte float prefilter records (outward-rounded box, strut, element)
   // blacklist (sorted keys lo<<32|hi)
   const unsigned long long *black;
   int64_t nblack;

@@ -6,7 +6,8 @@
 //   kc_cells     integer cell of each AABB's min corner (clamped to the grid),
 //                per-cell counts with integer atomics
 //   scan         exclusive scan of the counts (3 launches)
-//   kc_scatter   elements into cell order; kc_sortcells sorts each cell by id
+//   kc_scatter   element ids into their cells; kc_place writes each element's
+//                record at its stable (id-ordered) position in the cell
 //   kc_narrow    per element X: every Y in the 27 neighbour cells that the
 //                reference would pair with X (different strut, no node of X in
 //                Y's strut and vice versa, AABB overlap) -> closest points ->
@@ -161,10 +162,24 @@ struct __align__(64) CRec {
   int32_t e, s, cell, pad;
 };
 
+// element ids into their cells (order within a cell set by the atomics)
 __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (e >= c.E) return;
+  c.tmp[c.cstart[c.cell[e]] + c.rank[e]] = (int32_t)e;
+}
+
+// stable placement: an element's slot in its cell is the number of elements
+// of the same cell with a smaller id, so every cell lists its elements in id
+// order whatever order the atomics gave; the 64-byte record goes there
+__global__ __launch_bounds__(CB) void kc_place(ContactDev c) {
+  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (e >= c.E) return;
   const int64_t E = c.E;
+  const int32_t cell = c.cell[e];
+  const int32_t b = c.cstart[cell], n = c.cstart[cell + 1] - b;
+  int32_t pos = b;
+  for (int32_t j = 0; j < n; j++) pos += (c.tmp[b + j] < (int32_t)e) ? 1 : 0;
   CRec r;
   for (int i = 0; i < 3; i++) {
     r.lo[i] = c.box[i * E + e];
@@ -172,27 +187,17 @@ __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
   }
   r.e = (int32_t)e;
   r.s = c.es[e];
-  r.cell = c.cell[e];
+  r.cell = cell;
   r.pad = 0;
-  ((CRec *)c.rec)[c.cstart[c.cell[e]] + c.rank[e]] = r;
-}
-
-// each cell's records in element order (deterministic traversal)
-__global__ __launch_bounds__(CB) void kc_sortcells(ContactDev c) {
-  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (k >= c.ncell) return;
-  const int32_t b = c.cstart[k], e = c.cstart[k + 1];
-  if (e - b < 2) return;
-  CRec *r = (CRec *)c.rec;
-  for (int32_t i = b + 1; i < e; i++) {
-    const CRec v = r[i];
-    int32_t j = i - 1;
-    while (j >= b && r[j].e > v.e) {
-      r[j + 1] = r[j];
-      j--;
-    }
-    r[j + 1] = v;
-  }
+  ((CRec *)c.rec)[pos] = r;
+  // compact prefilter record: bounds rounded outwards to float, strut id
+  float4 lo, hi;
+  lo.x = __double2float_rd(r.lo[0]); lo.y = __double2float_rd(r.lo[1]);
+  lo.z = __double2float_rd(r.lo[2]); lo.w = __int_as_float(r.s);
+  hi.x = __double2float_ru(r.hi[0]); hi.y = __double2float_ru(r.hi[1]);
+  hi.z = __double2float_ru(r.hi[2]); hi.w = __int_as_float(r.e);
+  ((float4 *)c.frec)[2 * pos] = lo;
+  ((float4 *)c.frec)[2 * pos + 1] = hi;
 }
 
 __device__ __forceinline__ bool node_in_strut(const ContactDev &c, int32_t n, int32_t s) {
@@ -245,6 +250,7 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
     const int32_t cx = x.cell % c.dim[0];
     const int32_t cy = (x.cell / c.dim[0]) % c.dim[1];
     const int32_t cz = x.cell / (c.dim[0] * c.dim[1]);
+    const float4 xfl = ((const float4 *)c.frec)[2 * k], xfh = ((const float4 *)c.frec)[2 * k + 1];
     double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
     bool any = false;
     int32_t nlist = 0;
@@ -260,9 +266,30 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
         const int32_t x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
         const int32_t row = (z * c.dim[1] + y) * c.dim[0];
         const int32_t j1 = c.cstart[row + x1 + 1];
-        for (int32_t j = c.cstart[row + x0]; j < j1; j++) {
+        const float4 *fr = (const float4 *)c.frec;
+        for (int32_t j0 = c.cstart[row + x0]; j0 < j1; j0 += 4) {
+          // four independent prefilter loads in flight
+          float4 flo[4], fhi[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int32_t jj = (j0 + q < j1) ? j0 + q : j1 - 1;
+            flo[q] = fr[2 * jj];
+            fhi[q] = fr[2 * jj + 1];
+          }
+          unsigned hit = 0u;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const bool ok = (j0 + q < j1) && (__float_as_int(flo[q].w) != sX) &&
+                            (xfh.x >= flo[q].x) && (fhi[q].x >= xfl.x) &&
+                            (xfh.y >= flo[q].y) && (fhi[q].y >= xfl.y) &&
+                            (xfh.z >= flo[q].z) && (fhi[q].z >= xfl.z);
+            hit |= ok ? (1u << q) : 0u;
+          }
+          while (hit) {
+          const int q = __builtin_ctz(hit);
+          hit &= hit - 1u;
+          const int32_t j = j0 + q;
           const CRec yr = rec[j];
-          if (yr.s == sX) continue;
           bool ov = true;
           for (int i = 0; i < 3; i++) ov = ov && (x.hi[i] >= yr.lo[i]) && (yr.hi[i] >= x.lo[i]);
           if (!ov) continue;
@@ -304,6 +331,7 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
               if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
               nlist++;
             }
+          }
           }
         }
       }
@@ -459,6 +487,8 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = calloc_dev(cs, &c.cstart, ncell + 1))) return rc;
   if ((rc = calloc_dev(cs, &c.bsum, cs.nscan_blk + 1))) return rc;
   if ((rc = calloc_dev(cs, (double **)&c.rec, 8 * (size_t)E))) return rc;
+  if ((rc = calloc_dev(cs, &c.tmp, E))) return rc;
+  if ((rc = calloc_dev(cs, (double **)&c.frec, 4 * (size_t)E))) return rc;
   if ((rc = calloc_dev(cs, &c.cidx, E))) return rc;
   if ((rc = calloc_dev(cs, &c.fcs, 6 * E))) return rc;
   if ((rc = calloc_dev(cs, &c.slot_ctr, 1))) return rc;
@@ -491,7 +521,7 @@ static void broadphase(ContactState &cs, const DevData &d, hipStream_t s) {
   hipLaunchKernelGGL(kc_cells, dim3(cs.nblk_e), dim3(CB), 0, s, c);
   scan_i32(cs, c.ccount, c.ncell, c.cstart, s);
   hipLaunchKernelGGL(kc_scatter, dim3(cs.nblk_e), dim3(CB), 0, s, c);
-  hipLaunchKernelGGL(kc_sortcells, dim3(nb(c.ncell)), dim3(CB), 0, s, c);
+  hipLaunchKernelGGL(kc_place, dim3(cs.nblk_e), dim3(CB), 0, s, c);
 }
 
 static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,

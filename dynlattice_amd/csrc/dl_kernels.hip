@@ -303,9 +303,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
   const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
   double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int32_t t = t0; t < t1; t++) {
-    const int64_t sl = d.jslot[t];
 #pragma unroll
-    for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + sl];
+    for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + t];
   }
   if (MODE == MODE_FORCE) {
 #pragma unroll
