@@ -102,8 +102,69 @@ def calibrate_copy(nbytes=1 << 30):
     print(json.dumps({"calibration_copy_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def sweep(args):
+    """Batched design sweep (256 seeded elastoplastic BCC lattices, per-instance
+    dt) in one context replayed from a captured graph, against the same 256
+    instances stepped one after another in their own contexts."""
+    from dynlattice_amd import batch as BT
+    from dynlattice_amd import lattice as L
+    from dynlattice_amd.gpu import GpuModel
+    n_inst = args.sweep_instances
+    steps = args.steps
+    probs = [L.c4_instance(i) for i in range(n_inst)]
+    bt = BT.concat(probs)
+    print(f"sweep: {n_inst} instances, {bt.problem.n_elems} elements, {bt.problem.n_nodes} nodes",
+          file=sys.stderr, flush=True)
+    g = BT.make_model(bt)
+    best, tune = BT.autotune_graph(g, candidates=(0, 8, 32), steps=16)
+    g.step(args.warmup, out=False)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps, out=False)
+    g.sync()
+    t_batch = time.perf_counter() - t0
+    g.step(0)
+    g.close()
+    # the same instances one by one: their own contexts, direct launches, all
+    # on ONE stream so that no two instances overlap on the device
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(stream), 1) == 0
+    singles = []
+    for p in probs:
+        m = GpuModel(p, stream=stream)
+        m.set_reaction_group(p.groups["bottom"], 2)
+        m.step(args.warmup, p.dt, out=False)
+        singles.append(m)
+    for m in singles:
+        m.sync()
+    t0 = time.perf_counter()
+    for m, p in zip(singles, probs):
+        m.step(steps, p.dt, out=False)
+    for m in singles:
+        m.sync()
+    t_seq = time.perf_counter() - t0
+    for m in singles:
+        m.close()
+    hip.hipStreamDestroy(stream)
+    E = bt.problem.n_elems
+    res = {"metric": "batched design-sweep element-GP updates/s (FP64)",
+           "value": E * steps / t_batch, "unit": "element-GP updates/s", "n_gpus": 1,
+           "steps": steps, "warmup": args.warmup, "higher_is_better": True,
+           "config": {"workload": f"sweep-{n_inst}-bcc-elastoplastic", "instances": n_inst,
+                      "elements": E, "nodes": bt.problem.n_nodes,
+                      "dt_range": [float(bt.dts.min()), float(bt.dts.max())]},
+           "graph_steps": best, "autotune_s_per_step": tune,
+           "sequential_value": E * steps / t_seq, "batched_over_sequential": t_seq / t_batch}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--sweep", action="store_true",
+                    help="batched design sweep vs sequential single-instance runs")
+    ap.add_argument("--sweep-instances", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +178,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.sweep:
+        sweep(args)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
@@ -132,6 +196,8 @@ def main():
     g = GpuModel(p, device=0, flags=flags)
     if "bottom" in p.groups:
         g.set_reaction_group(p.groups["bottom"], 2)
+    from dynlattice_amd.batch import autotune_graph
+    graph_steps, tune = autotune_graph(g, candidates=(0, 10), steps=10)
     g.step(args.warmup, out=False)
     g.sync()
     t0 = time.perf_counter()
@@ -180,6 +246,7 @@ def main():
                    "material": "elastoplastic" if p.plastic is not None else "elastic",
                    "contact": p.contact is not None, "so3": bool(p.so3)},
         "roofline": roof,
+        "graph_steps": graph_steps,
         "phase_ms": {k: round(v, 5) for k, v in phases.items()},
     }
     if not args.no_cpu:

@@ -40,7 +40,7 @@ extern "C" const char *dl_last_error(void) { return g_err.c_str(); }
 struct BcHost {
   std::vector<uint8_t> cmask;
   std::vector<double> cval, fext;
-  std::vector<int32_t> react;
+  std::vector<int32_t> react;     // reaction nodes, concatenated by group
 };
 
 struct dl_ctx {
@@ -62,8 +62,17 @@ struct dl_ctx {
   // host copies needed by actions
   std::vector<double> w_e;        // Gauss weight (L0) per element
   std::vector<int32_t> elem_perm; // identity (elements are stored in input order)
-  // reaction group
+  // reaction groups: slots [rgptr[g], rgptr[g+1]) of rbuf belong to group g
   int64_t n_react = 0;
+  int32_t n_groups = 0;
+  int32_t *rgptr = nullptr;
+  int32_t *step_ctr = nullptr;   // device step counter (history row)
+  double *ndt = nullptr;          // per-node dt
+  // captured step graphs
+  int32_t graph_steps = 0;
+  hipGraphExec_t gexec = nullptr;
+  int64_t gen = 0, graph_gen = -1;
+  double graph_dt = 0.0;
   // boundary records (host) for re-upload
   std::vector<int32_t> bc_host;
   // device buffers for history
@@ -563,6 +572,7 @@ extern "C" int dl_destroy(dl_ctx *c) {
   if (!c) return DL_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
   for (void *p : c->allocs) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -660,6 +670,7 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
     mask[b] = m;
   }
   for (size_t k = 0; k < react_nodes.size(); k++) rs[bc[react_nodes[k]]] = (int32_t)k;
+  c->gen++;
   hipSetDevice(c->device);
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy((void *)c->d.bc, bc.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -694,9 +705,15 @@ extern "C" int dl_set_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, c
                     h.fext.empty() ? nullptr : h.fext.data(), h.react);
 }
 
-extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n, int32_t dof) {
+extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int32_t *group_ptr,
+                                      int32_t n_groups, int32_t dof) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (dof < 0 || dof > 5) return fail(DL_ERR_INPUT, "dof must be in [0, 5]");
+  if (n_groups < 0 || (n_groups > 0 && (!group_ptr || group_ptr[0] != 0)))
+    return fail(DL_ERR_INPUT, "group_ptr must start at 0");
+  for (int32_t g = 0; g < n_groups; g++)
+    if (group_ptr[g + 1] < group_ptr[g]) return fail(DL_ERR_INPUT, "group_ptr must be non-decreasing");
+  const int64_t n = n_groups ? group_ptr[n_groups] : 0;
   BcHost &h = bch(c);
   h.react.assign(nodes, nodes + n);
   for (int32_t x : h.react)
@@ -704,16 +721,56 @@ extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n,
   std::vector<int32_t> srt = h.react;
   std::sort(srt.begin(), srt.end());
   if (std::adjacent_find(srt.begin(), srt.end()) != srt.end())
-    return fail(DL_ERR_INPUT, "duplicate reaction node");
+    return fail(DL_ERR_INPUT, "a node may belong to one reaction group only");
   c->d.react_dof = dof;
   c->n_react = n;
+  c->n_groups = n_groups;
   double *rb;
   int rc = dalloc(c, &rb, (size_t)std::max<int64_t>(n, 1));
   if (rc) return rc;
   c->d.rbuf = rb;
   HIPCHK(hipMemset(rb, 0, sizeof(double) * std::max<int64_t>(n, 1)));
+  std::vector<int32_t> gp(group_ptr, group_ptr + n_groups + 1);
+  if (n_groups == 0) gp.assign(1, 0);
+  if ((rc = upload(c, &c->rgptr, gp))) return rc;
+  c->hist_cap = 0; // history rows are n_groups wide: reallocate on the next step call
+  c->gen++;
   return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
                     h.fext.empty() ? nullptr : h.fext.data(), h.react);
+}
+
+extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n, int32_t dof) {
+  if (n < 0 || n >= INT32_MAX) return fail(DL_ERR_INPUT, "bad reaction group size");
+  const int32_t gp[2] = {0, (int32_t)n};
+  return dl_set_reaction_groups(c, nodes, gp, 1, dof);
+}
+
+extern "C" int dl_set_node_dt(dl_ctx *c, const double *dt, int64_t n_nodes) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!dt) {
+    c->d.ndt = nullptr;
+    c->gen++;
+    return DL_OK;
+  }
+  if (n_nodes != c->d.N) return fail(DL_ERR_INPUT, "n_nodes mismatch");
+  for (int64_t i = 0; i < n_nodes; i++)
+    if (!(dt[i] > 0.0)) return fail(DL_ERR_INPUT, "node time steps must be > 0");
+  if (!c->ndt) {
+    int rc = dalloc(c, &c->ndt, n_nodes);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(c->ndt, dt, n_nodes * sizeof(double), hipMemcpyHostToDevice));
+  c->d.ndt = c->ndt;
+  c->gen++;
+  return DL_OK;
+}
+
+extern "C" int dl_set_graph(dl_ctx *c, int32_t steps_per_graph) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (steps_per_graph < 0) return fail(DL_ERR_INPUT, "steps_per_graph must be >= 0");
+  c->graph_steps = steps_per_graph;
+  c->gen++;
+  return DL_OK;
 }
 
 // ------------------------------------------------------------------
@@ -836,47 +893,54 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   return DL_OK;
 }
 
+// one explicit step: kernels only (capturable into a hipGraph)
+static int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
+  if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  if (c->contact) {
+    int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc, c->hist_ncand, c->step_ctr);
+    if (rc) return rc;
+  }
+  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+  launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
+  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+  launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
+  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
+                  c->hist_react, c->hist_diss, c->step_ctr, c->stream);
+  return DL_OK;
+}
+
 extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_out *out) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (!c->initialized) return fail(DL_ERR_STATE, "explicit step before INIT");
   if (nsteps < 0 || !(dt > 0.0)) return fail(DL_ERR_INPUT, "nsteps >= 0 and dt > 0 required");
   hipSetDevice(c->device);
-  if (nsteps > c->hist_cap) {
+  const int64_t ng = std::max<int32_t>(c->n_groups, 1);
+  if (nsteps > c->hist_cap || !c->step_ctr) {
     int rc;
-    const int64_t cap = std::max<int64_t>(nsteps, 16);
-    if ((rc = dalloc(c, &c->hist_react, cap))) return rc;
+    const int64_t cap = std::max<int64_t>(nsteps, std::max<int64_t>(c->hist_cap, 16));
+    if ((rc = dalloc(c, &c->hist_react, cap * ng))) return rc;
     if ((rc = dalloc(c, &c->hist_diss, cap))) return rc;
     if ((rc = dalloc(c, &c->hist_nc, cap))) return rc;
     if ((rc = dalloc(c, &c->hist_ncand, cap))) return rc;
+    if (!c->step_ctr && (rc = dalloc(c, &c->step_ctr, 1))) return rc;
     c->hist_cap = cap;
+    c->gen++;
   }
-  HIPCHK(hipMemsetAsync(c->hist_react, 0, nsteps * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(c->hist_react, 0, nsteps * ng * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(c->hist_diss, 0, nsteps * sizeof(double), c->stream));
   HIPCHK(hipMemsetAsync(c->hist_nc, 0, nsteps * sizeof(int64_t), c->stream));
   HIPCHK(hipMemsetAsync(c->hist_ncand, 0, nsteps * sizeof(int64_t), c->stream));
+  launch_set_i32(c->step_ctr, 0, c->stream);
   StepParams sp{dt, 1, c->track_diss ? 1 : 0};
-  std::vector<hipEvent_t> ev;
   if (c->timing) {
-    ev.resize(4 * (size_t)nsteps + 1);
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 1);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-  }
-  for (int64_t st = 0; st < nsteps; st++) {
-    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st], c->stream));
-    if (c->contact) {
-      int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc + st, c->hist_ncand + st);
+    for (int64_t st = 0; st < nsteps; st++) {
+      int rc = launch_one_step(c, sp, &ev[4 * st]);
       if (rc) return rc;
     }
-    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 1], c->stream));
-    launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
-    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 2], c->stream));
-    launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
-    if (c->timing) HIPCHK(hipEventRecord(ev[4 * st + 3], c->stream));
-    if (c->n_react > 0) launch_reduce_sum(c->d.rbuf, c->n_react, c->hist_react + st, nullptr, c->stream);
-    if (c->track_diss) launch_reduce_sum(c->d.dbuf, c->strut_blocks, c->hist_diss + st, c->diss_total, c->stream);
-  }
-  if (c->timing) HIPCHK(hipEventRecord(ev[4 * nsteps], c->stream));
-  HIPCHK(hipGetLastError());
-  c->hist_n = nsteps;
-  if (c->timing) {
+    HIPCHK(hipEventRecord(ev[4 * nsteps], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->t_step = c->t_contact = c->t_strut = c->t_joint = 0.0;
     for (int64_t st = 0; st < nsteps; st++) {
@@ -884,7 +948,7 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
       HIPCHK(hipEventElapsedTime(&a, ev[4 * st], ev[4 * st + 1]));
       HIPCHK(hipEventElapsedTime(&b, ev[4 * st + 1], ev[4 * st + 2]));
       HIPCHK(hipEventElapsedTime(&cc, ev[4 * st + 2], ev[4 * st + 3]));
-      HIPCHK(hipEventElapsedTime(&dd, ev[4 * st], ev[4 * st + 4 > 4 * nsteps ? 4 * nsteps : 4 * st + 4]));
+      HIPCHK(hipEventElapsedTime(&dd, ev[4 * st], ev[4 * st + 4]));
       c->t_contact += a;
       c->t_strut += b;
       c->t_joint += cc;
@@ -892,7 +956,37 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     }
     c->t_count = nsteps;
     for (auto &e : ev) hipEventDestroy(e);
+  } else {
+    int64_t done = 0;
+    const int64_t K = c->graph_steps;
+    if (K > 0 && nsteps >= K) {
+      if (!c->gexec || c->graph_gen != c->gen || c->graph_dt != dt) {
+        if (c->gexec) {
+          hipGraphExecDestroy(c->gexec);
+          c->gexec = nullptr;
+        }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = DL_OK;
+        for (int64_t k = 0; k < K && !rc; k++) rc = launch_one_step(c, sp, nullptr);
+        hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+        c->graph_gen = c->gen;
+        c->graph_dt = dt;
+      }
+      for (; done + K <= nsteps; done += K) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < nsteps; done++) {
+      int rc = launch_one_step(c, sp, nullptr);
+      if (rc) return rc;
+    }
   }
+  HIPCHK(hipGetLastError());
+  c->hist_n = nsteps;
   if (out) {
     int rc = check_err(c);
     if (rc) return rc;
@@ -902,12 +996,23 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     out->n_contacts = 0;
     out->n_candidates = 0;
     if (nsteps > 0) {
-      HIPCHK(hipMemcpy(&out->reaction, c->hist_react + nsteps - 1, sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&out->reaction, c->hist_react + (nsteps - 1) * ng, sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(&out->n_contacts, c->hist_nc + nsteps - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(&out->n_candidates, c->hist_ncand + nsteps - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
     }
     HIPCHK(hipMemcpy(&out->e_diss, c->diss_total, sizeof(double), hipMemcpyDeviceToHost));
   }
+  return DL_OK;
+}
+
+extern "C" int dl_get_group_history(dl_ctx *c, double *reaction, int64_t cap_steps, int64_t *n) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t ng = std::max<int32_t>(c->n_groups, 1);
+  const int64_t k = std::min(cap_steps, c->hist_n);
+  if (n) *n = c->hist_n;
+  if (k > 0 && reaction)
+    HIPCHK(hipMemcpy(reaction, c->hist_react, k * ng * sizeof(double), hipMemcpyDeviceToHost));
   return DL_OK;
 }
 
@@ -918,7 +1023,12 @@ extern "C" int dl_get_history(dl_ctx *c, double *reaction, double *e_diss, int64
   const int64_t k = std::min(cap, c->hist_n);
   if (n) *n = c->hist_n;
   if (k <= 0) return DL_OK;
-  if (reaction) HIPCHK(hipMemcpy(reaction, c->hist_react, k * sizeof(double), hipMemcpyDeviceToHost));
+  if (reaction) {
+    const int64_t ng = std::max<int32_t>(c->n_groups, 1);
+    std::vector<double> h(k * ng);
+    HIPCHK(hipMemcpy(h.data(), c->hist_react, k * ng * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < k; i++) reaction[i] = h[i * ng];
+  }
   if (e_diss) {
     if (c->track_diss) HIPCHK(hipMemcpy(e_diss, c->hist_diss, k * sizeof(double), hipMemcpyDeviceToHost));
     else std::fill(e_diss, e_diss + k, 0.0);

@@ -67,9 +67,10 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
 int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s);
 // evaluation at the current state for the action path (also lists pairs)
 int contact_force(ContactState &cs, const DevData &d, hipStream_t s);
-// evaluation inside the explicit step: per-element forces + counts
+// evaluation inside the explicit step: per-element forces + counts added to
+// n_active[*step], n_cand[*step] (kernels only: safe inside a captured graph)
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand);
+                 int64_t *n_cand, const int32_t *step);
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,
                        int64_t cap, int64_t *n);
 int dl_set_error(int code, const char *msg);

@@ -237,7 +237,8 @@ __device__ __forceinline__ void elem_pos(const DevData &d, int32_t n0, int32_t n
 template <int MODE>
 __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kind,
                                                 unsigned long long *cnt_act,
-                                                unsigned long long *cnt_cand) {
+                                                unsigned long long *cnt_cand,
+                                                const int32_t *step) {
   const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
   unsigned long long nact = 0, ncand = 0;
   const CRec *rec = (const CRec *)c.rec;
@@ -363,9 +364,11 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
       }
       __syncthreads();
     }
-    if (threadIdx.x == 0) {
-      if (cnt_act && ra[0]) atomicAdd(cnt_act, ra[0]);
-      if (cnt_cand && rc[0]) atomicAdd(cnt_cand, rc[0]);
+    if (threadIdx.x == 0 && cnt_act) {
+      // history row of the current step (device counter: graph-replay safe)
+      const int64_t st = *step;
+      if (ra[0]) atomicAdd(cnt_act + st, ra[0]);
+      if (rc[0]) atomicAdd(cnt_cand + st, rc[0]);
     }
   }
 }
@@ -527,7 +530,8 @@ static void broadphase(ContactState &cs, const DevData &d, hipStream_t s) {
 static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,
                       std::vector<int32_t> &A, std::vector<int32_t> &B) {
   ContactDev &c = cs.c;
-  hipLaunchKernelGGL(kc_narrow<1>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr);
+  hipLaunchKernelGGL(kc_narrow<1>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr,
+                     nullptr);
   scan_i32(cs, c.lcount, c.E, c.loff, s);
   int32_t total = 0;
   CHK(hipMemcpyAsync(&total, c.loff + c.E, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -542,7 +546,8 @@ static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kin
   A.assign(total, 0);
   B.assign(total, 0);
   if (total > 0) {
-    hipLaunchKernelGGL(kc_narrow<2>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr);
+    hipLaunchKernelGGL(kc_narrow<2>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, kind, nullptr, nullptr,
+                       nullptr);
     CHK(hipGetLastError());
     CHK(hipMemcpyAsync(A.data(), c.la, sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
     CHK(hipMemcpyAsync(B.data(), c.lb, sizeof(int32_t) * total, hipMemcpyDeviceToHost, s));
@@ -586,7 +591,7 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   broadphase(cs, d, s);
   hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
-                     nullptr, nullptr);
+                     nullptr, nullptr, nullptr);
   CHK(hipGetLastError());
   int rc = list_pairs(cs, d, s, LIST_ACTIVE, cs.last_a, cs.last_b);
   if (rc) return rc;
@@ -594,11 +599,11 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
 }
 
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand) {
+                 int64_t *n_cand, const int32_t *step) {
   ContactDev &c = cs.c;
   broadphase(cs, d, s);
   hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
-                     (unsigned long long *)n_active, (unsigned long long *)n_cand);
+                     (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
   CHK(hipGetLastError());
   return DL_OK;
 }

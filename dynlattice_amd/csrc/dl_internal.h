@@ -21,6 +21,7 @@ struct DevData {
   double *rot;       // so3: quaternion 4 (w,x,y,z); additive: rotation vector 3
   double *v;         // 6
   const double *mt;  // translational lumped mass
+  const double *ndt; // per-node time step (independent batched instances) or null
   const double *J0;  // 6: rotational inertia at theta = 0 (xx yy zz xy xz yz)
   const double *gm;  // gyro multiplicity (number of struts holding the node)
   const int32_t *bc; // -1 or record index
@@ -87,7 +88,10 @@ void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams
                      bool so3, int mat, bool contact, hipStream_t s);
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s);
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
-void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s);
+void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
+                     int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
+                     int32_t *step, hipStream_t s);
+void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,

@@ -90,7 +90,7 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
 #pragma unroll
     for (int c = 0; c < 3; c++) a[3 + c] = (mask & (8u << c)) ? ac[3 + c] : (1.0 / J[4 * c]) * fr[3 + c];
   }
-  const double dt = sp.dt;
+  const double dt = d.ndt ? d.ndt[n] : sp.dt;
 #pragma unroll
   for (int c = 0; c < 6; c++) {
     v[c] = v[c] + dt * a[c];
@@ -345,11 +345,24 @@ __global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) 
 }
 
 // ------------------------------------------------------------------
-// deterministic single-block sum
+// end of step: per-group reaction sums and the dissipation increment, in a
+// fixed tree order (deterministic), written to history row *step
 // ------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_reduce_sum(const double *in, int64_t n, double *out,
-                                                     double *accum) {
+__global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int32_t *rgptr,
+                                                   int32_t ngroups, const double *dbuf, int64_t ndb,
+                                                   double *diss_total, double *hist_react,
+                                                   double *hist_diss, const int32_t *step) {
   __shared__ double red[1024];
+  const int g = blockIdx.x;
+  const double *in;
+  int64_t n;
+  if (g < ngroups) {
+    in = rbuf + rgptr[g];
+    n = rgptr[g + 1] - rgptr[g];
+  } else {
+    in = dbuf;
+    n = ndb;
+  }
   double s = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) s += in[i];
   red[threadIdx.x] = s;
@@ -359,13 +372,23 @@ __global__ __launch_bounds__(1024) void k_reduce_sum(const double *in, int64_t n
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double t = red[0];
-    if (accum) {
-      t = *accum + t;
-      *accum = t;
+    const int64_t st = *step;
+    if (g < ngroups) {
+      hist_react[st * ngroups + g] = red[0];
+    } else {
+      const double t = *diss_total + red[0];
+      *diss_total = t;
+      hist_diss[st] = t;
     }
-    *out = t;
   }
+}
+
+__global__ void k_step_inc(int32_t *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+__global__ void k_set_i32(int32_t *p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 // ------------------------------------------------------------------
@@ -538,8 +561,19 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
   else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, 1);
 }
 
-void launch_reduce_sum(const double *in, int64_t n, double *out, double *accum, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, s, in, n, out, accum);
+void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
+                     int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
+                     int32_t *step, hipStream_t s) {
+  const int nb = ngroups + (diss ? 1 : 0);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
+                       ngroups, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
+                       (const int32_t *)step);
+  hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step);
+}
+
+void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, p, v);
 }
 
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {

@@ -32,6 +32,7 @@ EXPORTS = (
     "dl_set_bcs", "dl_set_reaction_group", "dl_explicit_step", "dl_get_history",
     "dl_contact_pairs", "dl_contact_candidates", "dl_plastic_active", "dl_plastic_state",
     "dl_phase_time", "dl_set_timing", "dl_sync", "dl_last_error",
+    "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
 )
 
 dp = C.POINTER(C.c_double)
@@ -116,6 +117,10 @@ def load():
     L.dl_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.dl_sync.argtypes = [C.c_void_p]
     L.dl_last_error.restype = C.c_char_p
+    L.dl_set_reaction_groups.argtypes = [C.c_void_p, ip32, ip32, C.c_int32, C.c_int32]
+    L.dl_set_node_dt.argtypes = [C.c_void_p, dp, C.c_int64]
+    L.dl_set_graph.argtypes = [C.c_void_p, C.c_int32]
+    L.dl_get_group_history.argtypes = [C.c_void_p, dp, C.c_int64, ip64]
     _lib = L
     return L
 
@@ -256,6 +261,30 @@ class GpuModel:
     def set_reaction_group(self, nodes, dof=2):
         g = _c(nodes, np.int32)
         self._check(self.L.dl_set_reaction_group(self.h, _p(g, ip32), g.shape[0], int(dof)))
+
+    def set_reaction_groups(self, nodes, group_ptr, dof=2):
+        n = _c(nodes, np.int32)
+        g = _c(group_ptr, np.int32)
+        self._ngroups = g.shape[0] - 1
+        self._check(self.L.dl_set_reaction_groups(self.h, _p(n, ip32), _p(g, ip32),
+                                                  self._ngroups, int(dof)))
+
+    def set_node_dt(self, dt):
+        if dt is None:
+            self._check(self.L.dl_set_node_dt(self.h, None, self.p.n_nodes))
+        else:
+            a = _c(dt)
+            self._check(self.L.dl_set_node_dt(self.h, _p(a), a.shape[0]))
+
+    def set_graph(self, steps_per_graph):
+        self._check(self.L.dl_set_graph(self.h, int(steps_per_graph)))
+
+    def group_history(self, nsteps):
+        ng = max(getattr(self, "_ngroups", 1), 1)
+        r = np.zeros(nsteps * ng)
+        n = C.c_int64()
+        self._check(self.L.dl_get_group_history(self.h, _p(r), nsteps, C.byref(n)))
+        return r[: n.value * ng].reshape(-1, ng)
 
     # --- stepping ------------------------------------------------------
     def step(self, nsteps, dt=None, out=True):

@@ -174,6 +174,23 @@ int dl_set_bcs(dl_ctx *ctx, const uint8_t *cmask, const double *cval, const doub
  * (GroupOutputModule.cpp:100-116,162), recorded once per step. */
 int dl_set_reaction_group(dl_ctx *ctx, const int32_t *nodes, int64_t n, int32_t dof);
 
+/* Several reaction groups at once (e.g. one per instance of a batched design
+ * sweep): nodes[group_ptr[g] .. group_ptr[g+1]) form group g; every step
+ * records one sum per group (dl_get_group_history).  Generalises the
+ * GroupOutputModule resp sum (GroupOutputModule.cpp:121-169) to n_groups. */
+int dl_set_reaction_groups(dl_ctx *ctx, const int32_t *nodes, const int32_t *group_ptr,
+                           int32_t n_groups, int32_t dof);
+
+/* Per-node time step for independent instances packed into one context (a
+ * batched sweep: every instance keeps its own stable dt, ExplicitModule's
+ * deltaTime per model).  NULL restores the scalar dt of dl_explicit_step. */
+int dl_set_node_dt(dl_ctx *ctx, const double *dt, int64_t n_nodes);
+
+/* Replay the step sequence from a captured hipGraph of steps_per_graph steps
+ * (0 = direct launches).  The graph is re-captured when dt or any boundary /
+ * group data changes. */
+int dl_set_graph(dl_ctx *ctx, int32_t steps_per_graph);
+
 /* Replaces: ControlModule -> ExplicitModule::advance / LeapFrogModule::solve /
  * ExplicitModule::commit, nsteps times with fixed dt (maxDtime = minDtime =
  * deltaTime), with the intended store of u, v (LeapFrogModule.cpp:44-84,
@@ -185,6 +202,8 @@ int dl_explicit_step(dl_ctx *ctx, int64_t nsteps, double dt, dl_step_out *out);
  * active contact pairs; n = number of steps recorded. */
 int dl_get_history(dl_ctx *ctx, double *reaction, double *e_diss, int64_t *n_contacts,
                    int64_t cap, int64_t *n);
+/* All reaction groups of the last dl_explicit_step call, [steps][n_groups]. */
+int dl_get_group_history(dl_ctx *ctx, double *reaction, int64_t cap_steps, int64_t *n);
 
 /* Replaces: RodContactModel contactsA_/contactsB_ (RodContactModel.cpp:732-733):
  * active pairs of the last force evaluation, sorted by (a, b). */
