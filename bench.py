@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -160,11 +161,88 @@ def sweep(args):
     print(json.dumps(res), flush=True)
 
 
+def adjoint(args):
+    """Adjoint design gradient of the 20^3 BCC impact with contact: gradient
+    of J = absorbed energy + peak plate force with respect to all strut
+    radii, by the checkpointed forward run and the reverse sweep; reported as
+    (forward with checkpoints + reverse sweep) / plain forward time, with a
+    central finite-difference spot check of the GPU forward run."""
+    from dynlattice_amd import lattice as L
+    from dynlattice_amd.adjoint import ImpactSimulation
+    from dynlattice_amd.gpu import SECTION_FIELDS
+    p = L.bcc_c5(args.adjoint_n)
+    n, every = args.adjoint_steps, args.checkpoint_every
+    print(f"adjoint: {p.name}: {p.n_elems} elements, {p.n_struts} struts, {n} steps, "
+          f"checkpoint every {every}", file=sys.stderr, flush=True)
+    sim = ImpactSimulation(p, n, every)
+    m = sim.model
+    sec = {f: np.array(getattr(p.sections, f)) for f in SECTION_FIELDS}
+    r0 = p.contact.radius.copy()
+    v_plate = -20.0
+    sgn = np.sign(v_plate)
+
+    def objective(reaction):
+        F = reaction[:, 0] * sgn
+        return p.dt * abs(v_plate) * F.sum() + F.max(), F
+
+    reaction = sim.run(sec, r0)  # warm-up: allocations, graph, caches
+    J0, F = objective(reaction)
+    rbar = np.full((n, 1), p.dt * abs(v_plate) * sgn)
+    rbar[int(np.argmax(F)), 0] += sgn
+    sim.gradients(rbar)
+    # plain forward (no checkpoints)
+    m.set_state(p.disp0, p.velo0)
+    m.init()
+    t0 = time.perf_counter()
+    m.step(n)
+    t_plain = time.perf_counter() - t0
+    # checkpointed forward + reverse sweep
+    m.set_state(p.disp0, p.velo0)
+    m.init()
+    t0 = time.perf_counter()
+    m.adjoint_forward(n, p.dt, every)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = m.adjoint_backward(rbar)
+    t_bwd = time.perf_counter() - t0
+    pi = math.pi
+    dJdr = (g["area"] * 2 * pi * r0 + (g["I1"] + g["I2"]) * pi * r0 ** 3 +
+            g["J"] * 2 * pi * r0 ** 3 + g["contact_radius"])
+    # finite-difference spot check on the GPU forward run
+    fd = []
+    for s in np.random.default_rng(5).choice(p.n_struts, args.fd_struts, replace=False):
+        h = 1e-6 * r0[s]
+        Js = []
+        for sg in (1, -1):
+            r = r0.copy()
+            r[s] += sg * h
+            S = L.Sections.circle(r, p.sections.young, 0.3, p.sections.density)
+            Js.append(objective(sim.run({f: getattr(S, f) for f in SECTION_FIELDS}, r))[0])
+        fdv = (Js[0] - Js[1]) / (2 * h)
+        fd.append({"strut": int(s), "fd": fdv, "adjoint": float(dJdr[s]),
+                   "rel_err": abs(fdv - dJdr[s]) / abs(fdv)})
+    sim.close()
+    res = {"metric": "adjoint/forward time ratio (full radius gradient)",
+           "value": (t_fwd + t_bwd) / t_plain, "unit": "x", "higher_is_better": False,
+           "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded BCC lattice generator)",
+           "config": {"workload": p.name, "elements": p.n_elems, "struts": p.n_struts,
+                      "steps": n, "checkpoint_every": every, "design_variables": p.n_struts},
+           "t_forward_plain_s": t_plain, "t_forward_checkpointed_s": t_fwd,
+           "t_reverse_s": t_bwd, "objective": J0, "fd_check": fd}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sweep", action="store_true",
                     help="batched design sweep vs sequential single-instance runs")
     ap.add_argument("--sweep-instances", type=int, default=256)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="adjoint gradient of the 20^3 impact: adjoint/forward time ratio")
+    ap.add_argument("--adjoint-n", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=5000)
+    ap.add_argument("--checkpoint-every", type=int, default=100)
+    ap.add_argument("--fd-struts", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +259,9 @@ def main():
         return
     if args.sweep:
         sweep(args)
+        return
+    if args.adjoint:
+        adjoint(args)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "

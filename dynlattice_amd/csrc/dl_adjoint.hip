@@ -1,0 +1,149 @@
+// dl_adjoint.hip -- reverse sweep of one explicit step (SO(3) mode, elastic
+// material), as three gfx950 kernels plus the contact adjoint
+// (dl_contact.hip):
+//   k_adj_node    one thread per node: VJP of the node update (mass solve,
+//                 gyroscopic term, constraints, SO(3) composition) from the
+//                 taped start state; gives the adjoint of the node's internal
+//                 force (plus the objective's weight on the reaction), the
+//                 adjoints of its start velocity and rotation, and the mass
+//                 adjoints.
+//   k_adj_elem    one thread per element: VJP of the element internal force
+//                 (quaternion kinematics, strains, elastic stress resultants)
+//                 into per-element-end buffers and the stiffness adjoint.
+//   k_adj_gather  one thread per node: sums its element ends (CSR, fixed
+//                 order) into the displacement and rotation adjoints.
+// Like the forward step, assembly is a gather: no floating-point atomics,
+// the result is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "dl_adjoint.h"
+#include "dl_internal.h"
+
+#define AB 256
+
+__global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_all) {
+  const int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  const double *t = a.tape;
+  Quat q;
+  q.w = t[3 * N + n]; q.x = t[4 * N + n]; q.y = t[5 * N + n]; q.z = t[6 * N + n];
+  double v[6], f[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    v[c] = t[(7 + c) * N + n];
+    f[c] = t[(13 + c) * N + n];
+  }
+  const int32_t b = d.bc[n];
+  unsigned mask = 0u;
+  double ac[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double fe[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int32_t rs = -1;
+  if (b >= 0) {
+    const int64_t nb = d.nbc;
+    mask = d.bc_mask[b];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      ac[c] = d.bc_a[c * nb + b];
+      fe[c] = d.bc_f[c * nb + b];
+    }
+    rs = d.bc_rslot[b];
+  }
+  double J0[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  const double dt = d.ndt ? d.ndt[n] : dt_all;
+  double ubn[3], vbn[6];
+  Quat qbn;
+#pragma unroll
+  for (int c = 0; c < 3; c++) ubn[c] = a.ubar[c * N + n];
+  qbn.w = a.qbar[n]; qbn.x = a.qbar[N + n]; qbn.y = a.qbar[2 * N + n]; qbn.z = a.qbar[3 * N + n];
+#pragma unroll
+  for (int c = 0; c < 6; c++) vbn[c] = a.vbar[c * N + n];
+  NodeVJP r;
+  node_advance_vjp(f, q, v, d.mt[n], J0, d.gm[n], mask, ac, fe, dt, ubn, qbn, vbn, r);
+  // the recorded reaction is the internal force of the constrained dof
+  const double rw = (rs >= 0) ? a.rbar[a.slot_group[rs]] : 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; c++) r.f[c] += (c == d.react_dof) ? rw : 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    a.fbar[c * N + n] = r.f[c];
+    a.vbar[c * N + n] = r.v[c];
+  }
+  a.qbar[n] = r.q.w; a.qbar[N + n] = r.q.x; a.qbar[2 * N + n] = r.q.y; a.qbar[3 * N + n] = r.q.z;
+  a.mtbar[n] += r.mt;
+#pragma unroll
+  for (int c = 0; c < 9; c++) a.J0bar[c * N + n] += r.J0[c];
+}
+
+__global__ __launch_bounds__(AB) void k_adj_elem(DevData d, AdjDev a) {
+  const int64_t e = (int64_t)blockIdx.x * AB + threadIdx.x;
+  if (e >= d.E) return;
+  const int64_t N = d.N, E = d.E, R = d.R;
+  const int32_t na = d.en0[e], nb = d.en1[e], s = d.es[e];
+  const double *t = a.tape;
+  double Xa[3], ua[3], Xb[3], ub[3], fba[6], fbb[6], eps0[6], C[6];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    Xa[i] = d.X0[i * N + na]; ua[i] = t[i * N + na];
+    Xb[i] = d.X0[i * N + nb]; ub[i] = t[i * N + nb];
+  }
+  Quat qna, qnb, qNa, qNb;
+  qna.w = t[3 * N + na]; qna.x = t[4 * N + na]; qna.y = t[5 * N + na]; qna.z = t[6 * N + na];
+  qnb.w = t[3 * N + nb]; qnb.x = t[4 * N + nb]; qnb.y = t[5 * N + nb]; qnb.z = t[6 * N + nb];
+  qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
+  qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    fba[i] = a.fbar[i * N + na];
+    fbb[i] = a.fbar[i * N + nb];
+    eps0[i] = d.eps0[i * E + e];
+    C[i] = d.C[i * R + s];
+  }
+  ElemVJP r;
+  elem_force_vjp(Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, C, fba, fbb, r);
+  double *eb = a.ebuf;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    eb[i * E + e] = r.ua[i];
+    eb[(7 + i) * E + e] = r.ub[i];
+  }
+  eb[3 * E + e] = r.qa.w; eb[4 * E + e] = r.qa.x; eb[5 * E + e] = r.qa.y; eb[6 * E + e] = r.qa.z;
+  eb[10 * E + e] = r.qb.w; eb[11 * E + e] = r.qb.x; eb[12 * E + e] = r.qb.y; eb[13 * E + e] = r.qb.z;
+#pragma unroll
+  for (int i = 0; i < 6; i++) a.Cbar[i * E + e] += r.C[i];
+}
+
+__global__ __launch_bounds__(AB) void k_adj_gather(DevData d, AdjDev a) {
+  const int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N, E = d.E;
+  double ub[3] = {0.0, 0.0, 0.0}, qb[4] = {0.0, 0.0, 0.0, 0.0};
+  const int32_t t1 = a.nptr[n + 1];
+  for (int32_t t = a.nptr[n]; t < t1; t++) {
+    const int32_t code = a.nlist[t];
+    const int64_t e = code >> 1;
+    const int o = (code & 1) ? 7 : 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) ub[c] += a.ebuf[(o + c) * E + e];
+#pragma unroll
+    for (int c = 0; c < 4; c++) qb[c] += a.ebuf[(o + 3 + c) * E + e];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) a.ubar[c * N + n] += ub[c];
+#pragma unroll
+  for (int c = 0; c < 4; c++) a.qbar[c * N + n] += qb[c];
+}
+
+static unsigned nblk_a(int64_t n) { return (unsigned)((n + AB - 1) / AB); }
+
+void launch_adj_node(const DevData &d, const AdjDev &a, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_node, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a, dt);
+}
+void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_elem, dim3(nblk_a(d.E)), dim3(AB), 0, s, d, a);
+}
+void launch_adj_gather(const DevData &d, const AdjDev &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_gather, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a);
+}

@@ -18,159 +18,117 @@
 
 #include "../../include/dl_gpu.h"
 #include "dl_contact.h"
+#include "dl_ctx.h"
 #include "dl_internal.h"
+#include "dl_setup.h"
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg) {
+int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
 
-#define HIPCHK(x)                                                                        \
-  do {                                                                                   \
-    hipError_t _e = (x);                                                                 \
-    if (_e != hipSuccess)                                                                \
-      return fail(DL_ERR_HIP, std::string("HIP: ") + hipGetErrorString(_e) + " at " #x); \
-  } while (0)
-
 extern "C" const char *dl_last_error(void) { return g_err.c_str(); }
 
-// boundary data kept on the host for re-uploads (constraints, loads, reaction group)
-struct BcHost {
-  std::vector<uint8_t> cmask;
-  std::vector<double> cval, fext;
-  std::vector<int32_t> react;     // reaction nodes, concatenated by group
-};
-
-struct dl_ctx {
-  BcHost bch;
-  std::vector<double> ediss_acc_h; // per-element dissipation of the action path (energyDiss_)
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  bool so3 = true;
-  bool plastic = false;
-  int mat_kind = 0; // 0 elastic; 1/2/3 plastic with yield exponents all 1 / all 2 / general
-  bool contact = false;
-  bool initialized = false;
-  bool track_diss = false;
-  uint32_t flags = 0;
-  PlasticParams pp{};
-  DevData d{};
-  std::vector<void *> allocs;
-  // host copies needed by actions
-  std::vector<double> w_e;        // Gauss weight (L0) per element
-  std::vector<int32_t> elem_perm; // identity (elements are stored in input order)
-  // reaction groups: slots [rgptr[g], rgptr[g+1]) of rbuf belong to group g
-  int64_t n_react = 0;
-  int32_t n_groups = 0;
-  int32_t *rgptr = nullptr;
-  int32_t *step_ctr = nullptr;   // device step counter (history row)
-  double *ndt = nullptr;          // per-node dt
-  // captured step graphs
-  int32_t graph_steps = 0;
-  hipGraphExec_t gexec = nullptr;
-  int64_t gen = 0, graph_gen = -1;
-  double graph_dt = 0.0;
-  // boundary records (host) for re-upload
-  std::vector<int32_t> bc_host;
-  // device buffers for history
-  double *hist_react = nullptr, *hist_diss = nullptr;
-  int64_t *hist_nc = nullptr, *hist_ncand = nullptr;
-  int64_t hist_cap = 0, hist_n = 0;
-  double *diss_total = nullptr;
-  int32_t strut_blocks = 0;
-  // action path energy accumulators
-  double *ediss_acc = nullptr, *epot_e = nullptr, *ediss_inc = nullptr;
-  // timing
-  bool timing = false;
-  double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0;
-  int64_t t_count = 0;
-  // contact
-  ContactState cs{};
-};
-
-template <class T>
-static int dalloc(dl_ctx *c, T **p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
-  c->allocs.push_back((void *)*p);
-  return DL_OK;
-}
-
-template <class T>
-static int upload(dl_ctx *c, T **p, const std::vector<T> &h) {
-  int rc = dalloc(c, p, h.size());
-  if (rc) return rc;
-  if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-  return DL_OK;
-}
-
-// ------------------------------------------------------------------
-// host math for set-up (row-major 3x3)
-// ------------------------------------------------------------------
-static void mm3(const double A[9], const double B[9], double C[9]) {
-  double T[9];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      double s = A[3 * i] * B[j];
-      s += A[3 * i + 1] * B[3 + j];
-      s += A[3 * i + 2] * B[6 + j];
-      T[3 * i + j] = s;
+void dfree(dl_ctx *c, void *p) {
+  if (!p) return;
+  for (size_t i = 0; i < c->allocs.size(); i++)
+    if (c->allocs[i] == p) {
+      c->allocs[i] = c->allocs.back();
+      c->allocs.pop_back();
+      hipFree(p);
+      return;
     }
-  std::memcpy(C, T, sizeof T);
 }
 
-// rotation matrix -> unit quaternion (Shepperd's method)
-static Quat mat2quat(const double R[9]) {
-  const double tr = R[0] + R[4] + R[8];
-  Quat q;
-  if (tr >= R[0] && tr >= R[4] && tr >= R[8]) {
-    const double s = std::sqrt(1.0 + tr) * 2.0;
-    q.w = 0.25 * s;
-    q.x = (R[7] - R[5]) / s;
-    q.y = (R[2] - R[6]) / s;
-    q.z = (R[3] - R[1]) / s;
-  } else if (R[0] >= R[4] && R[0] >= R[8]) {
-    const double s = std::sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
-    q.w = (R[7] - R[5]) / s;
-    q.x = 0.25 * s;
-    q.y = (R[1] + R[3]) / s;
-    q.z = (R[2] + R[6]) / s;
-  } else if (R[4] >= R[8]) {
-    const double s = std::sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
-    q.w = (R[2] - R[6]) / s;
-    q.x = (R[1] + R[3]) / s;
-    q.y = 0.25 * s;
-    q.z = (R[5] + R[7]) / s;
-  } else {
-    const double s = std::sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
-    q.w = (R[3] - R[1]) / s;
-    q.x = (R[2] + R[6]) / s;
-    q.y = (R[5] + R[7]) / s;
-    q.z = 0.25 * s;
-  }
-  if (q.w < 0) { q.w = -q.w; q.x = -q.x; q.y = -q.y; q.z = -q.z; }
-  return qnormalize(q);
+// ------------------------------------------------------------------
+// section-dependent tables: stiffness diag C = (kGA, kGA, EA, EI1, EI2, GJ)
+// (ElasticRodMaterial.cpp:265-287), kinematic hardening Hk, lumped mass
+// ------------------------------------------------------------------
+static int store_sections(dl_ctx *c, const dl_sections *sec) {
+  const int32_t R = c->mesh.R;
+  SecHost h;
+  h.area.assign(sec->area, sec->area + R);
+  h.I1.assign(sec->I1, sec->I1 + R);
+  h.I2.assign(sec->I2, sec->I2 + R);
+  h.J.assign(sec->J, sec->J + R);
+  h.kappa.assign(sec->shear_factor, sec->shear_factor + R);
+  h.young.assign(sec->young, sec->young + R);
+  h.shear.assign(sec->shear_modulus, sec->shear_modulus + R);
+  h.rho.assign(sec->density, sec->density + R);
+  h.inertia_correct = sec->inertia_correct;
+  for (int32_t s = 0; s < R; s++)
+    if (!(h.area[s] > 0) || !(h.young[s] > 0) || !(h.shear[s] > 0) || !(h.I1[s] > 0) ||
+        !(h.I2[s] > 0) || !(h.J[s] > 0) || h.rho[s] < 0)
+      return fail(DL_ERR_INPUT, "invalid section/material data for strut " + std::to_string(s));
+  c->sec = std::move(h);
+  return DL_OK;
 }
 
-// initRotation_ frame for a unit tangent d: the rotation taking e3 to d
-// (SpecialCosseratRodModel.cpp:641-655), -I for the 180 degree turn.
-static void frame_from_tangent(const double d[3], double Rm[9]) {
-  const double v[3] = {-1 * d[1], 1 * d[0], 0.0};
-  const double cc = d[0] * 0. + d[1] * 0. + d[2] * 1.;
-  for (int i = 0; i < 9; i++) Rm[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  if (std::fabs(cc + 1.0) > 2.220446049250313e-16) {
-    const double Sv[9] = {0.0, -1 * v[2], 1 * v[1], 1 * v[2], 0.0, -1 * v[0], -1 * v[1], 1 * v[0], 0.0};
-    double SS[9];
-    mm3(Sv, Sv, SS);
-    const double f = 1 / (1 + cc);
-    for (int i = 0; i < 9; i++) Rm[i] += Sv[i] + f * SS[i];
-  } else {
-    for (int i = 0; i < 9; i++) Rm[i] *= -1.;
+// builds C, Hk (elastoplastic), mt, J0 from the stored sections; allocates
+// the device tables when first, else overwrites them in place
+static int section_tables(dl_ctx *c, bool first) {
+  const MeshHost &m = c->mesh;
+  const SecHost &sc = c->sec;
+  const int64_t N = m.N;
+  const int32_t R = m.R;
+  std::vector<double> Cd(6 * (size_t)R), mt(N, 0.0), J0(6 * N, 0.0), pw, Hk;
+  if (c->plastic) {
+    pw.resize(6 * (size_t)R);
+    Hk.resize(6 * (size_t)R);
   }
+  for (int32_t s = 0; s < R; s++) {
+    const double G = sc.shear[s], Y = sc.young[s], kap = sc.kappa[s], A = sc.area[s];
+    Cd[0 * R + s] = G * kap * A;
+    Cd[1 * R + s] = G * kap * A;
+    Cd[2 * R + s] = Y * A;
+    Cd[3 * R + s] = Y * sc.I1[s];
+    Cd[4 * R + s] = Y * sc.I2[s];
+    Cd[5 * R + s] = G * sc.J[s];
+    if (c->plastic)
+      for (int i = 0; i < 6; i++) {
+        pw[i * R + s] = c->pl_weights[6 * s + i];
+        Hk[i * R + s] = c->mat.kin_const[i] + c->mat.kin_ratio * Cd[i * R + s];
+      }
+  }
+  walk_elements(m, [&](int32_t s, int64_t, int32_t a, int32_t b, const double *La, const double *Lb) {
+    add_elem_mass(sc, s, elem_jac(m.X.data(), a, b), a, b, La, Lb, N, mt.data(), J0.data());
+  });
+  // mass checks (ExplicitModule.cpp:452-454): positive mass and inertia
+  for (int64_t n = 0; n < N; n++) {
+    const double a = J0[n], b = J0[N + n], cc = J0[2 * N + n], xy = J0[3 * N + n],
+                 xz = J0[4 * N + n], yz = J0[5 * N + n];
+    const double det = a * (b * cc - yz * yz) - xy * (xy * cc - yz * xz) + xz * (xy * yz - b * xz);
+    if (!(mt[n] > 2.2250738585072014e-308) || !(a > 0) || !(a * b - xy * xy > 0) || !(det > 0))
+      return fail(DL_ERR_ZERO_MASS, "Zero (or negative) masses cannot be inversed! (node " +
+                                        std::to_string(n) + ")");
+  }
+  DevData &d = c->d;
+  if (first) {
+    double *p;
+    int rc;
+    if ((rc = upload(c, &p, mt))) return rc;
+    d.mt = p;
+    if ((rc = upload(c, &p, J0))) return rc;
+    d.J0 = p;
+    if ((rc = upload(c, &p, Cd))) return rc;
+    d.C = p;
+    if (c->plastic) {
+      if ((rc = upload(c, &p, pw))) return rc;
+      d.pw = p;
+      if ((rc = upload(c, &p, Hk))) return rc;
+      d.Hk = p;
+    }
+    return DL_OK;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy((void *)d.mt, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.J0, J0.data(), J0.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)d.C, Cd.data(), Cd.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->plastic)
+    HIPCHK(hipMemcpy((void *)d.Hk, Hk.data(), Hk.size() * sizeof(double), hipMemcpyHostToDevice));
+  return DL_OK;
 }
 
 // ------------------------------------------------------------------
@@ -322,125 +280,32 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   }
   while (lane_e.size() % 256) lane_e.push_back(-1);
 
-  // frames per (strut, node) -> per (element, end) quaternions; masses
-  const double *X = mesh->coords;
-  std::vector<double> qN(8 * E), mt(N, 0.0), J0(6 * N, 0.0), Cd(6 * R), wE(E);
-  std::vector<double> pw, Hk;
+  // host copies of the mesh, sections and material (tables are rebuilt from
+  // them when the sections change; the adjoint maps back through them)
+  {
+    MeshHost &m = c->mesh;
+    m.N = N; m.E = E; m.R = R;
+    m.X.assign(mesh->coords, mesh->coords + 3 * N);
+    m.en.assign(en, en + 2 * E);
+    m.es.assign(es, es + E);
+    m.eptr = eptr;
+    m.cnode = cnode;
+  }
   if (plastic) {
-    pw.resize(6 * R);
-    Hk.resize(6 * R);
+    c->mat = *mat;
+    c->pl_weights.assign(mat->weights, mat->weights + 6 * (size_t)R);
+    c->mat.weights = nullptr;
   }
-  const double xi[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
-  std::vector<double> dirs, lam;
-  std::vector<int32_t> lnode;
-  for (int32_t s = 0; s < R; s++) {
-    const double G = sec->shear_modulus[s], Y = sec->young[s], kap = sec->shear_factor[s];
-    const double A = sec->area[s], I1 = sec->I1[s], I2 = sec->I2[s], Jp = sec->J[s], rho = sec->density[s];
-    if (!(A > 0) || !(Y > 0) || !(G > 0) || !(I1 > 0) || !(I2 > 0) || !(Jp > 0) || rho < 0) {
-      dl_destroy(c);
-      return fail(DL_ERR_INPUT, "invalid section/material data for strut " + std::to_string(s));
-    }
-    double *Cs = &Cd[0];
-    Cs[0 * R + s] = G * kap * A;
-    Cs[1 * R + s] = G * kap * A;
-    Cs[2 * R + s] = Y * A;
-    Cs[3 * R + s] = Y * I1;
-    Cs[4 * R + s] = Y * I2;
-    Cs[5 * R + s] = G * Jp;
-    if (plastic)
-      for (int i = 0; i < 6; i++) {
-        pw[i * R + s] = mat->weights[6 * s + i];
-        Hk[i * R + s] = mat->kin_const[i] + mat->kin_ratio * Cs[i * R + s];
-      }
-    double Mm[6] = {rho * A, rho * A, rho * A, rho * I1, rho * I2, rho * Jp};
-    if (sec->inertia_correct != 1.0)
-      for (int i = 3; i < 6; i++) Mm[i] *= sec->inertia_correct;
-    // rod nodes: unique chain nodes of this strut
-    const int64_t c0 = eptr[s] + s, c1 = eptr[s + 1] + s;
-    lnode.clear();
-    for (int64_t p = c0; p <= c1; p++)
-      if (std::find(lnode.begin(), lnode.end(), cnode[p]) == lnode.end()) lnode.push_back(cnode[p]);
-    const size_t nl = lnode.size();
-    dirs.assign(3 * nl, 0.0);
-    lam.assign(9 * nl, 0.0);
-    auto lid = [&](int32_t n) { return (size_t)(std::find(lnode.begin(), lnode.end(), n) - lnode.begin()); };
-    for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) {
-      const int32_t a = en[2 * e], b = en[2 * e + 1];
-      const size_t la = lid(a), lb = lid(b);
-      for (int k = 0; k < 3; k++) {
-        const double t = X[3 * b + k] - X[3 * a + k];
-        dirs[3 * la + k] += t;
-        dirs[3 * lb + k] += t;
-      }
-    }
-    for (size_t k = 0; k < nl; k++) {
-      double *dd = &dirs[3 * k];
-      const double nn = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-      dd[0] = dd[0] / nn; dd[1] = dd[1] / nn; dd[2] = dd[2] / nn;
-      frame_from_tangent(dd, &lam[9 * k]);
-    }
-    for (int64_t e = eptr[s]; e < eptr[s + 1]; e++) {
-      const int32_t a = en[2 * e], b = en[2 * e + 1];
-      const double *La = &lam[9 * lid(a)], *Lb = &lam[9 * lid(b)];
-      const Quat qa = mat2quat(La), qb = mat2quat(Lb);
-      qN[0 * E + e] = qa.w; qN[1 * E + e] = qa.x; qN[2 * E + e] = qa.y; qN[3 * E + e] = qa.z;
-      qN[4 * E + e] = qb.w; qN[5 * E + e] = qb.x; qN[6 * E + e] = qb.y; qN[7 * E + e] = qb.z;
-      // lengths, weights (Line3D.cpp:91-115)
-      double dd[3] = {X[3 * b] - X[3 * a], X[3 * b + 1] - X[3 * a + 1], X[3 * b + 2] - X[3 * a + 2]};
-      double ll = dd[0] * dd[0];
-      ll += dd[1] * dd[1];
-      ll += dd[2] * dd[2];
-      const double Lc = std::sqrt(ll) + 0.;
-      const double Jc = -0.5 * 0. + 0.5 * Lc;
-      wE[e] = 2. * Jc;
-      // mass (assembleM_): translational row sum over Gauss2, rotational lumped / 2
-      const double wts = 1. * Jc;
-      const double l = (wts + wts) / (2 - 1);
-      double sh[2][2];
-      for (int ip = 0; ip < 2; ip++) {
-        sh[0][ip] = 0.5 * (1. - xi[ip]);
-        sh[1][ip] = 0.5 * (1. + xi[ip]);
-      }
-      const int32_t nodes2[2] = {a, b};
-      for (int in = 0; in < 2; in++) {
-        double acc = 0.0;
-        for (int jn = 0; jn < 2; jn++)
-          for (int ip = 0; ip < 2; ip++) acc += wts * sh[in][ip] * sh[jn][ip] * Mm[0];
-        mt[nodes2[in]] += acc;
-      }
-      double Mr[3] = {Mm[3] * l, Mm[4] * l, Mm[5] * l};
-      const double add = A * rho * std::pow(l, 3) / 12.;
-      Mr[0] += add;
-      Mr[1] += add;
-      for (int in = 0; in < 2; in++) {
-        const double *L = in == 0 ? La : Lb;
-        // L diag(Mr) L^T / 2
-        double Jm[9];
-        for (int i = 0; i < 3; i++)
-          for (int j = 0; j < 3; j++)
-            Jm[3 * i + j] = (L[3 * i] * Mr[0] * L[3 * j] + L[3 * i + 1] * Mr[1] * L[3 * j + 1] +
-                             L[3 * i + 2] * Mr[2] * L[3 * j + 2]) / 2.;
-        const int32_t n = nodes2[in];
-        J0[0 * N + n] += Jm[0];
-        J0[1 * N + n] += Jm[4];
-        J0[2 * N + n] += Jm[8];
-        J0[3 * N + n] += Jm[1];
-        J0[4 * N + n] += Jm[2];
-        J0[5 * N + n] += Jm[5];
-      }
-    }
-  }
-  // mass checks (ExplicitModule.cpp:452-454): positive mass and inertia
-  for (int64_t n = 0; n < N; n++) {
-    const double a = J0[n], b = J0[N + n], cc = J0[2 * N + n], xy = J0[3 * N + n],
-                 xz = J0[4 * N + n], yz = J0[5 * N + n];
-    const double det = a * (b * cc - yz * yz) - xy * (xy * cc - yz * xz) + xz * (xy * yz - b * xz);
-    if (!(mt[n] > 2.2250738585072014e-308) || !(a > 0) || !(a * b - xy * xy > 0) || !(det > 0)) {
-      dl_destroy(c);
-      return fail(DL_ERR_ZERO_MASS, "Zero (or negative) masses cannot be inversed! (node " +
-                                        std::to_string(n) + ")");
-    }
-  }
+  TRY(store_sections(c, sec));
+  // per (element, end) frame quaternions and Gauss weights
+  std::vector<double> qN(8 * E), wE(E);
+  walk_elements(c->mesh, [&](int32_t, int64_t e, int32_t a, int32_t b, const double *La,
+                             const double *Lb) {
+    const Quat qa = mat2quat(La), qb = mat2quat(Lb);
+    qN[0 * E + e] = qa.w; qN[1 * E + e] = qa.x; qN[2 * E + e] = qa.y; qN[3 * E + e] = qa.z;
+    qN[4 * E + e] = qb.w; qN[5 * E + e] = qb.x; qN[6 * E + e] = qb.y; qN[7 * E + e] = qb.z;
+    wE[e] = 2. * elem_jac(c->mesh.X.data(), a, b);
+  });
   c->w_e = wE;
 
   // upload
@@ -451,7 +316,7 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   {
     std::vector<double> X0(3 * N);
     for (int64_t n = 0; n < N; n++)
-      for (int k = 0; k < 3; k++) X0[k * N + n] = X[3 * n + k];
+      for (int k = 0; k < 3; k++) X0[k * N + n] = mesh->coords[3 * n + k];
     double *p;
     TRY(upload(c, &p, X0));
     d.X0 = p;
@@ -467,18 +332,12 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   }
   {
     double *p;
-    TRY(upload(c, &p, mt)); d.mt = p;
-    TRY(upload(c, &p, J0)); d.J0 = p;
     std::vector<double> g(N);
     for (int64_t n = 0; n < N; n++) g[n] = (double)nstr[n];
     TRY(upload(c, &p, g)); d.gm = p;
-    TRY(upload(c, &p, Cd)); d.C = p;
     TRY(upload(c, &p, qN)); d.qN = p;
     TRY(dalloc(c, &d.eps0, 6 * E));
-    if (plastic) {
-      TRY(upload(c, &p, pw)); d.pw = p;
-      TRY(upload(c, &p, Hk)); d.Hk = p;
-    }
+    TRY(section_tables(c, true));
   }
   {
     std::vector<int32_t> bc(N, -1);
@@ -579,13 +438,35 @@ extern "C" int dl_destroy(dl_ctx *c) {
   return DL_OK;
 }
 
+extern "C" int dl_set_sections(dl_ctx *c, const dl_sections *sec, const double *contact_radius) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!sec && !contact_radius) return fail(DL_ERR_INPUT, "nothing to set");
+  if (contact_radius && !c->contact) return fail(DL_ERR_INPUT, "contact radii given without contact");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  if (sec) {
+    SecHost keep = c->sec;
+    int rc = store_sections(c, sec);
+    if (rc) return rc;
+    if ((rc = section_tables(c, false))) {
+      c->sec = std::move(keep);
+      return rc;
+    }
+  }
+  if (contact_radius) {
+    int rc = contact_set_radius(c->cs, contact_radius, c->stream);
+    if (rc) return rc;
+  }
+  return DL_OK;
+}
+
 extern "C" int dl_sync(dl_ctx *c) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   HIPCHK(hipStreamSynchronize(c->stream));
   return DL_OK;
 }
 
-static int check_err(dl_ctx *c) {
+int check_err(dl_ctx *c) {
   int32_t e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->d.err, sizeof e, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -603,6 +484,7 @@ static int check_err(dl_ctx *c) {
 extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, int64_t ndof) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
+  c->adj.valid = false;
   hipSetDevice(c->device);
   double *tmp = nullptr;
   HIPCHK(hipMalloc((void **)&tmp, 2 * ndof * sizeof(double)));
@@ -638,6 +520,7 @@ extern "C" int dl_get_state(dl_ctx *c, double *disp, double *velo, int64_t ndof)
 // (re)build the boundary record table from host masks; keeps reaction slots
 static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const double *fext,
                       const std::vector<int32_t> &react_nodes) {
+  c->adj.valid = false;
   const int64_t N = c->d.N;
   std::vector<int32_t> bc(N, -1), rs;
   std::vector<uint32_t> mask;
@@ -733,6 +616,7 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
   std::vector<int32_t> gp(group_ptr, group_ptr + n_groups + 1);
   if (n_groups == 0) gp.assign(1, 0);
   if ((rc = upload(c, &c->rgptr, gp))) return rc;
+  c->rgptr_h = gp;
   c->hist_cap = 0; // history rows are n_groups wide: reallocate on the next step call
   c->gen++;
   return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
@@ -747,6 +631,7 @@ extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n,
 
 extern "C" int dl_set_node_dt(dl_ctx *c, const double *dt, int64_t n_nodes) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  c->adj.valid = false;
   if (!dt) {
     c->d.ndt = nullptr;
     c->gen++;
@@ -778,6 +663,7 @@ extern "C" int dl_set_graph(dl_ctx *c, int32_t steps_per_graph) {
 // ------------------------------------------------------------------
 extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  c->adj.valid = false;
   hipSetDevice(c->device);
   const int64_t N = c->d.N, E = c->d.E;
   switch (action) {
@@ -894,7 +780,7 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
 }
 
 // one explicit step: kernels only (capturable into a hipGraph)
-static int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
+int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
   if (c->contact) {
     int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc, c->hist_ncand, c->step_ctr);
@@ -910,15 +796,15 @@ static int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   return DL_OK;
 }
 
-extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_out *out) {
-  if (!c) return fail(DL_ERR_INPUT, "NULL context");
-  if (!c->initialized) return fail(DL_ERR_STATE, "explicit step before INIT");
-  if (nsteps < 0 || !(dt > 0.0)) return fail(DL_ERR_INPUT, "nsteps >= 0 and dt > 0 required");
-  hipSetDevice(c->device);
+int prepare_history(dl_ctx *c, int64_t nsteps) {
   const int64_t ng = std::max<int32_t>(c->n_groups, 1);
   if (nsteps > c->hist_cap || !c->step_ctr) {
     int rc;
     const int64_t cap = std::max<int64_t>(nsteps, std::max<int64_t>(c->hist_cap, 16));
+    dfree(c, c->hist_react);
+    dfree(c, c->hist_diss);
+    dfree(c, c->hist_nc);
+    dfree(c, c->hist_ncand);
     if ((rc = dalloc(c, &c->hist_react, cap * ng))) return rc;
     if ((rc = dalloc(c, &c->hist_diss, cap))) return rc;
     if ((rc = dalloc(c, &c->hist_nc, cap))) return rc;
@@ -932,6 +818,52 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
   HIPCHK(hipMemsetAsync(c->hist_nc, 0, nsteps * sizeof(int64_t), c->stream));
   HIPCHK(hipMemsetAsync(c->hist_ncand, 0, nsteps * sizeof(int64_t), c->stream));
   launch_set_i32(c->step_ctr, 0, c->stream);
+  return DL_OK;
+}
+
+int advance_steps(dl_ctx *c, int64_t nsteps, const StepParams &sp) {
+  int64_t done = 0;
+  const int64_t K = c->graph_steps;
+  const double dt = sp.dt;
+  if (K > 0 && nsteps >= K) {
+    if (!c->gexec || c->graph_gen != c->gen || c->graph_dt != dt) {
+      if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+      }
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = DL_OK;
+      for (int64_t k = 0; k < K && !rc; k++) rc = launch_one_step(c, sp, nullptr);
+      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (rc) return rc;
+      if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+      ec = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+      c->graph_gen = c->gen;
+      c->graph_dt = dt;
+    }
+    for (; done + K <= nsteps; done += K) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  }
+  for (; done < nsteps; done++) {
+    int rc = launch_one_step(c, sp, nullptr);
+    if (rc) return rc;
+  }
+  return DL_OK;
+}
+
+extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_out *out) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!c->initialized) return fail(DL_ERR_STATE, "explicit step before INIT");
+  if (nsteps < 0 || !(dt > 0.0)) return fail(DL_ERR_INPUT, "nsteps >= 0 and dt > 0 required");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  const int64_t ng = std::max<int32_t>(c->n_groups, 1);
+  {
+    int rc = prepare_history(c, nsteps);
+    if (rc) return rc;
+  }
   StepParams sp{dt, 1, c->track_diss ? 1 : 0};
   if (c->timing) {
     std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 1);
@@ -957,33 +889,8 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     c->t_count = nsteps;
     for (auto &e : ev) hipEventDestroy(e);
   } else {
-    int64_t done = 0;
-    const int64_t K = c->graph_steps;
-    if (K > 0 && nsteps >= K) {
-      if (!c->gexec || c->graph_gen != c->gen || c->graph_dt != dt) {
-        if (c->gexec) {
-          hipGraphExecDestroy(c->gexec);
-          c->gexec = nullptr;
-        }
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = DL_OK;
-        for (int64_t k = 0; k < K && !rc; k++) rc = launch_one_step(c, sp, nullptr);
-        hipError_t ec = hipStreamEndCapture(c->stream, &g);
-        if (rc) return rc;
-        if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
-        ec = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (ec != hipSuccess) return fail(DL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
-        c->graph_gen = c->gen;
-        c->graph_dt = dt;
-      }
-      for (; done + K <= nsteps; done += K) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-    }
-    for (; done < nsteps; done++) {
-      int rc = launch_one_step(c, sp, nullptr);
-      if (rc) return rc;
-    }
+    int rc = advance_steps(c, nsteps, sp);
+    if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
   c->hist_n = nsteps;

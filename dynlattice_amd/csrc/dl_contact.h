@@ -71,6 +71,11 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s);
 // n_active[*step], n_cand[*step] (kernels only: safe inside a captured graph)
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
                  int64_t *n_cand, const int32_t *step);
+// adjoint of the contact forces at the state d (the taped start state of a
+// step): broadphase, then per closed pair the VJP into a.ebuf and a.crbar
+int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s);
+// replaces the per-strut contact radii (host array of n_struts values)
+int contact_set_radius(ContactState &cs, const double *r, hipStream_t s);
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,
                        int64_t cap, int64_t *n);
 int dl_set_error(int code, const char *msg);

@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "dl_adjoint.h"
 #include "dl_contact.h"
 #include "dl_contact_math.h"
 
@@ -229,6 +230,70 @@ __device__ __forceinline__ void elem_pos(const DevData &d, int32_t n0, int32_t n
   }
 }
 
+// Candidate scan of the element X of cell-order record k: every Y in the 27
+// neighbour cells with a different strut, overlapping AABBs (float prefilter
+// with outward-rounded bounds, then the exact double test) and no node of X
+// in Y's strut nor of Y in X's strut.  fn(Y, sY, ya, yb, XisA) is called per
+// candidate in cell order, three contiguous cells per row.
+template <class Fn>
+__device__ __forceinline__ void scan_candidates(const ContactDev &c, int64_t k, const CRec &x,
+                                                Fn fn) {
+  const CRec *rec = (const CRec *)c.rec;
+  const int32_t X = x.e, sX = x.s;
+  const int32_t xa = c.en0[X], xb = c.en1[X];
+  const int32_t cx = x.cell % c.dim[0];
+  const int32_t cy = (x.cell / c.dim[0]) % c.dim[1];
+  const int32_t cz = x.cell / (c.dim[0] * c.dim[1]);
+  const float4 xfl = ((const float4 *)c.frec)[2 * k], xfh = ((const float4 *)c.frec)[2 * k + 1];
+  for (int dz = -1; dz <= 1; dz++) {
+    const int32_t z = cz + dz;
+    if (z < 0 || z >= c.dim[2]) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int32_t y = cy + dy;
+      if (y < 0 || y >= c.dim[1]) continue;
+      // the three cells x-1..x+1 of a row are contiguous in cell order
+      const int32_t x0 = cx > 0 ? cx - 1 : 0;
+      const int32_t x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
+      const int32_t row = (z * c.dim[1] + y) * c.dim[0];
+      const int32_t j1 = c.cstart[row + x1 + 1];
+      const float4 *fr = (const float4 *)c.frec;
+      for (int32_t j0 = c.cstart[row + x0]; j0 < j1; j0 += 4) {
+        // four independent prefilter loads in flight
+        float4 flo[4], fhi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int32_t jj = (j0 + q < j1) ? j0 + q : j1 - 1;
+          flo[q] = fr[2 * jj];
+          fhi[q] = fr[2 * jj + 1];
+        }
+        unsigned hit = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const bool ok = (j0 + q < j1) && (__float_as_int(flo[q].w) != sX) &&
+                          (xfh.x >= flo[q].x) && (fhi[q].x >= xfl.x) &&
+                          (xfh.y >= flo[q].y) && (fhi[q].y >= xfl.y) &&
+                          (xfh.z >= flo[q].z) && (fhi[q].z >= xfl.z);
+          hit |= ok ? (1u << q) : 0u;
+        }
+        while (hit) {
+          const int q = __builtin_ctz(hit);
+          hit &= hit - 1u;
+          const CRec yr = rec[j0 + q];
+          bool ov = true;
+          for (int i = 0; i < 3; i++) ov = ov && (x.hi[i] >= yr.lo[i]) && (yr.hi[i] >= x.lo[i]);
+          if (!ov) continue;
+          const int32_t Y = yr.e, sY = yr.s;
+          const int32_t ya = c.en0[Y], yb = c.en1[Y];
+          if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
+              node_in_strut(c, yb, sX))
+            continue;
+          fn(Y, sY, ya, yb, sX < sY);
+        }
+      }
+    }
+  }
+}
+
 // One thread per element, in cell order (thread k owns the element of
 // record k).  MODE 0: contact forces on the element's own nodes (+ counts of
 // active pairs and candidates, each pair counted on its A side).  MODE 1:
@@ -241,102 +306,50 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
                                                 const int32_t *step) {
   const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
   unsigned long long nact = 0, ncand = 0;
-  const CRec *rec = (const CRec *)c.rec;
   if (k < c.E) {
-    const CRec x = rec[k];
+    const CRec x = ((const CRec *)c.rec)[k];
     const int32_t X = x.e, sX = x.s;
-    const int32_t xa = c.en0[X], xb = c.en1[X];
     double pX[2][3];
-    elem_pos(d, xa, xb, pX);
-    const int32_t cx = x.cell % c.dim[0];
-    const int32_t cy = (x.cell / c.dim[0]) % c.dim[1];
-    const int32_t cz = x.cell / (c.dim[0] * c.dim[1]);
-    const float4 xfl = ((const float4 *)c.frec)[2 * k], xfh = ((const float4 *)c.frec)[2 * k + 1];
+    elem_pos(d, c.en0[X], c.en1[X], pX);
     double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
     bool any = false;
     int32_t nlist = 0;
     const int32_t off = (MODE == 2) ? c.loff[k] : 0;
-    for (int dz = -1; dz <= 1; dz++) {
-      const int32_t z = cz + dz;
-      if (z < 0 || z >= c.dim[2]) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        const int32_t y = cy + dy;
-        if (y < 0 || y >= c.dim[1]) continue;
-        // the three cells x-1..x+1 of a row are contiguous in cell order
-        const int32_t x0 = cx > 0 ? cx - 1 : 0;
-        const int32_t x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
-        const int32_t row = (z * c.dim[1] + y) * c.dim[0];
-        const int32_t j1 = c.cstart[row + x1 + 1];
-        const float4 *fr = (const float4 *)c.frec;
-        for (int32_t j0 = c.cstart[row + x0]; j0 < j1; j0 += 4) {
-          // four independent prefilter loads in flight
-          float4 flo[4], fhi[4];
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int32_t jj = (j0 + q < j1) ? j0 + q : j1 - 1;
-            flo[q] = fr[2 * jj];
-            fhi[q] = fr[2 * jj + 1];
+    scan_candidates(c, k, x, [&](int32_t Y, int32_t sY, int32_t ya, int32_t yb, bool XisA) {
+      if (XisA) ncand++;
+      if (MODE != 0 && kind == LIST_CAND) {
+        if (XisA) {
+          if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
+          nlist++;
+        }
+        return;
+      }
+      if (MODE != 0 && !XisA) return;
+      if (kind != LIST_CLOSED && blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
+      double pY[2][3];
+      elem_pos(d, ya, yb, pY);
+      const double R = c.crad[sX] + c.crad[sY];
+      if (MODE == 0) {
+        // this pair's contribution summed from zero, then added
+        double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+        const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                                 : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+        if (closed) {
+          any = true;
+          for (int i = 0; i < 3; i++) {
+            fa[i] += g0[i];
+            fb[i] += g1[i];
           }
-          unsigned hit = 0u;
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const bool ok = (j0 + q < j1) && (__float_as_int(flo[q].w) != sX) &&
-                            (xfh.x >= flo[q].x) && (fhi[q].x >= xfl.x) &&
-                            (xfh.y >= flo[q].y) && (fhi[q].y >= xfl.y) &&
-                            (xfh.z >= flo[q].z) && (fhi[q].z >= xfl.z);
-            hit |= ok ? (1u << q) : 0u;
-          }
-          while (hit) {
-          const int q = __builtin_ctz(hit);
-          hit &= hit - 1u;
-          const int32_t j = j0 + q;
-          const CRec yr = rec[j];
-          bool ov = true;
-          for (int i = 0; i < 3; i++) ov = ov && (x.hi[i] >= yr.lo[i]) && (yr.hi[i] >= x.lo[i]);
-          if (!ov) continue;
-          const int32_t Y = yr.e, sY = yr.s;
-          const int32_t ya = c.en0[Y], yb = c.en1[Y];
-          if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
-              node_in_strut(c, yb, sX))
-            continue;
-          const bool XisA = sX < sY;
-          if (XisA) ncand++;
-          if (MODE != 0 && kind == LIST_CAND) {
-            if (XisA) {
-              if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
-              nlist++;
-            }
-            continue;
-          }
-          if (MODE != 0 && !XisA) continue;
-          if (kind != LIST_CLOSED && blacklisted(c, XisA ? X : Y, XisA ? Y : X)) continue;
-          double pY[2][3];
-          elem_pos(d, ya, yb, pY);
-          const double R = c.crad[sX] + c.crad[sY];
-          if (MODE == 0) {
-            // this pair's contribution summed from zero, then added
-            double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
-            const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                                     : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
-            if (closed) {
-              any = true;
-              for (int i = 0; i < 3; i++) {
-                fa[i] += g0[i];
-                fb[i] += g1[i];
-              }
-              if (XisA) nact++;
-            }
-          } else {
-            double g0[3], g1[3];
-            if (cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)) {
-              if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
-              nlist++;
-            }
-          }
-          }
+          if (XisA) nact++;
+        }
+      } else {
+        double g0[3], g1[3];
+        if (cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)) {
+          if (MODE == 2) { c.la[off + nlist] = X; c.lb[off + nlist] = Y; }
+          nlist++;
         }
       }
-    }
+    });
     if (MODE == 0) {
       if (any) {
         const int32_t slot = atomicAdd(c.slot_ctr, 1);
@@ -371,6 +384,47 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
       if (rc[0]) atomicAdd(cnt_cand + st, rc[0]);
     }
   }
+}
+
+// Adjoint of the contact forces of one step, one thread per element X in
+// cell order: for every closed pair, the pair VJP from X's side (adjoints of
+// both elements' nodal forces) adds to X's element-end displacement adjoints
+// and to X's contact-distance adjoint.  Each element is owned by one thread,
+// so the additions are plain and ordered.
+__global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, AdjDev a) {
+  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (k >= c.E) return;
+  const int64_t N = c.N, E = c.E;
+  const CRec x = ((const CRec *)c.rec)[k];
+  const int32_t X = x.e, sX = x.s;
+  const int32_t xa = c.en0[X], xb = c.en1[X];
+  double pX[2][3], fbX[6];
+  elem_pos(d, xa, xb, pX);
+  for (int i = 0; i < 3; i++) {
+    fbX[i] = a.fbar[i * N + xa];
+    fbX[3 + i] = a.fbar[i * N + xb];
+  }
+  double g[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  bool any = false;
+  scan_candidates(c, k, x, [&](int32_t Y, int32_t sY, int32_t ya, int32_t yb, bool XisA) {
+    if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
+    double pY[2][3], fbY[6], o[7];
+    elem_pos(d, ya, yb, pY);
+    for (int i = 0; i < 3; i++) {
+      fbY[i] = a.fbar[i * N + ya];
+      fbY[3 + i] = a.fbar[i * N + yb];
+    }
+    if (contact_pair_vjp(pX, pY, XisA, c.crad[sX] + c.crad[sY], c.kS, c.kN, fbX, fbY, o)) {
+      any = true;
+      for (int i = 0; i < 7; i++) g[i] += o[i];
+    }
+  });
+  if (!any) return;
+  for (int i = 0; i < 3; i++) {
+    a.ebuf[i * E + X] += g[i];
+    a.ebuf[(7 + i) * E + X] += g[3 + i];
+  }
+  a.crbar[X] += g[6];
 }
 
 // ------------------------------------------------------------------
@@ -619,5 +673,21 @@ int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a,
     if (a) a[i] = A[i];
     if (b) b[i] = B[i];
   }
+  return DL_OK;
+}
+
+int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s) {
+  broadphase(cs, d, s);
+  hipLaunchKernelGGL(kc_adj_narrow, dim3(cs.nblk_e), dim3(CB), 0, s, cs.c, d, a);
+  CHK(hipGetLastError());
+  return DL_OK;
+}
+
+int contact_set_radius(ContactState &cs, const double *r, hipStream_t s) {
+  const int32_t R = cs.c.R;
+  for (int32_t i = 0; i < R; i++)
+    if (!(r[i] > 0.0)) return dl_set_error(DL_ERR_INPUT, "contact radius must be > 0");
+  CHK(hipStreamSynchronize(s));
+  CHK(hipMemcpy((void *)cs.c.crad, r, sizeof(double) * R, hipMemcpyHostToDevice));
   return DL_OK;
 }

@@ -7,36 +7,46 @@
 // part) and computeNTS_ (:1157-1208, force part), in the same operation
 // order (no FMA contraction) so that the closed/open decision of every pair
 // is the same bit pattern as the oracle's.
+//
+// Every function is a template on the scalar type T: the step instantiates
+// it with double; the adjoint instantiates it with forward-mode dual numbers
+// (dl_dual.h), whose value parts follow the same operations bit for bit, so
+// both see the same branch of every pair.
 #ifndef DL_CONTACT_MATH_H
 #define DL_CONTACT_MATH_H
 
 #include "dl_device.h"
 
-// StdLine::containsLocalPoint(u): -1 <= u <= 1 (NaN is outside)
-DL_HD bool cm_contains(double u) { return u >= -1.0 && u <= 1.0; }
+DL_HD double dl_val(double x) { return x; }
 
-DL_HD double cm_dot(const double a[3], const double b[3]) {
+// StdLine::containsLocalPoint(u): -1 <= u <= 1 (NaN is outside)
+template <typename T>
+DL_HD bool cm_contains(const T &u) { return u >= -1.0 && u <= 1.0; }
+
+template <typename T>
+DL_HD T cm_dot(const T a[3], const T b[3]) {
 #pragma clang fp contract(off)
-  double s = a[0] * b[0];
+  T s = a[0] * b[0];
   s += a[1] * b[1];
   s += a[2] * b[2];
   return s;
 }
 
-DL_HD void cm_point(double u, const double p0[3], const double p1[3], double x[3]) {
+template <typename T>
+DL_HD void cm_point(const T &u, const T p0[3], const T p1[3], T x[3]) {
 #pragma clang fp contract(off)
-  const double N0 = 0.5 * (1. - u), N1 = 0.5 * (1. + u);
+  const T N0 = 0.5 * (1. - u), N1 = 0.5 * (1. + u);
   for (int i = 0; i < 3; i++) {
-    double s = p0[i] * N0;
+    T s = p0[i] * N0;
     s += p1[i] * N1;
     x[i] = s;
   }
 }
 
-DL_HD void cm_closest_points(const double pA[2][3], const double pB[2][3], double *uA,
-                             double *uB) {
+template <typename T>
+DL_HD void cm_closest_points(const T pA[2][3], const T pB[2][3], T *uA, T *uB) {
 #pragma clang fp contract(off)
-  double bA[3], bB[3], tA[3], tB[3], d[3], vA[3], vB[3];
+  T bA[3], bB[3], tA[3], tB[3], d[3], vA[3], vB[3];
   for (int i = 0; i < 3; i++) {
     bA[i] = pA[1][i] + pA[0][i];
     bB[i] = pB[1][i] + pB[0][i];
@@ -44,19 +54,20 @@ DL_HD void cm_closest_points(const double pA[2][3], const double pB[2][3], doubl
     tB[i] = pB[1][i] - pB[0][i];
     d[i] = bB[i] - bA[i];
   }
-  const double tBtA = cm_dot(tB, tA), tBtB = cm_dot(tB, tB), tAtA = cm_dot(tA, tA);
+  const T tBtA = cm_dot(tB, tA), tBtB = cm_dot(tB, tB), tAtA = cm_dot(tA, tA);
   for (int i = 0; i < 3; i++) {
     vA[i] = tB[i] * tBtA - tA[i] * tBtB;
     vB[i] = tA[i] * tBtA - tB[i] * tAtA;
   }
-  const double den = tBtB * tAtA - tBtA * tBtA;
+  const T den = tBtB * tAtA - tBtA * tBtA;
   *uA = -1. * cm_dot(d, vA) / den;
   *uB = +1. * cm_dot(d, vB) / den;
 }
 
-DL_HD double cm_closest_point(const double s[3], const double pM[2][3]) {
+template <typename T>
+DL_HD T cm_closest_point(const T s[3], const T pM[2][3]) {
 #pragma clang fp contract(off)
-  double bM[3], tM[3];
+  T bM[3], tM[3];
   for (int i = 0; i < 3; i++) {
     bM[i] = pM[1][i] + pM[0][i];
     tM[i] = pM[1][i] - pM[0][i];
@@ -65,25 +76,27 @@ DL_HD double cm_closest_point(const double s[3], const double pM[2][3]) {
 }
 
 // jem::isTiny(d): |d| < DBL_MIN
-DL_HD bool cm_tiny(double d) { return fabs(d) < 2.2250738585072014e-308; }
+template <typename T>
+DL_HD bool cm_tiny(const T &d) { return fabs(dl_val(d)) < 2.2250738585072014e-308; }
 
 // segment-to-segment; f[12] += (A0, A1, B0, B1) when f != nullptr
-DL_HD bool cm_sts(const double pA[2][3], const double pB[2][3], double uA, double uB, double R,
-                  double kap, double *f) {
+template <typename T>
+DL_HD bool cm_sts(const T pA[2][3], const T pB[2][3], const T &uA, const T &uB, const T &R,
+                  double kap, T *f) {
 #pragma clang fp contract(off)
-  double xA[3], xB[3], d[3];
+  T xA[3], xB[3], d[3];
   cm_point(uA, pA[0], pA[1], xA);
   cm_point(uB, pB[0], pB[1], xB);
   for (int i = 0; i < 3; i++) d[i] = xB[i] - xA[i];
-  const double dist = sqrt(cm_dot(d, d));
+  const T dist = sqrt(cm_dot(d, d));
   if (cm_tiny(dist)) return false;
   if (dist > R) return false;
   if (!f) return true;
-  double n[3];
+  T n[3];
   for (int i = 0; i < 3; i++) n[i] = d[i] / dist;
-  const double NA0 = 0.5 * (1. - uA), NA1 = 0.5 * (1. + uA);
-  const double NB0 = 0.5 * (1. - uB), NB1 = 0.5 * (1. + uB);
-  const double sc = kap * (dist - R);
+  const T NA0 = 0.5 * (1. - uA), NA1 = 0.5 * (1. + uA);
+  const T NB0 = 0.5 * (1. - uB), NB1 = 0.5 * (1. + uB);
+  const T sc = kap * (dist - R);
   for (int i = 0; i < 3; i++) {
     f[0 + i] += sc * ((-1. * NA0) * n[i]);
     f[3 + i] += sc * ((-1. * NA1) * n[i]);
@@ -94,20 +107,20 @@ DL_HD bool cm_sts(const double pA[2][3], const double pB[2][3], double uA, doubl
 }
 
 // node-to-segment; f[9] += (slave, M0, M1) when f != nullptr
-DL_HD bool cm_nts(const double s[3], const double pM[2][3], double uM, double R, double kap,
-                  double *f) {
+template <typename T>
+DL_HD bool cm_nts(const T s[3], const T pM[2][3], const T &uM, const T &R, double kap, T *f) {
 #pragma clang fp contract(off)
-  double xM[3], d[3];
+  T xM[3], d[3];
   cm_point(uM, pM[0], pM[1], xM);
   for (int i = 0; i < 3; i++) d[i] = xM[i] - s[i];
-  const double dist = sqrt(cm_dot(d, d));
+  const T dist = sqrt(cm_dot(d, d));
   if (cm_tiny(dist)) return false;
   if (dist > R) return false;
   if (!f) return true;
-  double n[3];
+  T n[3];
   for (int i = 0; i < 3; i++) n[i] = (s[i] - xM[i]) / dist;
-  const double sc = kap * (dist - R);
-  const double a = -.5 * (1. - uM), b = -.5 * (1. + uM);
+  const T sc = kap * (dist - R);
+  const T a = -.5 * (1. - uM), b = -.5 * (1. + uM);
   for (int i = 0; i < 3; i++) {
     f[i] += sc * n[i];
     f[3 + i] += sc * (a * n[i]);
@@ -120,14 +133,14 @@ DL_HD bool cm_nts(const double s[3], const double pM[2][3], double uM, double R,
 // blacklist filters.  Returns whether the contact is closed.  When F is true
 // the translational contributions are added: SIDE 0 -> (A0, A1) into (f0, f1),
 // SIDE 1 -> (B0, B1) into (f0, f1).  All indices are static so nothing spills.
-template <bool F, int SIDE>
-DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R, double kS,
-                          double kN, double f0[3], double f1[3]) {
-  double uA, uB;
+template <bool F, int SIDE, typename T>
+DL_HD bool cm_narrowphase(const T pA[2][3], const T pB[2][3], const T &R, double kS, double kN,
+                          T f0[3], T f1[3]) {
+  T uA, uB;
   cm_closest_points(pA, pB, &uA, &uB);
   const bool inA = cm_contains(uA), inB = cm_contains(uB);
   if (inA && inB) {
-    double g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    T g[12] = {};
     if (!cm_sts(pA, pB, uA, uB, R, kS, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
@@ -139,19 +152,19 @@ DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R
   if (!inA && inB) {
     const bool i1 = !(uA < -1.);
     // RodContactModel.cpp:575 projects node iA of B (not of A) onto A
-    const double sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
+    const T sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
                           i1 ? pB[1][2] : pB[0][2]};
-    const double sA[3] = {i1 ? pA[1][0] : pA[0][0], i1 ? pA[1][1] : pA[0][1],
+    const T sA[3] = {i1 ? pA[1][0] : pA[0][0], i1 ? pA[1][1] : pA[0][1],
                           i1 ? pA[1][2] : pA[0][2]};
-    const double u = cm_closest_point(sB, pA);
+    const T u = cm_closest_point(sB, pA);
     if (!cm_contains(u)) return false;
-    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    T g[9] = {};
     if (!cm_nts(sA, pB, u, R, kN, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
         if (SIDE == 0) {
-          f0[i] += i1 ? 0.0 : g[i];
-          f1[i] += i1 ? g[i] : 0.0;
+          f0[i] += i1 ? T(0.0) : g[i];
+          f1[i] += i1 ? g[i] : T(0.0);
         } else {
           f0[i] += g[3 + i];
           f1[i] += g[6 + i];
@@ -161,17 +174,17 @@ DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R
   }
   if (!inB && inA) {
     const bool i1 = !(uB < -1.);
-    const double sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
+    const T sB[3] = {i1 ? pB[1][0] : pB[0][0], i1 ? pB[1][1] : pB[0][1],
                           i1 ? pB[1][2] : pB[0][2]};
-    const double u = cm_closest_point(sB, pA);
+    const T u = cm_closest_point(sB, pA);
     if (!cm_contains(u)) return false;
-    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    T g[9] = {};
     if (!cm_nts(sB, pA, u, R, kN, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
         if (SIDE == 1) {
-          f0[i] += i1 ? 0.0 : g[i];
-          f1[i] += i1 ? g[i] : 0.0;
+          f0[i] += i1 ? T(0.0) : g[i];
+          f1[i] += i1 ? g[i] : T(0.0);
         } else {
           f0[i] += g[3 + i];
           f1[i] += g[6 + i];
@@ -183,9 +196,9 @@ DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R
   bool closed = false;
 #pragma unroll
   for (int iA = 0; iA < 2; iA++) {
-    const double u = cm_closest_point(pA[iA], pB);
+    const T u = cm_closest_point(pA[iA], pB);
     if (!cm_contains(u)) continue;
-    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    T g[9] = {};
     if (cm_nts(pA[iA], pB, u, R, kN, F ? g : nullptr)) {
       if (F)
         for (int i = 0; i < 3; i++) {
@@ -201,9 +214,9 @@ DL_HD bool cm_narrowphase(const double pA[2][3], const double pB[2][3], double R
   }
 #pragma unroll
   for (int iB = 0; iB < 2; iB++) {
-    const double u = cm_closest_point(pB[iB], pA);
+    const T u = cm_closest_point(pB[iB], pA);
     if (!cm_contains(u)) continue;
-    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    T g[9] = {};
     if (cm_nts(pB[iB], pA, u, R, kN, F ? g : nullptr)) {
       if (F)
         for (int i = 0; i < 3; i++) {

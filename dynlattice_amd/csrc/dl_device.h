@@ -306,4 +306,47 @@ DL_HD void solve3_masked(const double J[9], const double f[3], unsigned mask, co
     if (mask & (1u << j)) a[j] = c[j];
 }
 
+// One node's central-difference update (ExplicitModule.cpp:339-392,
+// LeapFrogModule.cpp:44-84): a = M^-1 (f_ext - f - gyro) with constraints,
+// v += dt a, u += dt v, R <- exp(dt w) R (SO(3)) or theta += dt w (additive).
+//   f   in: element (+ contact) internal force; out: plus the gyroscopic term
+//   u, v, r in/out (r: quaternion w,x,y,z in SO(3) mode, rotation vector else)
+//   q   the node rotation as a quaternion (SO(3): r itself; additive: exp(r))
+template <bool SO3>
+DL_HD void node_advance(double f[6], const Quat q, double u[3], double v[6], double r[4],
+                        double mt, const double J0[6], double gm, unsigned mask,
+                        const double ac[6], const double fe[6], double dt) {
+  double R[9], J[9];
+  qmat(q, R);
+  spatial_inertia(R, J0, J);
+  if (SO3) {
+    // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
+    const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
+    const double jw1 = J[3] * v[3] + J[4] * v[4] + J[5] * v[5];
+    const double jw2 = J[6] * v[3] + J[7] * v[4] + J[8] * v[5];
+    f[3] += gm * (v[4] * jw2 - v[5] * jw1);
+    f[4] += gm * (v[5] * jw0 - v[3] * jw2);
+    f[5] += gm * (v[3] * jw1 - v[4] * jw0);
+  }
+  double fr[6];
+  for (int c = 0; c < 6; c++) fr[c] = fe[c] - f[c];
+  double a[6];
+  const double imt = 1.0 / mt;
+  for (int c = 0; c < 3; c++) a[c] = (mask & (1u << c)) ? ac[c] : imt * fr[c];
+  if (SO3) {
+    solve3_masked(J, fr + 3, mask >> 3, ac + 3, a + 3);
+  } else {
+    for (int c = 0; c < 3; c++) a[3 + c] = (mask & (8u << c)) ? ac[3 + c] : (1.0 / J[4 * c]) * fr[3 + c];
+  }
+  for (int c = 0; c < 6; c++) v[c] = v[c] + dt * a[c];
+  for (int c = 0; c < 3; c++) u[c] = u[c] + dt * v[c];
+  if (SO3) {
+    const Quat dq = qexp(dt * v[3], dt * v[4], dt * v[5]);
+    const Quat qn = qnormalize(qmul(dq, q));
+    r[0] = qn.w; r[1] = qn.x; r[2] = qn.y; r[3] = qn.z;
+  } else {
+    for (int c = 0; c < 3; c++) r[c] = r[c] + dt * v[3 + c];
+  }
+}
+
 #endif

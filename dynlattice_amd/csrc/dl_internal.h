@@ -72,6 +72,8 @@ struct DevData {
   double *rbuf;         // reaction contributions [n_react]
   double *dbuf;         // [R] per-strut dissipation increment
   int32_t *err;         // device error flag
+  // adjoint tape row of the current step ([19][N]: u 3, rot 4, v 6, f 6) or null
+  double *tape;
 };
 
 struct StepParams {
@@ -81,6 +83,26 @@ struct StepParams {
 };
 
 enum { MODE_STEP = 0, MODE_FORCE = 1 };
+
+// Reverse sweep of the explicit step (SO(3) mode, elastic material).
+// Adjoints ("bar") of the state after the step being reversed are turned
+// into those of the state before it; parameter adjoints accumulate.
+struct AdjDev {
+  const double *tape;   // [19][N] start state of the step: u 3, rot 4, v 6, f 6
+  double *ubar;         // [3][N]
+  double *qbar;         // [4][N]
+  double *vbar;         // [6][N]
+  double *fbar;         // [6][N] adjoint of the nodal internal force of the step
+  double *ebuf;         // [14][E] per element end: ubar_a 3, qbar_a 4, ubar_b 3, qbar_b 4
+  double *Cbar;         // [6][E] stiffness adjoint, summed over steps
+  double *mtbar;        // [N]
+  double *J0bar;        // [9][N] full 3x3
+  double *crbar;        // [E] contact-distance adjoint, summed over steps
+  const int32_t *nptr;  // [N+1] node -> element ends
+  const int32_t *nlist; // (element << 1) | end
+  const int32_t *slot_group; // reaction slot -> group
+  const double *rbar;   // [groups] objective adjoint of this step's reactions
+};
 
 // launchers (dl_kernels.hip)
 void launch_init_strain(const DevData &d, hipStream_t s);
@@ -96,5 +118,9 @@ void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *edis
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
                      hipStream_t s);
+// adjoint (dl_adjoint.hip): node update, element force, gather to nodes
+void launch_adj_node(const DevData &d, const AdjDev &a, double dt, hipStream_t s);
+void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
+void launch_adj_gather(const DevData &d, const AdjDev &a, hipStream_t s);
 
 #endif

@@ -38,6 +38,7 @@ __device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4
 // a = M^-1 (f_ext - f) with constraints, then v += dt a, u += dt v,
 // R <- exp(dt w) R (SO(3)) or theta += dt w (additive).
 // f: element (+ contact) internal force of the node at the step's start state.
+// With a tape, the node's start state and f are recorded for the adjoint.
 template <bool SO3>
 __device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
                                             double f[6], const Quat q, const double r[4]) {
@@ -57,54 +58,41 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
     const int32_t rs = d.bc_rslot[b];
     if (rs >= 0) d.rbuf[rs] = f[d.react_dof];
   }
-  double v[6];
+  double v[6], u[3], rr[4];
 #pragma unroll
   for (int c = 0; c < 6; c++) v[c] = d.v[c * N + n];
+#pragma unroll
+  for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
+#pragma unroll
+  for (int c = 0; c < 4; c++) rr[c] = r[c];
+  if (d.tape) {
+    // adjoint tape row: u(3) rot(4) v(6) f(6), component-major
+    double *t = d.tape;
+#pragma unroll
+    for (int c = 0; c < 3; c++) t[c * N + n] = u[c];
+#pragma unroll
+    for (int c = 0; c < 4; c++) t[(3 + c) * N + n] = rr[c];
+#pragma unroll
+    for (int c = 0; c < 6; c++) t[(7 + c) * N + n] = v[c];
+#pragma unroll
+    for (int c = 0; c < 6; c++) t[(13 + c) * N + n] = f[c];
+  }
   const double mt = d.mt[n];
   double J0[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
-  double R[9], J[9];
-  qmat(q, R);
-  spatial_inertia(R, J0, J);
-  if (SO3) {
-    // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
-    const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
-    const double jw1 = J[3] * v[3] + J[4] * v[4] + J[5] * v[5];
-    const double jw2 = J[6] * v[3] + J[7] * v[4] + J[8] * v[5];
-    const double gm = d.gm[n];
-    f[3] += gm * (v[4] * jw2 - v[5] * jw1);
-    f[4] += gm * (v[5] * jw0 - v[3] * jw2);
-    f[5] += gm * (v[3] * jw1 - v[4] * jw0);
-  }
-  double fr[6];
-#pragma unroll
-  for (int c = 0; c < 6; c++) fr[c] = fe[c] - f[c];
-  double a[6];
-  const double imt = 1.0 / mt;
-#pragma unroll
-  for (int c = 0; c < 3; c++) a[c] = (mask & (1u << c)) ? ac[c] : imt * fr[c];
-  if (SO3) {
-    solve3_masked(J, fr + 3, mask >> 3, ac + 3, a + 3);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; c++) a[3 + c] = (mask & (8u << c)) ? ac[3 + c] : (1.0 / J[4 * c]) * fr[3 + c];
-  }
   const double dt = d.ndt ? d.ndt[n] : sp.dt;
+  node_advance<SO3>(f, q, u, v, rr, mt, J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, dt);
 #pragma unroll
-  for (int c = 0; c < 6; c++) {
-    v[c] = v[c] + dt * a[c];
-    d.v[c * N + n] = v[c];
-  }
+  for (int c = 0; c < 6; c++) d.v[c * N + n] = v[c];
 #pragma unroll
-  for (int c = 0; c < 3; c++) d.u[c * N + n] = d.u[c * N + n] + dt * v[c];
+  for (int c = 0; c < 3; c++) d.u[c * N + n] = u[c];
   if (SO3) {
-    const Quat dq = qexp(dt * v[3], dt * v[4], dt * v[5]);
-    const Quat qn = qnormalize(qmul(dq, q));
-    d.rot[n] = qn.w; d.rot[N + n] = qn.x; d.rot[2 * N + n] = qn.y; d.rot[3 * N + n] = qn.z;
+#pragma unroll
+    for (int c = 0; c < 4; c++) d.rot[c * N + n] = rr[c];
   } else {
 #pragma unroll
-    for (int c = 0; c < 3; c++) d.rot[c * N + n] = r[c] + dt * v[3 + c];
+    for (int c = 0; c < 3; c++) d.rot[c * N + n] = rr[c];
   }
 }
 

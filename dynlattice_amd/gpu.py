@@ -33,7 +33,11 @@ EXPORTS = (
     "dl_contact_pairs", "dl_contact_candidates", "dl_plastic_active", "dl_plastic_state",
     "dl_phase_time", "dl_set_timing", "dl_sync", "dl_last_error",
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
+    "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward",
 )
+
+#: section fields of dl_sections / dl_section_grads, in ABI order
+SECTION_FIELDS = ("area", "I1", "I2", "J", "shear_factor", "young", "shear_modulus", "density")
 
 dp = C.POINTER(C.c_double)
 ip32 = C.POINTER(C.c_int32)
@@ -83,6 +87,10 @@ class dl_params(C.Structure):
                 ("e_diss", C.c_double)]
 
 
+class dl_section_grads(C.Structure):
+    _fields_ = [(f, dp) for f in SECTION_FIELDS] + [("contact_radius", dp)]
+
+
 class dl_step_out(C.Structure):
     _fields_ = [("reaction", C.c_double), ("e_diss", C.c_double), ("n_contacts", C.c_int64),
                 ("n_candidates", C.c_int64), ("steps_done", C.c_int64)]
@@ -121,6 +129,9 @@ def load():
     L.dl_set_node_dt.argtypes = [C.c_void_p, dp, C.c_int64]
     L.dl_set_graph.argtypes = [C.c_void_p, C.c_int32]
     L.dl_get_group_history.argtypes = [C.c_void_p, dp, C.c_int64, ip64]
+    L.dl_set_sections.argtypes = [C.c_void_p, C.POINTER(dl_sections), dp]
+    L.dl_adjoint_forward.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int32, dp]
+    L.dl_adjoint_backward.argtypes = [C.c_void_p, dp, C.POINTER(dl_section_grads)]
     _lib = L
     return L
 
@@ -338,6 +349,44 @@ class GpuModel:
 
     def contact_pairs(self):
         return self._pairs(self.L.dl_contact_pairs)
+
+    # --- design sensitivities -------------------------------------------
+    def set_sections(self, sections=None, contact_radius=None):
+        """Replace the per-strut section data (a lattice.Sections) and/or the
+        contact radii; call INIT afterwards to refresh the contact blacklist."""
+        sec = None
+        keep = []
+        if sections is not None:
+            arrs = [_c(getattr(sections, f)) for f in SECTION_FIELDS]
+            keep += arrs
+            sec = dl_sections(*[_p(a) for a in arrs], sections.inertia_correct)
+        cr = _c(contact_radius) if contact_radius is not None else None
+        self._check(self.L.dl_set_sections(self.h, C.byref(sec) if sec is not None else None,
+                                           _p(cr)))
+
+    def adjoint_forward(self, nsteps, dt=None, checkpoint_every=100):
+        """nsteps explicit steps with checkpoints; returns the reaction
+        history [nsteps, n_groups]."""
+        dt = self.p.dt if dt is None else dt
+        ng = max(getattr(self, "_ngroups", 1), 1)
+        r = np.zeros(nsteps * ng)
+        self._check(self.L.dl_adjoint_forward(self.h, int(nsteps), float(dt),
+                                              int(checkpoint_every), _p(r)))
+        self._adj_steps = int(nsteps)
+        return r.reshape(nsteps, ng)
+
+    def adjoint_backward(self, reaction_bar):
+        """Gradients of the section data (dict of [n_struts] arrays, plus
+        contact_radius) for the objective adjoint reaction_bar [nsteps, n_groups]."""
+        ng = max(getattr(self, "_ngroups", 1), 1)
+        rb = _c(reaction_bar).reshape(-1)
+        if rb.size != self._adj_steps * ng:
+            raise ValueError(f"reaction_bar must have {self._adj_steps} x {ng} entries")
+        R = self.p.n_struts
+        out = {f: np.zeros(R) for f in SECTION_FIELDS + ("contact_radius",)}
+        g = dl_section_grads(*[_p(out[f]) for f in SECTION_FIELDS + ("contact_radius",)])
+        self._check(self.L.dl_adjoint_backward(self.h, _p(rb), C.byref(g)))
+        return out
 
     def contact_candidates(self):
         return self._pairs(self.L.dl_contact_candidates)

@@ -328,6 +328,25 @@ def bcc_c5(n: int = 20) -> Problem:
     return p
 
 
+def bcc_adjoint_check(radius=None, dt=None) -> Problem:
+    """Small adjoint check case: 2^3 BCC cells of 10 mm, 3 elements per
+    strut (64 struts, 192 elements), r = 1.2 mm (1 + 0.1 U[-1, 1]), seed 5
+    (or the given per-strut radii), E = 70 GPa, nu = 0.3, rho = 2700,
+    elastic, penalty contact 1e6 N/m with contact radius = strut radius,
+    bottom clamped, top at -20 m/s.  Within 1600 steps about 30 contact pairs
+    close."""
+    n, m, cell = 2, 3, 10e-3
+    R = 8 * n ** 3
+    r = 1.2e-3 * (1.0 + 0.1 * (2.0 * uniform01(5, R) - 1.0)) if radius is None else \
+        np.asarray(radius, np.float64)
+    p = bcc_lattice(n, m, cell, r, 70e9, 0.3, 2700.0)
+    p.contact = Contact(radius=r.copy(), penalty_sts=1e6, penalty_nts=1e6)
+    crush_bcs(p, -20.0)
+    p.dt = stable_dt(p, 0.5) if dt is None else dt
+    p.name = "bcc-2x2x2-m3-elastic-contact"
+    return p
+
+
 def c4_instance(i: int) -> Problem:
     """Design-sweep instance i (seed 1000 + i): n ~ U{4..16}, m ~ U{2..6},
     r ~ U[0.2, 0.8] mm, yield ~ U[150, 350] MPa; elastoplastic BCC crush."""

@@ -224,6 +224,39 @@ int dl_plastic_state(dl_ctx *ctx, double *eps_p, double *alpha, int64_t n_elems)
 int dl_phase_time(dl_ctx *ctx, const char *phase, double *ms, int64_t *count);
 int dl_set_timing(dl_ctx *ctx, int32_t enabled);
 
+/* Replaces the per-strut section data (and, with contact, the per-strut
+ * contact radii; either may be NULL) of an existing context: stiffness,
+ * hardening and lumped mass are rebuilt as dl_create builds them, i.e. the
+ * ElasticRodMaterial configuration (ElasticRodMaterial.cpp:65-228) and
+ * ExplicitModule's mass set-up (ExplicitModule.cpp:58-196).  The contact
+ * blacklist keeps the radii of the last INIT. */
+int dl_set_sections(dl_ctx *ctx, const dl_sections *sec, const double *contact_radius);
+
+/* Gradients of the per-strut section data (each array [n_struts] or NULL to
+ * skip it). */
+typedef struct {
+  double *area, *I1, *I2, *J, *shear_factor, *young, *shear_modulus, *density;
+  double *contact_radius;
+} dl_section_grads;
+
+/* Adjoint design gradients (no counterpart in the reference, which has no
+ * sensitivities).  dl_adjoint_forward runs nsteps explicit steps like
+ * dl_explicit_step (same histories; reaction[nsteps][n_groups] returned when
+ * not NULL) and keeps a checkpoint of the state every checkpoint_every
+ * steps.  dl_adjoint_backward then takes the adjoint of an objective with
+ * respect to that reaction history, reaction_bar[nsteps][n_groups] (e.g.
+ * dt * v_plate for the absorbed energy, a one-hot row for the peak), and
+ * returns the gradient of the objective with respect to the section data by
+ * a reverse sweep: every segment is recomputed from its checkpoint, then
+ * reversed through hand-written adjoint kernels of the node update, the
+ * element internal force (elastic material) and the contact pair forces.
+ * SO(3) rotation update and elastic material only.  The forward run's final
+ * state is restored afterwards; backward may be called again with another
+ * reaction_bar until the state, boundary data or sections change. */
+int dl_adjoint_forward(dl_ctx *ctx, int64_t nsteps, double dt, int32_t checkpoint_every,
+                       double *reaction);
+int dl_adjoint_backward(dl_ctx *ctx, const double *reaction_bar, dl_section_grads *grads);
+
 /* Wait for the context's stream. */
 int dl_sync(dl_ctx *ctx);
 
