@@ -252,6 +252,7 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
   for (int64_t seg = nseg - 1; seg >= 0; seg--) {
     const int64_t t0 = seg * A.every, len = std::min<int64_t>(A.every, A.nsteps - t0);
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, false))) return rc;
+    if (c->contact) contact_invalidate(c->cs, s);
     // recompute the segment, taping every step's start state and forces
     for (int64_t j = 0; j < len; j++) {
       DevData dj = c->d;
@@ -276,6 +277,7 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
     HIPCHK(hipGetLastError());
   }
   if ((rc = state_copy(c, A.final_state, false))) return rc;
+  if (c->contact) contact_invalidate(c->cs, s);
   if ((rc = check_err(c))) return rc;
   return compose_gradients(c, grads);
 }

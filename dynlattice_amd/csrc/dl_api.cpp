@@ -456,6 +456,7 @@ extern "C" int dl_set_sections(dl_ctx *c, const dl_sections *sec, const double *
   if (contact_radius) {
     int rc = contact_set_radius(c->cs, contact_radius, c->stream);
     if (rc) return rc;
+    contact_invalidate(c->cs, c->stream);
   }
   return DL_OK;
 }
@@ -473,6 +474,11 @@ int check_err(dl_ctx *c) {
   if (e == DL_ERR_PLASTIC_ITER) {
     HIPCHK(hipMemsetAsync(c->d.err, 0, sizeof(int32_t), c->stream));
     return fail(DL_ERR_PLASTIC_ITER, "Too many iterations in plasticity loop");
+  }
+  if (e == DL_ERR_CAPACITY) {
+    HIPCHK(hipMemsetAsync(c->d.err, 0, sizeof(int32_t), c->stream));
+    return fail(DL_ERR_CAPACITY, "contact candidate lists exceed their capacity "
+                                 "(dl_contact.max_pairs); the step results are invalid");
   }
   if (e) return fail(e, "device error");
   return DL_OK;
@@ -493,6 +499,7 @@ extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, i
   if (velo) e2 = hipMemcpyAsync(tmp + ndof, velo, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e1 == hipSuccess && e2 == hipSuccess)
     launch_state_io(c->d, c->so3, false, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
+  if (c->contact) contact_invalidate(c->cs, c->stream);
   hipError_t e3 = hipStreamSynchronize(c->stream);
   hipFree(tmp);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
@@ -669,6 +676,7 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   switch (action) {
   case DL_INIT: {
     launch_init_strain(c->d, c->stream);
+    if (c->contact) contact_invalidate(c->cs, c->stream);
     HIPCHK(hipGetLastError());
     if (c->contact) {
       int rc = contact_init_blacklist(c->cs, c->d, c->stream);

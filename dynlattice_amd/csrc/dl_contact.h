@@ -49,6 +49,17 @@ struct ContactDev {
   int32_t *loff;        // [E+1]
   int32_t *la, *lb;     // [lcap]
   int64_t lcap;
+  // per-element candidate lists of the step (skin lists): partners whose
+  // boxes inflated by the skin overlapped at the last build, sorted by id;
+  // rebuilt when a node has moved by more than the trigger distance
+  double skin;          // > 0: lists on
+  int32_t *gate;        // [1] 1 = rebuild the lists at the next step
+  double *uref;         // [3][N] displacement at the last build
+  int32_t *vcount;      // [E]
+  int32_t *vptr;        // [E+1]
+  int32_t *vlist;       // [vcap]
+  int64_t vcap;
+  int32_t *err;         // device error flag (capacity)
 };
 
 struct ContactState {
@@ -74,6 +85,8 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
 // adjoint of the contact forces at the state d (the taped start state of a
 // step): broadphase, then per closed pair the VJP into a.ebuf and a.crbar
 int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s);
+// marks the candidate lists stale (the state was set from outside the step)
+void contact_invalidate(ContactState &cs, hipStream_t s);
 // replaces the per-strut contact radii (host array of n_struts values)
 int contact_set_radius(ContactState &cs, const double *r, hipStream_t s);
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,

@@ -36,21 +36,37 @@
 enum { LIST_CAND = 0, LIST_ACTIVE = 1, LIST_CLOSED = 2 };
 
 // ------------------------------------------------------------------
-__global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d) {
+// A gated kernel does nothing unless *gate != 0 (null gate: always runs):
+// the candidate-list rebuild is launched every step and runs only when the
+// list has expired, which keeps the step sequence fixed for graph capture.
+#define GATED(gate) \
+  if ((gate) && *(gate) == 0) return
+
+// AABB of an element inflated by r (the element's contact radius, plus the
+// list skin when building candidate lists), in one fixed operation order
+__device__ __forceinline__ void elem_box(const DevData &d, int64_t N, int32_t a, int32_t b,
+                                         double r, double lo[3], double hi[3]) {
+  for (int i = 0; i < 3; i++) {
+    const double xa = d.X0[i * N + a] + d.u[i * N + a];
+    const double xb = d.X0[i * N + b] + d.u[i * N + b];
+    lo[i] = (xa < xb ? xa : xb) - r;
+    hi[i] = (xa > xb ? xa : xb) + r;
+  }
+}
+
+__global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d, double infl,
+                                               const int32_t *gate) {
+  GATED(gate);
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   double ext = 0.0;
   if (e < c.E) {
     const int64_t N = c.N, E = c.E;
-    const int32_t a = c.en0[e], b = c.en1[e];
-    const double r = c.crad[c.es[e]];
+    double lo[3], hi[3];
+    elem_box(d, N, c.en0[e], c.en1[e], c.crad[c.es[e]] + infl, lo, hi);
     for (int i = 0; i < 3; i++) {
-      const double xa = d.X0[i * N + a] + d.u[i * N + a];
-      const double xb = d.X0[i * N + b] + d.u[i * N + b];
-      const double lo = (xa < xb ? xa : xb) - r;
-      const double hi = (xa > xb ? xa : xb) + r;
-      c.box[i * E + e] = lo;
-      c.box[(3 + i) * E + e] = hi;
-      ext = fmax(ext, hi - lo);
+      c.box[i * E + e] = lo[i];
+      c.box[(3 + i) * E + e] = hi[i];
+      ext = fmax(ext, hi[i] - lo[i]);
     }
   }
   __shared__ double red[CB];
@@ -63,7 +79,8 @@ __global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d) {
   if (threadIdx.x == 0) c.ext_part[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(1024) void kc_grid(ContactDev c, int32_t nblk) {
+__global__ __launch_bounds__(1024) void kc_grid(ContactDev c, int32_t nblk, const int32_t *gate) {
+  GATED(gate);
   __shared__ double red[1024];
   double m = 0.0;
   for (int i = threadIdx.x; i < nblk; i += 1024) m = fmax(m, c.ext_part[i]);
@@ -79,7 +96,9 @@ __global__ __launch_bounds__(1024) void kc_grid(ContactDev c, int32_t nblk) {
   }
 }
 
-__global__ __launch_bounds__(CB) void kc_fill_i32(int32_t *p, int64_t n, int32_t v) {
+__global__ __launch_bounds__(CB) void kc_fill_i32(int32_t *p, int64_t n, int32_t v,
+                                                  const int32_t *gate) {
+  GATED(gate);
   const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -90,7 +109,8 @@ __device__ __forceinline__ int32_t cell_coord(double x, double o, double h, int3
   return (int32_t)v;
 }
 
-__global__ __launch_bounds__(CB) void kc_cells(ContactDev c) {
+__global__ __launch_bounds__(CB) void kc_cells(ContactDev c, const int32_t *gate) {
+  GATED(gate);
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (e >= c.E) return;
   const double h = c.hbuf[0];
@@ -105,7 +125,8 @@ __global__ __launch_bounds__(CB) void kc_cells(ContactDev c) {
 
 // exclusive scan of n int32 values (chunks of 1024 per block)
 __global__ __launch_bounds__(CB) void kc_scan1(const int32_t *in, int64_t n, int32_t *out,
-                                               int32_t *bsum) {
+                                               int32_t *bsum, const int32_t *gate) {
+  GATED(gate);
   __shared__ int32_t s[CB];
   const int64_t base = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
   int32_t v[4];
@@ -130,7 +151,9 @@ __global__ __launch_bounds__(CB) void kc_scan1(const int32_t *in, int64_t n, int
   if (threadIdx.x == CB - 1) bsum[blockIdx.x] = s[CB - 1];
 }
 
-__global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int32_t *total) {
+__global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int32_t *total,
+                                                 const int32_t *gate) {
+  GATED(gate);
   __shared__ int32_t s[1024];
   int32_t carry = 0;
   for (int64_t b0 = 0; b0 < nb; b0 += 1024) {
@@ -152,7 +175,9 @@ __global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int3
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const int32_t *bsum) {
+__global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const int32_t *bsum,
+                                               const int32_t *gate) {
+  GATED(gate);
   const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (i < n) out[i] += bsum[i / 1024];
 }
@@ -164,7 +189,8 @@ struct __align__(64) CRec {
 };
 
 // element ids into their cells (order within a cell set by the atomics)
-__global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
+__global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *gate) {
+  GATED(gate);
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (e >= c.E) return;
   c.tmp[c.cstart[c.cell[e]] + c.rank[e]] = (int32_t)e;
@@ -173,7 +199,8 @@ __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c) {
 // stable placement: an element's slot in its cell is the number of elements
 // of the same cell with a smaller id, so every cell lists its elements in id
 // order whatever order the atomics gave; the 64-byte record goes there
-__global__ __launch_bounds__(CB) void kc_place(ContactDev c) {
+__global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate) {
+  GATED(gate);
   const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   if (e >= c.E) return;
   const int64_t E = c.E;
@@ -428,6 +455,194 @@ __global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, Adj
 }
 
 // ------------------------------------------------------------------
+// candidate lists of the step (skin lists)
+//
+// Built from the broadphase with boxes inflated by the skin s: every
+// partner Y of X whose inflated box overlaps X's (plus the static filters:
+// other strut, no shared node with the other strut, not blacklisted).  While
+// no node has moved by more than the trigger distance (< s) since the build,
+// two boxes that overlap now overlapped inflated by s at the build, so the
+// list holds every current candidate; the step then tests only the listed
+// pairs with the exact boxes, i.e. it finds exactly the broadphase's
+// candidate set.  Lists are sorted by partner id, so the force on X is
+// summed in an order that does not depend on the cell layout or on when the
+// lists were built.
+// ------------------------------------------------------------------
+template <bool FILL>
+__global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
+  GATED(gate);
+  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (k >= c.E) return;
+  const CRec x = ((const CRec *)c.rec)[k];
+  const int32_t X = x.e;
+  int32_t n = 0;
+  const int64_t o = FILL ? c.vptr[X] : 0;
+  const int64_t len = FILL ? c.vptr[X + 1] - o : 0;
+  int32_t *lst = c.vlist + o;
+  if (FILL && o + len > c.vcap) {
+    *c.err = DL_ERR_CAPACITY;
+    return;
+  }
+  scan_candidates(c, k, x, [&](int32_t Y, int32_t, int32_t, int32_t, bool XisA) {
+    if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
+    if (FILL && n < len) {
+      // insertion by partner id
+      int32_t i = n;
+      while (i > 0 && lst[i - 1] > Y) {
+        lst[i] = lst[i - 1];
+        i--;
+      }
+      lst[i] = Y;
+    }
+    n++;
+  });
+  if (!FILL) c.vcount[X] = n;
+}
+
+// the lists are built: remember the displacement they were built at
+__global__ __launch_bounds__(CB) void kc_vcommit(ContactDev c, DevData d, const int32_t *gate) {
+  GATED(gate);
+  const int64_t n = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (n >= c.N) return;
+  for (int i = 0; i < 3; i++) c.uref[i * c.N + n] = d.u[i * c.N + n];
+}
+
+__global__ void kc_gate_clear(int32_t *gate) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *gate = 0;
+}
+
+// contact forces of the step from the lists, one thread per element (id
+// order): exact boxes of X and of each listed Y, then the pair as in
+// kc_narrow; per-element force rows (cidx[X] = X) and per-step counts
+__global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
+                                                 unsigned long long *cnt_act,
+                                                 unsigned long long *cnt_cand,
+                                                 const int32_t *step) {
+  const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  unsigned long long nact = 0, ncand = 0;
+  if (X < c.E) {
+    const int64_t N = c.N, E = c.E;
+    const int32_t sX = c.es[X];
+    const double rX = c.crad[sX];
+    double pX[2][3], xlo[3], xhi[3];
+    elem_pos(d, c.en0[X], c.en1[X], pX);
+    elem_box(d, N, c.en0[X], c.en1[X], rX, xlo, xhi);
+    double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
+    bool any = false;
+    // a list that did not fit (error flagged at the build) is not read
+    const int32_t t1 = (c.vptr[X + 1] <= c.vcap) ? c.vptr[X + 1] : c.vptr[X];
+    for (int32_t t = c.vptr[X]; t < t1; t++) {
+      const int32_t Y = c.vlist[t];
+      const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
+      const double rY = c.crad[sY];
+      double ylo[3], yhi[3];
+      elem_box(d, N, ya, yb, rY, ylo, yhi);
+      bool ov = true;
+      for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
+      if (!ov) continue;
+      const bool XisA = sX < sY;
+      if (XisA) ncand++;
+      double pY[2][3];
+      elem_pos(d, ya, yb, pY);
+      const double R = rX + rY;
+      double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+      const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                               : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+      if (closed) {
+        any = true;
+        for (int i = 0; i < 3; i++) {
+          fa[i] += g0[i];
+          fb[i] += g1[i];
+        }
+        if (XisA) nact++;
+      }
+    }
+    if (any) {
+      c.cidx[X] = (int32_t)X;
+      for (int i = 0; i < 3; i++) {
+        c.fcs[i * E + X] = fa[i];
+        c.fcs[(3 + i) * E + X] = fb[i];
+      }
+    } else {
+      c.cidx[X] = -1;
+    }
+  }
+  __shared__ unsigned long long ra[CB], rc[CB];
+  ra[threadIdx.x] = nact;
+  rc[threadIdx.x] = ncand;
+  __syncthreads();
+  for (int w = CB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ra[threadIdx.x] += ra[threadIdx.x + w];
+      rc[threadIdx.x] += rc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && cnt_act) {
+    const int64_t st = *step;
+    if (ra[0]) atomicAdd(cnt_act + st, ra[0]);
+    if (rc[0]) atomicAdd(cnt_cand + st, rc[0]);
+  }
+}
+
+// list trigger for a state that was not produced by a step (the reverse
+// sweep walks backwards through taped states): any node moved by more than
+// the trigger distance since the build -> rebuild
+__global__ __launch_bounds__(CB) void kc_vcheck(ContactDev c, DevData d, double trig) {
+  const int64_t n = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (n >= c.N) return;
+  bool moved = false;
+  for (int i = 0; i < 3; i++) moved = moved || fabs(d.u[i * c.N + n] - c.uref[i * c.N + n]) > trig;
+  if (moved) *c.gate = 1;
+}
+
+// Adjoint of the contact forces from the lists (see kc_adj_narrow), one
+// thread per element in id order.
+__global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, AdjDev a) {
+  const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (X >= c.E) return;
+  const int64_t N = c.N, E = c.E;
+  const int32_t sX = c.es[X], xa = c.en0[X], xb = c.en1[X];
+  const double rX = c.crad[sX];
+  double pX[2][3], xlo[3], xhi[3], fbX[6];
+  elem_pos(d, xa, xb, pX);
+  elem_box(d, N, xa, xb, rX, xlo, xhi);
+  for (int i = 0; i < 3; i++) {
+    fbX[i] = a.fbar[i * N + xa];
+    fbX[3 + i] = a.fbar[i * N + xb];
+  }
+  double g[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  bool any = false;
+  const int32_t t1 = (c.vptr[X + 1] <= c.vcap) ? c.vptr[X + 1] : c.vptr[X];
+  for (int32_t t = c.vptr[X]; t < t1; t++) {
+    const int32_t Y = c.vlist[t];
+    const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
+    const double rY = c.crad[sY];
+    double ylo[3], yhi[3];
+    elem_box(d, N, ya, yb, rY, ylo, yhi);
+    bool ov = true;
+    for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
+    if (!ov) continue;
+    double pY[2][3], fbY[6], o[7];
+    elem_pos(d, ya, yb, pY);
+    for (int i = 0; i < 3; i++) {
+      fbY[i] = a.fbar[i * N + ya];
+      fbY[3 + i] = a.fbar[i * N + yb];
+    }
+    if (contact_pair_vjp(pX, pY, sX < sY, rX + rY, c.kS, c.kN, fbX, fbY, o)) {
+      any = true;
+      for (int i = 0; i < 7; i++) g[i] += o[i];
+    }
+  }
+  if (!any) return;
+  for (int i = 0; i < 3; i++) {
+    a.ebuf[i * E + X] += g[i];
+    a.ebuf[(7 + i) * E + X] += g[3 + i];
+  }
+  a.crbar[X] += g[6];
+}
+
+// ------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------
 static unsigned nb(int64_t n, int b = CB) { return (unsigned)((n + b - 1) / b); }
@@ -558,27 +773,51 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   CHK(hipMemset(c.cidx, 0xff, sizeof(int32_t) * E)); // -1: no contact before the first evaluation
   d.cidx = c.cidx;
   d.fcs = c.fcs;
+  // candidate lists of the step: skin 0 = half the largest radius, < 0 = off
+  c.skin = co->skin < 0.0 ? 0.0 : (co->skin > 0.0 ? co->skin : 0.5 * rmax);
+  c.err = d.err;
+  d.uref = nullptr;
+  d.cgate = nullptr;
+  d.ctrig = 0.0;
+  if (c.skin > 0.0) {
+    c.vcap = co->max_pairs > 0 ? co->max_pairs : 32 * E;
+    if ((rc = calloc_dev(cs, &c.gate, 1))) return rc;
+    if ((rc = calloc_dev(cs, &c.uref, 3 * N))) return rc;
+    if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
+    if ((rc = calloc_dev(cs, &c.vptr, E + 1))) return rc;
+    if ((rc = calloc_dev(cs, &c.vlist, c.vcap))) return rc;
+    const int32_t one = 1; // lists are built at the first step
+    CHK(hipMemcpy(c.gate, &one, sizeof one, hipMemcpyHostToDevice));
+    d.uref = c.uref;
+    d.cgate = c.gate;
+    // rebuild once a node has moved by 0.95 s: the margin covers rounding
+    // of the box bounds, far below 5 % of s
+    d.ctrig = 0.95 * c.skin;
+  }
   return DL_OK;
 }
 
-static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *out, hipStream_t s) {
+static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *out, hipStream_t s,
+                     const int32_t *gate = nullptr) {
   const int64_t nblk = (n + 1023) / 1024;
   ContactDev &c = cs.c;
-  hipLaunchKernelGGL(kc_scan1, dim3((unsigned)nblk), dim3(CB), 0, s, in, n, out, c.bsum);
-  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.bsum, nblk, out + n);
-  hipLaunchKernelGGL(kc_scan3, dim3(nb(n)), dim3(CB), 0, s, out, n, (const int32_t *)c.bsum);
+  hipLaunchKernelGGL(kc_scan1, dim3((unsigned)nblk), dim3(CB), 0, s, in, n, out, c.bsum, gate);
+  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.bsum, nblk, out + n, gate);
+  hipLaunchKernelGGL(kc_scan3, dim3(nb(n)), dim3(CB), 0, s, out, n, (const int32_t *)c.bsum, gate);
 }
 
-// broadphase up to sorted cell lists
-static void broadphase(ContactState &cs, const DevData &d, hipStream_t s) {
+// broadphase up to sorted cell lists, boxes inflated by infl beyond the
+// contact radii; gated (see GATED) when gate is not null
+static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double infl = 0.0,
+                       const int32_t *gate = nullptr) {
   ContactDev &c = cs.c;
-  hipLaunchKernelGGL(kc_boxes, dim3(cs.nblk_e), dim3(CB), 0, s, c, d);
-  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, cs.nblk_e);
-  hipLaunchKernelGGL(kc_fill_i32, dim3(nb(c.ncell)), dim3(CB), 0, s, c.ccount, c.ncell, 0);
-  hipLaunchKernelGGL(kc_cells, dim3(cs.nblk_e), dim3(CB), 0, s, c);
-  scan_i32(cs, c.ccount, c.ncell, c.cstart, s);
-  hipLaunchKernelGGL(kc_scatter, dim3(cs.nblk_e), dim3(CB), 0, s, c);
-  hipLaunchKernelGGL(kc_place, dim3(cs.nblk_e), dim3(CB), 0, s, c);
+  hipLaunchKernelGGL(kc_boxes, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, infl, gate);
+  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, cs.nblk_e, gate);
+  hipLaunchKernelGGL(kc_fill_i32, dim3(nb(c.ncell)), dim3(CB), 0, s, c.ccount, c.ncell, 0, gate);
+  hipLaunchKernelGGL(kc_cells, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
+  scan_i32(cs, c.ccount, c.ncell, c.cstart, s, gate);
+  hipLaunchKernelGGL(kc_scatter, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
+  hipLaunchKernelGGL(kc_place, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
 }
 
 static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,
@@ -652,14 +891,40 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
   return list_pairs(cs, d, s, LIST_CAND, cs.cand_a, cs.cand_b);
 }
 
+// rebuilds the lists when stale (gated), as the step does
+static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
+  ContactDev &c = cs.c;
+  const int32_t *g = c.gate;
+  broadphase(cs, d, s, c.skin, g);
+  hipLaunchKernelGGL(kc_vbuild<false>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, g);
+  scan_i32(cs, c.vcount, c.E, c.vptr, s, g);
+  hipLaunchKernelGGL(kc_vbuild<true>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vcommit, dim3(nb(c.N)), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_gate_clear, dim3(1), dim3(1), 0, s, c.gate);
+}
+
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
                  int64_t *n_cand, const int32_t *step) {
   ContactDev &c = cs.c;
-  broadphase(cs, d, s);
-  hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
+  if (!(c.skin > 0.0)) {
+    // no lists: full broadphase and cell scan every step
+    broadphase(cs, d, s);
+    hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
+                       (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
+    CHK(hipGetLastError());
+    return DL_OK;
+  }
+  // list rebuild, gated on the trigger (no-ops while the lists are valid)
+  list_refresh(cs, d, s);
+  hipLaunchKernelGGL(kc_vnarrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d,
                      (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
   CHK(hipGetLastError());
   return DL_OK;
+}
+
+void contact_invalidate(ContactState &cs, hipStream_t s) {
+  if (cs.c.skin > 0.0) hipLaunchKernelGGL(kc_fill_i32, dim3(1), dim3(CB), 0, s, cs.c.gate, 1, 1,
+                                          (const int32_t *)nullptr);
 }
 
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,
@@ -677,8 +942,15 @@ int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a,
 }
 
 int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s) {
-  broadphase(cs, d, s);
-  hipLaunchKernelGGL(kc_adj_narrow, dim3(cs.nblk_e), dim3(CB), 0, s, cs.c, d, a);
+  ContactDev &c = cs.c;
+  if (!(c.skin > 0.0)) {
+    broadphase(cs, d, s);
+    hipLaunchKernelGGL(kc_adj_narrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
+  } else {
+    hipLaunchKernelGGL(kc_vcheck, dim3(nb(c.N)), dim3(CB), 0, s, c, d, 0.95 * c.skin);
+    list_refresh(cs, d, s);
+    hipLaunchKernelGGL(kc_adj_vnarrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
+  }
   CHK(hipGetLastError());
   return DL_OK;
 }

@@ -74,6 +74,11 @@ struct DevData {
   int32_t *err;         // device error flag
   // adjoint tape row of the current step ([19][N]: u 3, rot 4, v 6, f 6) or null
   double *tape;
+  // contact candidate lists: a node whose displacement has moved by more
+  // than ctrig (any component) since the lists' build sets *cgate
+  const double *uref;
+  int32_t *cgate;
+  double ctrig;
 };
 
 struct StepParams {

@@ -87,6 +87,12 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
   for (int c = 0; c < 6; c++) d.v[c * N + n] = v[c];
 #pragma unroll
   for (int c = 0; c < 3; c++) d.u[c * N + n] = u[c];
+  if (d.uref) {
+    bool moved = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) moved = moved || fabs(u[c] - d.uref[c * N + n]) > d.ctrig;
+    if (moved) *d.cgate = 1;
+  }
   if (SO3) {
 #pragma unroll
     for (int c = 0; c < 4; c++) d.rot[c * N + n] = rr[c];

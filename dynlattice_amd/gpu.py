@@ -73,7 +73,8 @@ class dl_material(C.Structure):
 
 class dl_contact(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("penalty_sts", C.c_double),
-                ("penalty_nts", C.c_double), ("radius", dp), ("max_pairs", C.c_int64)]
+                ("penalty_nts", C.c_double), ("radius", dp), ("max_pairs", C.c_int64),
+                ("skin", C.c_double)]
 
 
 class dl_options(C.Structure):
@@ -190,6 +191,7 @@ class GpuModel:
             ct.penalty_nts = p.contact.penalty_nts
             ct.radius = _p(r)
             ct.max_pairs = int(max_pairs)
+            ct.skin = float(getattr(p.contact, "skin", 0.0))
         opt = dl_options(int(p.so3), int(device), stream, int(flags))
         h = C.c_void_p()
         self._check(self.L.dl_create(C.byref(mesh), C.byref(sec), C.byref(mat), C.byref(ct),

@@ -128,6 +128,7 @@ class Contact:
     radius: np.ndarray
     penalty_sts: float = 1e6
     penalty_nts: float = 1e6
+    skin: float = 0.0   # step candidate-list skin (0: half the largest radius, < 0: off)
 
 
 @dataclass

@@ -116,6 +116,8 @@ typedef struct {
   double penalty_nts;
   const double *radius; /* [n_struts] */
   int64_t max_pairs;    /* candidate-pair capacity; 0 = 32 * n_elems */
+  double skin;          /* candidate-list skin distance of the step: 0 = half the
+                           largest radius, < 0 = full broadphase every step */
 } dl_contact;
 
 typedef struct {

@@ -72,3 +72,57 @@ def test_colliding_segments_leapfrog():
     assert out["n_contacts"].max() > 5
     assert np.array_equal(h["n_contacts"], out["n_contacts"])
     assert rel_inf(d, r.disp.reshape(-1, 6)) <= 1e-8
+
+
+def _run_with_skin(p, skin, K, max_pairs=0):
+    import copy
+
+    from dynlattice_amd.gpu import GpuModel
+    q = copy.deepcopy(p)
+    q.contact.skin = skin
+    g = GpuModel(q, flags=0, max_pairs=max_pairs)
+    if "top" in q.groups:
+        g.set_reaction_group(q.groups["top"], 2)
+    o = g.step(K)
+    h = g.history(K)
+    d, v = g.get_state()
+    g.close()
+    return d, v, h, o.n_candidates
+
+
+def _cases():
+    from dynlattice_amd import lattice as L
+    p = random_segments(seed=31, kS=1e4, kN=1e4)
+    c = p.coords.mean(0)
+    K = 300
+    p.velo0[:, :3] = -(p.coords + p.disp0[:, :3] - c) * (0.8 / (K * p.dt))
+    return [(p, K), (L.bcc_adjoint_check(), 1600)]
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_step_candidate_lists_equal_full_broadphase(case):
+    """The step's skin lists find exactly the broadphase's pairs: per-step
+    active counts and the last step's candidate count equal those of the full
+    cell scan; the lists are sorted by partner id, so any skin (rebuilt at
+    different steps) gives the same bits."""
+    p, K = _cases()[case]
+    r = float(p.contact.radius.max())
+    d0, v0, h0, c0 = _run_with_skin(p, -1.0, K)
+    cap = 400 * p.n_elems  # room for the widest skin
+    runs = [_run_with_skin(p, s, K, cap) for s in (0.05 * r, 0.0, 2.0 * r)]
+    assert h0["n_contacts"].max() > 5
+    for d, v, h, cand in runs:
+        assert np.array_equal(h["n_contacts"], h0["n_contacts"])
+        assert cand == c0
+        assert rel_inf(d, d0) <= 1e-8
+    for d, v, h, cand in runs[1:]:
+        assert np.array_equal(d, runs[0][0]) and np.array_equal(v, runs[0][1])
+        assert np.array_equal(h["reaction"], runs[0][2]["reaction"])
+
+
+def test_step_candidate_list_capacity_is_checked():
+    from dynlattice_amd.gpu import DL_ERR_CAPACITY, DLError
+    p, K = _cases()[0]
+    with pytest.raises(DLError) as e:
+        _run_with_skin(p, 0.0, 5, max_pairs=4)
+    assert e.value.code == DL_ERR_CAPACITY
