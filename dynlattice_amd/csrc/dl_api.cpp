@@ -361,6 +361,9 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     TRY(upload(c, &pi, sb)); d.sb = pi;
     TRY(upload(c, &pi, lane_e)); d.lane_e = pi;
     d.NL = (int64_t)lane_e.size();
+    d.lane_ident = 1;
+    for (int64_t L = 0; L < d.NL && d.lane_ident; L++)
+      d.lane_ident = lane_e[L] == (L < E ? (int32_t)L : -1);
     uint8_t *pu;
     TRY(upload(c, &pu, eflag)); d.eflag = pu;
     d.NJ = NJ;
@@ -394,6 +397,8 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       P.has_kin = mat->has_kin;
       P.max_iter = mat->max_iter;
       for (int i = 0; i < 6; i++) P.exponent[i] = mat->exponent[i];
+      for (int i = 0; i < 6; i++) P.kin_const[i] = mat->kin_const[i];
+      P.kin_ratio = mat->kin_ratio;
       bool all1 = true, all2 = true;
       for (int i = 0; i < 6; i++) {
         all1 = all1 && mat->exponent[i] == 1.0;

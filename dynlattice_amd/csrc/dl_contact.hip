@@ -20,6 +20,13 @@
 // own nodes' forces, in a fixed order, so no floating-point atomics are used
 // and the result is bitwise deterministic.  The strut kernel adds the
 // per-element forces to its nodal sums.
+//
+// The explicit step does not run this per step: it keeps per-element
+// candidate lists built by the same broadphase with boxes inflated by a skin
+// (kc_vbuild) and rebuilds them only when a node has moved by nearly the
+// skin (kc_vnarrow evaluates the listed pairs; see the list section below).
+// The action path (GET_INT_VECTOR, pair listings, blacklist) and the adjoint
+// without lists use the cell scan directly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +49,13 @@ enum { LIST_CAND = 0, LIST_ACTIVE = 1, LIST_CLOSED = 2 };
 #define GATED(gate) \
   if ((gate) && *(gate) == 0) return
 
+// The broadphase and list-build kernels loop over their range with a grid
+// stride and are launched with at most GS_CAP blocks: full-chip occupancy
+// when they work, and a cheap dispatch when they are gated off.
+#define GS_CAP 2048
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < (n); i += (int64_t)gridDim.x * CB)
+
 // AABB of an element inflated by r (the element's contact radius, plus the
 // list skin when building candidate lists), in one fixed operation order
 __device__ __forceinline__ void elem_box(const DevData &d, int64_t N, int32_t a, int32_t b,
@@ -57,9 +71,8 @@ __device__ __forceinline__ void elem_box(const DevData &d, int64_t N, int32_t a,
 __global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d, double infl,
                                                const int32_t *gate) {
   GATED(gate);
-  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
   double ext = 0.0;
-  if (e < c.E) {
+  GRID_STRIDE(e, c.E) {
     const int64_t N = c.N, E = c.E;
     double lo[3], hi[3];
     elem_box(d, N, c.en0[e], c.en1[e], c.crad[c.es[e]] + infl, lo, hi);
@@ -109,18 +122,21 @@ __device__ __forceinline__ int32_t cell_coord(double x, double o, double h, int3
   return (int32_t)v;
 }
 
+// cell of every element (clamped cell of its box's min corner) and its rank
+// among the cell's elements; ccount is all zero on entry (kc_scatter leaves
+// it so)
 __global__ __launch_bounds__(CB) void kc_cells(ContactDev c, const int32_t *gate) {
   GATED(gate);
-  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (e >= c.E) return;
   const double h = c.hbuf[0];
   const int64_t E = c.E;
-  const int32_t cx = cell_coord(c.box[e], c.org[0], h, c.dim[0]);
-  const int32_t cy = cell_coord(c.box[E + e], c.org[1], h, c.dim[1]);
-  const int32_t cz = cell_coord(c.box[2 * E + e], c.org[2], h, c.dim[2]);
-  const int32_t lin = (cz * c.dim[1] + cy) * c.dim[0] + cx;
-  c.cell[e] = lin;
-  c.rank[e] = atomicAdd(&c.ccount[lin], 1);
+  GRID_STRIDE(e, E) {
+    const int32_t cx = cell_coord(c.box[e], c.org[0], h, c.dim[0]);
+    const int32_t cy = cell_coord(c.box[E + e], c.org[1], h, c.dim[1]);
+    const int32_t cz = cell_coord(c.box[2 * E + e], c.org[2], h, c.dim[2]);
+    const int32_t lin = (cz * c.dim[1] + cy) * c.dim[0] + cx;
+    c.cell[e] = lin;
+    c.rank[e] = atomicAdd(&c.ccount[lin], 1);
+  }
 }
 
 // exclusive scan of n int32 values (chunks of 1024 per block)
@@ -128,27 +144,31 @@ __global__ __launch_bounds__(CB) void kc_scan1(const int32_t *in, int64_t n, int
                                                int32_t *bsum, const int32_t *gate) {
   GATED(gate);
   __shared__ int32_t s[CB];
-  const int64_t base = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
-  int32_t v[4];
-  int32_t t = 0;
-  for (int k = 0; k < 4; k++) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
-    t += v[k];
-  }
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < CB; off <<= 1) {
-    const int32_t x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+  const int64_t nchunk = (n + 1023) / 1024;
+  for (int64_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+    const int64_t base = ch * 1024 + 4 * threadIdx.x;
+    int32_t v[4];
+    int32_t t = 0;
+    for (int k = 0; k < 4; k++) {
+      v[k] = (base + k < n) ? in[base + k] : 0;
+      t += v[k];
+    }
+    s[threadIdx.x] = t;
     __syncthreads();
-    s[threadIdx.x] += x;
+    for (int off = 1; off < CB; off <<= 1) {
+      const int32_t x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += x;
+      __syncthreads();
+    }
+    int32_t run = s[threadIdx.x] - t;
+    for (int k = 0; k < 4; k++) {
+      if (base + k < n) out[base + k] = run;
+      run += v[k];
+    }
+    if (threadIdx.x == CB - 1) bsum[ch] = s[CB - 1];
     __syncthreads();
   }
-  int32_t run = s[threadIdx.x] - t;
-  for (int k = 0; k < 4; k++) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == CB - 1) bsum[blockIdx.x] = s[CB - 1];
 }
 
 __global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int32_t *total,
@@ -178,8 +198,7 @@ __global__ __launch_bounds__(1024) void kc_scan2(int32_t *bsum, int64_t nb, int3
 __global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const int32_t *bsum,
                                                const int32_t *gate) {
   GATED(gate);
-  const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (i < n) out[i] += bsum[i / 1024];
+  GRID_STRIDE(i, n) out[i] += bsum[i / 1024];
 }
 
 // cell-ordered element record: AABB, element, strut, cell (one 64-byte line)
@@ -189,20 +208,21 @@ struct __align__(64) CRec {
 };
 
 // element ids into their cells (order within a cell set by the atomics)
+// element ids into their cells; the counts are consumed by now and are
+// reset to zero for the next broadphase
 __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *gate) {
   GATED(gate);
-  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (e >= c.E) return;
-  c.tmp[c.cstart[c.cell[e]] + c.rank[e]] = (int32_t)e;
+  GRID_STRIDE(e, c.E) {
+    const int32_t cell = c.cell[e];
+    c.tmp[c.cstart[cell] + c.rank[e]] = (int32_t)e;
+    c.ccount[cell] = 0;
+  }
 }
 
 // stable placement: an element's slot in its cell is the number of elements
 // of the same cell with a smaller id, so every cell lists its elements in id
 // order whatever order the atomics gave; the 64-byte record goes there
-__global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate) {
-  GATED(gate);
-  const int64_t e = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (e >= c.E) return;
+__device__ __forceinline__ void place_one(const ContactDev &c, int64_t e) {
   const int64_t E = c.E;
   const int32_t cell = c.cell[e];
   const int32_t b = c.cstart[cell], n = c.cstart[cell + 1] - b;
@@ -226,6 +246,11 @@ __global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate
   hi.z = __double2float_ru(r.hi[2]); hi.w = __int_as_float(r.e);
   ((float4 *)c.frec)[2 * pos] = lo;
   ((float4 *)c.frec)[2 * pos + 1] = hi;
+}
+
+__global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate) {
+  GATED(gate);
+  GRID_STRIDE(e, c.E) place_one(c, e);
 }
 
 __device__ __forceinline__ bool node_in_strut(const ContactDev &c, int32_t n, int32_t s) {
@@ -471,40 +496,40 @@ __global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, Adj
 template <bool FILL>
 __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
-  const int64_t k = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (k >= c.E) return;
-  const CRec x = ((const CRec *)c.rec)[k];
-  const int32_t X = x.e;
-  int32_t n = 0;
-  const int64_t o = FILL ? c.vptr[X] : 0;
-  const int64_t len = FILL ? c.vptr[X + 1] - o : 0;
-  int32_t *lst = c.vlist + o;
-  if (FILL && o + len > c.vcap) {
-    *c.err = DL_ERR_CAPACITY;
-    return;
-  }
-  scan_candidates(c, k, x, [&](int32_t Y, int32_t, int32_t, int32_t, bool XisA) {
-    if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
-    if (FILL && n < len) {
-      // insertion by partner id
-      int32_t i = n;
-      while (i > 0 && lst[i - 1] > Y) {
-        lst[i] = lst[i - 1];
-        i--;
-      }
-      lst[i] = Y;
+  GRID_STRIDE(k, c.E) {
+    const CRec x = ((const CRec *)c.rec)[k];
+    const int32_t X = x.e;
+    int32_t n = 0;
+    const int64_t o = FILL ? c.vptr[X] : 0;
+    const int64_t len = FILL ? c.vptr[X + 1] - o : 0;
+    int32_t *lst = c.vlist + o;
+    if (FILL && o + len > c.vcap) {
+      *c.err = DL_ERR_CAPACITY;
+      continue;
     }
-    n++;
-  });
-  if (!FILL) c.vcount[X] = n;
+    scan_candidates(c, k, x, [&](int32_t Y, int32_t, int32_t, int32_t, bool XisA) {
+      if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
+      if (FILL && n < len) {
+        // insertion by partner id
+        int32_t i = n;
+        while (i > 0 && lst[i - 1] > Y) {
+          lst[i] = lst[i - 1];
+          i--;
+        }
+        lst[i] = Y;
+      }
+      n++;
+    });
+    if (!FILL) c.vcount[X] = n;
+  }
 }
 
 // the lists are built: remember the displacement they were built at
 __global__ __launch_bounds__(CB) void kc_vcommit(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
-  const int64_t n = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (n >= c.N) return;
-  for (int i = 0; i < 3; i++) c.uref[i * c.N + n] = d.u[i * c.N + n];
+  GRID_STRIDE(n, c.N) {
+    for (int i = 0; i < 3; i++) c.uref[i * c.N + n] = d.u[i * c.N + n];
+  }
 }
 
 __global__ void kc_gate_clear(int32_t *gate) {
@@ -756,6 +781,7 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = calloc_dev(cs, &c.cell, E))) return rc;
   if ((rc = calloc_dev(cs, &c.rank, E))) return rc;
   if ((rc = calloc_dev(cs, &c.ccount, ncell))) return rc;
+  CHK(hipMemset(c.ccount, 0, sizeof(int32_t) * ncell)); // kc_cells expects zero counts
   if ((rc = calloc_dev(cs, &c.cstart, ncell + 1))) return rc;
   if ((rc = calloc_dev(cs, &c.bsum, cs.nscan_blk + 1))) return rc;
   if ((rc = calloc_dev(cs, (double **)&c.rec, 8 * (size_t)E))) return rc;
@@ -797,13 +823,17 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   return DL_OK;
 }
 
+// grid of a grid-stride kernel over n items
+static unsigned gs(int64_t n) { return std::min<unsigned>(nb(n), GS_CAP); }
+
 static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *out, hipStream_t s,
                      const int32_t *gate = nullptr) {
-  const int64_t nblk = (n + 1023) / 1024;
+  const int64_t nchunk = (n + 1023) / 1024;
   ContactDev &c = cs.c;
-  hipLaunchKernelGGL(kc_scan1, dim3((unsigned)nblk), dim3(CB), 0, s, in, n, out, c.bsum, gate);
-  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.bsum, nblk, out + n, gate);
-  hipLaunchKernelGGL(kc_scan3, dim3(nb(n)), dim3(CB), 0, s, out, n, (const int32_t *)c.bsum, gate);
+  const unsigned g1 = (unsigned)std::min<int64_t>(nchunk, GS_CAP);
+  hipLaunchKernelGGL(kc_scan1, dim3(g1), dim3(CB), 0, s, in, n, out, c.bsum, gate);
+  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.bsum, nchunk, out + n, gate);
+  hipLaunchKernelGGL(kc_scan3, dim3(gs(n)), dim3(CB), 0, s, out, n, (const int32_t *)c.bsum, gate);
 }
 
 // broadphase up to sorted cell lists, boxes inflated by infl beyond the
@@ -811,13 +841,13 @@ static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *ou
 static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double infl = 0.0,
                        const int32_t *gate = nullptr) {
   ContactDev &c = cs.c;
-  hipLaunchKernelGGL(kc_boxes, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, infl, gate);
-  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, cs.nblk_e, gate);
-  hipLaunchKernelGGL(kc_fill_i32, dim3(nb(c.ncell)), dim3(CB), 0, s, c.ccount, c.ncell, 0, gate);
-  hipLaunchKernelGGL(kc_cells, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
+  const unsigned ge = gs(c.E);
+  hipLaunchKernelGGL(kc_boxes, dim3(ge), dim3(CB), 0, s, c, d, infl, gate);
+  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, (int32_t)ge, gate);
+  hipLaunchKernelGGL(kc_cells, dim3(ge), dim3(CB), 0, s, c, gate);
   scan_i32(cs, c.ccount, c.ncell, c.cstart, s, gate);
-  hipLaunchKernelGGL(kc_scatter, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
-  hipLaunchKernelGGL(kc_place, dim3(cs.nblk_e), dim3(CB), 0, s, c, gate);
+  hipLaunchKernelGGL(kc_scatter, dim3(ge), dim3(CB), 0, s, c, gate);
+  hipLaunchKernelGGL(kc_place, dim3(ge), dim3(CB), 0, s, c, gate);
 }
 
 static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,
@@ -896,10 +926,10 @@ static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   const int32_t *g = c.gate;
   broadphase(cs, d, s, c.skin, g);
-  hipLaunchKernelGGL(kc_vbuild<false>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vbuild<false>, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
   scan_i32(cs, c.vcount, c.E, c.vptr, s, g);
-  hipLaunchKernelGGL(kc_vbuild<true>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, g);
-  hipLaunchKernelGGL(kc_vcommit, dim3(nb(c.N)), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vbuild<true>, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vcommit, dim3(gs(c.N)), dim3(CB), 0, s, c, d, g);
   hipLaunchKernelGGL(kc_gate_clear, dim3(1), dim3(1), 0, s, c.gate);
 }
 

@@ -178,8 +178,19 @@ struct PlasticParams {
   double iso_modulus;
   double precision;
   double exponent[6];
+  double kin_const[6];  // H_kin = kin_const + kin_ratio * C
+  double kin_ratio;
   int32_t has_iso, has_kin, max_iter;
 };
+
+// kinematic hardening modulus of component i: the diagonal of the
+// "kinematicTensor" hardening matrix (ElastoPlasticRodMaterial.cpp:76-106),
+// given as kin_const + kin_ratio * C (kin_ratio = 1/100: "H = E/100 on the
+// resultants"); unfused, like the oracle's set-up of the same table
+DL_HD double kin_modulus(const PlasticParams &P, int i, double C) {
+#pragma clang fp contract(off)
+  return P.kin_const[i] + P.kin_ratio * C;
+}
 
 DL_HD double dl_sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
 // |x|^p and d|x|^p/d|x| of the power-sum yield family.  EXPK selects the

@@ -37,6 +37,7 @@ struct DevData {
   // element -> lane packing: chains of <= 64 elements in consecutive lanes
   int64_t NL;             // lanes (multiple of the block size)
   const int32_t *lane_e;  // [NL] element of the lane or -1
+  int32_t lane_ident;     // lane L holds element L for L < E (no padding): lane_e not read
   const int32_t *en0;     // [E] node a
   const int32_t *en1;     // [E] node b
   const int32_t *es;      // [E] strut

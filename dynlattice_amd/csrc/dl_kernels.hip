@@ -39,9 +39,11 @@ __device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4
 // R <- exp(dt w) R (SO(3)) or theta += dt w (additive).
 // f: element (+ contact) internal force of the node at the step's start state.
 // With a tape, the node's start state and f are recorded for the adjoint.
+// The rotation is (re)loaded here rather than kept live from the element
+// part: fewer registers across the material update (the line is in cache).
 template <bool SO3>
 __device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
-                                            double f[6], const Quat q, const double r[4]) {
+                                            double f[6]) {
   const int64_t N = d.N;
   const int32_t b = d.bc[n];
   unsigned mask = 0u;
@@ -59,12 +61,15 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
     if (rs >= 0) d.rbuf[rs] = f[d.react_dof];
   }
   double v[6], u[3], rr[4];
+  Quat q;
+  // compiler barrier: reload the rotation instead of reusing the element
+  // part's copy (which would stay live in registers)
+  asm volatile("" ::: "memory");
+  load_rot<SO3>(d, n, rr, q);
 #pragma unroll
   for (int c = 0; c < 6; c++) v[c] = d.v[c * N + n];
 #pragma unroll
   for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
-#pragma unroll
-  for (int c = 0; c < 4; c++) rr[c] = r[c];
   if (d.tape) {
     // adjoint tape row: u(3) rot(4) v(6) f(6), component-major
     double *t = d.tape;
@@ -141,11 +146,11 @@ template <int MODE, bool SO3, int MAT, bool CONTACT>
 __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp) {
   constexpr bool PLASTIC = MAT != 0;
   const int64_t L = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
-  const int32_t e = (L < d.NL) ? d.lane_e[L] : -1;
+  // element of the lane: identity packing (every chain fills whole
+  // wavefronts) saves a dependent load before the element data
+  const int32_t e = d.lane_ident ? (L < d.E ? (int32_t)L : -1) : ((L < d.NL) ? d.lane_e[L] : -1);
   const int64_t N = d.N, E = d.E, R = d.R;
   double fa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double ra[4], rb[4];
-  Quat qna, qnb;
   int32_t na = 0, nb = 0;
   uint32_t fl = 0;
   double diss = 0.0;
@@ -163,6 +168,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
       Xa[i] = d.X0[i * N + na]; ua[i] = d.u[i * N + na];
       Xb[i] = d.X0[i * N + nb]; ub[i] = d.u[i * N + nb];
     }
+    double ra[4], rb[4];
+    Quat qna, qnb;
     load_rot<SO3>(d, na, ra, qna);
     load_rot<SO3>(d, nb, rb, qnb);
     Quat qNa, qNb;
@@ -183,15 +190,42 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     } else {
       double pw[6], Hk[6];
 #pragma unroll
-      for (int i = 0; i < 6; i++) { pw[i] = d.pw[i * R + s]; Hk[i] = d.Hk[i * R + s]; }
-      double ep[6], al[7], epo[6];
+      for (int i = 0; i < 6; i++) { pw[i] = d.pw[i * R + s]; Hk[i] = kin_modulus(pp, i, C[i]); }
+      double ep[6], al[7];
 #pragma unroll
-      for (int i = 0; i < 6; i++) { ep[i] = d.ep[i * E + e]; epo[i] = ep[i]; }
+      for (int i = 0; i < 6; i++) ep[i] = d.ep[i * E + e];
+      al[0] = pp.has_iso ? d.al[e] : 0.0; // isotropic parameter only when in use
 #pragma unroll
-      for (int i = 0; i < 7; i++) al[i] = d.al[i * E + e];
+      for (int i = 1; i < 7; i++) al[i] = d.al[i * E + e];
       double dfl;
+      // The kinematics needed after the return map (Lambda, phi', weights)
+      // and the strain wait in LDS meanwhile: the return map's working set
+      // then fits 168 VGPRs, i.e. 3 waves per SIMD instead of 2.  The
+      // compiler barriers keep the values from being forwarded in registers.
+      __shared__ double stash[21][DL_BLOCK];
+      const int tx = threadIdx.x;
+#pragma unroll
+      for (int i = 0; i < 9; i++) stash[i][tx] = k.R[i];
+#pragma unroll
+      for (int i = 0; i < 3; i++) stash[9 + i][tx] = k.phiP[i];
+#pragma unroll
+      for (int i = 0; i < 6; i++) stash[12 + i][tx] = strain[i];
+      stash[18][tx] = k.w;
+      stash[19][tx] = k.g0;
+      stash[20][tx] = k.g1;
+      asm volatile("" ::: "memory");
       const int it = return_map<MAT == 3 ? 0 : MAT>(pp, C, pw, Hk, strain, ep, al, sm, &dfl,
                                                      sp.inelastic);
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int i = 0; i < 9; i++) k.R[i] = stash[i][tx];
+#pragma unroll
+      for (int i = 0; i < 3; i++) k.phiP[i] = stash[9 + i][tx];
+#pragma unroll
+      for (int i = 0; i < 6; i++) strain[i] = stash[12 + i][tx];
+      k.w = stash[18][tx];
+      k.g0 = stash[19][tx];
+      k.g1 = stash[20][tx];
       if (it < 0) *d.err = -3;
       if (MODE == MODE_STEP) {
         if (sp.track_diss) {
@@ -200,8 +234,9 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
             double t = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-              const double so = C[i] * (d.eo[i * E + e] - epo[i]);
-              t += ((so + sm[i]) / 2.0) * (ep[i] - epo[i]);
+              const double epo = d.ep[i * E + e]; // committed plastic strain (updated below)
+              const double so = C[i] * (d.eo[i * E + e] - epo);
+              t += ((so + sm[i]) / 2.0) * (ep[i] - epo);
             }
             diss = k.w * t;
           }
@@ -211,8 +246,9 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
         if (it > 0) {
 #pragma unroll
           for (int i = 0; i < 6; i++) d.ep[i * E + e] = ep[i];
+          if (pp.has_iso) d.al[e] = al[0];
 #pragma unroll
-          for (int i = 0; i < 7; i++) d.al[i * E + e] = al[i];
+          for (int i = 1; i < 7; i++) d.al[i * E + e] = al[i];
         }
       } else {
 #pragma unroll
@@ -256,7 +292,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 #pragma unroll
       for (int i = 0; i < 6; i++) d.fout[i * N + na] = f[i];
     } else {
-      node_update<SO3>(d, sp, na, f, qna, ra);
+      node_update<SO3>(d, sp, na, f);
     }
     if (fl & 2u) { // last element of the chain: also node b
       const int32_t sl = d.sb[e];
@@ -267,7 +303,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 #pragma unroll
         for (int i = 0; i < 6; i++) d.fout[i * N + nb] = fb[i];
       } else {
-        node_update<SO3>(d, sp, nb, fb, qnb, rb);
+        node_update<SO3>(d, sp, nb, fb);
       }
     }
   }
@@ -296,19 +332,28 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
   const int32_t n = d.jnode[j];
   const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
   double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int32_t t = t0; t < t1; t++) {
+  // four slots per iteration with their loads in flight together; the
+  // additions keep the slot order, so the sum is the same bits as one by one
+  for (int32_t t = t0; t < t1; t += 4) {
+    double g[4][6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + t];
+    for (int q = 0; q < 4; q++) {
+      const int64_t tq = (t + q < t1) ? t + q : t;
+#pragma unroll
+      for (int i = 0; i < 6; i++) g[q][i] = (t + q < t1) ? d.sbuf[i * (int64_t)d.nslot + tq] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+        if (t + q < t1) f[i] += g[q][i];
   }
   if (MODE == MODE_FORCE) {
 #pragma unroll
     for (int i = 0; i < 6; i++) d.fout[i * d.N + n] = f[i];
     return;
   }
-  double r[4];
-  Quat q;
-  load_rot<SO3>(d, n, r, q);
-  node_update<SO3>(d, sp, n, f, q, r);
+  node_update<SO3>(d, sp, n, f);
 }
 
 // ------------------------------------------------------------------
