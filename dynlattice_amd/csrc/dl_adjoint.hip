@@ -1,7 +1,8 @@
 // dl_adjoint.hip -- reverse sweep of one explicit step (SO(3) mode, elastic
 // material), as three gfx950 kernels plus the contact adjoint
 // (dl_contact.hip):
-//   k_adj_node    one thread per node: VJP of the node update (mass solve,
+//   k_adj_node    one thread per node: gather of the element ends of the step
+//                 reversed before, then the VJP of the node update (mass solve,
 //                 gyroscopic term, constraints, SO(3) composition) from the
 //                 taped start state; gives the adjoint of the node's internal
 //                 force (plus the objective's weight on the reaction), the
@@ -10,8 +11,8 @@
 //   k_adj_elem    one thread per element: VJP of the element internal force
 //                 (quaternion kinematics, strains, elastic stress resultants)
 //                 into per-element-end buffers and the stiffness adjoint.
-//   k_adj_gather  one thread per node: sums its element ends (CSR, fixed
-//                 order) into the displacement and rotation adjoints.
+//   The node kernel of the next reversed step first sums the node's element
+//   ends (CSR, fixed order) into its displacement and rotation adjoints.
 // Like the forward step, assembly is a gather: no floating-point atomics,
 // the result is bitwise reproducible.
 #include <hip/hip_runtime.h>
@@ -21,11 +22,18 @@
 
 #define AB 256
 
-__global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_all) {
+__global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_all, int gather) {
   const int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x;
   if (n >= d.N) return;
-  const int64_t N = d.N;
+  const int64_t N = d.N, E = d.E;
   const double *t = a.tape;
+  if (d.uref) {
+    // contact candidate lists: stale for this taped state?
+    bool moved = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) moved = moved || fabs(t[c * N + n] - d.uref[c * N + n]) > d.ctrig;
+    if (moved) *d.cgate = 1;
+  }
   Quat q;
   q.w = t[3 * N + n]; q.x = t[4 * N + n]; q.y = t[5 * N + n]; q.z = t[6 * N + n];
   double v[6], f[6];
@@ -58,6 +66,24 @@ __global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_
 #pragma unroll
   for (int c = 0; c < 3; c++) ubn[c] = a.ubar[c * N + n];
   qbn.w = a.qbar[n]; qbn.x = a.qbar[N + n]; qbn.y = a.qbar[2 * N + n]; qbn.z = a.qbar[3 * N + n];
+  if (gather) {
+    // element ends of the step reversed before (the next one in time): their
+    // displacement and rotation adjoints, summed in CSR order
+    double ub[3] = {0.0, 0.0, 0.0}, qb[4] = {0.0, 0.0, 0.0, 0.0};
+    const int32_t t1 = a.nptr[n + 1];
+    for (int32_t k = a.nptr[n]; k < t1; k++) {
+      const int32_t code = a.nlist[k];
+      const int64_t e = code >> 1;
+      const int o = (code & 1) ? 7 : 0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) ub[c] += a.ebuf[(o + c) * E + e];
+#pragma unroll
+      for (int c = 0; c < 4; c++) qb[c] += a.ebuf[(o + 3 + c) * E + e];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) ubn[c] += ub[c];
+    qbn.w += qb[0]; qbn.x += qb[1]; qbn.y += qb[2]; qbn.z += qb[3];
+  }
 #pragma unroll
   for (int c = 0; c < 6; c++) vbn[c] = a.vbar[c * N + n];
   NodeVJP r;
@@ -70,6 +96,10 @@ __global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_
   for (int c = 0; c < 6; c++) {
     a.fbar[c * N + n] = r.f[c];
     a.vbar[c * N + n] = r.v[c];
+  }
+  if (gather) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) a.ubar[c * N + n] = r.u[c];
   }
   a.qbar[n] = r.q.w; a.qbar[N + n] = r.q.x; a.qbar[2 * N + n] = r.q.y; a.qbar[3 * N + n] = r.q.z;
   a.mtbar[n] += r.mt;
@@ -115,35 +145,11 @@ __global__ __launch_bounds__(AB) void k_adj_elem(DevData d, AdjDev a) {
   for (int i = 0; i < 6; i++) a.Cbar[i * E + e] += r.C[i];
 }
 
-__global__ __launch_bounds__(AB) void k_adj_gather(DevData d, AdjDev a) {
-  const int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x;
-  if (n >= d.N) return;
-  const int64_t N = d.N, E = d.E;
-  double ub[3] = {0.0, 0.0, 0.0}, qb[4] = {0.0, 0.0, 0.0, 0.0};
-  const int32_t t1 = a.nptr[n + 1];
-  for (int32_t t = a.nptr[n]; t < t1; t++) {
-    const int32_t code = a.nlist[t];
-    const int64_t e = code >> 1;
-    const int o = (code & 1) ? 7 : 0;
-#pragma unroll
-    for (int c = 0; c < 3; c++) ub[c] += a.ebuf[(o + c) * E + e];
-#pragma unroll
-    for (int c = 0; c < 4; c++) qb[c] += a.ebuf[(o + 3 + c) * E + e];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; c++) a.ubar[c * N + n] += ub[c];
-#pragma unroll
-  for (int c = 0; c < 4; c++) a.qbar[c * N + n] += qb[c];
-}
-
 static unsigned nblk_a(int64_t n) { return (unsigned)((n + AB - 1) / AB); }
 
-void launch_adj_node(const DevData &d, const AdjDev &a, double dt, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_node, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a, dt);
+void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_node, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a, dt, (int)gather);
 }
 void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_elem, dim3(nblk_a(d.E)), dim3(AB), 0, s, d, a);
-}
-void launch_adj_gather(const DevData &d, const AdjDev &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_gather, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a);
 }

@@ -6,7 +6,8 @@
 // Backward: segments in reverse order; each segment is recomputed from its
 // checkpoint with a tape (the start state and internal force of every node
 // at every step of the segment), then reversed step by step:
-//   k_adj_node -> k_adj_elem -> [contact broadphase + pair VJPs] -> k_adj_gather.
+//   k_adj_node (gather of the previous reversal's element ends + node VJP)
+//   -> k_adj_elem -> [contact list refresh + pair VJPs].
 // Parameter adjoints (per-element stiffness, per-node mass and inertia,
 // per-element contact distance) accumulate over all steps and are mapped to
 // the per-strut section data on the host, through the same set-up functions
@@ -261,18 +262,18 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
       launch_strut_pp(dj, sp, c->pp, MODE_STEP, true, c->mat_kind, c->contact, s);
       launch_joint(dj, sp, MODE_STEP, true, s);
     }
-    // reverse it
+    // reverse it; the node kernel first gathers the element ends of the
+    // step reversed before (none before the last step)
     for (int64_t j = len - 1; j >= 0; j--) {
       a.tape = A.tape + j * 19 * N;
       a.rbar = A.rbar + (t0 + j) * ng;
-      launch_adj_node(c->d, a, A.dt, s);
+      launch_adj_node(c->d, a, A.dt, t0 + j < A.nsteps - 1, s);
       launch_adj_elem(c->d, a, s);
       if (c->contact) {
         DevData dn = c->d;
         dn.u = (double *)a.tape; // rows 0..2 of the tape: the step's start displacement
         if ((rc = contact_adjoint(c->cs, dn, a, s))) return rc;
       }
-      launch_adj_gather(c->d, a, s);
     }
     HIPCHK(hipGetLastError());
   }

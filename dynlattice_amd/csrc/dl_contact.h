@@ -26,9 +26,8 @@ struct ContactDev {
   int32_t dim[3];
   int64_t ncell;
   double h0;
-  double *hbuf;         // [1] cell size of the current evaluation
+  unsigned long long *extbits; // [1] largest box extent (double bits) of the current evaluation
   double *box;          // [6][E]
-  double *ext_part;     // [nblk]
   int32_t *cell;        // [E]
   int32_t *rank;        // [E]
   int32_t *ccount;      // [ncell]
@@ -55,10 +54,9 @@ struct ContactDev {
   double skin;          // > 0: lists on
   int32_t *gate;        // [1] 1 = rebuild the lists at the next step
   double *uref;         // [3][N] displacement at the last build
-  int32_t *vcount;      // [E]
-  int32_t *vptr;        // [E+1]
-  int32_t *vlist;       // [vcap]
-  int64_t vcap;
+  int32_t *vcount;      // [E] partners listed
+  int32_t *vlist;       // [E][vk] partner ids
+  int32_t vk;           // list capacity per element
   int32_t *err;         // device error flag (capacity)
 };
 

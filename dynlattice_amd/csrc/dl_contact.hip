@@ -1,10 +1,11 @@
 // dl_contact.hip -- rod-rod penalty contact on gfx950.
 //
 // Per force evaluation:
-//   kc_boxes     element AABBs (nodes +- own contact radius), block max extent
-//   kc_grid      cell size h = max(h0, 1.001 * largest extent)
-//   kc_cells     integer cell of each AABB's min corner (clamped to the grid),
-//                per-cell counts with integer atomics
+//   kc_boxes     element AABBs (nodes +- own contact radius), largest extent
+//                (integer atomic max of the bit patterns)
+//   kc_cells     cell size h = max(h0, 1.001 * largest extent); integer cell of
+//                each AABB's min corner (clamped to the grid), per-cell counts
+//                with integer atomics
 //   scan         exclusive scan of the counts (3 launches)
 //   kc_scatter   element ids into their cells; kc_place writes each element's
 //                record at its stable (id-ordered) position in the cell
@@ -89,24 +90,9 @@ __global__ __launch_bounds__(CB) void kc_boxes(ContactDev c, DevData d, double i
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) c.ext_part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(1024) void kc_grid(ContactDev c, int32_t nblk, const int32_t *gate) {
-  GATED(gate);
-  __shared__ double red[1024];
-  double m = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 1024) m = fmax(m, c.ext_part[i]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    c.hbuf[0] = fmax(c.h0, 1.001 * red[0]);
-    c.slot_ctr[0] = 0;
-  }
+  // largest extent over the blocks: the bit patterns of non-negative doubles
+  // order like the values, so an integer max is exact and order-independent
+  if (threadIdx.x == 0) atomicMax(c.extbits, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 __global__ __launch_bounds__(CB) void kc_fill_i32(int32_t *p, int64_t n, int32_t v,
@@ -127,7 +113,8 @@ __device__ __forceinline__ int32_t cell_coord(double x, double o, double h, int3
 // it so)
 __global__ __launch_bounds__(CB) void kc_cells(ContactDev c, const int32_t *gate) {
   GATED(gate);
-  const double h = c.hbuf[0];
+  // cell size h = max(h0, 1.001 * largest extent)
+  const double h = fmax(c.h0, 1.001 * __longlong_as_double((long long)*c.extbits));
   const int64_t E = c.E;
   GRID_STRIDE(e, E) {
     const int32_t cx = cell_coord(c.box[e], c.org[0], h, c.dim[0]);
@@ -212,6 +199,12 @@ struct __align__(64) CRec {
 // reset to zero for the next broadphase
 __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *gate) {
   GATED(gate);
+  // the extent maximum has been used (kc_cells): reset it for the next
+  // broadphase, and the force-slot counter of kc_narrow
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *c.extbits = 0ull;
+    c.slot_ctr[0] = 0;
+  }
   GRID_STRIDE(e, c.E) {
     const int32_t cell = c.cell[e];
     c.tmp[c.cstart[cell] + c.rank[e]] = (int32_t)e;
@@ -493,23 +486,17 @@ __global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, Adj
 // summed in an order that does not depend on the cell layout or on when the
 // lists were built.
 // ------------------------------------------------------------------
-template <bool FILL>
 __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
   GRID_STRIDE(k, c.E) {
     const CRec x = ((const CRec *)c.rec)[k];
     const int32_t X = x.e;
     int32_t n = 0;
-    const int64_t o = FILL ? c.vptr[X] : 0;
-    const int64_t len = FILL ? c.vptr[X + 1] - o : 0;
-    int32_t *lst = c.vlist + o;
-    if (FILL && o + len > c.vcap) {
-      *c.err = DL_ERR_CAPACITY;
-      continue;
-    }
+    int32_t *lst = c.vlist + (int64_t)X * c.vk;
+    const int32_t cap = c.vk;
     scan_candidates(c, k, x, [&](int32_t Y, int32_t, int32_t, int32_t, bool XisA) {
       if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
-      if (FILL && n < len) {
+      if (n < cap) {
         // insertion by partner id
         int32_t i = n;
         while (i > 0 && lst[i - 1] > Y) {
@@ -520,7 +507,8 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
       }
       n++;
     });
-    if (!FILL) c.vcount[X] = n;
+    if (n > cap) *c.err = DL_ERR_CAPACITY;
+    c.vcount[X] = n < cap ? n : cap;
   }
 }
 
@@ -532,10 +520,6 @@ __global__ __launch_bounds__(CB) void kc_vcommit(ContactDev c, DevData d, const 
   }
 }
 
-__global__ void kc_gate_clear(int32_t *gate) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *gate = 0;
-}
-
 // contact forces of the step from the lists, one thread per element (id
 // order): exact boxes of X and of each listed Y, then the pair as in
 // kc_narrow; per-element force rows (cidx[X] = X) and per-step counts
@@ -544,6 +528,9 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
                                                  unsigned long long *cnt_cand,
                                                  const int32_t *step) {
   const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  // the lists are current (list_refresh ran before): clear the rebuild flag;
+  // the step's node updates may set it again
+  if (X == 0) *c.gate = 0;
   unsigned long long nact = 0, ncand = 0;
   if (X < c.E) {
     const int64_t N = c.N, E = c.E;
@@ -554,10 +541,10 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
     elem_box(d, N, c.en0[X], c.en1[X], rX, xlo, xhi);
     double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
     bool any = false;
-    // a list that did not fit (error flagged at the build) is not read
-    const int32_t t1 = (c.vptr[X + 1] <= c.vcap) ? c.vptr[X + 1] : c.vptr[X];
-    for (int32_t t = c.vptr[X]; t < t1; t++) {
-      const int32_t Y = c.vlist[t];
+    const int32_t *lst = c.vlist + X * c.vk;
+    const int32_t t1 = c.vcount[X];
+    for (int32_t t = 0; t < t1; t++) {
+      const int32_t Y = lst[t];
       const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
       const double rY = c.crad[sY];
       double ylo[3], yhi[3];
@@ -610,21 +597,11 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
   }
 }
 
-// list trigger for a state that was not produced by a step (the reverse
-// sweep walks backwards through taped states): any node moved by more than
-// the trigger distance since the build -> rebuild
-__global__ __launch_bounds__(CB) void kc_vcheck(ContactDev c, DevData d, double trig) {
-  const int64_t n = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (n >= c.N) return;
-  bool moved = false;
-  for (int i = 0; i < 3; i++) moved = moved || fabs(d.u[i * c.N + n] - c.uref[i * c.N + n]) > trig;
-  if (moved) *c.gate = 1;
-}
-
 // Adjoint of the contact forces from the lists (see kc_adj_narrow), one
 // thread per element in id order.
 __global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, AdjDev a) {
   const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (X == 0) *c.gate = 0; // lists current for this state (see kc_vnarrow)
   if (X >= c.E) return;
   const int64_t N = c.N, E = c.E;
   const int32_t sX = c.es[X], xa = c.en0[X], xb = c.en1[X];
@@ -638,9 +615,10 @@ __global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, Ad
   }
   double g[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   bool any = false;
-  const int32_t t1 = (c.vptr[X + 1] <= c.vcap) ? c.vptr[X + 1] : c.vptr[X];
-  for (int32_t t = c.vptr[X]; t < t1; t++) {
-    const int32_t Y = c.vlist[t];
+  const int32_t *lst = c.vlist + X * c.vk;
+  const int32_t t1 = c.vcount[X];
+  for (int32_t t = 0; t < t1; t++) {
+    const int32_t Y = lst[t];
     const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
     const double rY = c.crad[sY];
     double ylo[3], yhi[3];
@@ -775,9 +753,9 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = up(cs, &pi, nslist))) return rc; c.nslist = pi;
   cs.nblk_e = (int32_t)nb(E);
   cs.nscan_blk = (std::max<int64_t>(ncell, E) + 1023) / 1024 + 1;
-  if ((rc = calloc_dev(cs, &c.hbuf, 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.extbits, 1))) return rc;
+  CHK(hipMemset(c.extbits, 0, sizeof(unsigned long long)));
   if ((rc = calloc_dev(cs, &c.box, 6 * E))) return rc;
-  if ((rc = calloc_dev(cs, &c.ext_part, cs.nblk_e))) return rc;
   if ((rc = calloc_dev(cs, &c.cell, E))) return rc;
   if ((rc = calloc_dev(cs, &c.rank, E))) return rc;
   if ((rc = calloc_dev(cs, &c.ccount, ncell))) return rc;
@@ -806,12 +784,14 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   d.cgate = nullptr;
   d.ctrig = 0.0;
   if (c.skin > 0.0) {
-    c.vcap = co->max_pairs > 0 ? co->max_pairs : 32 * E;
+    // list capacity per element: max_pairs / E partners, default 48
+    const int64_t vk = co->max_pairs > 0 ? std::max<int64_t>(1, co->max_pairs / E) : 48;
+    if (vk > INT32_MAX) return dl_set_error(DL_ERR_INPUT, "max_pairs too large");
+    c.vk = (int32_t)vk;
     if ((rc = calloc_dev(cs, &c.gate, 1))) return rc;
     if ((rc = calloc_dev(cs, &c.uref, 3 * N))) return rc;
     if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
-    if ((rc = calloc_dev(cs, &c.vptr, E + 1))) return rc;
-    if ((rc = calloc_dev(cs, &c.vlist, c.vcap))) return rc;
+    if ((rc = calloc_dev(cs, &c.vlist, (size_t)E * c.vk))) return rc;
     const int32_t one = 1; // lists are built at the first step
     CHK(hipMemcpy(c.gate, &one, sizeof one, hipMemcpyHostToDevice));
     d.uref = c.uref;
@@ -843,7 +823,6 @@ static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double
   ContactDev &c = cs.c;
   const unsigned ge = gs(c.E);
   hipLaunchKernelGGL(kc_boxes, dim3(ge), dim3(CB), 0, s, c, d, infl, gate);
-  hipLaunchKernelGGL(kc_grid, dim3(1), dim3(1024), 0, s, c, (int32_t)ge, gate);
   hipLaunchKernelGGL(kc_cells, dim3(ge), dim3(CB), 0, s, c, gate);
   scan_i32(cs, c.ccount, c.ncell, c.cstart, s, gate);
   hipLaunchKernelGGL(kc_scatter, dim3(ge), dim3(CB), 0, s, c, gate);
@@ -926,11 +905,8 @@ static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   const int32_t *g = c.gate;
   broadphase(cs, d, s, c.skin, g);
-  hipLaunchKernelGGL(kc_vbuild<false>, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
-  scan_i32(cs, c.vcount, c.E, c.vptr, s, g);
-  hipLaunchKernelGGL(kc_vbuild<true>, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vbuild, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
   hipLaunchKernelGGL(kc_vcommit, dim3(gs(c.N)), dim3(CB), 0, s, c, d, g);
-  hipLaunchKernelGGL(kc_gate_clear, dim3(1), dim3(1), 0, s, c.gate);
 }
 
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
@@ -977,7 +953,6 @@ int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStre
     broadphase(cs, d, s);
     hipLaunchKernelGGL(kc_adj_narrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
   } else {
-    hipLaunchKernelGGL(kc_vcheck, dim3(nb(c.N)), dim3(CB), 0, s, c, d, 0.95 * c.skin);
     list_refresh(cs, d, s);
     hipLaunchKernelGGL(kc_adj_vnarrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
   }

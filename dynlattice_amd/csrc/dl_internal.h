@@ -124,9 +124,8 @@ void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *edis
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
                      hipStream_t s);
-// adjoint (dl_adjoint.hip): node update, element force, gather to nodes
-void launch_adj_node(const DevData &d, const AdjDev &a, double dt, hipStream_t s);
+// adjoint (dl_adjoint.hip): [gather of the element ends +] node update, element force
+void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s);
 void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
-void launch_adj_gather(const DevData &d, const AdjDev &a, hipStream_t s);
 
 #endif

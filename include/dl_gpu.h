@@ -115,7 +115,8 @@ typedef struct {
   double penalty_sts;
   double penalty_nts;
   const double *radius; /* [n_struts] */
-  int64_t max_pairs;    /* candidate-pair capacity; 0 = 32 * n_elems */
+  int64_t max_pairs;    /* step candidate-list capacity (partners summed over the
+                           elements; max_pairs / n_elems per element); 0 = 48 * n_elems */
   double skin;          /* candidate-list skin distance of the step: 0 = half the
                            largest radius, < 0 = full broadphase every step */
 } dl_contact;
