@@ -85,6 +85,49 @@ def cpu_baseline(p, budget_s, steps_gpu):
                       f"{dt:.1f} s"}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "bcc40_elastoplastic_contact.md")
+
+
+def profiled_traffic(workload):
+    """HBM bytes per k_strut launch measured by rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE, scaled by the copy calibration) and recorded in the profile
+    summary of this workload, or None."""
+    if workload != "bcc-40x40x40-m4-elastoplastic-contact" or not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        if line.startswith("k_strut HBM bytes per launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes on the library's device, timed with
+    HIP events: bytes read + written per second (GB/s), best of reps."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    best = 0.0
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        best = max(best, 2.0 * nbytes / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return best
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A device-to-device copy of known size (the runtime's copy kernel), used
     to scale the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs."""
@@ -304,10 +347,18 @@ def main():
         bytes_strut = b_elem * p.n_elems + B_NODE * npriv
         t_strut = phases["strut"] * 1e-3
         achieved = bytes_strut / t_strut / 1e9
+        traffic = profiled_traffic(p.name)
+        copy_gbs = copy_bandwidth()
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_strut", "algorithmic_bytes_per_launch": bytes_strut,
-                "avg_launch_ms": round(phases["strut"], 5)}
+                "avg_launch_ms": round(phases["strut"], 5),
+                # practical ceiling on this device: a 1 GiB device copy
+                "device_copy_GBs": round(copy_gbs, 1)}
+        if traffic:
+            actual = traffic / t_strut / 1e9
+            roof["actual_GBs"] = round(actual, 1)
+            roof["actual_frac_of_device_copy"] = round(actual / copy_gbs, 4)
 
     res = {
         "metric": "element-GP updates/s per explicit step (FP64)",
