@@ -66,11 +66,21 @@ def private_node_count(p):
     return int((cnt == 1).sum())
 
 
-def cpu_baseline(p, budget_s, steps_gpu):
-    """The oracle (single-threaded C restatement) on this host, bounded sample:
-    the same lattice, K steps with K chosen so the timed part takes ~budget_s."""
+def cpu_threads():
+    """host threads the all-cores oracle uses (OpenMP's OMP_NUM_THREADS)"""
+    try:
+        return int(os.environ["OMP_NUM_THREADS"])
+    except (KeyError, ValueError):
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(p, budget_s, steps_gpu, omp):
+    """The oracle (C restatement, oracle/ref_cpu.c) on this host, bounded
+    sample: the same lattice, K steps with K chosen so the timed part takes
+    about budget_s.  omp: the all-cores build (struts and pairs threaded
+    with OpenMP) on cpu_threads() threads; else the serial build."""
     from oracle.ref_cpu import RefModel
-    ref = RefModel(p, grid=True)
+    ref = RefModel(p, grid=True, omp=omp)
     t0 = time.perf_counter()
     ref.leapfrog(1)
     t1 = time.perf_counter() - t0
@@ -78,11 +88,13 @@ def cpu_baseline(p, budget_s, steps_gpu):
     t0 = time.perf_counter()
     ref.leapfrog(k)
     dt = time.perf_counter() - t0
-    return {"value": p.n_elems * k / dt, "unit": "element-GP updates/s", "cores": 1,
+    cores = cpu_threads() if omp else 1
+    build = "OpenMP over struts and contact pairs" if omp else "1 thread"
+    return {"value": p.n_elems * k / dt, "unit": "element-GP updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"oracle (oracle/ref_cpu.c, gcc -O3 -ffp-contract=off, 1 thread) on the "
-                      f"same {p.name} lattice, {k} LeapFrog steps after 1 warm-up step, "
-                      f"{dt:.1f} s"}
+            "sample": f"oracle (oracle/ref_cpu.c, gcc -O3 -ffp-contract=off, {build}, "
+                      f"{cores} thread(s)) on the same {p.name} lattice, {k} LeapFrog steps "
+                      f"after 1 warm-up step, {dt:.1f} s"}
 
 
 PROFILE = os.path.join(ROOT, "profiles", "bcc40_elastoplastic_contact.md")
@@ -382,9 +394,12 @@ def main():
         "phase_ms": {k: round(v, 5) for k, v in phases.items()},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(p, args.cpu_budget, args.steps)
+        cb = cpu_baseline(p, args.cpu_budget, args.steps, omp=True)
+        c1 = cpu_baseline(p, args.cpu_budget, args.steps, omp=False)
         res["cpu_baseline"] = cb
         res["speedup_vs_cpu"] = value / cb["value"]
+        res["cpu_baseline_serial"] = c1
+        res["speedup_vs_cpu_serial"] = value / c1["value"]
     print(json.dumps(res), flush=True)
 
 

@@ -32,6 +32,11 @@
 #include <string.h>
 
 #define TINY_H (DBL_EPSILON * 1e6) /* helpers.h:55 */
+#ifdef REF_OMP
+#define REF_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define REF_PARALLEL_FOR
+#endif
 #define REF_IS_TINY(x) (fabs(x) < DBL_MIN)
 
 /* ------------------------------------------------------------------ */
@@ -233,6 +238,9 @@ struct ref_model {
   int so3;
   double inertia_correct;
   int err;
+  /* all-cores variant: strut-local node rows and the node -> rows map */
+  int64_t *st_off, *nd_ptr, *nd_row;
+  double *loc_blk, *loc_f, *loc_g;
 };
 
 int ref_last_error(const ref_model *m) { return m->err; }
@@ -392,6 +400,8 @@ void ref_destroy(ref_model *m) {
   free(m->dflow); free(m->ediss); free(m->ehard); free(m->active);
   free(m->crad); free(m->black); free(m->actA); free(m->actB);
   free(m->cmask); free(m->cval); free(m->fext);
+  free(m->st_off); free(m->nd_ptr); free(m->nd_row);
+  free(m->loc_blk); free(m->loc_f); free(m->loc_g);
   free(m);
 }
 
@@ -927,7 +937,23 @@ static void broadphase_grid(ref_model *m, const double *disp, pairbuf *out) {
   int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * ncell);
   memcpy(fill, start, sizeof(int64_t) * ncell);
   for (int64_t e = 0; e < E; e++) sorted[fill[cell[e]]++] = (int32_t)e;
+#ifdef REF_OMP
+  /* all-cores variant: count per element, then fill at the prefix offsets,
+   * which keeps the serial candidate order */
+  int64_t *cnt = (int64_t *)calloc(E + 1, sizeof(int64_t));
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      for (int64_t a = 0; a < E; a++) cnt[a + 1] += cnt[a];
+      out->a = (int32_t *)malloc(sizeof(int32_t) * (cnt[E] ? cnt[E] : 1));
+      out->b = (int32_t *)malloc(sizeof(int32_t) * (cnt[E] ? cnt[E] : 1));
+      out->n = out->cap = cnt[E];
+    }
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t a = 0; a < E; a++) {
+      int64_t k_out = pass ? cnt[a] : 0;
+#else
   for (int64_t a = 0; a < E; a++) {
+#endif
     const int64_t ca = cell[a];
     const int64_t cx = ca % dim[0], cy = (ca / dim[0]) % dim[1], cz = ca / (dim[0] * dim[1]);
     const int32_t sa = m->es[a];
@@ -942,10 +968,21 @@ static void broadphase_grid(ref_model *m, const double *disp, pairbuf *out) {
             const int32_t sb = m->es[b];
             if (sb <= sa) continue; /* A belongs to the lower strut index */
             if (!elem_free_of(m, a, sb) || !elem_free_of(m, b, sa)) continue;
-            if (box_overlap(boxes + 6 * a, boxes + 6 * b)) pb_push(out, (int32_t)a, b);
+            if (box_overlap(boxes + 6 * a, boxes + 6 * b)) {
+#ifdef REF_OMP
+              if (pass) { out->a[k_out] = (int32_t)a; out->b[k_out] = b; k_out++; }
+              else cnt[a + 1]++;
+#else
+              pb_push(out, (int32_t)a, b);
+#endif
+            }
           }
         }
   }
+#ifdef REF_OMP
+  }
+  free(cnt);
+#endif
   free(boxes); free(cell); free(start); free(sorted); free(fill);
 }
 
@@ -1081,16 +1118,33 @@ int64_t ref_contact_force(ref_model *m, const double *disp, double *fint) {
   if (!m->co.enabled) return 0;
   pairbuf pb = {0};
   broadphase(m, disp, &pb);
+#ifdef REF_OMP
+  /* all-cores variant: pair forces in parallel, added in candidate order */
+  double *pf = (double *)malloc(sizeof(double) * 12 * (pb.n ? pb.n : 1));
+  uint8_t *closed = (uint8_t *)malloc(pb.n ? pb.n : 1);
+  #pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < pb.n; i++)
+    closed[i] = !is_blacklisted(m, pb.a[i], pb.b[i]) &&
+                narrowphase(m, pb.a[i], pb.b[i], disp, pf + 12 * i);
+#endif
   for (int64_t i = 0; i < pb.n; i++) {
     const int32_t ea = pb.a[i], eb = pb.b[i];
+#ifdef REF_OMP
+    if (!closed[i]) continue;
+    const double *f = pf + 12 * i;
+#else
     if (is_blacklisted(m, ea, eb)) continue;
     double f[12];
     if (!narrowphase(m, ea, eb, disp, f)) continue;
+#endif
     const int32_t nd[4] = {m->en[2 * ea], m->en[2 * ea + 1], m->en[2 * eb], m->en[2 * eb + 1]};
     for (int k = 0; k < 4; k++)
       for (int d = 0; d < 3; d++) fint[6 * nd[k] + d] += f[3 * k + d];
     act_push(m, ea, eb);
   }
+#ifdef REF_OMP
+  free(pf); free(closed);
+#endif
   free(pb.a); free(pb.b);
   return m->n_act;
 }
@@ -1147,84 +1201,206 @@ int ref_frames(ref_model *m, double *lambda_n, double *strain0) {
 /* ------------------------------------------------------------------ */
 /* mass (assembleM_, SpecialCosseratRodModel.cpp:969-1031)             */
 /* ------------------------------------------------------------------ */
+/* assembleM_ contributions of one element (SpecialCosseratRodModel.cpp:
+ * 969-1031): SI[in][jn] is the 6x6 block that row node in receives for
+ * column node jn, in the order the reference adds them */
+static void elem_mass_si(const ref_model *m, int32_t s, int64_t e, const double *disp,
+                         double SI[2][2][36]) {
+  const double xi[2] = {-1. / sqrt(3.), 1. / sqrt(3.)};
+  const double *Mm = m->Mm + 6 * s;
+  double c[2][3], u[2][3], L[2][9];
+  elem_gather(m, e, disp, c, u, L);
+  /* Gauss2 rotations (getRotations with massShape) */
+  double Lr[9], psi0[3], psi1[3], Lip[2][9];
+  node_rot_vecs(L[0], L[1], Lr, psi0, psi1);
+  double sh[2][2]; /* sh[node][ip] */
+  for (int ip = 0; ip < 2; ip++) {
+    sh[0][ip] = 0.5 * (1. - xi[ip]);
+    sh[1][ip] = 0.5 * (1. + xi[ip]);
+    double ps[3], Ex[9];
+    for (int i = 0; i < 3; i++) {
+      double t = psi0[i] * sh[0][ip];
+      t += psi1[i] * sh[1][ip];
+      ps[i] = t;
+    }
+    ref_expvec(ps, Ex);
+    mm3(Lr, Ex, Lip[ip]);
+  }
+  /* getIntegrationWeights: w_ip = 1 * |J|, J = 0.5 L */
+  double d[3] = {c[1][0] - c[0][0], c[1][1] - c[0][1], c[1][2] - c[0][2]};
+  const double Lc = norm3(d) + 0.;
+  const double Jc = -0.5 * 0. + 0.5 * Lc;
+  const double wts[2] = {1. * Jc, 1. * Jc};
+  const double l = (wts[0] + wts[1]) / (2 - 1);
+  /* material masses */
+  const double Mt[9] = {Mm[0], 0, 0, 0, Mm[1], 0, 0, 0, Mm[2]};
+  double Mr[9] = {Mm[3] * l, 0, 0, 0, Mm[4] * l, 0, 0, 0, Mm[5] * l};
+  const double add = m->area[s] * m->rho[s] * pow(l, 3) / 12.;
+  Mr[0] += add;
+  Mr[4] += add;
+  for (int in = 0; in < 2; in++) {
+    for (int jn = 0; jn < 2; jn++) {
+      double *S = SI[in][jn];
+      memset(S, 0, 36 * sizeof(double));
+      for (int ip = 0; ip < 2; ip++) {
+        double T[9], T2[9];
+        mm3(Lip[ip], Mt, T);
+        mmt3(T, Lip[ip], T2);
+        const double sc = wts[ip] * sh[in][ip] * sh[jn][ip];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) S[6 * i + j] += sc * T2[3 * i + j];
+      }
+      if (in == jn) {
+        double T[9], T2[9];
+        mm3(L[in], Mr, T);
+        mmt3(T, L[in], T2);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) S[6 * (3 + i) + 3 + j] += T2[3 * i + j];
+        if (in == 0 || in == 1)
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) S[6 * (3 + i) + 3 + j] /= 2.;
+      }
+    }
+  }
+}
+
+static void add_block(double *B, const double *S, int lumped) {
+  if (lumped) {
+    for (int i = 0; i < 6; i++) B[7 * i] += S[7 * i];
+  } else {
+    for (int i = 0; i < 36; i++) B[i] += S[i];
+  }
+}
+
+#ifndef REF_OMP
 int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped) {
   const int64_t N = m->N;
   memset(blocks, 0, sizeof(double) * 36 * N);
-  const double xi[2] = {-1. / sqrt(3.), 1. / sqrt(3.)};
   for (int32_t s = 0; s < m->R; s++) {
     const strut_t *S = &m->st[s];
-    const double *Mm = m->Mm + 6 * s;
     for (int ie = 0; ie < S->n_el; ie++) {
       const int64_t e = S->elems[ie];
-      double c[2][3], u[2][3], L[2][9];
-      elem_gather(m, e, disp, c, u, L);
-      /* Gauss2 rotations (getRotations with massShape) */
-      double Lr[9], psi0[3], psi1[3], Lip[2][9];
-      node_rot_vecs(L[0], L[1], Lr, psi0, psi1);
-      double sh[2][2]; /* sh[node][ip] */
-      for (int ip = 0; ip < 2; ip++) {
-        sh[0][ip] = 0.5 * (1. - xi[ip]);
-        sh[1][ip] = 0.5 * (1. + xi[ip]);
-        double ps[3], Ex[9];
-        for (int i = 0; i < 3; i++) {
-          double t = psi0[i] * sh[0][ip];
-          t += psi1[i] * sh[1][ip];
-          ps[i] = t;
-        }
-        ref_expvec(ps, Ex);
-        mm3(Lr, Ex, Lip[ip]);
-      }
-      /* getIntegrationWeights: w_ip = 1 * |J|, J = 0.5 L */
-      double d[3] = {c[1][0] - c[0][0], c[1][1] - c[0][1], c[1][2] - c[0][2]};
-      const double Lc = norm3(d) + 0.;
-      const double Jc = -0.5 * 0. + 0.5 * Lc;
-      const double wts[2] = {1. * Jc, 1. * Jc};
-      const double l = (wts[0] + wts[1]) / (2 - 1);
-      /* material masses */
-      const double Mt[9] = {Mm[0], 0, 0, 0, Mm[1], 0, 0, 0, Mm[2]};
-      double Mr[9] = {Mm[3] * l, 0, 0, 0, Mm[4] * l, 0, 0, 0, Mm[5] * l};
-      const double add = m->area[s] * m->rho[s] * pow(l, 3) / 12.;
-      Mr[0] += add;
-      Mr[4] += add;
-      for (int in = 0; in < 2; in++) {
-        const int32_t gn = m->en[2 * e + in];
-        for (int jn = 0; jn < 2; jn++) {
-          double SI[36];
-          memset(SI, 0, sizeof SI);
-          for (int ip = 0; ip < 2; ip++) {
-            double T[9], T2[9];
-            mm3(Lip[ip], Mt, T);
-            mmt3(T, Lip[ip], T2);
-            const double sc = wts[ip] * sh[in][ip] * sh[jn][ip];
-            for (int i = 0; i < 3; i++)
-              for (int j = 0; j < 3; j++) SI[6 * i + j] += sc * T2[3 * i + j];
-          }
-          if (in == jn) {
-            double T[9], T2[9];
-            mm3(L[in], Mr, T);
-            mmt3(T, L[in], T2);
-            for (int i = 0; i < 3; i++)
-              for (int j = 0; j < 3; j++) SI[6 * (3 + i) + 3 + j] += T2[3 * i + j];
-            if (in == 0 || in == 1)
-              for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) SI[6 * (3 + i) + 3 + j] /= 2.;
-          }
-          double *B = blocks + 36 * gn;
-          if (lumped) {
-            for (int i = 0; i < 6; i++) B[7 * i] += SI[7 * i];
-          } else {
-            for (int i = 0; i < 36; i++) B[i] += SI[i];
-          }
-        }
-      }
+      double SI[2][2][36];
+      elem_mass_si(m, s, e, disp, SI);
+      for (int in = 0; in < 2; in++)
+        for (int jn = 0; jn < 2; jn++) add_block(blocks + 36 * m->en[2 * e + in], SI[in][jn], lumped);
     }
   }
   return REF_OK;
 }
+#endif
 
 /* ------------------------------------------------------------------ */
 /* GET_INT_VECTOR                                                      */
 /* ------------------------------------------------------------------ */
+#ifdef REF_OMP
+/* All-cores variant (libref_cpu_omp.so, the CPU baseline on every host
+ * core): struts run in parallel, each accumulating its elements in order
+ * into strut-local node rows; every node then sums its struts' rows in strut
+ * order.  Deterministic for any thread count; equal to the serial path up
+ * to the association of the sums at nodes shared by several struts. */
+static int omp_prepare(ref_model *m) {
+  if (m->nd_ptr) return REF_OK;
+  const int64_t N = m->N;
+  m->st_off = (int64_t *)malloc(sizeof(int64_t) * (m->R + 1));
+  m->st_off[0] = 0;
+  for (int32_t s = 0; s < m->R; s++) m->st_off[s + 1] = m->st_off[s] + m->st[s].n_nd;
+  m->nd_ptr = (int64_t *)calloc(N + 1, sizeof(int64_t));
+  for (int32_t s = 0; s < m->R; s++)
+    for (int k = 0; k < m->st[s].n_nd; k++) m->nd_ptr[m->st[s].nodes[k] + 1]++;
+  for (int64_t n = 0; n < N; n++) m->nd_ptr[n + 1] += m->nd_ptr[n];
+  m->nd_row = (int64_t *)malloc(sizeof(int64_t) * (m->nd_ptr[N] ? m->nd_ptr[N] : 1));
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (N ? N : 1));
+  memcpy(fill, m->nd_ptr, sizeof(int64_t) * N);
+  for (int32_t s = 0; s < m->R; s++)
+    for (int k = 0; k < m->st[s].n_nd; k++)
+      m->nd_row[fill[m->st[s].nodes[k]]++] = m->st_off[s] + k;
+  free(fill);
+  const int64_t rows = m->st_off[m->R];
+  m->loc_blk = (double *)malloc(sizeof(double) * 36 * (rows ? rows : 1));
+  m->loc_f = (double *)malloc(sizeof(double) * 6 * (rows ? rows : 1));
+  m->loc_g = (double *)malloc(sizeof(double) * 6 * (rows ? rows : 1));
+  return m->loc_blk && m->loc_f && m->loc_g ? REF_OK : REF_ERR_INPUT;
+}
+
+int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped) {
+  if (omp_prepare(m)) return REF_ERR_INPUT;
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    double *loc = m->loc_blk + 36 * m->st_off[s];
+    memset(loc, 0, sizeof(double) * 36 * S->n_nd);
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      double SI[2][2][36];
+      elem_mass_si(m, s, e, disp, SI);
+      for (int in = 0; in < 2; in++)
+        for (int jn = 0; jn < 2; jn++)
+          add_block(loc + 36 * rod_local_node(S, m->en[2 * e + in]), SI[in][jn], lumped);
+    }
+  }
+  #pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < m->N; n++) {
+    double *B = blocks + 36 * n;
+    memset(B, 0, sizeof(double) * 36);
+    for (int64_t t = m->nd_ptr[n]; t < m->nd_ptr[n + 1]; t++)
+      for (int i = 0; i < 36; i++) B[i] += m->loc_blk[36 * m->nd_row[t] + i];
+  }
+  return REF_OK;
+}
+
+static int int_vector(ref_model *m, const double *disp, const double *velo, double *fint,
+                      double *fgyro, const double *blocks, int inelastic) {
+  if (omp_prepare(m)) return REF_ERR_INPUT;
+  int rc_all = REF_OK;
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    double *lf = m->loc_f + 6 * m->st_off[s], *lg = m->loc_g + 6 * m->st_off[s];
+    memset(lf, 0, sizeof(double) * 6 * S->n_nd);
+    memset(lg, 0, sizeof(double) * 6 * S->n_nd);
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      double f[12];
+      const int rc = ref_elem_fint(m, e, disp, f, inelastic);
+      if (rc) {
+        #pragma omp critical
+        rc_all = rc;
+        break;
+      }
+      for (int k = 0; k < 2; k++) {
+        double *r = lf + 6 * rod_local_node(S, m->en[2 * e + k]);
+        for (int i = 0; i < 6; i++) r[i] += f[6 * k + i];
+      }
+    }
+    if (fgyro) /* assembleGyro_: per rod, its nodes */
+      for (int k = 0; k < S->n_nd; k++) {
+        const int32_t n = S->nodes[k];
+        const double *B = blocks + 36 * n;
+        const double *v = velo + 6 * n;
+        double t[3], Sw[9], g[3];
+        for (int i = 0; i < 3; i++) {
+          double a = B[6 * (3 + i)] * v[0];
+          for (int j = 1; j < 6; j++) a += B[6 * (3 + i) + j] * v[j];
+          t[i] = a;
+        }
+        skew3(v + 3, Sw);
+        mv3(Sw, t, g);
+        for (int i = 0; i < 3; i++) lg[6 * k + 3 + i] += g[i];
+      }
+  }
+  if (rc_all) return rc_all;
+  #pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < m->N; n++)
+    for (int64_t t = m->nd_ptr[n]; t < m->nd_ptr[n + 1]; t++) {
+      const int64_t r = m->nd_row[t];
+      for (int i = 0; i < 6; i++) fint[6 * n + i] += m->loc_f[6 * r + i];
+      if (fgyro)
+        for (int i = 0; i < 6; i++) fgyro[6 * n + i] += m->loc_g[6 * r + i];
+    }
+  return REF_OK;
+}
+#else
 static int int_vector(ref_model *m, const double *disp, const double *velo, double *fint,
                       double *fgyro, const double *blocks, int inelastic) {
   for (int32_t s = 0; s < m->R; s++) {
@@ -1258,6 +1434,7 @@ static int int_vector(ref_model *m, const double *disp, const double *velo, doub
   }
   return REF_OK;
 }
+#endif
 
 int ref_get_int_vector(ref_model *m, const double *disp, const double *velo, double *fint,
                        int inelastic, int with_gyro, int with_contact) {
@@ -1375,6 +1552,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
       react[st] = r;
     }
     /* fres = fext - fint; getAcce */
+    REF_PARALLEL_FOR
     for (int64_t n = 0; n < N; n++) {
       double fr[6];
       for (int i = 0; i < 6; i++) fr[i] = m->fext[6 * n + i] - (fint[6 * n + i] + fg[6 * n + i]);
@@ -1388,7 +1566,9 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
       }
     }
     /* v_new = v + dt a;  u_new = u + dt v_new (+ SO(3) composition) */
+    REF_PARALLEL_FOR
     for (int64_t i = 0; i < nd; i++) velo[i] = velo[i] + dt * acc[i];
+    REF_PARALLEL_FOR
     for (int64_t n = 0; n < N; n++) {
       double *u = disp + 6 * n, *v = velo + 6 * n;
       double th_old[3] = {u[3], u[4], u[5]};
