@@ -218,6 +218,27 @@ def cantilever_c1() -> Problem:
     return p
 
 
+def swinging_rod(n_elem: int = 10) -> Problem:
+    """Swinging flexible rod under gravity (Lang, Linn, Arnold 2011,
+    Example 1; the reference's tests/transient/test2.pro): 1 m along x,
+    circle r = 5 mm, E = 5e6, nu = 0.4999999, shear correction 1,
+    rho = 1100; translations of the node at x = 0 fixed, rotations free;
+    gravity as nodal loads rho A L_e g along -z, half at both ends
+    (the Neumann factors -1 on all nodes, +0.5 on the two ends); at rest at
+    t = 0; first time step 1e-6 s (MilneDevice, adaptive)."""
+    p = straight_rod(n_elem, 1.0, 5e-3, 5e6, 0.4999999, 1.1e3)
+    p.sections.shear_factor[:] = 1.0
+    fixed, free = int(p.groups["fixed"][0]), int(p.groups["tip"][0])
+    p.cmask[fixed, :3] = 1
+    w = math.pi * (5e-3) ** 2 * (1.0 / n_elem) * 1.1e3 * 9.81
+    p.fext[:, 2] = -w
+    p.fext[fixed, 2] += 0.5 * w
+    p.fext[free, 2] += 0.5 * w
+    p.dt = 1e-6
+    p.name = f"swinging-rod-{n_elem}el"
+    return p
+
+
 def bcc_lattice(n: int, m: int, cell: float, radii: np.ndarray, young, poisson,
                 density) -> Problem:
     """BCC lattice of n^3 cells; 8 struts per cell from the cell centre to its

@@ -1608,6 +1608,150 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
   return rc;
 }
 
+/* SO(3)-aware state update y_new = y_old (+) dy (ExplicitModule.cpp:360-388):
+ * translations add; rotation vectors compose as log(exp(dr) exp(r_old)) */
+static void update_vec(const ref_model *m, const double *y_old, const double *dy,
+                       double *y_new, int rot) {
+  const int64_t N = m->N;
+  REF_PARALLEL_FOR
+  for (int64_t n = 0; n < N; n++) {
+    for (int i = 0; i < 6; i++) y_new[6 * n + i] = y_old[6 * n + i] + dy[6 * n + i];
+    if (rot && m->so3) {
+      double Ro[9], Vu[9], Rn[9];
+      ref_expvec(y_old + 6 * n + 3, Ro);
+      ref_expvec(dy + 6 * n + 3, Vu);
+      mm3(Vu, Ro, Rn);
+      ref_logmat(Rn, y_new + 6 * n + 3);
+    }
+  }
+}
+
+/* getQuality (ExplicitModule.cpp:490-499): RMS of y_pre - y_cor over all
+ * dofs, translational dofs divided by the length scale */
+static double quality(const ref_model *m, const double *y_pre, const double *y_cor,
+                      double len_scale) {
+  const int64_t nd = 6 * m->N;
+  double s = 0.;
+  for (int64_t i = 0; i < nd; i++) {
+    double d = y_pre[i] - y_cor[i];
+    if (i % 6 < 3 || !m->so3) d /= len_scale;
+    s += d * d;
+  }
+  return sqrt(s) / sqrt((double)nd);
+}
+
+/* forces and accelerations at (disp, velo) with the mass blocks of the
+ * step's start state: fres = fext - fint - fgyro, a = M^-1 fres (getForce /
+ * updForce + getAcce) */
+static int milne_acce(ref_model *m, const double *disp, const double *velo,
+                      const double *blocks, double *fint, double *fg, double *acc,
+                      int64_t *nc) {
+  const int64_t N = m->N, nd = 6 * N;
+  memset(fint, 0, sizeof(double) * nd);
+  memset(fg, 0, sizeof(double) * nd);
+  int rc = int_vector(m, disp, velo, fint, m->so3 ? fg : NULL, blocks, 1);
+  if (rc) return rc;
+  const int64_t c = m->co.enabled ? ref_contact_force(m, disp, fint) : 0;
+  if (nc) *nc = c;
+  REF_PARALLEL_FOR
+  for (int64_t n = 0; n < N; n++) {
+    double fr[6];
+    for (int i = 0; i < 6; i++) fr[i] = m->fext[6 * n + i] - (fint[6 * n + i] + fg[6 * n + i]);
+    if (m->so3) {
+      solve_block(blocks + 36 * n, fr, m->cmask + 6 * n, m->cval + 6 * n, acc + 6 * n);
+    } else {
+      for (int i = 0; i < 6; i++) {
+        acc[6 * n + i] = (1 / blocks[36 * n + 7 * i]) * fr[i];
+        if (m->cmask[6 * n + i]) acc[6 * n + i] = m->cval[6 * n + i];
+      }
+    }
+  }
+  return REF_OK;
+}
+
+/* MilneDevice attempts (MilneDeviceModule.cpp:41-112) with the adaptive step
+ * of ExplicitModule::commit (ExplicitModule.cpp:290-335), order 1:
+ *   a_cur = a(u, v);  v_pre = v + dt a_cur;  u_pre = u (+) dt v
+ *   a_pre = a(u_pre, v_pre) (mass of the start state);
+ *   v_new = v + dt a_pre;  u_new = u (+) dt v_pre
+ *   err = (quality(u_pre, u_new) + dt quality(v_pre, v_new)) / 2
+ *   dt_opt = dt (prec / err)^(1/2); accepted if err <= prec or dt <= dt_min,
+ *   then dt = max(min(s dt_opt, incr dt, dt_max), decr dt, dt_min), the
+ *   material state of the last evaluation is committed and the state moves
+ *   to (u_new, v_new); otherwise dt = max(s dt_opt, decr dt, dt_min).
+ * Per attempt: out[6*k] = dt used, err, accepted, reaction at the start
+ * state (group sum of f_int), contact pairs at the start state, new dt. */
+int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
+              int64_t n_attempts, const int32_t *group, int64_t group_n, int32_t group_dof,
+              double *out) {
+  const int64_t N = m->N, nd = 6 * N;
+  double *blocks = (double *)malloc(sizeof(double) * 36 * N);
+  double *fint = (double *)malloc(sizeof(double) * nd);
+  double *fg = (double *)malloc(sizeof(double) * nd);
+  double *acc = (double *)malloc(sizeof(double) * nd);
+  double *dy = (double *)malloc(sizeof(double) * nd);
+  double *u_pre = (double *)malloc(sizeof(double) * nd);
+  double *v_pre = (double *)malloc(sizeof(double) * nd);
+  double *u_new = (double *)malloc(sizeof(double) * nd);
+  double *v_new = (double *)malloc(sizeof(double) * nd);
+  double dt = o->dt;
+  int rc = REF_OK;
+  for (int64_t k = 0; k < n_attempts && !rc; k++) {
+    ref_mass_blocks(m, disp, blocks, !m->so3);
+    int64_t nc = 0;
+    if ((rc = milne_acce(m, disp, velo, blocks, fint, fg, acc, &nc))) break;
+    double r = 0.;
+    for (int64_t g = 0; g < group_n; g++) r += fint[6 * (int64_t)group[g] + group_dof];
+    /* predictor (ABupdate: dy = dt f) */
+    for (int64_t i = 0; i < nd; i++) dy[i] = dt * acc[i];
+    update_vec(m, velo, dy, v_pre, 0);
+    for (int64_t i = 0; i < nd; i++) dy[i] = dt * velo[i];
+    update_vec(m, disp, dy, u_pre, 1);
+    if ((rc = milne_acce(m, u_pre, v_pre, blocks, fint, fg, acc, NULL))) break;
+    /* corrector (AMupdate_: dy = dt f_pre) */
+    for (int64_t i = 0; i < nd; i++) dy[i] = dt * acc[i];
+    update_vec(m, velo, dy, v_new, 0);
+    for (int64_t i = 0; i < nd; i++) dy[i] = dt * v_pre[i];
+    update_vec(m, disp, dy, u_new, 1);
+    double err = 0.;
+    err += quality(m, u_pre, u_new, o->len_scale);
+    err += quality(m, v_pre, v_new, o->len_scale) * dt;
+    err = 0.5 * err;
+    /* ExplicitModule::commit, order 1 */
+    const double dt_opt = dt * pow(o->precision / err, 1. / (1. + 1.));
+    const int accept = (err <= o->precision || dt <= o->dt_min);
+    double dt_next;
+    if (accept) {
+      double t = o->safety * dt_opt;
+      if (o->incr * dt < t) t = o->incr * dt;
+      if (o->dt_max < t) t = o->dt_max;
+      if (t < o->decr * dt) t = o->decr * dt;
+      if (t < o->dt_min) t = o->dt_min;
+      dt_next = t;
+      apply_deform(m);
+      memcpy(disp, u_new, sizeof(double) * nd);
+      memcpy(velo, v_new, sizeof(double) * nd);
+    } else {
+      double t = o->safety * dt_opt;
+      if (t < o->decr * dt) t = o->decr * dt;
+      if (t < o->dt_min) t = o->dt_min;
+      dt_next = t;
+    }
+    if (out) {
+      out[6 * k] = dt;
+      out[6 * k + 1] = err;
+      out[6 * k + 2] = accept;
+      out[6 * k + 3] = r;
+      out[6 * k + 4] = (double)nc;
+      out[6 * k + 5] = dt_next;
+    }
+    dt = dt_next;
+  }
+  free(blocks); free(fint); free(fg); free(acc); free(dy);
+  free(u_pre); free(v_pre); free(u_new); free(v_new);
+  return rc;
+}
+
 int ref_plastic_state(ref_model *m, double *eps_p, double *alpha, double *dflow, uint8_t *active) {
   const int64_t E = m->E;
   if (eps_p) memcpy(eps_p, m->curr_ep, 6 * E * sizeof(double));

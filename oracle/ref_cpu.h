@@ -133,6 +133,21 @@ void ref_set_grid(ref_model *m, int use_grid);
  * committed state: applyDeform, then potential and dissipated energy passes. */
 int ref_commit(ref_model *m, const double *disp, double *e_pot, double *e_diss);
 
+/* MilneDevice predictor-corrector with the adaptive step of
+ * ExplicitModule::commit (MilneDeviceModule.cpp:41-112,
+ * ExplicitModule.cpp:290-335); see ref_cpu.c */
+typedef struct {
+  double dt;         /* first step size (deltaTime) */
+  double dt_min, dt_max;
+  double precision;  /* prec_ */
+  double safety;     /* stepSaftey, 0.9 */
+  double incr, decr; /* increaseFactor 1.2, decreaseFactor 0.8 */
+  double len_scale;  /* lengthScale, 1e-3 */
+} ref_milne_opts;
+int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
+              int64_t n_attempts, const int32_t *group, int64_t group_n, int32_t group_dof,
+              double *out);
+
 /* Contact broadphase, faithful O(R^2) (RodContactModel.cpp:245-446).
  * Writes candidate pairs (A,B), returns the count (or, if more than cap,
  * the count without writing beyond cap). */

@@ -20,6 +20,22 @@ ip64 = C.POINTER(C.c_int64)
 u8p = C.POINTER(C.c_uint8)
 
 
+class MilneOpts(C.Structure):
+    """ref_milne_opts: first step, bounds, precision and the step-size
+    factors of ExplicitModule (defaults: safety 0.9, incr 1.2, decr 0.8,
+    length scale 1e-3)"""
+    _fields_ = [("dt", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("precision", C.c_double), ("safety", C.c_double), ("incr", C.c_double),
+                ("decr", C.c_double), ("len_scale", C.c_double)]
+
+
+def milne_opts(dt, precision, dt_min=None, dt_max=None, safety=0.9, incr=1.2, decr=0.8,
+               len_scale=1e-3):
+    return MilneOpts(dt, dt / 100.0 if dt_min is None else dt_min,
+                     dt * 100.0 if dt_max is None else dt_max, precision, safety, incr, decr,
+                     len_scale)
+
+
 class _Mesh(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_elems", C.c_int64), ("n_struts", C.c_int32),
                 ("coords", dp), ("elem_nodes", ip32), ("elem_strut", ip32)]
@@ -74,6 +90,8 @@ def lib(omp=False):
     L.ref_leapfrog.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int64, ip32, C.c_int64,
                                C.c_int32, dp, C.c_int, dp, dp, ip64]
     L.ref_commit.argtypes = [C.c_void_p, dp, dp, dp]
+    L.ref_milne.argtypes = [C.c_void_p, dp, dp, C.POINTER(MilneOpts), C.c_int64, ip32,
+                            C.c_int64, C.c_int32, dp]
     L.ref_contact_candidates.restype = C.c_int64
     L.ref_contact_candidates.argtypes = [C.c_void_p, dp, ip32, ip32, C.c_int64]
     L.ref_contact_active.restype = C.c_int64
@@ -229,6 +247,20 @@ class RefModel:
         if energy:
             out["energies"] = en.reshape(-1, 3)
         return out
+
+    def milne(self, n_attempts, opts, group=None, group_dof=2):
+        """n_attempts MilneDevice attempts from the current state; returns a
+        dict of per-attempt arrays: dt, err, accepted, reaction, n_contacts,
+        dt_next"""
+        out = np.zeros(6 * n_attempts)
+        g = _c(group if group is not None else np.zeros(0), np.int32)
+        rc = self.L.ref_milne(self.h, _p(self.disp), _p(self.velo), C.byref(opts),
+                              int(n_attempts), _p(g, ip32), int(g.shape[0]), int(group_dof),
+                              _p(out))
+        self._check(rc)
+        o = out.reshape(-1, 6)
+        return {"dt": o[:, 0], "err": o[:, 1], "accepted": o[:, 2] > 0, "reaction": o[:, 3],
+                "n_contacts": o[:, 4].astype(np.int64), "dt_next": o[:, 5]}
 
     def contact_candidates(self, disp=None):
         d = self.disp if disp is None else _c(disp).reshape(-1)
