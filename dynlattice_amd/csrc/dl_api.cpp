@@ -437,6 +437,7 @@ extern "C" int dl_destroy(dl_ctx *c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->milne.gexec) hipGraphExecDestroy(c->milne.gexec);
   for (void *p : c->allocs) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
   delete c;

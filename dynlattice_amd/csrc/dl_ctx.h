@@ -68,6 +68,16 @@ struct AdjState {
   int64_t rbar_cap = 0;
 };
 
+// MilneDevice attempts: device buffers and the captured attempt graph
+struct MilneState {
+  bool ready = false;
+  MilneDev m{};
+  int64_t log_cap = 0;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_gen = -1;
+  MilneCtl graph_ctl{};
+};
+
 struct dl_ctx {
   BcHost bch;
   std::vector<double> ediss_acc_h; // per-element dissipation of the action path (energyDiss_)
@@ -123,6 +133,8 @@ struct dl_ctx {
   std::vector<int32_t> rgptr_h;   // host copy of the reaction group pointer
   // adjoint
   AdjState adj;
+  // MilneDevice
+  MilneState milne;
 };
 
 template <class T>

@@ -323,6 +323,49 @@ DL_HD void solve3_masked(const double J[9], const double f[3], unsigned mask, co
 //   f   in: element (+ contact) internal force; out: plus the gyroscopic term
 //   u, v, r in/out (r: quaternion w,x,y,z in SO(3) mode, rotation vector else)
 //   q   the node rotation as a quaternion (SO(3): r itself; additive: exp(r))
+// Nodal acceleration a = M^-1 (f_ext - f - gyro) with constraints
+// (getAcce, ExplicitModule.cpp:337-354): mass at rotation q, gyroscopic term
+// with velocity v.  f: internal force in, plus the gyroscopic term out.
+template <bool SO3>
+DL_HD void node_accel(double f[6], const Quat q, const double v[6], double mt,
+                      const double J0[6], double gm, unsigned mask, const double ac[6],
+                      const double fe[6], double a[6]) {
+  double R[9], J[9];
+  qmat(q, R);
+  spatial_inertia(R, J0, J);
+  if (SO3) {
+    // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
+    const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
+    const double jw1 = J[3] * v[3] + J[4] * v[4] + J[5] * v[5];
+    const double jw2 = J[6] * v[3] + J[7] * v[4] + J[8] * v[5];
+    f[3] += gm * (v[4] * jw2 - v[5] * jw1);
+    f[4] += gm * (v[5] * jw0 - v[3] * jw2);
+    f[5] += gm * (v[3] * jw1 - v[4] * jw0);
+  }
+  double fr[6];
+  for (int c = 0; c < 6; c++) fr[c] = fe[c] - f[c];
+  const double imt = 1.0 / mt;
+  for (int c = 0; c < 3; c++) a[c] = (mask & (1u << c)) ? ac[c] : imt * fr[c];
+  if (SO3) {
+    solve3_masked(J, fr + 3, mask >> 3, ac + 3, a + 3);
+  } else {
+    for (int c = 0; c < 3; c++) a[3 + c] = (mask & (8u << c)) ? ac[3 + c] : (1.0 / J[4 * c]) * fr[3 + c];
+  }
+}
+
+// r <- r (+) dt w: SO(3) composition exp(dt w) q (quaternion r), or the
+// additive rotation vector update
+template <bool SO3>
+DL_HD void rot_advance(double r[4], const Quat q, const double w[3], double dt) {
+  if (SO3) {
+    const Quat dq = qexp(dt * w[0], dt * w[1], dt * w[2]);
+    const Quat qn = qnormalize(qmul(dq, q));
+    r[0] = qn.w; r[1] = qn.x; r[2] = qn.y; r[3] = qn.z;
+  } else {
+    for (int c = 0; c < 3; c++) r[c] = r[c] + dt * w[c];
+  }
+}
+
 template <bool SO3>
 DL_HD void node_advance(double f[6], const Quat q, double u[3], double v[6], double r[4],
                         double mt, const double J0[6], double gm, unsigned mask,
@@ -358,6 +401,73 @@ DL_HD void node_advance(double f[6], const Quat q, double u[3], double v[6], dou
   } else {
     for (int c = 0; c < 3; c++) r[c] = r[c] + dt * v[3 + c];
   }
+}
+
+// MilneDevice predictor at one node (MilneDeviceModule.cpp:64-96):
+//   a = M(q)^-1 (f_ext - f - gyro(v));  v_pre = v + dt a;  u_pre = u + dt v;
+//   r_pre = r (+) dt w.  Outputs in *_p; the current state is read only.
+template <bool SO3>
+DL_HD void node_milne_pred(double f[6], const Quat q, const double u[3], const double v[6],
+                           const double r[4], double mt, const double J0[6], double gm,
+                           unsigned mask, const double ac[6], const double fe[6], double dt,
+                           double u_p[3], double v_p[6], double r_p[4]) {
+  double a[6];
+  node_accel<SO3>(f, q, v, mt, J0, gm, mask, ac, fe, a);
+  for (int c = 0; c < 6; c++) v_p[c] = v[c] + dt * a[c];
+  for (int c = 0; c < 3; c++) u_p[c] = u[c] + dt * v[c];
+  for (int c = 0; c < 4; c++) r_p[c] = r[c];
+  rot_advance<SO3>(r_p, q, v + 3, dt);
+}
+
+// rotation vector of a node rotation (the reference's state vector holds
+// log(R), angle in [0, pi])
+template <bool SO3>
+DL_HD void rot_vector(const double r[4], double t[3]) {
+  if (SO3) {
+    Quat q;
+    q.w = r[0]; q.x = r[1]; q.y = r[2]; q.z = r[3];
+    qlog(q, t);
+  } else {
+    t[0] = r[0]; t[1] = r[1]; t[2] = r[2];
+  }
+}
+
+// MilneDevice corrector at one node (MilneDeviceModule.cpp:98-112) with the
+// node's contribution to the error measure (getQuality,
+// ExplicitModule.cpp:490-499):
+//   a = M(q)^-1 (f_ext - f_pre - gyro(v_pre))  (mass of the start state);
+//   v_new = v + dt a;  u_new = u + dt v_pre;  r_new = r (+) dt w_pre.
+//   e[0] = sum of squared (u_pre - u_new) over the node's dofs, e[1] the same
+//   for v; translational dofs (all dofs in the additive mode) divided by the
+//   length scale.  Outputs overwrite the predictor values in place.
+template <bool SO3>
+DL_HD void node_milne_corr(double f[6], const Quat q, const double u[3], const double v[6],
+                           const double r[4], double mt, const double J0[6], double gm,
+                           unsigned mask, const double ac[6], const double fe[6], double dt,
+                           double ls, double u_p[3], double v_p[6], double r_p[4],
+                           double e[2]) {
+  double a[6], un[3], vn[6], rn[4];
+  node_accel<SO3>(f, q, v_p, mt, J0, gm, mask, ac, fe, a);
+  for (int c = 0; c < 6; c++) vn[c] = v[c] + dt * a[c];
+  for (int c = 0; c < 3; c++) un[c] = u[c] + dt * v_p[c];
+  for (int c = 0; c < 4; c++) rn[c] = r[c];
+  rot_advance<SO3>(rn, q, v_p + 3, dt);
+  double tp[3], tn[3];
+  rot_vector<SO3>(r_p, tp);
+  rot_vector<SO3>(rn, tn);
+  const double rs = SO3 ? 1.0 : ls;
+  double eu = 0.0, ev = 0.0;
+  for (int c = 0; c < 3; c++) {
+    const double du = (u_p[c] - un[c]) / ls, dr = (tp[c] - tn[c]) / rs;
+    const double dv = (v_p[c] - vn[c]) / ls, dw = (v_p[3 + c] - vn[3 + c]) / rs;
+    eu += du * du + dr * dr;
+    ev += dv * dv + dw * dw;
+  }
+  e[0] = eu;
+  e[1] = ev;
+  for (int c = 0; c < 3; c++) u_p[c] = un[c];
+  for (int c = 0; c < 6; c++) v_p[c] = vn[c];
+  for (int c = 0; c < 4; c++) r_p[c] = rn[c];
 }
 
 #endif

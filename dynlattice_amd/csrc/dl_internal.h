@@ -110,6 +110,26 @@ struct AdjDev {
   const double *rbar;   // [groups] objective adjoint of this step's reactions
 };
 
+// MilneDevice attempts: predicted then corrected state, error partials and
+// the step-size control state, all on the device
+struct MilneDev {
+  double *u1;     // [3][N] predicted, then corrected displacement
+  double *r1;     // [4][N] rotation (quaternion; additive mode: 3)
+  double *v1;     // [6][N]
+  double *part;   // [2][nblk] per-block sums of squared differences (u, v)
+  int32_t nblk;
+  double *dt;     // [1] step size of the current attempt
+  double *tacc;   // [1] sum of the accepted step sizes
+  int32_t *acc;   // [1] last attempt accepted
+  int64_t *nacc;  // [1] accepted attempts
+  double *log;    // [attempts][4]: dt, error, accepted, next dt
+};
+
+struct MilneCtl {
+  double dt_min, dt_max, prec, safety, incr, decr, len_scale;
+  int64_t ndof;
+};
+
 // launchers (dl_kernels.hip)
 void launch_init_strain(const DevData &d, hipStream_t s);
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
@@ -120,12 +140,22 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s);
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
-void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
+// gate (may be null): commit only if *gate != 0; epot, ediss may be null
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s,
+                      const int32_t *gate = nullptr);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
                      hipStream_t s);
 // adjoint (dl_adjoint.hip): [gather of the element ends +] node update, element force
 void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s);
 void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
+// MilneDevice (dl_milne.hip): predictor and corrector node passes over the
+// internal force in d.fout, step-size control, and the accept/reject commit
+void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
+void launch_milne_corr(const DevData &d, const MilneDev &m, double len_scale, bool so3,
+                       hipStream_t s);
+void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step, hipStream_t s);
+void launch_milne_commit(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
+void launch_set_f64(double *p, double v, hipStream_t s);
 
 #endif

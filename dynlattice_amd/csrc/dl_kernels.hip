@@ -435,9 +435,11 @@ __global__ void k_set_i32(int32_t *p, int32_t v) {
 // increment, old <- curr
 // ------------------------------------------------------------------
 template <bool PLASTIC>
-__global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, double *ediss) {
+__global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, double *ediss,
+                                                     const int32_t *gate) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.E) return;
+  if (gate && *gate == 0) return;
   const int64_t E = d.E;
   const int32_t lo = d.es[e];
   const int64_t R = d.R;
@@ -449,8 +451,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
       const double c = d.ecur[i * E + e];
       t += c * (C[i] * c);
     }
-    epot[e] = 0.5 * t;
-    ediss[e] = 0.0;
+    if (epot) epot[e] = 0.5 * t;
+    if (ediss) ediss[e] = 0.0;
     for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
     return;
   }
@@ -463,8 +465,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
     t += cel * cs;
     ds += ((os + cs) / 2.0) * dep;
   }
-  epot[e] = 0.5 * t;
-  ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
+  if (epot) epot[e] = 0.5 * t;
+  if (ediss) ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
   for (int i = 0; i < 6; i++) {
     d.ep[i * E + e] = d.ep_cur[i * E + e];
     d.eo[i * E + e] = d.ecur[i * E + e];
@@ -615,10 +617,13 @@ void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, p, v);
 }
 
-void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s,
+                      const int32_t *gate) {
   if (d.E == 0) return;
-  if (plastic) hipLaunchKernelGGL(k_commit<true>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
-  else hipLaunchKernelGGL(k_commit<false>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
+  if (plastic)
+    hipLaunchKernelGGL(k_commit<true>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss, gate);
+  else
+    hipLaunchKernelGGL(k_commit<false>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss, gate);
 }
 
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s) {

@@ -1,0 +1,254 @@
+// dl_milne.hip -- gfx950 kernels of the MilneDevice predictor-corrector
+// integrator with adaptive step size (FP64).
+//
+// One attempt (MilneDeviceModule.cpp:41-112 with ExplicitModule::commit,
+// ExplicitModule.cpp:290-335) is, on one stream and without a host round
+// trip:
+//   forces at the start state (u, v)        k_strut/k_joint in force mode
+//   k_milne_pred   a = M(u)^-1 (f_ext - f - gyro); v_pre = v + dt a;
+//                  u_pre = u (+) dt v            -> (u1, r1, v1)
+//   forces at (u_pre, v_pre)                 k_strut/k_joint on (u1, r1)
+//   k_milne_corr   a_pre with the start state's mass; v_new = v + dt a_pre;
+//                  u_new = u (+) dt v_pre        -> (u1, r1, v1) in place,
+//                  per-block sums of the squared differences pre - new
+//   k_milne_ctl    error, accept or reject, next step size (one block)
+//   k_milne_commit on accept the state moves to (u1, r1, v1)
+//   material commit of the predicted-state evaluation on accept
+// The step size lives in device memory, so a sequence of attempts can be
+// replayed from a captured graph.
+#include <hip/hip_runtime.h>
+
+#include "dl_internal.h"
+
+#define DL_BLOCK 256
+
+namespace {
+
+// boundary record of node n: constrained dofs, prescribed acceleration,
+// external force; the reaction slot receives f[react_dof]
+__device__ __forceinline__ void node_bc(const DevData &d, int32_t n, const double f[6],
+                                        unsigned &mask, double ac[6], double fe[6],
+                                        bool reaction) {
+  const int32_t b = d.bc[n];
+  mask = 0u;
+#pragma unroll
+  for (int c = 0; c < 6; c++) { ac[c] = 0.0; fe[c] = 0.0; }
+  if (b < 0) return;
+  const int64_t nb = d.nbc;
+  mask = d.bc_mask[b];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    ac[c] = d.bc_a[c * nb + b];
+    fe[c] = d.bc_f[c * nb + b];
+  }
+  const int32_t rs = d.bc_rslot[b];
+  if (reaction && rs >= 0) d.rbuf[rs] = f[d.react_dof];
+}
+
+template <bool SO3>
+__device__ __forceinline__ void load_node(const double *u, const double *rot, const double *v,
+                                          int64_t N, int64_t n, double uu[3], double rr[4],
+                                          double vv[6], Quat &q) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) uu[c] = u[c * N + n];
+#pragma unroll
+  for (int c = 0; c < 6; c++) vv[c] = v[c * N + n];
+  if (SO3) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) rr[c] = rot[c * N + n];
+    q.w = rr[0]; q.x = rr[1]; q.y = rr[2]; q.z = rr[3];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) rr[c] = rot[c * N + n];
+    rr[3] = 0.0;
+    q = qexp(rr[0], rr[1], rr[2]);
+  }
+}
+
+template <bool SO3>
+__device__ __forceinline__ void store_node(double *u, double *rot, double *v, int64_t N, int64_t n,
+                                           const double uu[3], const double rr[4],
+                                           const double vv[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) u[c * N + n] = uu[c];
+#pragma unroll
+  for (int c = 0; c < 6; c++) v[c * N + n] = vv[c];
+#pragma unroll
+  for (int c = 0; c < (SO3 ? 4 : 3); c++) rot[c * N + n] = rr[c];
+}
+
+// contact candidate lists: flag a rebuild when the node has left the skin
+__device__ __forceinline__ void list_trigger(const DevData &d, int64_t n, const double u[3]) {
+  if (!d.uref) return;
+  const int64_t N = d.N;
+  bool moved = false;
+#pragma unroll
+  for (int c = 0; c < 3; c++) moved = moved || fabs(u[c] - d.uref[c * N + n]) > d.ctrig;
+  if (moved) *d.cgate = 1;
+}
+
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_pred(DevData d, MilneDev m) {
+  const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  double f[6], ac[6], fe[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
+  unsigned mask;
+  node_bc(d, (int32_t)n, f, mask, ac, fe, true);
+  double u[3], r[4], v[6];
+  Quat q;
+  load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
+  double J0[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  double up[3], rp[4], vp[6];
+  node_milne_pred<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
+                       up, vp, rp);
+  store_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp);
+  list_trigger(d, n, up);
+}
+
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_corr(DevData d, MilneDev m, double ls) {
+  const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  const int64_t N = d.N;
+  double e[2] = {0.0, 0.0};
+  if (n < N) {
+    double f[6], ac[6], fe[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
+    unsigned mask;
+    node_bc(d, (int32_t)n, f, mask, ac, fe, false);
+    double u[3], r[4], v[6], up[3], rp[4], vp[6];
+    Quat q, qp;
+    load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
+    load_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp, qp);
+    double J0[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+    node_milne_corr<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
+                         ls, up, vp, rp, e);
+    store_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp);
+  }
+  // per-block sums in a fixed tree order
+  __shared__ double red[2][DL_BLOCK];
+  red[0][threadIdx.x] = e[0];
+  red[1][threadIdx.x] = e[1];
+  __syncthreads();
+  for (int w = DL_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    m.part[blockIdx.x] = red[0][0];
+    m.part[m.nblk + blockIdx.x] = red[1][0];
+  }
+}
+
+// error = (rms(u_pre - u_new) + dt rms(v_pre - v_new)) / 2 over all dofs
+// (getQuality, ExplicitModule.cpp:490-499; MilneDeviceModule.cpp:105-110),
+// then the step-size rule of ExplicitModule::commit (order 1)
+__global__ __launch_bounds__(1024) void k_milne_ctl(MilneDev m, MilneCtl mc, const int32_t *step) {
+  __shared__ double red[2][1024];
+  double su = 0.0, sv = 0.0;
+  for (int32_t i = threadIdx.x; i < m.nblk; i += 1024) {
+    su += m.part[i];
+    sv += m.part[m.nblk + i];
+  }
+  red[0][threadIdx.x] = su;
+  red[1][threadIdx.x] = sv;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double dt = *m.dt;
+  const double sn = sqrt((double)mc.ndof);
+  double err = 0.0;
+  err += sqrt(red[0][0]) / sn;
+  err += (sqrt(red[1][0]) / sn) * dt;
+  err = 0.5 * err;
+  const double dt_opt = dt * pow(mc.prec / err, 0.5);
+  const bool accept = err <= mc.prec || dt <= mc.dt_min;
+  double t = mc.safety * dt_opt;
+  if (accept) {
+    if (mc.incr * dt < t) t = mc.incr * dt;
+    if (mc.dt_max < t) t = mc.dt_max;
+  }
+  if (t < mc.decr * dt) t = mc.decr * dt;
+  if (t < mc.dt_min) t = mc.dt_min;
+  const int64_t k = *step;
+  double *row = m.log + 4 * k;
+  row[0] = dt;
+  row[1] = err;
+  row[2] = accept ? 1.0 : 0.0;
+  row[3] = t;
+  *m.acc = accept ? 1 : 0;
+  if (accept) {
+    *m.tacc += dt;
+    *m.nacc += 1;
+  }
+  *m.dt = t;
+}
+
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_commit(DevData d, MilneDev m) {
+  const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  if (n >= d.N) return;
+  const int64_t N = d.N;
+  double u[3];
+  if (*m.acc) {
+    double r[4], v[6];
+    Quat q;
+    load_node<SO3>(m.u1, m.r1, m.v1, N, n, u, r, v, q);
+    store_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
+  }
+  // the next attempt evaluates contact at this state
+  list_trigger(d, n, u);
+}
+
+__global__ void k_set_f64(double *p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + DL_BLOCK - 1) / DL_BLOCK); }
+
+} // namespace
+
+void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_milne_pred<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+  else hipLaunchKernelGGL(k_milne_pred<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+}
+
+void launch_milne_corr(const DevData &d, const MilneDev &m, double len_scale, bool so3,
+                       hipStream_t s) {
+  if (so3)
+    hipLaunchKernelGGL(k_milne_corr<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m, len_scale);
+  else
+    hipLaunchKernelGGL(k_milne_corr<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m, len_scale);
+}
+
+void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step, hipStream_t s) {
+  hipLaunchKernelGGL(k_milne_ctl, dim3(1), dim3(1024), 0, s, m, mc, step);
+}
+
+void launch_milne_commit(const DevData &d, const MilneDev &m, bool so3, hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_milne_commit<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+  else hipLaunchKernelGGL(k_milne_commit<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+}
+
+void launch_set_f64(double *p, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, s, p, v);
+}

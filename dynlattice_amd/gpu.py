@@ -33,7 +33,7 @@ EXPORTS = (
     "dl_contact_pairs", "dl_contact_candidates", "dl_plastic_active", "dl_plastic_state",
     "dl_phase_time", "dl_set_timing", "dl_sync", "dl_last_error",
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
-    "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward",
+    "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
 )
 
 #: section fields of dl_sections / dl_section_grads, in ABI order
@@ -97,6 +97,27 @@ class dl_step_out(C.Structure):
                 ("n_candidates", C.c_int64), ("steps_done", C.c_int64)]
 
 
+class dl_milne_opts(C.Structure):
+    """MilneDevice settings (defaults of the reference's solver module:
+    dt_min = dt / 100, dt_max = 100 dt, safety 0.9, incr 1.2, decr 0.8,
+    length scale 1e-3)"""
+    _fields_ = [("dt", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("precision", C.c_double), ("safety", C.c_double), ("incr", C.c_double),
+                ("decr", C.c_double), ("len_scale", C.c_double)]
+
+
+def milne_opts(dt, precision, dt_min=None, dt_max=None, safety=0.9, incr=1.2, decr=0.8,
+               len_scale=1e-3):
+    return dl_milne_opts(dt, dt / 100.0 if dt_min is None else dt_min,
+                         dt * 100.0 if dt_max is None else dt_max, precision, safety, incr,
+                         decr, len_scale)
+
+
+class dl_milne_out(C.Structure):
+    _fields_ = [("attempts", C.c_int64), ("accepted", C.c_int64), ("time", C.c_double),
+                ("dt_next", C.c_double)]
+
+
 _lib = None
 
 
@@ -133,6 +154,8 @@ def load():
     L.dl_set_sections.argtypes = [C.c_void_p, C.POINTER(dl_sections), dp]
     L.dl_adjoint_forward.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int32, dp]
     L.dl_adjoint_backward.argtypes = [C.c_void_p, dp, C.POINTER(dl_section_grads)]
+    L.dl_milne_step.argtypes = [C.c_void_p, C.c_int64, C.POINTER(dl_milne_opts), dp,
+                                C.POINTER(dl_milne_out)]
     _lib = L
     return L
 
@@ -314,6 +337,21 @@ class GpuModel:
         n = C.c_int64()
         self._check(self.L.dl_get_history(self.h, _p(r), _p(e), _p(c, ip64), nsteps, C.byref(n)))
         return {"reaction": r[:n.value], "e_diss": e[:n.value], "n_contacts": c[:n.value]}
+
+    def milne(self, n_attempts, opts):
+        """n_attempts MilneDevice attempts from the current state (opts:
+        :func:`milne_opts`); returns per-attempt arrays dt, err, accepted,
+        reaction, n_contacts, dt_next and the totals (accepted count, time
+        advanced)"""
+        n = int(n_attempts)
+        log = np.zeros(4 * max(n, 1))
+        o = dl_milne_out()
+        self._check(self.L.dl_milne_step(self.h, n, C.byref(opts), _p(log), C.byref(o)))
+        h = self.history(n)
+        lg = log[: 4 * n].reshape(-1, 4)
+        return {"dt": lg[:, 0], "err": lg[:, 1], "accepted": lg[:, 2] > 0,
+                "reaction": h["reaction"], "n_contacts": h["n_contacts"], "dt_next": lg[:, 3],
+                "n_accepted": int(o.accepted), "time": float(o.time)}
 
     def sync(self):
         self._check(self.L.dl_sync(self.h))

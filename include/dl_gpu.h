@@ -260,6 +260,45 @@ int dl_adjoint_forward(dl_ctx *ctx, int64_t nsteps, double dt, int32_t checkpoin
                        double *reaction);
 int dl_adjoint_backward(dl_ctx *ctx, const double *reaction_bar, dl_section_grads *grads);
 
+/* MilneDevice integrator settings: the solver module's configuration
+ * (defaults ExplicitModule.cpp:37-46, lengthScale :76, the rest :128-149;
+ * MilneDeviceModule.cpp:28-31 sets order 1).  dt_min and dt_max default to
+ * dt / 100 and dt * 100 there. */
+typedef struct {
+  double dt;        /* step size of the first attempt (deltaTime) */
+  double dt_min;    /* minDtime: attempts at or below it are always accepted */
+  double dt_max;    /* maxDtime */
+  double precision; /* precision: error bound for acceptance */
+  double safety;    /* stepSaftey, 0.9 */
+  double incr;      /* increaseFactor, 1.2 */
+  double decr;      /* decreaseFactor, 0.8 */
+  double len_scale; /* lengthScale: translational dofs are divided by it in the error, 1e-3 */
+} dl_milne_opts;
+
+/* Output of dl_milne_step, filled after a stream synchronisation. */
+typedef struct {
+  int64_t attempts; /* attempts made */
+  int64_t accepted; /* of which accepted */
+  double time;      /* sum of the accepted step sizes */
+  double dt_next;   /* step size for the next attempt (pass it as opts.dt to continue) */
+} dl_milne_out;
+
+/* Replaces: ControlModule -> ExplicitModule::advance / MilneDeviceModule::solve /
+ * ExplicitModule::commit|cancel with adaptive step size, n_attempts times
+ * (MilneDeviceModule.cpp:41-112, ExplicitModule.cpp:259-335).  Each attempt:
+ * forces at the start state and the predictor, forces at the predicted state
+ * and the corrector (mass of the start state), the error estimate, then
+ * accept (the material commits its predicted-state update and the state
+ * moves to the corrected one) or reject, and the next step size -- all on the
+ * device, without a host round trip (dl_set_graph replays attempts from a
+ * captured graph).  Histories as dl_explicit_step, one row per attempt
+ * (reaction and contact pairs at the attempt's start state).  attempts (may
+ * be NULL): host [n_attempts][4] = dt, error, accepted (0/1), next dt.
+ * Elastic and elastoplastic material, contact, both rotation updates; not
+ * with per-node time steps (dl_set_node_dt). */
+int dl_milne_step(dl_ctx *ctx, int64_t n_attempts, const dl_milne_opts *opts, double *attempts,
+                  dl_milne_out *out);
+
 /* Wait for the context's stream. */
 int dl_sync(dl_ctx *ctx);
 
