@@ -287,6 +287,84 @@ def adjoint(args):
     print(json.dumps(res), flush=True)
 
 
+def milne(args):
+    """MilneDevice attempts (predictor-corrector, adaptive step with the
+    error estimate, accept/reject on the device) on the given lattice: one
+    "step" is one attempt over the whole lattice, i.e. two internal-force
+    evaluations, the predictor, corrector, step-size control and commit.
+    value = elements x attempts / wall time."""
+    from dynlattice_amd.gpu import GpuModel, milne_opts
+    p = workload(args.workload)
+    prec = args.milne_precision
+    print(f"milne: {p.name}: {p.n_elems} elements, {p.n_nodes} nodes, first dt {p.dt:.3e}, "
+          f"precision {prec:g}", file=sys.stderr, flush=True)
+    g = GpuModel(p, device=0)
+    if "bottom" in p.groups:
+        g.set_reaction_group(p.groups["bottom"], 2)
+    g.set_graph(args.milne_graph)
+    o = milne_opts(p.dt, prec)
+    q = g.milne(args.warmup, o)
+    o = milne_opts(float(q["dt_next"][-1]), prec, dt_min=p.dt / 100, dt_max=p.dt * 100)
+    g.sync()
+    t0 = time.perf_counter()
+    q = g.milne(args.steps, o)
+    wall = time.perf_counter() - t0
+    value = p.n_elems * args.steps / wall
+    o = milne_opts(float(q["dt_next"][-1]), prec, dt_min=p.dt / 100, dt_max=p.dt * 100)
+    g.set_timing(True)
+    g.milne(args.steps, o)
+    phases = {}
+    for ph in ("step", "contact", "strut", "joint", "node"):
+        ms, cnt = g.phase_time(ph)
+        phases[ph] = ms / max(cnt, 1)
+    g.set_timing(False)
+    g.close()
+    # force-mode strut kernel: element data + plastic state read, trial state
+    # written (eps_p, alpha, strain, flow increment), private nodes: x, theta
+    # read and the nodal force written
+    npriv = private_node_count(p)
+    b_elem = B_ELEM + (B_ELEM_PLASTIC + 8.0 * 7 if p.plastic is not None else 48.0)
+    bytes_strut = b_elem * p.n_elems + 96.0 * npriv
+    t_launch = phases["strut"] / 2 * 1e-3
+    achieved = bytes_strut / t_launch / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_strut (force mode)", "algorithmic_bytes_per_launch": bytes_strut,
+            "avg_launch_ms": round(phases["strut"] / 2, 5)}
+    res = {"metric": "MilneDevice element-GP attempts/s (FP64, 2 force evaluations each)",
+           "value": value, "unit": "element-GP attempts/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded BCC lattice generator, random per-strut radii)",
+           "config": {"workload": p.name, "elements": p.n_elems, "nodes": p.n_nodes,
+                      "precision": prec, "graph_attempts": args.milne_graph,
+                      "material": "elastoplastic" if p.plastic is not None else "elastic",
+                      "contact": p.contact is not None},
+           "accepted_fraction": float(np.mean(q["accepted"])),
+           "dt_range": [float(q["dt"].min()), float(q["dt"].max())],
+           "roofline": roof, "phase_ms": {k: round(v, 5) for k, v in phases.items()}}
+    if not args.no_cpu:
+        from oracle.ref_cpu import RefModel
+        from oracle.ref_cpu import milne_opts as ref_milne_opts
+        for omp, key in ((True, "cpu_baseline"), (False, "cpu_baseline_serial")):
+            ref = RefModel(p, grid=True, omp=omp)
+            ro = ref_milne_opts(p.dt, prec)
+            t0 = time.perf_counter()
+            ref.milne(1, ro)
+            t1 = time.perf_counter() - t0
+            k = int(max(1, min(args.steps, args.cpu_budget / max(t1, 1e-9))))
+            t0 = time.perf_counter()
+            ref.milne(k, ro)
+            dt = time.perf_counter() - t0
+            cores = cpu_threads() if omp else 1
+            res[key] = {"value": p.n_elems * k / dt, "unit": "element-GP attempts/s",
+                        "cores": cores, "kind": "port",
+                        "sample": f"oracle ref_milne (oracle/ref_cpu.c, {cores} thread(s)) on "
+                                  f"the same {p.name} lattice, {k} attempts after 1, {dt:.1f} s"}
+        res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sweep", action="store_true",
@@ -298,6 +376,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=5000)
     ap.add_argument("--checkpoint-every", type=int, default=100)
     ap.add_argument("--fd-struts", type=int, default=4)
+    ap.add_argument("--milne", action="store_true",
+                    help="MilneDevice adaptive predictor-corrector attempts on --workload")
+    ap.add_argument("--milne-precision", type=float, default=1e-4)
+    ap.add_argument("--milne-graph", type=int, default=10,
+                    help="attempts per captured graph (0: direct launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +400,9 @@ def main():
         return
     if args.adjoint:
         adjoint(args)
+        return
+    if args.milne:
+        milne(args)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "

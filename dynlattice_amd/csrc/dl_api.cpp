@@ -787,6 +787,7 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   else if (ph == "contact") v = c->t_contact;
   else if (ph == "strut") v = c->t_strut;
   else if (ph == "joint") v = c->t_joint;
+  else if (ph == "node") v = c->t_node;
   else return fail(DL_ERR_INPUT, "unknown phase " + ph);
   if (ms) *ms = v;
   if (count) *count = c->t_count;
@@ -888,7 +889,7 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     }
     HIPCHK(hipEventRecord(ev[4 * nsteps], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->t_step = c->t_contact = c->t_strut = c->t_joint = 0.0;
+    c->t_step = c->t_contact = c->t_strut = c->t_joint = c->t_node = 0.0;
     for (int64_t st = 0; st < nsteps; st++) {
       float a, b, cc, dd;
       HIPCHK(hipEventElapsedTime(&a, ev[4 * st], ev[4 * st + 1]));

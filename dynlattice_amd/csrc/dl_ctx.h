@@ -120,7 +120,7 @@ struct dl_ctx {
   double *ediss_acc = nullptr, *epot_e = nullptr, *ediss_inc = nullptr;
   // timing
   bool timing = false;
-  double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0;
+  double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0, t_node = 0;
   int64_t t_count = 0;
   // contact
   ContactState cs{};

@@ -113,7 +113,7 @@ struct AdjDev {
 // MilneDevice attempts: predicted then corrected state, error partials and
 // the step-size control state, all on the device
 struct MilneDev {
-  double *u1;     // [3][N] predicted, then corrected displacement
+  double *u1;     // [3][N] predicted displacement, then the start state's
   double *r1;     // [4][N] rotation (quaternion; additive mode: 3)
   double *v1;     // [6][N]
   double *part;   // [2][nblk] per-block sums of squared differences (u, v)
@@ -140,9 +140,7 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s);
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
-// gate (may be null): commit only if *gate != 0; epot, ediss may be null
-void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s,
-                      const int32_t *gate = nullptr);
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,
                      hipStream_t s);
@@ -155,7 +153,8 @@ void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_
 void launch_milne_corr(const DevData &d, const MilneDev &m, double len_scale, bool so3,
                        hipStream_t s);
 void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step, hipStream_t s);
-void launch_milne_commit(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
+void launch_milne_restore(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
+void launch_milne_commit_pl(const DevData &d, const MilneDev &m, hipStream_t s);
 void launch_set_f64(double *p, double v, hipStream_t s);
 
 #endif

@@ -251,10 +251,17 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
           for (int i = 1; i < 7; i++) d.al[i * E + e] = al[i];
         }
       } else {
+        // trial plastic state: written only where the return map corrected
+        // it (elsewhere it equals the committed state, and COMMIT reads the
+        // committed one through the active flag)
+        if (it > 0) {
 #pragma unroll
-        for (int i = 0; i < 6; i++) { d.ep_cur[i * E + e] = ep[i]; d.ecur[i * E + e] = strain[i]; }
+          for (int i = 0; i < 6; i++) d.ep_cur[i * E + e] = ep[i];
 #pragma unroll
-        for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
+          for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
         d.dflow[e] = dfl;
       }
       d.active[e] = (uint8_t)(it > 0);
@@ -435,11 +442,9 @@ __global__ void k_set_i32(int32_t *p, int32_t v) {
 // increment, old <- curr
 // ------------------------------------------------------------------
 template <bool PLASTIC>
-__global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, double *ediss,
-                                                     const int32_t *gate) {
+__global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, double *ediss) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.E) return;
-  if (gate && *gate == 0) return;
   const int64_t E = d.E;
   const int32_t lo = d.es[e];
   const int64_t R = d.R;
@@ -451,27 +456,32 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
       const double c = d.ecur[i * E + e];
       t += c * (C[i] * c);
     }
-    if (epot) epot[e] = 0.5 * t;
-    if (ediss) ediss[e] = 0.0;
+    epot[e] = 0.5 * t;
+    ediss[e] = 0.0;
     for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
     return;
   }
+  // the trial plastic state is in ep_cur/al_cur only where the last
+  // evaluation corrected it (active); elsewhere it is the committed one
+  const bool act = d.active[e] != 0;
   double t = 0.0, ds = 0.0;
   for (int i = 0; i < 6; i++) {
-    const double oel = d.eo[i * E + e] - d.ep[i * E + e];
-    const double cel = d.ecur[i * E + e] - d.ep_cur[i * E + e];
-    const double dep = d.ep_cur[i * E + e] - d.ep[i * E + e];
+    const double epo = d.ep[i * E + e];
+    const double epc = act ? d.ep_cur[i * E + e] : epo;
+    const double oel = d.eo[i * E + e] - epo;
+    const double cel = d.ecur[i * E + e] - epc;
+    const double dep = epc - epo;
     const double os = C[i] * oel, cs = C[i] * cel;
     t += cel * cs;
     ds += ((os + cs) / 2.0) * dep;
   }
-  if (epot) epot[e] = 0.5 * t;
-  if (ediss) ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
-  for (int i = 0; i < 6; i++) {
-    d.ep[i * E + e] = d.ep_cur[i * E + e];
-    d.eo[i * E + e] = d.ecur[i * E + e];
+  epot[e] = 0.5 * t;
+  ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
+  for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+  if (act) {
+    for (int i = 0; i < 6; i++) d.ep[i * E + e] = d.ep_cur[i * E + e];
+    for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
   }
-  for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
   d.dflow[e] = 0.0;
 }
 
@@ -617,13 +627,10 @@ void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, p, v);
 }
 
-void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s,
-                      const int32_t *gate) {
+void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {
   if (d.E == 0) return;
-  if (plastic)
-    hipLaunchKernelGGL(k_commit<true>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss, gate);
-  else
-    hipLaunchKernelGGL(k_commit<false>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss, gate);
+  if (plastic) hipLaunchKernelGGL(k_commit<true>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
+  else hipLaunchKernelGGL(k_commit<false>, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d, epot, ediss);
 }
 
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s) {

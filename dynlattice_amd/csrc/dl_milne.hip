@@ -9,13 +9,16 @@
 //                  u_pre = u (+) dt v            -> (u1, r1, v1)
 //   forces at (u_pre, v_pre)                 k_strut/k_joint on (u1, r1)
 //   k_milne_corr   a_pre with the start state's mass; v_new = v + dt a_pre;
-//                  u_new = u (+) dt v_pre        -> (u1, r1, v1) in place,
-//                  per-block sums of the squared differences pre - new
+//                  u_new = u (+) dt v_pre        -> (u, rot, v), and the
+//                  start state -> (u1, r1, v1); per-block sums of the
+//                  squared differences pre - new
 //   k_milne_ctl    error, accept or reject, next step size (one block)
-//   k_milne_commit on accept the state moves to (u1, r1, v1)
-//   material commit of the predicted-state evaluation on accept
-// The step size lives in device memory, so a sequence of attempts can be
-// replayed from a captured graph.
+//   k_milne_restore  on reject the start state comes back from (u1, r1, v1)
+//   k_milne_commit_pl  on accept the material commits the predicted-state
+//                  evaluation
+// Accepted attempts (the usual case) thus move no state beyond the
+// corrector's own writes.  The step size lives in device memory, so a
+// sequence of attempts can be replayed from a captured graph.
 #include <hip/hip_runtime.h>
 
 #include "dl_internal.h"
@@ -130,7 +133,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_milne_corr(DevData d, MilneDev m, 
     for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
     node_milne_corr<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
                          ls, up, vp, rp, e);
-    store_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp);
+    // the corrected state becomes current; the start state is kept for a
+    // rejection (each thread touches its own node only)
+    store_node<SO3>(d.u, d.rot, d.v, N, n, up, rp, vp);
+    store_node<SO3>(m.u1, m.r1, m.v1, N, n, u, r, v);
+    list_trigger(d, n, up);
   }
   // per-block sums in a fixed tree order
   __shared__ double red[2][DL_BLOCK];
@@ -200,23 +207,37 @@ __global__ __launch_bounds__(1024) void k_milne_ctl(MilneDev m, MilneCtl mc, con
   *m.dt = t;
 }
 
+// rejected attempt: the start state comes back (the next attempt
+// evaluates contact there, so the list trigger checks it)
 template <bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_milne_commit(DevData d, MilneDev m) {
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_restore(DevData d, MilneDev m) {
+  if (*m.acc) return;
   const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
   if (n >= d.N) return;
   const int64_t N = d.N;
-  double u[3];
-  if (*m.acc) {
-    double r[4], v[6];
-    Quat q;
-    load_node<SO3>(m.u1, m.r1, m.v1, N, n, u, r, v, q);
-    store_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
-  }
-  // the next attempt evaluates contact at this state
+  double u[3], r[4], v[6];
+  Quat q;
+  load_node<SO3>(m.u1, m.r1, m.v1, N, n, u, r, v, q);
+  store_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v);
   list_trigger(d, n, u);
+}
+
+// accepted attempt: applyDeform of the predicted-state evaluation (the last
+// one).  The trial plastic state was written only where the return map
+// corrected it (active); the committed total strain follows the trial one.
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_commit_pl(DevData d, const int32_t *acc) {
+  if (!*acc) return;
+  const int64_t e = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  if (e >= d.E) return;
+  const int64_t E = d.E;
+#pragma unroll
+  for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+  if (d.active[e]) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) d.ep[i * E + e] = d.ep_cur[i * E + e];
+#pragma unroll
+    for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
+  }
 }
 
 __global__ void k_set_f64(double *p, double v) {
@@ -244,9 +265,15 @@ void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step
   hipLaunchKernelGGL(k_milne_ctl, dim3(1), dim3(1024), 0, s, m, mc, step);
 }
 
-void launch_milne_commit(const DevData &d, const MilneDev &m, bool so3, hipStream_t s) {
-  if (so3) hipLaunchKernelGGL(k_milne_commit<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
-  else hipLaunchKernelGGL(k_milne_commit<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+void launch_milne_restore(const DevData &d, const MilneDev &m, bool so3, hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_milne_restore<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+  else hipLaunchKernelGGL(k_milne_restore<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+}
+
+void launch_milne_commit_pl(const DevData &d, const MilneDev &m, hipStream_t s) {
+  if (d.E == 0) return;
+  hipLaunchKernelGGL(k_milne_commit_pl, dim3(nblk(d.E)), dim3(DL_BLOCK), 0, s, d,
+                     (const int32_t *)m.acc);
 }
 
 void launch_set_f64(double *p, double v, hipStream_t s) {

@@ -42,20 +42,29 @@ static int milne_buffers(dl_ctx *c, int64_t n_attempts) {
   return DL_OK;
 }
 
-// one attempt, kernels only (capturable)
-static int launch_attempt(dl_ctx *c, const MilneCtl &mc) {
+// events per timed attempt: start, after contact/strut/joint/predictor of
+// the start-state evaluation, after contact/strut/joint of the predicted one, end
+#define MILNE_EVENTS 9
+
+// one attempt, kernels only (capturable); ev: MILNE_EVENTS events or null
+static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
   const DevData &d = c->d;
   const MilneDev &m = c->milne.m;
   hipStream_t s = c->stream;
   const StepParams sp{0.0, 1, 0};
+  if (ev) HIPCHK(hipEventRecord(ev[0], s));
   // forces at the start state; contact pairs counted in the attempt's row
   if (c->contact) {
     int rc = contact_step(c->cs, d, s, c->hist_nc, c->hist_ncand, c->step_ctr);
     if (rc) return rc;
   }
+  if (ev) HIPCHK(hipEventRecord(ev[1], s));
   launch_strut_pp(d, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, s);
+  if (ev) HIPCHK(hipEventRecord(ev[2], s));
   launch_joint(d, sp, MODE_FORCE, c->so3, s);
+  if (ev) HIPCHK(hipEventRecord(ev[3], s));
   launch_milne_pred(d, m, c->so3, s);
+  if (ev) HIPCHK(hipEventRecord(ev[4], s));
   // forces at the predicted state
   DevData dp = d;
   dp.u = m.u1;
@@ -65,16 +74,47 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc) {
     int rc = contact_step(c->cs, dp, s, nullptr, nullptr, nullptr);
     if (rc) return rc;
   }
+  if (ev) HIPCHK(hipEventRecord(ev[5], s));
   launch_strut_pp(dp, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, s);
+  if (ev) HIPCHK(hipEventRecord(ev[6], s));
   launch_joint(dp, sp, MODE_FORCE, c->so3, s);
+  if (ev) HIPCHK(hipEventRecord(ev[7], s));
   launch_milne_corr(d, m, mc.len_scale, c->so3, s);
   launch_milne_ctl(m, mc, c->step_ctr, s);
-  launch_milne_commit(d, m, c->so3, s);
-  // the material commits the predicted-state update (the last evaluation)
-  if (c->plastic) launch_commit_pl(d, true, nullptr, nullptr, s, m.acc);
+  launch_milne_restore(d, m, c->so3, s);
+  // the material commits the predicted-state update (the last evaluation);
+  // the elastic material keeps no state that anything reads back
+  if (c->plastic) launch_milne_commit_pl(d, m, s);
   launch_step_end(d, c->rgptr, c->n_groups, false, 0, nullptr, c->hist_react, c->hist_diss,
                   c->step_ctr, s);
+  if (ev) HIPCHK(hipEventRecord(ev[8], s));
   return DL_OK;
+}
+
+// attempts with event timing: phase sums over both evaluations per attempt
+// ("node": predictor, corrector, control, commit)
+static int timed_attempts(dl_ctx *c, int64_t n, const MilneCtl &mc) {
+  std::vector<hipEvent_t> ev(MILNE_EVENTS * (size_t)n);
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = DL_OK;
+  for (int64_t k = 0; k < n && !rc; k++) rc = launch_attempt(c, mc, &ev[MILNE_EVENTS * k]);
+  if (!rc) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->t_step = c->t_contact = c->t_strut = c->t_joint = c->t_node = 0.0;
+    for (int64_t k = 0; k < n; k++) {
+      const hipEvent_t *e = &ev[MILNE_EVENTS * k];
+      float t[MILNE_EVENTS - 1];
+      for (int i = 0; i + 1 < MILNE_EVENTS; i++) HIPCHK(hipEventElapsedTime(&t[i], e[i], e[i + 1]));
+      c->t_contact += t[0] + t[4];
+      c->t_strut += t[1] + t[5];
+      c->t_joint += t[2] + t[6];
+      c->t_node += t[3] + t[7];
+      for (int i = 0; i + 1 < MILNE_EVENTS; i++) c->t_step += t[i];
+    }
+    c->t_count = n;
+  }
+  for (auto &e : ev) hipEventDestroy(e);
+  return rc;
 }
 
 static bool same_ctl(const MilneCtl &a, const MilneCtl &b) {
@@ -94,7 +134,7 @@ static int milne_attempts(dl_ctx *c, int64_t n, const MilneCtl &mc) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int rc = DL_OK;
-      for (int64_t k = 0; k < K && !rc; k++) rc = launch_attempt(c, mc);
+      for (int64_t k = 0; k < K && !rc; k++) rc = launch_attempt(c, mc, nullptr);
       hipError_t ec = hipStreamEndCapture(c->stream, &g);
       if (rc) return rc;
       if (ec != hipSuccess)
@@ -109,7 +149,7 @@ static int milne_attempts(dl_ctx *c, int64_t n, const MilneCtl &mc) {
     for (; done + K <= n; done += K) HIPCHK(hipGraphLaunch(M.gexec, c->stream));
   }
   for (; done < n; done++) {
-    int rc = launch_attempt(c, mc);
+    int rc = launch_attempt(c, mc, nullptr);
     if (rc) return rc;
   }
   return DL_OK;
@@ -137,7 +177,8 @@ extern "C" int dl_milne_step(dl_ctx *c, int64_t n_attempts, const dl_milne_opts 
   HIPCHK(hipMemsetAsync(m.nacc, 0, sizeof(int64_t), c->stream));
   const MilneCtl mc{o->dt_min, o->dt_max, o->precision, o->safety, o->incr, o->decr,
                     o->len_scale, 6 * c->d.N};
-  if ((rc = milne_attempts(c, n_attempts, mc))) return rc;
+  if ((rc = c->timing ? timed_attempts(c, n_attempts, mc) : milne_attempts(c, n_attempts, mc)))
+    return rc;
   HIPCHK(hipGetLastError());
   c->hist_n = n_attempts;
   if ((rc = check_err(c))) return rc;

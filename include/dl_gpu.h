@@ -221,9 +221,12 @@ int dl_contact_candidates(dl_ctx *ctx, int32_t *a, int32_t *b, int64_t cap, int6
 int dl_plastic_active(dl_ctx *ctx, uint8_t *mask, int64_t n);
 int dl_plastic_state(dl_ctx *ctx, double *eps_p, double *alpha, int64_t n_elems);
 
-/* Device timing of the kernels of the last dl_explicit_step call, measured
- * with HIP events on the context's stream: total milliseconds of the named
- * phase ("step", "contact", "strut", "joint") and the number of steps. */
+/* Device timing of the kernels of the last dl_explicit_step (or
+ * dl_milne_step) call, measured with HIP events on the context's stream:
+ * total milliseconds of the named phase ("step", "contact", "strut",
+ * "joint"; for MilneDevice both force evaluations of an attempt, and "node":
+ * predictor, corrector, step control and commit) and the number of steps
+ * (attempts). */
 int dl_phase_time(dl_ctx *ctx, const char *phase, double *ms, int64_t *count);
 int dl_set_timing(dl_ctx *ctx, int32_t enabled);
 
