@@ -95,6 +95,14 @@ static int section_tables(dl_ctx *c, bool first) {
   walk_elements(m, [&](int32_t s, int64_t, int32_t a, int32_t b, const double *La, const double *Lb) {
     add_elem_mass(sc, s, elem_jac(m.X.data(), a, b), a, b, La, Lb, N, mt.data(), J0.data());
   });
+  // point masses (MassModel.cpp:92-108): isotropic additions to the
+  // translational mass and to the rotational diagonal (R (J0 + m I) R^T =
+  // R J0 R^T + m I, so the addition can sit in the reference inertia)
+  if (!c->pm_mass.empty())
+    for (int64_t n = 0; n < N; n++) mt[n] += c->pm_mass[n];
+  if (!c->pm_inertia.empty())
+    for (int64_t n = 0; n < N; n++)
+      for (int i = 0; i < 3; i++) J0[i * N + n] += c->pm_inertia[n];
   // mass checks (ExplicitModule.cpp:452-454): positive mass and inertia
   for (int64_t n = 0; n < N; n++) {
     const double a = J0[n], b = J0[N + n], cc = J0[2 * N + n], xy = J0[3 * N + n],
@@ -444,6 +452,25 @@ extern "C" int dl_destroy(dl_ctx *c) {
   return DL_OK;
 }
 
+extern "C" int dl_set_point_masses(dl_ctx *c, const double *mass, const double *inertia,
+                                   int64_t n_nodes) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if ((mass || inertia) && n_nodes != c->d.N) return fail(DL_ERR_INPUT, "n_nodes mismatch");
+  for (const double *a : {mass, inertia})
+    if (a)
+      for (int64_t n = 0; n < n_nodes; n++)
+        if (!(a[n] >= 0.0) || !std::isfinite(a[n]))
+          return fail(DL_ERR_INPUT, "point masses must be finite and >= 0");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  std::vector<double> pm(mass ? mass : (const double *)nullptr, mass ? mass + n_nodes : nullptr);
+  std::vector<double> pi(inertia ? inertia : (const double *)nullptr,
+                         inertia ? inertia + n_nodes : nullptr);
+  c->pm_mass.swap(pm);
+  c->pm_inertia.swap(pi);
+  return section_tables(c, false);
+}
+
 extern "C" int dl_set_sections(dl_ctx *c, const dl_sections *sec, const double *contact_radius) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (!sec && !contact_radius) return fail(DL_ERR_INPUT, "nothing to set");
@@ -584,7 +611,7 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
   c->d.bc_f = pf;
   c->d.bc_rslot = pr;
   c->d.nbc = nbc;
-  return DL_OK;
+  return prog_entries(c);
 }
 
 static BcHost &bch(dl_ctx *c) { return c->bch; }
@@ -644,6 +671,7 @@ extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n,
 
 extern "C" int dl_set_node_dt(dl_ctx *c, const double *dt, int64_t n_nodes) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (dt && c->pd.nprog > 0) return fail(DL_ERR_STATE, "per-node time steps with time programs");
   c->adj.valid = false;
   if (!dt) {
     c->d.ndt = nullptr;
@@ -797,6 +825,8 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
 // one explicit step: kernels only (capturable into a hipGraph)
 int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  // boundary values at the end time of the step (after advance)
+  launch_bc_program(c->pd, sp.dt, nullptr, c->stream);
   if (c->contact) {
     int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc, c->hist_ncand, c->step_ctr);
     if (rc) return rc;
@@ -808,6 +838,7 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
                   c->hist_react, c->hist_diss, c->step_ctr, c->stream);
+  launch_clock_advance(c->pd, sp.dt, c->stream);
   return DL_OK;
 }
 

@@ -68,6 +68,13 @@ struct AdjState {
   int64_t rbar_cap = 0;
 };
 
+// boundary-data time programs (host copy)
+struct ProgState {
+  std::vector<int32_t> kind, cptr, dof_prog;
+  std::vector<double> abc;
+  std::vector<ProgOp> code;
+};
+
 // MilneDevice attempts: device buffers and the captured attempt graph
 struct MilneState {
   bool ready = false;
@@ -135,6 +142,10 @@ struct dl_ctx {
   AdjState adj;
   // MilneDevice
   MilneState milne;
+  // boundary programs, point masses (per node, added to mt and J0's diagonal)
+  ProgState prog;
+  ProgDev pd{};
+  std::vector<double> pm_mass, pm_inertia;
 };
 
 template <class T>
@@ -164,5 +175,7 @@ int prepare_history(dl_ctx *c, int64_t nsteps);
 // nsteps further steps (captured graph when configured), continuing the step counter
 int advance_steps(dl_ctx *c, int64_t nsteps, const StepParams &sp);
 int check_err(dl_ctx *c);
+// rebuilds the device list of program-driven boundary values (dl_program.cpp)
+int prog_entries(dl_ctx *c);
 
 #endif

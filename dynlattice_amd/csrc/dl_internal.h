@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "dl_device.h"
+#include "dl_program.h"
 
 // Nodes, elements, struts and the shared-node gather map.
 struct DevData {
@@ -110,6 +111,26 @@ struct AdjDev {
   const double *rbar;   // [groups] objective adjoint of this step's reactions
 };
 
+// boundary-data time programs: scales evaluated on the device from the
+// clock (t, k), written into the boundary records' values
+struct ProgDev {
+  int32_t nprog;
+  const int32_t *kind;   // [nprog]
+  const double *abc;     // [nprog][3]
+  const int32_t *cptr;   // [nprog+1] expression code ranges
+  const ProgOp *code;
+  int32_t nent;          // program-driven record values
+  const int32_t *ent_rec;
+  const int32_t *ent_dof;  // dof 0..5, + 8 when constrained (acceleration)
+  const int32_t *ent_prog;
+  const double *ent_base;  // dl_set_bcs value
+  double *bc_a;          // record arrays written ([6][nbc])
+  double *bc_f;
+  int32_t nbc;
+  double *clock_t;       // [1] time
+  int64_t *clock_k;      // [1] steps taken
+};
+
 // MilneDevice attempts: predicted then corrected state, error partials and
 // the step-size control state, all on the device
 struct MilneDev {
@@ -123,6 +144,8 @@ struct MilneDev {
   int32_t *acc;   // [1] last attempt accepted
   int64_t *nacc;  // [1] accepted attempts
   double *log;    // [attempts][4]: dt, error, accepted, next dt
+  double *clock_t;   // program clock advanced on accept, or null
+  int64_t *clock_k;
 };
 
 struct MilneCtl {
@@ -156,5 +179,9 @@ void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step
 void launch_milne_restore(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
 void launch_milne_commit_pl(const DevData &d, const MilneDev &m, hipStream_t s);
 void launch_set_f64(double *p, double v, hipStream_t s);
+// boundary programs (dl_kernels.hip): values at clock time + dt (dtp: device
+// dt, else dt); clock advance by dt
+void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s);
+void launch_clock_advance(const ProgDev &P, double dt, hipStream_t s);
 
 #endif

@@ -438,6 +438,37 @@ __global__ void k_set_i32(int32_t *p, int32_t v) {
 }
 
 // ------------------------------------------------------------------
+// boundary programs: the scale of every program at (clock time + dt, steps
+// taken), then every program-driven record value = its base x its scale.
+// One block; programs and values are few (loaded groups, supports).
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bc_program(ProgDev P, double dt, const double *dtp) {
+  __shared__ double scale[DL_PROG_MAXPROGS];
+  const double t = *P.clock_t + (dtp ? *dtp : dt);
+  const double k = (double)*P.clock_k;
+  for (int32_t q = threadIdx.x; q < P.nprog; q += blockDim.x) {
+    const int32_t c0 = P.cptr[q];
+    scale[q] = prog_scale(P.kind[q], P.abc + 3 * q, P.code + c0, P.cptr[q + 1] - c0, t, k);
+  }
+  __syncthreads();
+  const int64_t nb = P.nbc;
+  for (int32_t j = threadIdx.x; j < P.nent; j += blockDim.x) {
+    const int32_t dof = P.ent_dof[j];
+    const double v = P.ent_base[j] * scale[P.ent_prog[j]];
+    const int64_t at = (int64_t)(dof & 7) * nb + P.ent_rec[j];
+    if (dof & 8) P.bc_a[at] = v;
+    else P.bc_f[at] = v;
+  }
+}
+
+__global__ void k_clock_advance(double *t, int64_t *k, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *t += dt;
+    *k += 1;
+  }
+}
+
+// ------------------------------------------------------------------
 // COMMIT (applyDeform): per-element potential energy and dissipation
 // increment, old <- curr
 // ------------------------------------------------------------------
@@ -625,6 +656,16 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
 
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, p, v);
+}
+
+void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s) {
+  if (P.nprog == 0) return;
+  hipLaunchKernelGGL(k_bc_program, dim3(1), dim3(256), 0, s, P, dt, dtp);
+}
+
+void launch_clock_advance(const ProgDev &P, double dt, hipStream_t s) {
+  if (P.nprog == 0) return;
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(1), 0, s, P.clock_t, P.clock_k, dt);
 }
 
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {

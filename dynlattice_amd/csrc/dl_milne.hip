@@ -203,6 +203,10 @@ __global__ __launch_bounds__(1024) void k_milne_ctl(MilneDev m, MilneCtl mc, con
   if (accept) {
     *m.tacc += dt;
     *m.nacc += 1;
+    if (m.clock_t) {
+      *m.clock_t += dt;
+      *m.clock_k += 1;
+    }
   }
   *m.dt = t;
 }

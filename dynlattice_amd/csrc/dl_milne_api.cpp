@@ -49,10 +49,16 @@ static int milne_buffers(dl_ctx *c, int64_t n_attempts) {
 // one attempt, kernels only (capturable); ev: MILNE_EVENTS events or null
 static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
   const DevData &d = c->d;
-  const MilneDev &m = c->milne.m;
+  MilneDev m = c->milne.m;
   hipStream_t s = c->stream;
   const StepParams sp{0.0, 1, 0};
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
+  // boundary values at the end time of the attempt; the clock moves on accept
+  if (c->pd.nprog > 0) {
+    launch_bc_program(c->pd, 0.0, m.dt, s);
+    m.clock_t = c->pd.clock_t;
+    m.clock_k = c->pd.clock_k;
+  }
   // forces at the start state; contact pairs counted in the attempt's row
   if (c->contact) {
     int rc = contact_step(c->cs, d, s, c->hist_nc, c->hist_ncand, c->step_ctr);

@@ -34,7 +34,9 @@ EXPORTS = (
     "dl_phase_time", "dl_set_timing", "dl_sync", "dl_last_error",
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
+    "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
 )
+DL_PROG_CONST, DL_PROG_STEP, DL_PROG_EXPR = 0, 1, 2
 
 #: section fields of dl_sections / dl_section_grads, in ABI order
 SECTION_FIELDS = ("area", "I1", "I2", "J", "shear_factor", "young", "shear_modulus", "density")
@@ -113,6 +115,22 @@ def milne_opts(dt, precision, dt_min=None, dt_max=None, safety=0.9, incr=1.2, de
                          decr, len_scale)
 
 
+class dl_program(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
+                ("expr", C.c_char_p)]
+
+
+def program_eval(expr, t=0.0, i=0.0):
+    """value of a program expression through the device's compiler and
+    evaluator, run on the host"""
+    L = load()
+    v = C.c_double()
+    rc = L.dl_program_eval(expr.encode(), float(t), float(i), C.byref(v))
+    if rc != DL_OK:
+        raise DLError(rc, L.dl_last_error().decode())
+    return v.value
+
+
 class dl_milne_out(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("accepted", C.c_int64), ("time", C.c_double),
                 ("dt_next", C.c_double)]
@@ -156,6 +174,11 @@ def load():
     L.dl_adjoint_backward.argtypes = [C.c_void_p, dp, C.POINTER(dl_section_grads)]
     L.dl_milne_step.argtypes = [C.c_void_p, C.c_int64, C.POINTER(dl_milne_opts), dp,
                                 C.POINTER(dl_milne_out)]
+    L.dl_set_programs.argtypes = [C.c_void_p, C.POINTER(dl_program), C.c_int32, ip32, C.c_int64]
+    L.dl_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+    L.dl_get_time.argtypes = [C.c_void_p, dp, ip64]
+    L.dl_program_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
+    L.dl_set_point_masses.argtypes = [C.c_void_p, dp, dp, C.c_int64]
     _lib = L
     return L
 
@@ -221,6 +244,11 @@ class GpuModel:
                                      C.byref(opt), C.byref(h)))
         self.h = h
         self.set_bcs(p.cmask, p.cval, p.fext)
+        if getattr(p, "point_mass", None) is not None or getattr(p, "point_inertia", None) is not None:
+            self.set_point_masses(p.point_mass, p.point_inertia)
+        if getattr(p, "programs", None):
+            self.set_programs(p.programs, p.dof_prog)
+            self.set_time(p.t0, 0)
         self.set_state(p.disp0, p.velo0)
         if init:
             self.init()
@@ -337,6 +365,33 @@ class GpuModel:
         n = C.c_int64()
         self._check(self.L.dl_get_history(self.h, _p(r), _p(e), _p(c, ip64), nsteps, C.byref(n)))
         return {"reaction": r[:n.value], "e_diss": e[:n.value], "n_contacts": c[:n.value]}
+
+    def set_programs(self, programs, dof_prog):
+        """programs: list of lattice.Program; dof_prog [N, 6] (-1: none)"""
+        n = len(programs)
+        arr = (dl_program * max(n, 1))()
+        enc = [pr.expr.encode() for pr in programs]
+        for j, pr in enumerate(programs):
+            arr[j] = dl_program(pr.KINDS[pr.kind], pr.a, pr.b, pr.c, enc[j])
+        dpg = _c(dof_prog if dof_prog is not None else np.full(6 * self.p.n_nodes, -1), np.int32)
+        self._prog_keep = (arr, enc, dpg)
+        self._check(self.L.dl_set_programs(self.h, arr, n, _p(dpg, ip32), dpg.size))
+
+    def set_time(self, t, k=0):
+        self._check(self.L.dl_set_time(self.h, float(t), int(k)))
+
+    def get_time(self):
+        t = C.c_double()
+        k = C.c_int64()
+        self._check(self.L.dl_get_time(self.h, C.byref(t), C.byref(k)))
+        return t.value, k.value
+
+    def set_point_masses(self, mass=None, inertia=None):
+        m = None if mass is None else _c(mass)
+        i = None if inertia is None else _c(inertia)
+        self._check(self.L.dl_set_point_masses(self.h, _p(m) if m is not None else None,
+                                               _p(i) if i is not None else None,
+                                               self.p.n_nodes))
 
     def milne(self, n_attempts, opts):
         """n_attempts MilneDevice attempts from the current state (opts:

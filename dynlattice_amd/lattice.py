@@ -132,6 +132,34 @@ class Contact:
 
 
 @dataclass
+class Program:
+    """Time program of boundary values: the value of a dof that follows it
+    is (its cval / fext entry) x scale, evaluated at the end time of every
+    step.  kind "const": scale = a; "step": scale = a + b (k + c), k = steps
+    taken (Neumann initLoad/loadIncr: c = 1; Dirichlet initDisp/dispIncr:
+    c = 0); "expr": scale = expr(t, i) in JEM function syntax (LoadScale
+    scaleFunc)."""
+    kind: str = "const"
+    a: float = 1.0
+    b: float = 0.0
+    c: float = 0.0
+    expr: str = ""
+
+    KINDS = {"const": 0, "step": 1, "expr": 2}
+
+
+def add_program(p: "Problem", prog: Program, nodes, dofs) -> int:
+    """attaches prog to the given nodes' dofs; returns its index"""
+    if p.dof_prog is None:
+        p.dof_prog = np.full((p.n_nodes, 6), -1, np.int32)
+    p.programs.append(prog)
+    q = len(p.programs) - 1
+    for d in np.atleast_1d(dofs):
+        p.dof_prog[np.asarray(nodes), int(d)] = q
+    return q
+
+
+@dataclass
 class Problem:
     """One explicit-dynamics case: mesh, materials, contact, boundary data."""
     coords: np.ndarray
@@ -151,6 +179,14 @@ class Problem:
     dt: float = 0.0
     groups: Dict[str, np.ndarray] = field(default_factory=dict)
     name: str = ""
+    # time programs of the boundary values (see Program); dof_prog [N, 6]
+    # holds each dof's program index or -1
+    programs: list = field(default_factory=list)
+    dof_prog: Optional[np.ndarray] = None
+    t0: float = 0.0
+    # point masses per node (MassModel): translational mass, rotational inertia
+    point_mass: Optional[np.ndarray] = None
+    point_inertia: Optional[np.ndarray] = None
 
     @property
     def n_nodes(self):

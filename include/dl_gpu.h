@@ -302,6 +302,55 @@ typedef struct {
 int dl_milne_step(dl_ctx *ctx, int64_t n_attempts, const dl_milne_opts *opts, double *attempts,
                   dl_milne_out *out);
 
+/* Time programs of the boundary data.  Every prescribed acceleration
+ * (constrained dof) and external force (free dof) of dl_set_bcs may follow a
+ * program: its value becomes (the dl_set_bcs value) x (the program's scale).
+ * Replaces the time dependence the reference's boundary models give the
+ * solver through GET_CONSTRAINTS / GET_EXT_VECTOR after
+ * ExplicitModule::advance (ExplicitModule.cpp:259-274, 397-419):
+ *   DL_PROG_STEP, scale = a + b (k + c): NeumannModel initLoad/loadIncr
+ *     (a = initLoad, b = loadIncr, c = 1; NeumannModel.cpp:118-147,347-368)
+ *     and DirichletModel initDisp/dispIncr (c = 0; DirichletModel.cpp:163-172,
+ *     290-306,400-406);
+ *   DL_PROG_EXPR, scale = f(t, i): JIVE's LoadScale scaleFunc (e.g.
+ *     tests/transient/test1.pro, test3.pro), f in JEM function syntax:
+ *     numbers, t, i, PI, + - * / % ^, comparisons, && ||, !, if(c,a,b),
+ *     sin cos tan asin acos atan exp log sqrt abs floor ceil min max pow;
+ *   DL_PROG_CONST, scale = a.
+ * t is the time at the end of the step being taken (clock time + dt) and
+ * k (i) the number of steps taken on the clock; both are device-resident:
+ * every dl_explicit_step step and every accepted dl_milne_step attempt
+ * advances them.  Scales are evaluated on the device before every step /
+ * attempt (no host round trip; graph replay included).  dof_prog[6N]: the
+ * program of each dof or -1.  n_progs = 0 removes all programs.  Not with
+ * per-node time steps or the adjoint. */
+#define DL_PROG_CONST 0
+#define DL_PROG_STEP 1
+#define DL_PROG_EXPR 2
+#define DL_MAX_PROGRAMS 64
+typedef struct {
+  int32_t kind;
+  double a, b, c;
+  const char *expr; /* DL_PROG_EXPR */
+} dl_program;
+int dl_set_programs(dl_ctx *ctx, const dl_program *progs, int32_t n_progs, const int32_t *dof_prog,
+                    int64_t ndof);
+/* The programs' clock: time t and steps taken. */
+int dl_set_time(dl_ctx *ctx, double t, int64_t step);
+int dl_get_time(dl_ctx *ctx, double *t, int64_t *step);
+/* Evaluates a program expression on the host with the compiler and
+ * evaluator the device uses (to check an expression before a run). */
+int dl_program_eval(const char *expr, double t, double i, double *value);
+
+/* Replaces: MassModel's GET_MATRIX2 contribution (MassModel.cpp:92-108):
+ * per node, a point mass added to the three translational dofs and a
+ * rotational inertia added to the three rotational dofs (either array may
+ * be NULL; NULL for both removes them).  The reference adds a group's
+ * totalMass / group size to each listed dof; a group listing all
+ * translational and/or all rotational dofs maps onto these arrays exactly
+ * (an isotropic addition commutes with the node rotation). */
+int dl_set_point_masses(dl_ctx *ctx, const double *mass, const double *inertia, int64_t n_nodes);
+
 /* Wait for the context's stream. */
 int dl_sync(dl_ctx *ctx);
 

@@ -234,6 +234,15 @@ struct ref_model {
   /* bcs */
   uint8_t *cmask;
   double *cval, *fext;
+  /* boundary programs: dl_set_bcs-style base values scaled per step */
+  int n_prog;
+  ref_prog prog[REF_MAX_PROG];
+  int32_t *dof_prog;  /* [6N] or NULL */
+  double *cval0, *fext0;
+  double clock_t;
+  int64_t clock_k;
+  /* point masses (MassModel) */
+  double *pm_mass, *pm_inertia;
   /* options */
   int so3;
   double inertia_correct;
@@ -400,6 +409,9 @@ void ref_destroy(ref_model *m) {
   free(m->dflow); free(m->ediss); free(m->ehard); free(m->active);
   free(m->crad); free(m->black); free(m->actA); free(m->actB);
   free(m->cmask); free(m->cval); free(m->fext);
+  for (int q = 0; q < m->n_prog; q++) free(m->prog[q].expr);
+  free(m->dof_prog); free(m->cval0); free(m->fext0);
+  free(m->pm_mass); free(m->pm_inertia);
   free(m->st_off); free(m->nd_ptr); free(m->nd_row);
   free(m->loc_blk); free(m->loc_f); free(m->loc_g);
   free(m);
@@ -411,7 +423,280 @@ int ref_set_bcs(ref_model *m, const uint8_t *cmask, const double *cval,
   if (cmask) memcpy(m->cmask, cmask, n); else memset(m->cmask, 0, n);
   if (cval) memcpy(m->cval, cval, n * sizeof(double)); else memset(m->cval, 0, n * sizeof(double));
   if (fext) memcpy(m->fext, fext, n * sizeof(double)); else memset(m->fext, 0, n * sizeof(double));
+  if (m->cval0) {
+    memcpy(m->cval0, m->cval, n * sizeof(double));
+    memcpy(m->fext0, m->fext, n * sizeof(double));
+  }
   return REF_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary programs                                                   */
+/* ------------------------------------------------------------------ */
+/* Expression evaluation by recursive descent directly on the string (JEM
+ * function syntax subset of LoadScale scaleFunc strings): numbers, t, i,
+ * PI, + - * / % ^ (unary minus looser than ^), comparisons, && ||, !,
+ * if(c,a,b) and the one- and two-argument functions listed below. */
+typedef struct {
+  const char *s;
+  size_t p;
+  double t, i;
+  int err;
+} ref_ex;
+
+static double ex_or(ref_ex *e);
+static void ex_ws(ref_ex *e) {
+  while (e->s[e->p] == ' ' || e->s[e->p] == '\t' || e->s[e->p] == '\n') e->p++;
+}
+static int ex_eat(ref_ex *e, const char *tok) {
+  ex_ws(e);
+  size_t n = strlen(tok);
+  if (strncmp(e->s + e->p, tok, n) == 0) {
+    e->p += n;
+    return 1;
+  }
+  return 0;
+}
+static double ex_unary(ref_ex *e);
+static double ex_primary(ref_ex *e) {
+  ex_ws(e);
+  char c = e->s[e->p];
+  if ((c >= '0' && c <= '9') || c == '.') {
+    char *end;
+    double v = strtod(e->s + e->p, &end);
+    if (end == e->s + e->p) { e->err = 1; return 0.; }
+    e->p = (size_t)(end - e->s);
+    return v;
+  }
+  if (c == '(') {
+    e->p++;
+    double v = ex_or(e);
+    if (!ex_eat(e, ")")) e->err = 1;
+    return v;
+  }
+  if ((c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || c == '_') {
+    char id[16];
+    int n = 0;
+    while (((e->s[e->p] >= 'a' && e->s[e->p] <= 'z') || (e->s[e->p] >= 'A' && e->s[e->p] <= 'Z') ||
+            (e->s[e->p] >= '0' && e->s[e->p] <= '9') || e->s[e->p] == '_') && n < 15)
+      id[n++] = e->s[e->p++];
+    id[n] = 0;
+    if (!strcmp(id, "t")) return e->t;
+    if (!strcmp(id, "i")) return e->i;
+    if (!strcmp(id, "PI")) return 3.14159265358979323846;
+    double a[3] = {0., 0., 0.};
+    int na = (!strcmp(id, "if")) ? 3 : (!strcmp(id, "min") || !strcmp(id, "max") || !strcmp(id, "pow")) ? 2 : 1;
+    if (!ex_eat(e, "(")) { e->err = 1; return 0.; }
+    for (int k = 0; k < na; k++) {
+      if (k > 0 && !ex_eat(e, ",")) { e->err = 1; return 0.; }
+      a[k] = ex_or(e);
+    }
+    if (!ex_eat(e, ")")) { e->err = 1; return 0.; }
+    if (!strcmp(id, "if")) return a[0] != 0. ? a[1] : a[2];
+    if (!strcmp(id, "min")) return a[0] < a[1] ? a[0] : a[1];
+    if (!strcmp(id, "max")) return a[0] > a[1] ? a[0] : a[1];
+    if (!strcmp(id, "pow")) return pow(a[0], a[1]);
+    if (!strcmp(id, "sin")) return sin(a[0]);
+    if (!strcmp(id, "cos")) return cos(a[0]);
+    if (!strcmp(id, "tan")) return tan(a[0]);
+    if (!strcmp(id, "asin")) return asin(a[0]);
+    if (!strcmp(id, "acos")) return acos(a[0]);
+    if (!strcmp(id, "atan")) return atan(a[0]);
+    if (!strcmp(id, "exp")) return exp(a[0]);
+    if (!strcmp(id, "log")) return log(a[0]);
+    if (!strcmp(id, "sqrt")) return sqrt(a[0]);
+    if (!strcmp(id, "abs")) return fabs(a[0]);
+    if (!strcmp(id, "floor")) return floor(a[0]);
+    if (!strcmp(id, "ceil")) return ceil(a[0]);
+    e->err = 1;
+    return 0.;
+  }
+  e->err = 1;
+  return 0.;
+}
+static double ex_power(ref_ex *e) {
+  double x = ex_primary(e);
+  ex_ws(e);
+  if (e->s[e->p] == '^') {
+    e->p++;
+    return pow(x, ex_unary(e));
+  }
+  return x;
+}
+static double ex_unary(ref_ex *e) {
+  ex_ws(e);
+  if (e->s[e->p] == '-') { e->p++; return -ex_unary(e); }
+  if (e->s[e->p] == '+') { e->p++; return ex_unary(e); }
+  if (e->s[e->p] == '!' && e->s[e->p + 1] != '=') { e->p++; return ex_unary(e) == 0. ? 1. : 0.; }
+  return ex_power(e);
+}
+static double ex_mul(ref_ex *e) {
+  double x = ex_unary(e);
+  for (;;) {
+    ex_ws(e);
+    char c = e->s[e->p];
+    if (c == '*') { e->p++; x = x * ex_unary(e); }
+    else if (c == '/') { e->p++; x = x / ex_unary(e); }
+    else if (c == '%') { e->p++; x = fmod(x, ex_unary(e)); }
+    else return x;
+  }
+}
+static double ex_add(ref_ex *e) {
+  double x = ex_mul(e);
+  for (;;) {
+    ex_ws(e);
+    char c = e->s[e->p];
+    if (c == '+') { e->p++; x = x + ex_mul(e); }
+    else if (c == '-') { e->p++; x = x - ex_mul(e); }
+    else return x;
+  }
+}
+static double ex_cmp(ref_ex *e) {
+  double x = ex_add(e);
+  if (ex_eat(e, "<=")) return x <= ex_add(e);
+  if (ex_eat(e, ">=")) return x >= ex_add(e);
+  if (ex_eat(e, "==")) return x == ex_add(e);
+  if (ex_eat(e, "!=")) return x != ex_add(e);
+  if (ex_eat(e, "<")) return x < ex_add(e);
+  if (ex_eat(e, ">")) return x > ex_add(e);
+  return x;
+}
+static double ex_and(ref_ex *e) {
+  double x = ex_cmp(e);
+  while (ex_eat(e, "&&")) {
+    double y = ex_cmp(e);
+    x = (x != 0.) && (y != 0.);
+  }
+  return x;
+}
+static double ex_or(ref_ex *e) {
+  double x = ex_and(e);
+  while (ex_eat(e, "||")) {
+    double y = ex_and(e);
+    x = (x != 0.) || (y != 0.);
+  }
+  return x;
+}
+
+int ref_expr_eval(const char *expr, double t, double i, double *value) {
+  ref_ex e = {expr, 0, t, i, 0};
+  double v = ex_or(&e);
+  ex_ws(&e);
+  if (e.err || e.s[e.p]) return REF_ERR_INPUT;
+  *value = v;
+  return REF_OK;
+}
+
+/* STEP accumulator as the reference iterates it: NeumannModel keeps
+ * loadScale0 and adds loadIncr at every advance, committing the sum
+ * (NeumannModel.cpp:347-368); DirichletModel starts from initDisp - dispIncr
+ * (DirichletModel.cpp:165-172) and does the same: scale_k = acc_k + b,
+ * acc_{k+1} = scale_k, acc_0 = a + b (c - 1) */
+static void prog_reset_acc(ref_model *m) {
+  for (int q = 0; q < m->n_prog; q++) {
+    ref_prog *g = &m->prog[q];
+    g->acc = g->a + g->b * ((double)m->clock_k + g->c - 1.);
+  }
+}
+
+int ref_set_programs(ref_model *m, int n, const int32_t *kind, const double *abc,
+                     const char *const *exprs, const int32_t *dof_prog) {
+  if (n < 0 || n > REF_MAX_PROG) return REF_ERR_INPUT;
+  const int64_t nd = 6 * m->N;
+  for (int q = 0; q < m->n_prog; q++) free(m->prog[q].expr);
+  m->n_prog = 0;
+  free(m->dof_prog);
+  m->dof_prog = NULL;
+  if (n == 0) {
+    if (m->cval0) {
+      memcpy(m->cval, m->cval0, nd * sizeof(double));
+      memcpy(m->fext, m->fext0, nd * sizeof(double));
+    }
+    return REF_OK;
+  }
+  for (int q = 0; q < n; q++) {
+    ref_prog *g = &m->prog[q];
+    g->kind = kind[q];
+    g->a = abc[3 * q]; g->b = abc[3 * q + 1]; g->c = abc[3 * q + 2];
+    g->expr = NULL;
+    if (g->kind == REF_PROG_EXPR) {
+      double v;
+      if (!exprs || !exprs[q] || ref_expr_eval(exprs[q], 0., 0., &v)) { m->n_prog = q; return REF_ERR_INPUT; }
+      const size_t len = strlen(exprs[q]) + 1;
+      g->expr = (char *)malloc(len);
+      memcpy(g->expr, exprs[q], len);
+    }
+  }
+  m->n_prog = n;
+  m->dof_prog = (int32_t *)malloc(sizeof(int32_t) * nd);
+  memcpy(m->dof_prog, dof_prog, sizeof(int32_t) * nd);
+  if (!m->cval0) {
+    m->cval0 = (double *)malloc(nd * sizeof(double));
+    m->fext0 = (double *)malloc(nd * sizeof(double));
+    memcpy(m->cval0, m->cval, nd * sizeof(double));
+    memcpy(m->fext0, m->fext, nd * sizeof(double));
+  }
+  prog_reset_acc(m);
+  return REF_OK;
+}
+
+int ref_set_time(ref_model *m, double t, int64_t k) {
+  m->clock_t = t;
+  m->clock_k = k;
+  prog_reset_acc(m);
+  return REF_OK;
+}
+
+int ref_get_time(const ref_model *m, double *t, int64_t *k) {
+  if (t) *t = m->clock_t;
+  if (k) *k = m->clock_k;
+  return REF_OK;
+}
+
+/* boundary values of the step that ends at time t_end */
+static int prog_apply(ref_model *m, double t_end, double *scale) {
+  for (int q = 0; q < m->n_prog; q++) {
+    const ref_prog *g = &m->prog[q];
+    if (g->kind == REF_PROG_CONST) scale[q] = g->a;
+    else if (g->kind == REF_PROG_STEP) scale[q] = g->acc + g->b;
+    else if (ref_expr_eval(g->expr, t_end, (double)m->clock_k, &scale[q])) return REF_ERR_INPUT;
+  }
+  const int64_t nd = 6 * m->N;
+  for (int64_t i = 0; i < nd; i++) {
+    const int32_t q = m->dof_prog[i];
+    if (q < 0) continue;
+    if (m->cmask[i]) m->cval[i] = m->cval0[i] * scale[q];
+    else m->fext[i] = m->fext0[i] * scale[q];
+  }
+  return REF_OK;
+}
+
+/* the step is taken (committed): the clock and the accumulators move */
+static void prog_commit(ref_model *m, double dt, const double *scale) {
+  m->clock_t += dt;
+  m->clock_k += 1;
+  for (int q = 0; q < m->n_prog; q++)
+    if (m->prog[q].kind == REF_PROG_STEP) m->prog[q].acc = scale[q];
+}
+
+int ref_set_point_masses(ref_model *m, const double *mass, const double *inertia) {
+  const int64_t N = m->N;
+  free(m->pm_mass); free(m->pm_inertia);
+  m->pm_mass = m->pm_inertia = NULL;
+  if (mass) { m->pm_mass = (double *)malloc(N * sizeof(double)); memcpy(m->pm_mass, mass, N * sizeof(double)); }
+  if (inertia) { m->pm_inertia = (double *)malloc(N * sizeof(double)); memcpy(m->pm_inertia, inertia, N * sizeof(double)); }
+  return REF_OK;
+}
+
+/* MassModel's GET_MATRIX2 (MassModel.cpp:92-108): diagonal additions */
+static void add_point_masses(const ref_model *m, double *blocks) {
+  for (int64_t n = 0; n < m->N; n++) {
+    double *B = blocks + 36 * n;
+    if (m->pm_mass)
+      for (int i = 0; i < 3; i++) B[7 * i] += m->pm_mass[n];
+    if (m->pm_inertia)
+      for (int i = 3; i < 6; i++) B[7 * i] += m->pm_inertia[n];
+  }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1286,6 +1571,7 @@ int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped
         for (int jn = 0; jn < 2; jn++) add_block(blocks + 36 * m->en[2 * e + in], SI[in][jn], lumped);
     }
   }
+  add_point_masses(m, blocks);
   return REF_OK;
 }
 #endif
@@ -1346,6 +1632,7 @@ int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped
     for (int64_t t = m->nd_ptr[n]; t < m->nd_ptr[n + 1]; t++)
       for (int i = 0; i < 36; i++) B[i] += m->loc_blk[36 * m->nd_row[t] + i];
   }
+  add_point_masses(m, blocks);
   return REF_OK;
 }
 
@@ -1529,8 +1816,11 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
   double *fg = (double *)malloc(sizeof(double) * nd);
   double *acc = (double *)malloc(sizeof(double) * nd);
   int rc = REF_OK;
+  double scale[REF_MAX_PROG];
   for (int64_t st = 0; st < nsteps && !rc; st++) {
-    /* advance: updCond default true -> updateMass_ at the current state */
+    /* advance: boundary values at the new time; updCond default true ->
+     * updateMass_ at the current state */
+    if (m->n_prog && (rc = prog_apply(m, m->clock_t + dt, scale))) break;
     ref_mass_blocks(m, disp, blocks, !m->so3);
     if (!m->so3) {
       for (int64_t n = 0; n < N; n++)
@@ -1603,6 +1893,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
     } else {
       apply_deform(m);
     }
+    if (m->n_prog) prog_commit(m, dt, scale);
   }
   free(blocks); free(fint); free(fg); free(acc);
   return rc;
@@ -1696,7 +1987,10 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
   double *v_new = (double *)malloc(sizeof(double) * nd);
   double dt = o->dt;
   int rc = REF_OK;
+  double scale[REF_MAX_PROG];
   for (int64_t k = 0; k < n_attempts && !rc; k++) {
+    /* advance: boundary values at the attempt's end time */
+    if (m->n_prog && (rc = prog_apply(m, m->clock_t + dt, scale))) break;
     ref_mass_blocks(m, disp, blocks, !m->so3);
     int64_t nc = 0;
     if ((rc = milne_acce(m, disp, velo, blocks, fint, fg, acc, &nc))) break;
@@ -1731,6 +2025,7 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
       apply_deform(m);
       memcpy(disp, u_new, sizeof(double) * nd);
       memcpy(velo, v_new, sizeof(double) * nd);
+      if (m->n_prog) prog_commit(m, dt, scale);
     } else {
       double t = o->safety * dt_opt;
       if (t < o->decr * dt) t = o->decr * dt;

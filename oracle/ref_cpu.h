@@ -133,6 +133,32 @@ void ref_set_grid(ref_model *m, int use_grid);
  * committed state: applyDeform, then potential and dissipated energy passes. */
 int ref_commit(ref_model *m, const double *disp, double *e_pot, double *e_diss);
 
+/* Boundary programs: every constrained dof's prescribed acceleration and
+ * every free dof's external force with dof_prog[i] = q >= 0 becomes (its
+ * ref_set_bcs value) x scale_q, evaluated before every step / attempt at the
+ * step's end time (clock + dt) after ExplicitModule::advance
+ * (ExplicitModule.cpp:259-274): CONST a; STEP as NeumannModel / Dirichlet
+ * iterate it (scale_k = a + b (k + c) by repeated addition); EXPR a LoadScale
+ * scaleFunc of t and i.  The clock (t, k) moves with each taken step. */
+#define REF_MAX_PROG 64
+#define REF_PROG_CONST 0
+#define REF_PROG_STEP 1
+#define REF_PROG_EXPR 2
+typedef struct {
+  int kind;
+  double a, b, c;
+  char *expr;
+  double acc;
+} ref_prog;
+int ref_set_programs(ref_model *m, int n, const int32_t *kind, const double *abc,
+                     const char *const *exprs, const int32_t *dof_prog);
+int ref_set_time(ref_model *m, double t, int64_t k);
+int ref_get_time(const ref_model *m, double *t, int64_t *k);
+int ref_expr_eval(const char *expr, double t, double i, double *value);
+/* MassModel point masses: per node, added to the translational and the
+ * rotational diagonal of the mass blocks (either may be NULL) */
+int ref_set_point_masses(ref_model *m, const double *mass, const double *inertia);
+
 /* MilneDevice predictor-corrector with the adaptive step of
  * ExplicitModule::commit (MilneDeviceModule.cpp:41-112,
  * ExplicitModule.cpp:290-335); see ref_cpu.c */

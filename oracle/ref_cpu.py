@@ -92,6 +92,11 @@ def lib(omp=False):
     L.ref_commit.argtypes = [C.c_void_p, dp, dp, dp]
     L.ref_milne.argtypes = [C.c_void_p, dp, dp, C.POINTER(MilneOpts), C.c_int64, ip32,
                             C.c_int64, C.c_int32, dp]
+    L.ref_set_programs.argtypes = [C.c_void_p, C.c_int, ip32, dp, C.POINTER(C.c_char_p), ip32]
+    L.ref_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+    L.ref_get_time.argtypes = [C.c_void_p, dp, ip64]
+    L.ref_expr_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
+    L.ref_set_point_masses.argtypes = [C.c_void_p, dp, dp]
     L.ref_contact_candidates.restype = C.c_int64
     L.ref_contact_candidates.argtypes = [C.c_void_p, dp, ip32, ip32, C.c_int64]
     L.ref_contact_active.restype = C.c_int64
@@ -135,6 +140,15 @@ def expvecp(psi, psiP):
     out = np.zeros(9)
     lib().ref_expvecp(_p(_c(psi)), _p(_c(psiP)), _p(out))
     return out.reshape(3, 3)
+
+
+def expr_eval(expr, t=0.0, i=0.0, omp=False):
+    """the oracle's evaluation of a program expression (raises ValueError
+    on a syntax error)"""
+    v = C.c_double()
+    if lib(omp).ref_expr_eval(expr.encode(), float(t), float(i), C.byref(v)):
+        raise ValueError(f"bad expression: {expr!r}")
+    return v.value
 
 
 class RefModel:
@@ -189,6 +203,11 @@ class RefModel:
         self.L.ref_set_grid(self.h, int(grid))
         self._check()
         self.set_bcs(p.cmask, p.cval, p.fext)
+        if getattr(p, "point_mass", None) is not None or getattr(p, "point_inertia", None) is not None:
+            self.set_point_masses(p.point_mass, p.point_inertia)
+        if getattr(p, "programs", None):
+            self.set_programs(p.programs, p.dof_prog)
+            self.set_time(p.t0, 0)
         self.disp = _c(p.disp0).reshape(-1).copy()
         self.velo = _c(p.velo0).reshape(-1).copy()
         if init:
@@ -247,6 +266,33 @@ class RefModel:
         if energy:
             out["energies"] = en.reshape(-1, 3)
         return out
+
+    def set_programs(self, programs, dof_prog):
+        """programs: list of lattice.Program; dof_prog [N, 6] (-1: none)"""
+        n = len(programs)
+        kind = np.array([pr.KINDS[pr.kind] for pr in programs], np.int32)
+        abc = np.array([[pr.a, pr.b, pr.c] for pr in programs], np.float64).reshape(-1)
+        ex = (C.c_char_p * max(n, 1))(*[pr.expr.encode() for pr in programs])
+        dpg = _c(dof_prog if dof_prog is not None else np.full(6 * self.p.n_nodes, -1), np.int32)
+        self._prog_keep = (kind, abc, ex, dpg)
+        self._check(self.L.ref_set_programs(self.h, n, _p(kind, ip32), _p(abc), ex,
+                                            _p(dpg, ip32)))
+
+    def set_time(self, t, k=0):
+        self._check(self.L.ref_set_time(self.h, float(t), int(k)))
+
+    def get_time(self):
+        t = C.c_double()
+        k = C.c_int64()
+        self.L.ref_get_time(self.h, C.byref(t), C.byref(k))
+        return t.value, k.value
+
+    def set_point_masses(self, mass=None, inertia=None):
+        m = None if mass is None else _c(mass)
+        i = None if inertia is None else _c(inertia)
+        self._pm_keep = (m, i)
+        self._check(self.L.ref_set_point_masses(self.h, _p(m) if m is not None else None,
+                                                _p(i) if i is not None else None))
 
     def milne(self, n_attempts, opts, group=None, group_dof=2):
         """n_attempts MilneDevice attempts from the current state; returns a
