@@ -179,6 +179,12 @@ void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step
 void launch_milne_restore(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
 void launch_milne_commit_pl(const DevData &d, const MilneDev &m, hipStream_t s);
 void launch_set_f64(double *p, double v, hipStream_t s);
+// observables (dl_tables.hip): element table kind 0 strain, 1 mat_strain,
+// 2 stress, 3 mat_stress into out [E][6] (may be null), element potential
+// energy into epot [E] (may be null); node kinetic energy [N]
+void launch_elem_table(const DevData &d, const PlasticParams &pp, bool so3, int mat, int kind,
+                       double *out, double *epot, hipStream_t s);
+void launch_node_ekin(const DevData &d, bool so3, double *ekin, hipStream_t s);
 // boundary programs (dl_kernels.hip): values at clock time + dt (dtp: device
 // dt, else dt); clock advance by dt
 void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s);

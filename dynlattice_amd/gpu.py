@@ -35,7 +35,10 @@ EXPORTS = (
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
+    "dl_get_table", "dl_get_energies",
 )
+ELEM_TABLES = ("strain", "mat_strain", "stress", "mat_stress")
+NODE_TABLES = ("potentialEnergy", "kineticEnergy", "mass")
 DL_PROG_CONST, DL_PROG_STEP, DL_PROG_EXPR = 0, 1, 2
 
 #: section fields of dl_sections / dl_section_grads, in ABI order
@@ -179,6 +182,8 @@ def load():
     L.dl_get_time.argtypes = [C.c_void_p, dp, ip64]
     L.dl_program_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
     L.dl_set_point_masses.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+    L.dl_get_table.argtypes = [C.c_void_p, C.c_char_p, dp, C.c_int64, C.c_int32]
+    L.dl_get_energies.argtypes = [C.c_void_p, dp, dp, dp, dp]
     _lib = L
     return L
 
@@ -392,6 +397,23 @@ class GpuModel:
         self._check(self.L.dl_set_point_masses(self.h, _p(m) if m is not None else None,
                                                _p(i) if i is not None else None,
                                                self.p.n_nodes))
+
+    def table(self, name):
+        """GET_TABLE at the current state: [n_elems, 6] for strain, mat_strain,
+        stress, mat_stress; [n_nodes] for potentialEnergy, kineticEnergy, mass"""
+        if name in ELEM_TABLES:
+            rows, cols = self.p.n_elems, 6
+        else:
+            rows, cols = self.p.n_nodes, 1
+        out = np.zeros(rows * cols)
+        self._check(self.L.dl_get_table(self.h, name.encode(), _p(out), rows, cols))
+        return out.reshape(rows, cols) if cols > 1 else out
+
+    def energies(self):
+        """{kinetic, potential, dissipated, mass} totals at the current state"""
+        v = [C.c_double() for _ in range(4)]
+        self._check(self.L.dl_get_energies(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("kinetic", "potential", "dissipated", "mass"), [x.value for x in v]))
 
     def milne(self, n_attempts, opts):
         """n_attempts MilneDevice attempts from the current state (opts:

@@ -351,6 +351,26 @@ int dl_program_eval(const char *expr, double t, double i, double *value);
  * (an isotropic addition commutes with the node rotation). */
 int dl_set_point_masses(dl_ctx *ctx, const double *mass, const double *inertia, int64_t n_nodes);
 
+/* Replaces: Actions::GET_TABLE of the rod models (SpecialCosseratRodModel.cpp:
+ * 221-262, 426-531) and LatticeModel's kinetic energy and mass tables
+ * (LatticeModel.cpp:195-275), evaluated at the current state:
+ *   "strain", "mat_strain", "stress", "mat_stress"  [n_elems][6]
+ *       (gamma_x,y,z, kappa_x,y,z / n_x,y,z, m_x,y,z; spatial = PI x material;
+ *       stress of the elastoplastic material: the return map from the
+ *       committed state, which stays unchanged);
+ *   "potentialEnergy", "kineticEnergy", "mass"  [n_nodes][1]
+ *       (element potential energy w 1/2 eps_e . sigma shared equally by the
+ *       element's two nodes; 1/2 v . M v with the block-lumped mass; the
+ *       translational mass).
+ * n_rows and n_cols must match the table. */
+int dl_get_table(dl_ctx *ctx, const char *name, double *out, int64_t n_rows, int32_t n_cols);
+/* Totals at the current state: kinetic and potential energy as above, the
+ * dissipated energy accumulated so far (DL_TRACK_DISSIPATION steps and
+ * COMMIT actions) and the total mass (LatticeModel.cpp:160-190 sets
+ * kineticEnergy and mass, SpecialCosseratRodModel.cpp:367-388
+ * potentialEnergy and dissipatedEnergy).  Any pointer may be NULL. */
+int dl_get_energies(dl_ctx *ctx, double *e_kin, double *e_pot, double *e_diss, double *mass);
+
 /* Wait for the context's stream. */
 int dl_sync(dl_ctx *ctx);
 

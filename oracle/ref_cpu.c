@@ -2047,6 +2047,59 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
   return rc;
 }
 
+/* GET_TABLE at the state disp (SpecialCosseratRodModel.cpp:221-262,
+ * 426-531): kind 0 strain (PI x material), 1 mat_strain, 2 stress (PI x
+ * material), 3 mat_stress, the material stress with the default load case
+ * (return map from the committed state; getStresses_'s loadCase = "").
+ * out [E][6] (may be NULL); epot [E] (may be NULL): w 1/2 (eps - eps_p) .
+ * sigma of that evaluation (ElastoPlasticRodMaterial.cpp:291-292). */
+int ref_elem_table(ref_model *m, const double *disp, int kind, double *out, double *epot) {
+  for (int64_t e = 0; e < m->E; e++) {
+    double c[2][3], u[2][3], L[2][9], strain[6], Lip[9], phiP[3], g[2], w, sm[6];
+    elem_gather(m, e, disp, c, u, L);
+    elem_strains(c, u, L, m->strain0 + 6 * e, strain, Lip, phiP, g, &w);
+    int rc = material_stress(m, e, strain, sm, 1);
+    if (rc) return rc;
+    if (epot) {
+      const double *ep = (m->pl.model == REF_ELASTIC) ? NULL : m->curr_ep + 6 * e;
+      double t = 0.;
+      for (int i = 0; i < 6; i++) t += (strain[i] - (ep ? ep[i] : 0.)) * sm[i];
+      epot[e] = w * (0.5 * t);
+    }
+    if (!out) continue;
+    const double *v = (kind <= 1) ? strain : sm;
+    double *o = out + 6 * e;
+    if (kind == 1 || kind == 3) {
+      for (int i = 0; i < 6; i++) o[i] = v[i];
+    } else {
+      mv3(Lip, v, o);
+      mv3(Lip, v + 3, o + 3);
+    }
+  }
+  return REF_OK;
+}
+
+/* LatticeModel::getKineticEnergy (LatticeModel.cpp:195-222): per node
+ * 1/2 v . M v with the solver's mass blocks at disp (lumped in the additive
+ * rotation mode) */
+int ref_kinetic_energy(ref_model *m, const double *disp, const double *velo, double *ekin) {
+  const int64_t N = m->N;
+  double *blocks = (double *)malloc(sizeof(double) * 36 * N);
+  ref_mass_blocks(m, disp, blocks, !m->so3);
+  for (int64_t n = 0; n < N; n++) {
+    const double *B = blocks + 36 * n, *v = velo + 6 * n;
+    double t = 0.;
+    for (int i = 0; i < 6; i++) {
+      double a = 0.;
+      for (int j = 0; j < 6; j++) a += B[6 * i + j] * v[j];
+      t += v[i] * a;
+    }
+    ekin[n] = 0.5 * t;
+  }
+  free(blocks);
+  return REF_OK;
+}
+
 int ref_plastic_state(ref_model *m, double *eps_p, double *alpha, double *dflow, uint8_t *active) {
   const int64_t E = m->E;
   if (eps_p) memcpy(eps_p, m->curr_ep, 6 * E * sizeof(double));

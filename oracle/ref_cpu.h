@@ -155,6 +155,11 @@ int ref_set_programs(ref_model *m, int n, const int32_t *kind, const double *abc
 int ref_set_time(ref_model *m, double t, int64_t k);
 int ref_get_time(const ref_model *m, double *t, int64_t *k);
 int ref_expr_eval(const char *expr, double t, double i, double *value);
+/* GET_TABLE at disp: kind 0 strain, 1 mat_strain, 2 stress, 3 mat_stress
+ * into out [E][6]; element potential energy into epot [E] (either NULL) */
+int ref_elem_table(ref_model *m, const double *disp, int kind, double *out, double *epot);
+/* per-node kinetic energy 1/2 v . M v (mass blocks at disp) */
+int ref_kinetic_energy(ref_model *m, const double *disp, const double *velo, double *ekin);
 /* MassModel point masses: per node, added to the translational and the
  * rotational diagonal of the mass blocks (either may be NULL) */
 int ref_set_point_masses(ref_model *m, const double *mass, const double *inertia);

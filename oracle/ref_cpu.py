@@ -97,6 +97,8 @@ def lib(omp=False):
     L.ref_get_time.argtypes = [C.c_void_p, dp, ip64]
     L.ref_expr_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
     L.ref_set_point_masses.argtypes = [C.c_void_p, dp, dp]
+    L.ref_elem_table.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
+    L.ref_kinetic_energy.argtypes = [C.c_void_p, dp, dp, dp]
     L.ref_contact_candidates.restype = C.c_int64
     L.ref_contact_candidates.argtypes = [C.c_void_p, dp, ip32, ip32, C.c_int64]
     L.ref_contact_active.restype = C.c_int64
@@ -293,6 +295,28 @@ class RefModel:
         self._pm_keep = (m, i)
         self._check(self.L.ref_set_point_masses(self.h, _p(m) if m is not None else None,
                                                 _p(i) if i is not None else None))
+
+    TABLE_KINDS = {"strain": 0, "mat_strain": 1, "stress": 2, "mat_stress": 3}
+
+    def elem_table(self, name, disp=None):
+        """[E, 6] strain / stress table at disp (default: the current state)"""
+        d = self.disp if disp is None else _c(disp).reshape(-1)
+        out = np.zeros(6 * self.p.n_elems)
+        self._check(self.L.ref_elem_table(self.h, _p(d), self.TABLE_KINDS[name], _p(out), None))
+        return out.reshape(-1, 6)
+
+    def elem_potential_energy(self, disp=None):
+        d = self.disp if disp is None else _c(disp).reshape(-1)
+        ep = np.zeros(self.p.n_elems)
+        self._check(self.L.ref_elem_table(self.h, _p(d), 0, None, _p(ep)))
+        return ep
+
+    def kinetic_energy(self, disp=None, velo=None):
+        d = self.disp if disp is None else _c(disp).reshape(-1)
+        v = self.velo if velo is None else _c(velo).reshape(-1)
+        ek = np.zeros(self.p.n_nodes)
+        self._check(self.L.ref_kinetic_energy(self.h, _p(d), _p(v), _p(ek)))
+        return ek
 
     def milne(self, n_attempts, opts, group=None, group_dof=2):
         """n_attempts MilneDevice attempts from the current state; returns a
