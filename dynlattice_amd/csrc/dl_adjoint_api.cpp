@@ -42,6 +42,7 @@ static int adjoint_supported(dl_ctx *c) {
   if (!c->so3) return fail(DL_ERR_STATE, "the adjoint supports the SO(3) rotation update only");
   if (c->plastic) return fail(DL_ERR_STATE, "the adjoint supports the elastic material only");
   if (c->pd.nprog > 0) return fail(DL_ERR_STATE, "the adjoint does not support time programs");
+  if (c->imp.ngroups > 0) return fail(DL_ERR_STATE, "the adjoint does not support impact groups");
   return DL_OK;
 }
 

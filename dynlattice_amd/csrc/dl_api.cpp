@@ -567,9 +567,13 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
   std::vector<double> a, f;
   std::vector<char> inreact(N, 0);
   for (int32_t n : react_nodes) inreact[n] = 1;
+  const BcHost &hb = c->bch;
+  std::vector<int32_t> impg(N, -1);
+  for (size_t g = 0; g + 1 < hb.imp_ptr.size(); g++)
+    for (int32_t j = hb.imp_ptr[g]; j < hb.imp_ptr[g + 1]; j++) impg[hb.imp_nodes[j]] = (int32_t)g;
   int32_t nbc = 0;
   for (int64_t n = 0; n < N; n++) {
-    bool any = inreact[n];
+    bool any = inreact[n] || impg[n] >= 0;
     for (int i = 0; i < 6 && !any; i++)
       any = (cmask && cmask[6 * n + i]) || (fext && fext[6 * n + i] != 0.0) ||
             (cval && cmask && cmask[6 * n + i] && cval[6 * n + i] != 0.0);
@@ -593,6 +597,15 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
     mask[b] = m;
   }
   for (size_t k = 0; k < react_nodes.size(); k++) rs[bc[react_nodes[k]]] = (int32_t)k;
+  std::vector<int32_t> bimp;
+  if (!hb.imp_ptr.empty()) {
+    bimp.assign(std::max(nbc, 1), -1);
+    for (int64_t n = 0; n < N; n++)
+      if (impg[n] >= 0) {
+        bimp[bc[n]] = impg[n];
+        mask[bc[n]] |= 1u << hb.imp_dof[impg[n]];
+      }
+  }
   c->gen++;
   hipSetDevice(c->device);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -611,6 +624,14 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
   c->d.bc_f = pf;
   c->d.bc_rslot = pr;
   c->d.nbc = nbc;
+  c->d.bc_imp = nullptr;
+  if (!bimp.empty()) {
+    int32_t *pb;
+    if ((rc = upload(c, &pb, bimp))) return rc;
+    c->d.bc_imp = pb;
+  }
+  c->imp.bc_a = pa;
+  c->imp.nbc = nbc;
   return prog_entries(c);
 }
 
@@ -663,6 +684,68 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
                     h.fext.empty() ? nullptr : h.fext.data(), h.react);
 }
 
+extern "C" int dl_set_impact_groups(dl_ctx *c, const int32_t *nodes, const int32_t *group_ptr,
+                                    int32_t n_groups, const int32_t *dofs, const double *weights) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (n_groups < 0 || (n_groups > 0 && (!group_ptr || !nodes || !dofs || !weights || group_ptr[0] != 0)))
+    return fail(DL_ERR_INPUT, "impact groups need nodes, group_ptr (from 0), dofs, weights");
+  const int64_t N = c->d.N;
+  BcHost &h = bch(c);
+  std::vector<int32_t> seen(N, 0);
+  for (int32_t g = 0; g < n_groups; g++) {
+    if (group_ptr[g + 1] < group_ptr[g]) return fail(DL_ERR_INPUT, "group_ptr must be non-decreasing");
+    if (dofs[g] < 0 || dofs[g] > 5) return fail(DL_ERR_INPUT, "impact dof must be in [0, 5]");
+    if (!(weights[g] != 0.0) || !std::isfinite(weights[g]))
+      return fail(DL_ERR_INPUT, "impact weights must be finite and nonzero");
+    for (int32_t j = group_ptr[g]; j < group_ptr[g + 1]; j++) {
+      const int32_t n = nodes[j];
+      if (n < 0 || n >= N) return fail(DL_ERR_INPUT, "impact node out of range");
+      if (seen[n]++) return fail(DL_ERR_INPUT, "a node may belong to one impact group only");
+      if (!h.cmask.empty() && h.cmask[6 * (int64_t)n + dofs[g]])
+        return fail(DL_ERR_INPUT, "impact dof already constrained by dl_set_bcs");
+    }
+  }
+  if (n_groups > 0 && c->d.ndt) return fail(DL_ERR_STATE, "impact groups with per-node time steps");
+  hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t nn = n_groups ? group_ptr[n_groups] : 0;
+  h.imp_nodes.assign(nodes, nodes + nn);
+  h.imp_ptr.clear();
+  h.imp_dof.clear();
+  h.imp_w.clear();
+  if (n_groups > 0) {
+    h.imp_ptr.assign(group_ptr, group_ptr + n_groups + 1);
+    h.imp_dof.assign(dofs, dofs + n_groups);
+    h.imp_w.assign(weights, weights + n_groups);
+  }
+  ImpDev &I = c->imp;
+  for (void *q : {(void *)I.ptr, (void *)I.nodes, (void *)I.dof, (void *)I.w, (void *)I.R})
+    dfree(c, q);
+  I.ptr = I.nodes = I.dof = nullptr;
+  I.w = nullptr;
+  I.R = nullptr;
+  I.ngroups = n_groups;
+  int rc;
+  if (n_groups > 0) {
+    int32_t *pi;
+    double *pd;
+    if ((rc = upload(c, &pi, h.imp_ptr))) return rc;
+    I.ptr = pi;
+    if ((rc = upload(c, &pi, h.imp_nodes))) return rc;
+    I.nodes = pi;
+    if ((rc = upload(c, &pi, h.imp_dof))) return rc;
+    I.dof = pi;
+    if ((rc = upload(c, &pd, h.imp_w))) return rc;
+    I.w = pd;
+    if ((rc = dalloc(c, &I.R, n_groups))) return rc;
+    if (!I.ready && (rc = dalloc(c, &I.ready, 1))) return rc;
+    HIPCHK(hipMemset(I.ready, 0, sizeof(int32_t)));
+  }
+  c->adj.valid = false;
+  return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
+                    h.fext.empty() ? nullptr : h.fext.data(), h.react);
+}
+
 extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n, int32_t dof) {
   if (n < 0 || n >= INT32_MAX) return fail(DL_ERR_INPUT, "bad reaction group size");
   const int32_t gp[2] = {0, (int32_t)n};
@@ -672,6 +755,7 @@ extern "C" int dl_set_reaction_group(dl_ctx *c, const int32_t *nodes, int64_t n,
 extern "C" int dl_set_node_dt(dl_ctx *c, const double *dt, int64_t n_nodes) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (dt && c->pd.nprog > 0) return fail(DL_ERR_STATE, "per-node time steps with time programs");
+  if (dt && c->imp.ngroups > 0) return fail(DL_ERR_STATE, "per-node time steps with impact groups");
   c->adj.valid = false;
   if (!dt) {
     c->d.ndt = nullptr;
@@ -710,6 +794,8 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   switch (action) {
   case DL_INIT: {
     launch_init_strain(c->d, c->stream);
+    // no output has run yet: ImpactBC dofs start free
+    if (c->imp.ready) launch_set_i32(c->imp.ready, 0, c->stream);
     if (c->contact) contact_invalidate(c->cs, c->stream);
     HIPCHK(hipGetLastError());
     if (c->contact) {
@@ -835,6 +921,7 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
   launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
+  launch_impact_step(c->d, sp, c->imp, c->so3, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
                   c->hist_react, c->hist_diss, c->step_ctr, c->stream);

@@ -28,6 +28,9 @@ struct BcHost {
   std::vector<uint8_t> cmask;
   std::vector<double> cval, fext;
   std::vector<int32_t> react;     // reaction nodes, concatenated by group
+  // ImpactBC groups: nodes concatenated by group, per group dof and weight
+  std::vector<int32_t> imp_nodes, imp_ptr, imp_dof;
+  std::vector<double> imp_w;
 };
 
 // host copy of the mesh (set-up tables are rebuilt from it when the
@@ -145,6 +148,7 @@ struct dl_ctx {
   // boundary programs, point masses (per node, added to mt and J0's diagonal)
   ProgState prog;
   ProgDev pd{};
+  ImpDev imp{};
   std::vector<double> pm_mass, pm_inertia;
 };
 

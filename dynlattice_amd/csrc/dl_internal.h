@@ -31,6 +31,7 @@ struct DevData {
   const double *bc_a;      // [6][nbc] prescribed acceleration
   const double *bc_f;      // [6][nbc] external force
   const int32_t *bc_rslot; // reaction slot or -1
+  const int32_t *bc_imp;   // impact group of the record or -1 (null: no impact groups)
   int32_t nbc;
   int32_t react_dof;
   // struts
@@ -131,6 +132,22 @@ struct ProgDev {
   int64_t *clock_k;      // [1] steps taken
 };
 
+// ImpactBC groups (ImpactModel.cpp:74-104): the group's dof gets the
+// prescribed acceleration -R / w, R the group's internal-force sum at the
+// step's start state (the reaction the reference's GroupOutput computed
+// after the previous step); before the first output the dof is free
+struct ImpDev {
+  int32_t ngroups;
+  const int32_t *ptr;    // [ngroups+1]
+  const int32_t *nodes;
+  const int32_t *dof;    // [ngroups]
+  const double *w;       // [ngroups]
+  double *R;             // [ngroups]
+  int32_t *ready;        // [1]
+  double *bc_a;          // record accelerations written ([6][nbc])
+  int32_t nbc;
+};
+
 // MilneDevice attempts: predicted then corrected state, error partials and
 // the step-size control state, all on the device
 struct MilneDev {
@@ -146,6 +163,8 @@ struct MilneDev {
   double *log;    // [attempts][4]: dt, error, accepted, next dt
   double *clock_t;   // program clock advanced on accept, or null
   int64_t *clock_k;
+  int32_t *imp_ready; // impact groups: set on accept, or null
+  const int32_t *imp_dof; // impact group -> dof
 };
 
 struct MilneCtl {
@@ -189,5 +208,11 @@ void launch_node_ekin(const DevData &d, bool so3, double *ekin, hipStream_t s);
 // dt, else dt); clock advance by dt
 void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s);
 void launch_clock_advance(const ProgDev &P, double dt, hipStream_t s);
+// ImpactBC (dl_kernels.hip): after the joint kernel of a step, the deferred
+// impact nodes (their force in fout) are summed per group and updated
+void launch_impact_step(const DevData &d, const StepParams &sp, const ImpDev &I, bool so3,
+                        hipStream_t s);
+// group sums of fout into I.R and the records' accelerations (MilneDevice)
+void launch_impact_sum(const ImpDev &I, const DevData &d, hipStream_t s);
 
 #endif

@@ -41,17 +41,27 @@ __device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4
 // With a tape, the node's start state and f are recorded for the adjoint.
 // The rotation is (re)loaded here rather than kept live from the element
 // part: fewer registers across the material update (the line is in cache).
+// Nodes of ImpactBC groups are deferred: their force goes to fout and the
+// impact kernel updates them once the group's reaction is known
+// (impact_pass, with free_mask clearing the impact dof before the first
+// output).
 template <bool SO3>
 __device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
-                                            double f[6]) {
+                                            double f[6], unsigned free_mask = 0u,
+                                            bool impact_pass = false) {
   const int64_t N = d.N;
   const int32_t b = d.bc[n];
+  if (!impact_pass && b >= 0 && d.bc_imp && d.bc_imp[b] >= 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) d.fout[c * N + n] = f[c];
+    return;
+  }
   unsigned mask = 0u;
   double ac[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double fe[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (b >= 0) {
     const int64_t nb = d.nbc;
-    mask = d.bc_mask[b];
+    mask = d.bc_mask[b] & ~free_mask;
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       ac[c] = d.bc_a[c * nb + b];
@@ -461,6 +471,50 @@ __global__ __launch_bounds__(256) void k_bc_program(ProgDev P, double dt, const 
   }
 }
 
+// ------------------------------------------------------------------
+// ImpactBC: one block per group.  R = the group's internal-force sum on its
+// dof (fixed tree order), acceleration -R / w into the group's records,
+// then the deferred node updates (dof free before the first output).
+// ------------------------------------------------------------------
+__device__ __forceinline__ double impact_group_sum(const DevData &d, const ImpDev &I, int32_t g) {
+  __shared__ double red[256];
+  const int32_t t0 = I.ptr[g], t1 = I.ptr[g + 1], dof = I.dof[g];
+  double s = 0.0;
+  for (int32_t j = t0 + (int32_t)threadIdx.x; j < t1; j += 256) s += d.fout[(int64_t)dof * d.N + I.nodes[j]];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double R = red[0];
+  const double a = -R / I.w[g];
+  for (int32_t j = t0 + (int32_t)threadIdx.x; j < t1; j += 256)
+    I.bc_a[(int64_t)dof * I.nbc + d.bc[I.nodes[j]]] = a;
+  if (threadIdx.x == 0) I.R[g] = R;
+  return R;
+}
+
+template <bool SO3>
+__global__ __launch_bounds__(256) void k_impact_step(DevData d, StepParams sp, ImpDev I) {
+  const int32_t g = blockIdx.x;
+  impact_group_sum(d, I, g);
+  __syncthreads();
+  const unsigned free_mask = *I.ready ? 0u : (1u << I.dof[g]);
+  const int64_t N = d.N;
+  for (int32_t j = I.ptr[g] + (int32_t)threadIdx.x; j < I.ptr[g + 1]; j += 256) {
+    const int32_t n = I.nodes[j];
+    double f[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
+    node_update<SO3>(d, sp, n, f, free_mask, true);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_impact_sum(ImpDev I, DevData d) {
+  impact_group_sum(d, I, blockIdx.x);
+}
+
 __global__ void k_clock_advance(double *t, int64_t *k, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *t += dt;
@@ -661,6 +715,19 @@ void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
 void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s) {
   if (P.nprog == 0) return;
   hipLaunchKernelGGL(k_bc_program, dim3(1), dim3(256), 0, s, P, dt, dtp);
+}
+
+void launch_impact_step(const DevData &d, const StepParams &sp, const ImpDev &I, bool so3,
+                        hipStream_t s) {
+  if (I.ngroups == 0) return;
+  if (so3) hipLaunchKernelGGL(k_impact_step<true>, dim3(I.ngroups), dim3(256), 0, s, d, sp, I);
+  else hipLaunchKernelGGL(k_impact_step<false>, dim3(I.ngroups), dim3(256), 0, s, d, sp, I);
+  hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, I.ready, 1);
+}
+
+void launch_impact_sum(const ImpDev &I, const DevData &d, hipStream_t s) {
+  if (I.ngroups == 0) return;
+  hipLaunchKernelGGL(k_impact_sum, dim3(I.ngroups), dim3(256), 0, s, I, d);
 }
 
 void launch_clock_advance(const ProgDev &P, double dt, hipStream_t s) {

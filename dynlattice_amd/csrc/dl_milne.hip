@@ -28,10 +28,11 @@
 namespace {
 
 // boundary record of node n: constrained dofs, prescribed acceleration,
-// external force; the reaction slot receives f[react_dof]
-__device__ __forceinline__ void node_bc(const DevData &d, int32_t n, const double f[6],
-                                        unsigned &mask, double ac[6], double fe[6],
-                                        bool reaction) {
+// external force; the reaction slot receives f[react_dof].  An ImpactBC
+// dof is free until the first accepted attempt (no group reaction yet).
+__device__ __forceinline__ void node_bc(const DevData &d, const MilneDev &m, int32_t n,
+                                        const double f[6], unsigned &mask, double ac[6],
+                                        double fe[6], bool reaction) {
   const int32_t b = d.bc[n];
   mask = 0u;
 #pragma unroll
@@ -39,6 +40,7 @@ __device__ __forceinline__ void node_bc(const DevData &d, int32_t n, const doubl
   if (b < 0) return;
   const int64_t nb = d.nbc;
   mask = d.bc_mask[b];
+  if (d.bc_imp && d.bc_imp[b] >= 0 && !*m.imp_ready) mask &= ~(1u << m.imp_dof[d.bc_imp[b]]);
 #pragma unroll
   for (int c = 0; c < 6; c++) {
     ac[c] = d.bc_a[c * nb + b];
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_milne_pred(DevData d, MilneDev m) 
 #pragma unroll
   for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
   unsigned mask;
-  node_bc(d, (int32_t)n, f, mask, ac, fe, true);
+  node_bc(d, m, (int32_t)n, f, mask, ac, fe, true);
   double u[3], r[4], v[6];
   Quat q;
   load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_milne_corr(DevData d, MilneDev m, 
 #pragma unroll
     for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
     unsigned mask;
-    node_bc(d, (int32_t)n, f, mask, ac, fe, false);
+    node_bc(d, m, (int32_t)n, f, mask, ac, fe, false);
     double u[3], r[4], v[6], up[3], rp[4], vp[6];
     Quat q, qp;
     load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(1024) void k_milne_ctl(MilneDev m, MilneCtl mc, con
       *m.clock_t += dt;
       *m.clock_k += 1;
     }
+    if (m.imp_ready) *m.imp_ready = 1;
   }
   *m.dt = t;
 }

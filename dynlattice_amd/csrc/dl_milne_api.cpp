@@ -68,6 +68,12 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
   launch_strut_pp(d, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, s);
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
   launch_joint(d, sp, MODE_FORCE, c->so3, s);
+  // ImpactBC: the groups' reactions at the start state set their accelerations
+  if (c->imp.ngroups > 0) {
+    launch_impact_sum(c->imp, d, s);
+    m.imp_ready = c->imp.ready;
+    m.imp_dof = c->imp.dof;
+  }
   if (ev) HIPCHK(hipEventRecord(ev[3], s));
   launch_milne_pred(d, m, c->so3, s);
   if (ev) HIPCHK(hipEventRecord(ev[4], s));

@@ -35,7 +35,7 @@ EXPORTS = (
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
-    "dl_get_table", "dl_get_energies",
+    "dl_get_table", "dl_get_energies", "dl_set_impact_groups",
 )
 ELEM_TABLES = ("strain", "mat_strain", "stress", "mat_stress")
 NODE_TABLES = ("potentialEnergy", "kineticEnergy", "mass")
@@ -184,6 +184,7 @@ def load():
     L.dl_set_point_masses.argtypes = [C.c_void_p, dp, dp, C.c_int64]
     L.dl_get_table.argtypes = [C.c_void_p, C.c_char_p, dp, C.c_int64, C.c_int32]
     L.dl_get_energies.argtypes = [C.c_void_p, dp, dp, dp, dp]
+    L.dl_set_impact_groups.argtypes = [C.c_void_p, ip32, ip32, C.c_int32, ip32, dp]
     _lib = L
     return L
 
@@ -254,6 +255,8 @@ class GpuModel:
         if getattr(p, "programs", None):
             self.set_programs(p.programs, p.dof_prog)
             self.set_time(p.t0, 0)
+        if getattr(p, "impact", None):
+            self.set_impact_groups(p.impact)
         self.set_state(p.disp0, p.velo0)
         if init:
             self.init()
@@ -390,6 +393,17 @@ class GpuModel:
         k = C.c_int64()
         self._check(self.L.dl_get_time(self.h, C.byref(t), C.byref(k)))
         return t.value, k.value
+
+    def set_impact_groups(self, groups):
+        """groups: list of (nodes, dof, weight) (ImpactBC)"""
+        nodes = _c(np.concatenate([np.asarray(g[0]) for g in groups]) if groups else
+                   np.zeros(0), np.int32)
+        ptr = _c(np.cumsum([0] + [len(g[0]) for g in groups]), np.int32)
+        dofs = _c([g[1] for g in groups], np.int32)
+        w = _c([g[2] for g in groups])
+        self._imp_keep = (nodes, ptr, dofs, w)
+        self._check(self.L.dl_set_impact_groups(self.h, _p(nodes, ip32), _p(ptr, ip32), len(groups),
+                                                _p(dofs, ip32), _p(w)))
 
     def set_point_masses(self, mass=None, inertia=None):
         m = None if mass is None else _c(mass)

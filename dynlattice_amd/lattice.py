@@ -187,6 +187,8 @@ class Problem:
     # point masses per node (MassModel): translational mass, rotational inertia
     point_mass: Optional[np.ndarray] = None
     point_inertia: Optional[np.ndarray] = None
+    # ImpactBC groups: (nodes, dof, weight)
+    impact: list = field(default_factory=list)
 
     @property
     def n_nodes(self):

@@ -177,6 +177,18 @@ int dl_set_bcs(dl_ctx *ctx, const uint8_t *cmask, const double *cval, const doub
  * (GroupOutputModule.cpp:100-116,162), recorded once per step. */
 int dl_set_reaction_group(dl_ctx *ctx, const int32_t *nodes, int64_t n, int32_t dof);
 
+/* Replaces: ImpactModel (ImpactBC, ImpactModel.cpp:74-104): every node of
+ * group g gets on dof dofs[g] the prescribed acceleration -R_g / weights[g],
+ * R_g the group's internal-force sum on that dof at the step's start state
+ * (the "resp" the reference's GroupOutput computed after the previous step).
+ * Until the first step (LeapFrog) or accepted attempt (MilneDevice) after
+ * INIT there is no such output and the dof stays free, as in the reference.
+ * nodes[group_ptr[g] .. group_ptr[g+1]); a node belongs to one group at
+ * most, and the dof must not be constrained by dl_set_bcs.  n_groups = 0
+ * removes them.  Not with per-node time steps or the adjoint. */
+int dl_set_impact_groups(dl_ctx *ctx, const int32_t *nodes, const int32_t *group_ptr,
+                         int32_t n_groups, const int32_t *dofs, const double *weights);
+
 /* Several reaction groups at once (e.g. one per instance of a batched design
  * sweep): nodes[group_ptr[g] .. group_ptr[g+1]) form group g; every step
  * records one sum per group (dl_get_group_history).  Generalises the

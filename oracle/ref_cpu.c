@@ -243,6 +243,10 @@ struct ref_model {
   int64_t clock_k;
   /* point masses (MassModel) */
   double *pm_mass, *pm_inertia;
+  /* ImpactBC groups */
+  int imp_n, imp_ready;
+  int32_t *imp_nodes, *imp_ptr, *imp_dof;
+  double *imp_w;
   /* options */
   int so3;
   double inertia_correct;
@@ -253,6 +257,8 @@ struct ref_model {
 };
 
 int ref_last_error(const ref_model *m) { return m->err; }
+
+static void impact_constraints(ref_model *m, const double *fint);
 
 static int cmp_i32(const void *a, const void *b) {
   int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
@@ -412,6 +418,7 @@ void ref_destroy(ref_model *m) {
   for (int q = 0; q < m->n_prog; q++) free(m->prog[q].expr);
   free(m->dof_prog); free(m->cval0); free(m->fext0);
   free(m->pm_mass); free(m->pm_inertia);
+  free(m->imp_nodes); free(m->imp_ptr); free(m->imp_dof); free(m->imp_w);
   free(m->st_off); free(m->nd_ptr); free(m->nd_row);
   free(m->loc_blk); free(m->loc_f); free(m->loc_g);
   free(m);
@@ -1468,6 +1475,7 @@ int ref_init(ref_model *m, const double *disp) {
     elem_strains(c, u, L, NULL, m->strain0 + 6 * e, Lip, phiP, g, &w);
   }
   if (m->co.enabled) compute_blacklist(m, disp);
+  m->imp_ready = 0; /* no output yet: ImpactBC dofs start free */
   return m->err;
 }
 
@@ -1841,6 +1849,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
       for (int64_t k = 0; k < group_n; k++) r += fint[6 * (int64_t)group[k] + group_dof];
       react[st] = r;
     }
+    if (m->imp_n) impact_constraints(m, fint);
     /* fres = fext - fint; getAcce */
     REF_PARALLEL_FOR
     for (int64_t n = 0; n < N; n++) {
@@ -1894,6 +1903,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
       apply_deform(m);
     }
     if (m->n_prog) prog_commit(m, dt, scale);
+    m->imp_ready = 1;
   }
   free(blocks); free(fint); free(fg); free(acc);
   return rc;
@@ -1931,19 +1941,63 @@ static double quality(const ref_model *m, const double *y_pre, const double *y_c
   return sqrt(s) / sqrt((double)nd);
 }
 
-/* forces and accelerations at (disp, velo) with the mass blocks of the
- * step's start state: fres = fext - fint - fgyro, a = M^-1 fres (getForce /
- * updForce + getAcce) */
-static int milne_acce(ref_model *m, const double *disp, const double *velo,
-                      const double *blocks, double *fint, double *fg, double *acc,
-                      int64_t *nc) {
-  const int64_t N = m->N, nd = 6 * N;
+/* ImpactBC (ImpactModel.cpp:74-104): group g's dof gets the prescribed
+ * acceleration -R_g / w_g, R_g the group's internal-force sum at the start
+ * state (the reference reads it from GroupOutput's last output); before the
+ * first output (m->imp_ready == 0) the dof is free. */
+static void impact_constraints(ref_model *m, const double *fint) {
+  for (int g = 0; g < m->imp_n; g++) {
+    const int32_t dof = m->imp_dof[g];
+    double R = 0.;
+    for (int32_t j = m->imp_ptr[g]; j < m->imp_ptr[g + 1]; j++)
+      R += fint[6 * (int64_t)m->imp_nodes[j] + dof];
+    const double a = -R / m->imp_w[g];
+    for (int32_t j = m->imp_ptr[g]; j < m->imp_ptr[g + 1]; j++) {
+      const int64_t i = 6 * (int64_t)m->imp_nodes[j] + dof;
+      m->cmask[i] = (uint8_t)(m->imp_ready ? 1 : 0);
+      m->cval[i] = a;
+    }
+  }
+}
+
+int ref_set_impact_groups(ref_model *m, int n, const int32_t *nodes, const int32_t *ptr,
+                          const int32_t *dofs, const double *weights) {
+  free(m->imp_nodes); free(m->imp_ptr); free(m->imp_dof); free(m->imp_w);
+  m->imp_nodes = m->imp_ptr = m->imp_dof = NULL;
+  m->imp_w = NULL;
+  m->imp_n = n;
+  m->imp_ready = 0;
+  if (n <= 0) { m->imp_n = 0; return REF_OK; }
+  const int32_t nn = ptr[n];
+  m->imp_ptr = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+  m->imp_nodes = (int32_t *)malloc(sizeof(int32_t) * (nn ? nn : 1));
+  m->imp_dof = (int32_t *)malloc(sizeof(int32_t) * n);
+  m->imp_w = (double *)malloc(sizeof(double) * n);
+  memcpy(m->imp_ptr, ptr, sizeof(int32_t) * (n + 1));
+  memcpy(m->imp_nodes, nodes, sizeof(int32_t) * nn);
+  memcpy(m->imp_dof, dofs, sizeof(int32_t) * n);
+  memcpy(m->imp_w, weights, sizeof(double) * n);
+  return REF_OK;
+}
+
+/* internal forces at (disp, velo) with the mass blocks of the step's start
+ * state (getForce / updForce): fint (+ contact), gyro into fg */
+static int milne_forces(ref_model *m, const double *disp, const double *velo,
+                        const double *blocks, double *fint, double *fg, int64_t *nc) {
+  const int64_t nd = 6 * m->N;
   memset(fint, 0, sizeof(double) * nd);
   memset(fg, 0, sizeof(double) * nd);
   int rc = int_vector(m, disp, velo, fint, m->so3 ? fg : NULL, blocks, 1);
   if (rc) return rc;
   const int64_t c = m->co.enabled ? ref_contact_force(m, disp, fint) : 0;
   if (nc) *nc = c;
+  return REF_OK;
+}
+
+/* a = M^-1 (fext - fint - fgyro) with the constraints (getAcce) */
+static void milne_solve(ref_model *m, const double *blocks, const double *fint, const double *fg,
+                        double *acc) {
+  const int64_t N = m->N;
   REF_PARALLEL_FOR
   for (int64_t n = 0; n < N; n++) {
     double fr[6];
@@ -1957,7 +2011,6 @@ static int milne_acce(ref_model *m, const double *disp, const double *velo,
       }
     }
   }
-  return REF_OK;
 }
 
 /* MilneDevice attempts (MilneDeviceModule.cpp:41-112) with the adaptive step
@@ -1993,7 +2046,9 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
     if (m->n_prog && (rc = prog_apply(m, m->clock_t + dt, scale))) break;
     ref_mass_blocks(m, disp, blocks, !m->so3);
     int64_t nc = 0;
-    if ((rc = milne_acce(m, disp, velo, blocks, fint, fg, acc, &nc))) break;
+    if ((rc = milne_forces(m, disp, velo, blocks, fint, fg, &nc))) break;
+    if (m->imp_n) impact_constraints(m, fint);
+    milne_solve(m, blocks, fint, fg, acc);
     double r = 0.;
     for (int64_t g = 0; g < group_n; g++) r += fint[6 * (int64_t)group[g] + group_dof];
     /* predictor (ABupdate: dy = dt f) */
@@ -2001,7 +2056,8 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
     update_vec(m, velo, dy, v_pre, 0);
     for (int64_t i = 0; i < nd; i++) dy[i] = dt * velo[i];
     update_vec(m, disp, dy, u_pre, 1);
-    if ((rc = milne_acce(m, u_pre, v_pre, blocks, fint, fg, acc, NULL))) break;
+    if ((rc = milne_forces(m, u_pre, v_pre, blocks, fint, fg, NULL))) break;
+    milne_solve(m, blocks, fint, fg, acc);
     /* corrector (AMupdate_: dy = dt f_pre) */
     for (int64_t i = 0; i < nd; i++) dy[i] = dt * acc[i];
     update_vec(m, velo, dy, v_new, 0);
@@ -2026,6 +2082,7 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
       memcpy(disp, u_new, sizeof(double) * nd);
       memcpy(velo, v_new, sizeof(double) * nd);
       if (m->n_prog) prog_commit(m, dt, scale);
+      m->imp_ready = 1;
     } else {
       double t = o->safety * dt_opt;
       if (t < o->decr * dt) t = o->decr * dt;

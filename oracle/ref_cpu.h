@@ -160,6 +160,11 @@ int ref_expr_eval(const char *expr, double t, double i, double *value);
 int ref_elem_table(ref_model *m, const double *disp, int kind, double *out, double *epot);
 /* per-node kinetic energy 1/2 v . M v (mass blocks at disp) */
 int ref_kinetic_energy(ref_model *m, const double *disp, const double *velo, double *ekin);
+/* ImpactBC groups: nodes[ptr[g] .. ptr[g+1]) get on dofs[g] the prescribed
+ * acceleration -R_g / weights[g], R_g their internal-force sum on that dof
+ * at the step's start state; free until the first taken step after INIT */
+int ref_set_impact_groups(ref_model *m, int n, const int32_t *nodes, const int32_t *ptr,
+                          const int32_t *dofs, const double *weights);
 /* MassModel point masses: per node, added to the translational and the
  * rotational diagonal of the mass blocks (either may be NULL) */
 int ref_set_point_masses(ref_model *m, const double *mass, const double *inertia);

@@ -98,6 +98,7 @@ def lib(omp=False):
     L.ref_expr_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
     L.ref_set_point_masses.argtypes = [C.c_void_p, dp, dp]
     L.ref_elem_table.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
+    L.ref_set_impact_groups.argtypes = [C.c_void_p, C.c_int, ip32, ip32, ip32, dp]
     L.ref_kinetic_energy.argtypes = [C.c_void_p, dp, dp, dp]
     L.ref_contact_candidates.restype = C.c_int64
     L.ref_contact_candidates.argtypes = [C.c_void_p, dp, ip32, ip32, C.c_int64]
@@ -210,6 +211,8 @@ class RefModel:
         if getattr(p, "programs", None):
             self.set_programs(p.programs, p.dof_prog)
             self.set_time(p.t0, 0)
+        if getattr(p, "impact", None):
+            self.set_impact_groups(p.impact)
         self.disp = _c(p.disp0).reshape(-1).copy()
         self.velo = _c(p.velo0).reshape(-1).copy()
         if init:
@@ -288,6 +291,17 @@ class RefModel:
         k = C.c_int64()
         self.L.ref_get_time(self.h, C.byref(t), C.byref(k))
         return t.value, k.value
+
+    def set_impact_groups(self, groups):
+        """groups: list of (nodes, dof, weight)"""
+        nodes = _c(np.concatenate([np.asarray(g[0]) for g in groups]) if groups else
+                   np.zeros(0), np.int32)
+        ptr = _c(np.cumsum([0] + [len(g[0]) for g in groups]), np.int32)
+        dofs = _c([g[1] for g in groups], np.int32)
+        w = _c([g[2] for g in groups])
+        self._imp_keep = (nodes, ptr, dofs, w)
+        self._check(self.L.ref_set_impact_groups(self.h, len(groups), _p(nodes, ip32), _p(ptr, ip32),
+                                                 _p(dofs, ip32), _p(w)))
 
     def set_point_masses(self, mass=None, inertia=None):
         m = None if mass is None else _c(mass)
