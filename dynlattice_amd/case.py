@@ -13,7 +13,8 @@ case describe:
   ElasticRodMaterial.cpp:65-190), Dirichlet (INCREMENT or under LoadScale;
   the value is a prescribed acceleration in dynamics), Neumann (loadIncr /
   initLoad, or under LoadScale), LoadScale (scaleFunc of t), InitLoad
-  (initial displacements and velocities), Mass (point masses) and None;
+  (initial displacements and velocities), Mass (point masses), ImpactBC
+  (group-reaction-driven accelerations) and None;
   time dependence becomes device programs (:class:`~.lattice.Program`);
 * ``Solver.integrator``: MilneDevice or LeapFrog with deltaTime, precision,
   min/maxDtime, stepSaftey, increase/decreaseFactor, lengthScale; the
@@ -261,6 +262,10 @@ def _apply_models(props, p, dofidx, mdl_root="model.model"):
                     p.point_mass[nodes] += share
                 if rot:
                     p.point_inertia[nodes] += share
+            return
+        if typ == "ImpactBC":
+            for g, d, w in zip(spec["nodeGroups"], spec["dofs"], spec["weights"]):
+                p.impact.append((np.asarray(p.groups[g], np.int32), dofidx[d], P.number(w)))
             return
         raise CaseError(f"model type {typ} is not supported")
 
