@@ -510,11 +510,8 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     if (n > cap) *c.err = DL_ERR_CAPACITY;
     c.vcount[X] = n < cap ? n : cap;
   }
-}
-
-// the lists are built: remember the displacement they were built at
-__global__ __launch_bounds__(CB) void kc_vcommit(ContactDev c, DevData d, const int32_t *gate) {
-  GATED(gate);
+  // the lists are built: remember the displacement they were built at (no
+  // kernel here reads uref; the node updates that follow compare against it)
   GRID_STRIDE(n, c.N) {
     for (int i = 0; i < 3; i++) c.uref[i * c.N + n] = d.u[i * c.N + n];
   }
@@ -905,8 +902,7 @@ static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   const int32_t *g = c.gate;
   broadphase(cs, d, s, c.skin, g);
-  hipLaunchKernelGGL(kc_vbuild, dim3(gs(c.E)), dim3(CB), 0, s, c, d, g);
-  hipLaunchKernelGGL(kc_vcommit, dim3(gs(c.N)), dim3(CB), 0, s, c, d, g);
+  hipLaunchKernelGGL(kc_vbuild, dim3(gs(std::max(c.E, c.N))), dim3(CB), 0, s, c, d, g);
 }
 
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,

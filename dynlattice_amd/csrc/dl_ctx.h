@@ -82,6 +82,7 @@ struct ProgState {
 struct MilneState {
   bool ready = false;
   MilneDev m{};
+  int64_t part_cap = 0;
   int64_t log_cap = 0;
   hipGraphExec_t gexec = nullptr;
   int64_t graph_gen = -1;

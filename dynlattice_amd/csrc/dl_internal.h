@@ -90,7 +90,10 @@ struct StepParams {
   int32_t track_diss;
 };
 
-enum { MODE_STEP = 0, MODE_FORCE = 1 };
+// strut / joint kernel modes: the explicit step (node update), force only
+// (nodal force to fout), and the MilneDevice predictor / corrector fused
+// into the force evaluation
+enum { MODE_STEP = 0, MODE_FORCE = 1, MODE_MPRED = 2, MODE_MCORR = 3 };
 
 // Reverse sweep of the explicit step (SO(3) mode, elastic material).
 // Adjoints ("bar") of the state after the step being reversed are turned
@@ -151,11 +154,16 @@ struct ImpDev {
 // MilneDevice attempts: predicted then corrected state, error partials and
 // the step-size control state, all on the device
 struct MilneDev {
+  double *u0;     // the context's state (start of the attempt, then corrected)
+  double *r0;
+  double *v0;
   double *u1;     // [3][N] predicted displacement, then the start state's
   double *r1;     // [4][N] rotation (quaternion; additive mode: 3)
   double *v1;     // [6][N]
   double *part;   // [2][nblk] per-block sums of squared differences (u, v)
-  int32_t nblk;
+  int32_t nblk;   // partial sums per quantity
+  int32_t jofs;   // joint kernel's first slot (fused corrector), else 0
+  double ls;      // length scale of the error measure
   double *dt;     // [1] step size of the current attempt
   double *tacc;   // [1] sum of the accepted step sizes
   int32_t *acc;   // [1] last attempt accepted
@@ -174,9 +182,11 @@ struct MilneCtl {
 
 // launchers (dl_kernels.hip)
 void launch_init_strain(const DevData &d, hipStream_t s);
+// m: the MilneDevice buffers of the fused modes (null otherwise)
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
-                     bool so3, int mat, bool contact, hipStream_t s);
-void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s);
+                     bool so3, int mat, bool contact, hipStream_t s, const MilneDev *m = nullptr);
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s,
+                  const MilneDev *m = nullptr);
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
@@ -192,8 +202,7 @@ void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
 // MilneDevice (dl_milne.hip): predictor and corrector node passes over the
 // internal force in d.fout, step-size control, and the accept/reject commit
 void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
-void launch_milne_corr(const DevData &d, const MilneDev &m, double len_scale, bool so3,
-                       hipStream_t s);
+void launch_milne_corr(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
 void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step, hipStream_t s);
 void launch_milne_restore(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);
 void launch_milne_commit_pl(const DevData &d, const MilneDev &m, hipStream_t s);

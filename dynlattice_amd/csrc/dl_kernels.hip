@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dl_internal.h"
+#include "dl_milne_node.h"
 
 #define DL_BLOCK 256
 
@@ -153,7 +154,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
 // ------------------------------------------------------------------
 // MAT: 0 elastic; 1, 2, 3 elastoplastic with yield exponents all 1, all 2, general
 template <int MODE, bool SO3, int MAT, bool CONTACT>
-__global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp) {
+__global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp,
+                                                    MilneDev m) {
   constexpr bool PLASTIC = MAT != 0;
   const int64_t L = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
   // element of the lane: identity packing (every chain fills whole
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     if (!PLASTIC) {
 #pragma unroll
       for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
-      if (MODE == MODE_FORCE) {
+      if (MODE != MODE_STEP) {
 #pragma unroll
         for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
       }
@@ -290,6 +292,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     }
   }
   // node a of lane k receives fb of lane k-1 (same chain, same wavefront)
+  double em[2] = {0.0, 0.0};  // corrector error sums of this lane's nodes
   double f[6];
 #pragma unroll
   for (int i = 0; i < 6; i++) f[i] = __shfl_up(fb[i], 1, 64);
@@ -308,8 +311,12 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     } else if (MODE == MODE_FORCE) {
 #pragma unroll
       for (int i = 0; i < 6; i++) d.fout[i * N + na] = f[i];
-    } else {
+    } else if (MODE == MODE_STEP) {
       node_update<SO3>(d, sp, na, f);
+    } else if (MODE == MODE_MPRED) {
+      milne_pred_node<SO3>(d, m, na, f);
+    } else {
+      milne_corr_node<SO3>(d, m, na, f, em);
     }
     if (fl & 2u) { // last element of the chain: also node b
       const int32_t sl = d.sb[e];
@@ -319,11 +326,16 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
       } else if (MODE == MODE_FORCE) {
 #pragma unroll
         for (int i = 0; i < 6; i++) d.fout[i * N + nb] = fb[i];
-      } else {
+      } else if (MODE == MODE_STEP) {
         node_update<SO3>(d, sp, nb, fb);
+      } else if (MODE == MODE_MPRED) {
+        milne_pred_node<SO3>(d, m, nb, fb);
+      } else {
+        milne_corr_node<SO3>(d, m, nb, fb, em);
       }
     }
   }
+  if (MODE == MODE_MCORR) milne_block_partials(m, em, blockIdx.x);
   if (PLASTIC && MODE == MODE_STEP) {
     if (sp.track_diss) {
       // per-block partial of the dissipation increment, fixed tree order
@@ -343,8 +355,22 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 // joint kernel: shared nodes
 // ------------------------------------------------------------------
 template <int MODE, bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
+__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE == MODE_MCORR) {
+    // every thread joins the block's error partial sums
+    double em[2] = {0.0, 0.0};
+    if (j < d.NJ) {
+      const int32_t n = d.jnode[j];
+      double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int32_t t = d.jptr[j]; t < d.jptr[j + 1]; t++)
+#pragma unroll
+        for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + t];
+      milne_corr_node<SO3>(d, m, n, f, em);
+    }
+    milne_block_partials(m, em, m.jofs + blockIdx.x);
+    return;
+  }
   if (j >= d.NJ) return;
   const int32_t n = d.jnode[j];
   const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
@@ -368,6 +394,10 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp) {
   if (MODE == MODE_FORCE) {
 #pragma unroll
     for (int i = 0; i < 6; i++) d.fout[i * d.N + n] = f[i];
+    return;
+  }
+  if (MODE == MODE_MPRED) {
+    milne_pred_node<SO3>(d, m, n, f);
     return;
   }
   node_update<SO3>(d, sp, n, f);
@@ -404,12 +434,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) 
 // end of step: per-group reaction sums and the dissipation increment, in a
 // fixed tree order (deterministic), written to history row *step
 // ------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int32_t *rgptr,
-                                                   int32_t ngroups, const double *dbuf, int64_t ndb,
-                                                   double *diss_total, double *hist_react,
-                                                   double *hist_diss, const int32_t *step) {
-  __shared__ double red[1024];
-  const int g = blockIdx.x;
+// one sum (reaction group g < ngroups, else the dissipation partials)
+__device__ __forceinline__ void step_end_sum(int g, const double *rbuf, const int32_t *rgptr,
+                                             int32_t ngroups, const double *dbuf, int64_t ndb,
+                                             double *diss_total, double *hist_react,
+                                             double *hist_diss, int64_t st, double *red) {
   const double *in;
   int64_t n;
   if (g < ngroups) {
@@ -428,7 +457,6 @@ __global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const int64_t st = *step;
     if (g < ngroups) {
       hist_react[st * ngroups + g] = red[0];
     } else {
@@ -437,6 +465,31 @@ __global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int
       hist_diss[st] = t;
     }
   }
+  __syncthreads();
+}
+
+// several sums, one block each (the step counter moves in k_step_inc)
+__global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int32_t *rgptr,
+                                                   int32_t ngroups, const double *dbuf, int64_t ndb,
+                                                   double *diss_total, double *hist_react,
+                                                   double *hist_diss, const int32_t *step) {
+  __shared__ double red[1024];
+  step_end_sum(blockIdx.x, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss,
+               *step, red);
+}
+
+// few sums: one block does them in turn and then moves the step counter
+// (saves the counter's own launch every step)
+__global__ __launch_bounds__(1024) void k_step_end1(const double *rbuf, const int32_t *rgptr,
+                                                    int32_t ngroups, int32_t nsums,
+                                                    const double *dbuf, int64_t ndb,
+                                                    double *diss_total, double *hist_react,
+                                                    double *hist_diss, int32_t *step) {
+  __shared__ double red[1024];
+  const int64_t st = *step;
+  for (int g = 0; g < nsums; g++)
+    step_end_sum(g, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss, st, red);
+  if (threadIdx.x == 0) *step = (int32_t)(st + 1);
 }
 
 __global__ void k_step_inc(int32_t *step) {
@@ -643,52 +696,67 @@ void launch_init_strain(const DevData &d, hipStream_t s) {
 }
 
 template <int MODE, bool SO3, int MAT, bool CONTACT>
-static void ls(const DevData &d, const StepParams &sp, const PlasticParams &pp, hipStream_t s) {
+static void ls(const DevData &d, const StepParams &sp, const PlasticParams &pp, const MilneDev &m,
+               hipStream_t s) {
   hipLaunchKernelGGL((k_strut<MODE, SO3, MAT, CONTACT>), dim3(nblk(d.NL)), dim3(DL_BLOCK), 0, s,
-                     d, sp, pp);
+                     d, sp, pp, m);
 }
 
 template <int MODE, bool SO3, int MAT>
-static void ls_c(const DevData &d, const StepParams &sp, const PlasticParams &pp, bool contact,
-                 hipStream_t s) {
-  if (contact) ls<MODE, SO3, MAT, true>(d, sp, pp, s);
-  else ls<MODE, SO3, MAT, false>(d, sp, pp, s);
+static void ls_c(const DevData &d, const StepParams &sp, const PlasticParams &pp,
+                 const MilneDev &m, bool contact, hipStream_t s) {
+  if (contact) ls<MODE, SO3, MAT, true>(d, sp, pp, m, s);
+  else ls<MODE, SO3, MAT, false>(d, sp, pp, m, s);
 }
 
 template <int MODE, bool SO3>
-static void ls_m(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mat,
-                 bool contact, hipStream_t s) {
+static void ls_m(const DevData &d, const StepParams &sp, const PlasticParams &pp,
+                 const MilneDev &m, int mat, bool contact, hipStream_t s) {
   switch (mat) {
-  case 0: ls_c<MODE, SO3, 0>(d, sp, pp, contact, s); break;
-  case 1: ls_c<MODE, SO3, 1>(d, sp, pp, contact, s); break;
-  case 2: ls_c<MODE, SO3, 2>(d, sp, pp, contact, s); break;
-  default: ls_c<MODE, SO3, 3>(d, sp, pp, contact, s); break;
+  case 0: ls_c<MODE, SO3, 0>(d, sp, pp, m, contact, s); break;
+  case 1: ls_c<MODE, SO3, 1>(d, sp, pp, m, contact, s); break;
+  case 2: ls_c<MODE, SO3, 2>(d, sp, pp, m, contact, s); break;
+  default: ls_c<MODE, SO3, 3>(d, sp, pp, m, contact, s); break;
   }
+}
+
+template <int MODE>
+static void ls_s(const DevData &d, const StepParams &sp, const PlasticParams &pp,
+                 const MilneDev &m, bool so3, int mat, bool contact, hipStream_t s) {
+  if (so3) ls_m<MODE, true>(d, sp, pp, m, mat, contact, s);
+  else ls_m<MODE, false>(d, sp, pp, m, mat, contact, s);
 }
 
 // mat: 0 elastic, 1/2/3 elastoplastic with yield exponents all 1 / all 2 / general
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
-                     bool so3, int mat, bool contact, hipStream_t s) {
+                     bool so3, int mat, bool contact, hipStream_t s, const MilneDev *m) {
   if (d.NL == 0) return;
-  if (mode == MODE_STEP) {
-    if (so3) ls_m<MODE_STEP, true>(d, sp, pp, mat, contact, s);
-    else ls_m<MODE_STEP, false>(d, sp, pp, mat, contact, s);
-  } else {
-    if (so3) ls_m<MODE_FORCE, true>(d, sp, pp, mat, contact, s);
-    else ls_m<MODE_FORCE, false>(d, sp, pp, mat, contact, s);
+  const MilneDev mm = m ? *m : MilneDev{};
+  switch (mode) {
+  case MODE_STEP: ls_s<MODE_STEP>(d, sp, pp, mm, so3, mat, contact, s); break;
+  case MODE_FORCE: ls_s<MODE_FORCE>(d, sp, pp, mm, so3, mat, contact, s); break;
+  case MODE_MPRED: ls_s<MODE_MPRED>(d, sp, pp, mm, so3, mat, contact, s); break;
+  default: ls_s<MODE_MCORR>(d, sp, pp, mm, so3, mat, contact, s); break;
   }
 }
 
-void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s) {
-  if (d.NJ == 0) return;
+template <int MODE>
+static void lj(const DevData &d, const StepParams &sp, const MilneDev &m, bool so3, hipStream_t s) {
   const dim3 g(nblk(d.NJ)), b(DL_BLOCK);
-  if (mode == MODE_FORCE) {
-    if (so3) hipLaunchKernelGGL((k_joint<MODE_FORCE, true>), g, b, 0, s, d, sp);
-    else hipLaunchKernelGGL((k_joint<MODE_FORCE, false>), g, b, 0, s, d, sp);
-    return;
+  if (so3) hipLaunchKernelGGL((k_joint<MODE, true>), g, b, 0, s, d, sp, m);
+  else hipLaunchKernelGGL((k_joint<MODE, false>), g, b, 0, s, d, sp, m);
+}
+
+void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s,
+                  const MilneDev *m) {
+  if (d.NJ == 0) return;
+  const MilneDev mm = m ? *m : MilneDev{};
+  switch (mode) {
+  case MODE_STEP: lj<MODE_STEP>(d, sp, mm, so3, s); break;
+  case MODE_FORCE: lj<MODE_FORCE>(d, sp, mm, so3, s); break;
+  case MODE_MPRED: lj<MODE_MPRED>(d, sp, mm, so3, s); break;
+  default: lj<MODE_MCORR>(d, sp, mm, so3, s); break;
   }
-  if (so3) hipLaunchKernelGGL((k_joint<MODE_STEP, true>), g, b, 0, s, d, sp);
-  else hipLaunchKernelGGL((k_joint<MODE_STEP, false>), g, b, 0, s, d, sp);
 }
 
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
@@ -701,10 +769,15 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s) {
   const int nb = ngroups + (diss ? 1 : 0);
-  if (nb > 0)
-    hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
-                       ngroups, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
-                       (const int32_t *)step);
+  if (nb <= 8) {
+    hipLaunchKernelGGL(k_step_end1, dim3(1), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
+                       ngroups, nb, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
+                       step);
+    return;
+  }
+  hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
+                     ngroups, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
+                     (const int32_t *)step);
   hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step);
 }
 

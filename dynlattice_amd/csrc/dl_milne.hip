@@ -22,141 +22,35 @@
 #include <hip/hip_runtime.h>
 
 #include "dl_internal.h"
+#include "dl_milne_node.h"
 
 #define DL_BLOCK 256
 
 namespace {
-
-// boundary record of node n: constrained dofs, prescribed acceleration,
-// external force; the reaction slot receives f[react_dof].  An ImpactBC
-// dof is free until the first accepted attempt (no group reaction yet).
-__device__ __forceinline__ void node_bc(const DevData &d, const MilneDev &m, int32_t n,
-                                        const double f[6], unsigned &mask, double ac[6],
-                                        double fe[6], bool reaction) {
-  const int32_t b = d.bc[n];
-  mask = 0u;
-#pragma unroll
-  for (int c = 0; c < 6; c++) { ac[c] = 0.0; fe[c] = 0.0; }
-  if (b < 0) return;
-  const int64_t nb = d.nbc;
-  mask = d.bc_mask[b];
-  if (d.bc_imp && d.bc_imp[b] >= 0 && !*m.imp_ready) mask &= ~(1u << m.imp_dof[d.bc_imp[b]]);
-#pragma unroll
-  for (int c = 0; c < 6; c++) {
-    ac[c] = d.bc_a[c * nb + b];
-    fe[c] = d.bc_f[c * nb + b];
-  }
-  const int32_t rs = d.bc_rslot[b];
-  if (reaction && rs >= 0) d.rbuf[rs] = f[d.react_dof];
-}
-
-template <bool SO3>
-__device__ __forceinline__ void load_node(const double *u, const double *rot, const double *v,
-                                          int64_t N, int64_t n, double uu[3], double rr[4],
-                                          double vv[6], Quat &q) {
-#pragma unroll
-  for (int c = 0; c < 3; c++) uu[c] = u[c * N + n];
-#pragma unroll
-  for (int c = 0; c < 6; c++) vv[c] = v[c * N + n];
-  if (SO3) {
-#pragma unroll
-    for (int c = 0; c < 4; c++) rr[c] = rot[c * N + n];
-    q.w = rr[0]; q.x = rr[1]; q.y = rr[2]; q.z = rr[3];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; c++) rr[c] = rot[c * N + n];
-    rr[3] = 0.0;
-    q = qexp(rr[0], rr[1], rr[2]);
-  }
-}
-
-template <bool SO3>
-__device__ __forceinline__ void store_node(double *u, double *rot, double *v, int64_t N, int64_t n,
-                                           const double uu[3], const double rr[4],
-                                           const double vv[6]) {
-#pragma unroll
-  for (int c = 0; c < 3; c++) u[c * N + n] = uu[c];
-#pragma unroll
-  for (int c = 0; c < 6; c++) v[c * N + n] = vv[c];
-#pragma unroll
-  for (int c = 0; c < (SO3 ? 4 : 3); c++) rot[c * N + n] = rr[c];
-}
-
-// contact candidate lists: flag a rebuild when the node has left the skin
-__device__ __forceinline__ void list_trigger(const DevData &d, int64_t n, const double u[3]) {
-  if (!d.uref) return;
-  const int64_t N = d.N;
-  bool moved = false;
-#pragma unroll
-  for (int c = 0; c < 3; c++) moved = moved || fabs(u[c] - d.uref[c * N + n]) > d.ctrig;
-  if (moved) *d.cgate = 1;
-}
 
 template <bool SO3>
 __global__ __launch_bounds__(DL_BLOCK) void k_milne_pred(DevData d, MilneDev m) {
   const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
   if (n >= d.N) return;
   const int64_t N = d.N;
-  double f[6], ac[6], fe[6];
+  double f[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
-  unsigned mask;
-  node_bc(d, m, (int32_t)n, f, mask, ac, fe, true);
-  double u[3], r[4], v[6];
-  Quat q;
-  load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
-  double J0[6];
-#pragma unroll
-  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
-  double up[3], rp[4], vp[6];
-  node_milne_pred<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
-                       up, vp, rp);
-  store_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp);
-  list_trigger(d, n, up);
+  milne_pred_node<SO3>(d, m, n, f);
 }
 
 template <bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_milne_corr(DevData d, MilneDev m, double ls) {
+__global__ __launch_bounds__(DL_BLOCK) void k_milne_corr(DevData d, MilneDev m) {
   const int64_t n = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
   const int64_t N = d.N;
   double e[2] = {0.0, 0.0};
   if (n < N) {
-    double f[6], ac[6], fe[6];
+    double f[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) f[c] = d.fout[c * N + n];
-    unsigned mask;
-    node_bc(d, m, (int32_t)n, f, mask, ac, fe, false);
-    double u[3], r[4], v[6], up[3], rp[4], vp[6];
-    Quat q, qp;
-    load_node<SO3>(d.u, d.rot, d.v, N, n, u, r, v, q);
-    load_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp, qp);
-    double J0[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
-    node_milne_corr<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
-                         ls, up, vp, rp, e);
-    // the corrected state becomes current; the start state is kept for a
-    // rejection (each thread touches its own node only)
-    store_node<SO3>(d.u, d.rot, d.v, N, n, up, rp, vp);
-    store_node<SO3>(m.u1, m.r1, m.v1, N, n, u, r, v);
-    list_trigger(d, n, up);
+    milne_corr_node<SO3>(d, m, n, f, e);
   }
-  // per-block sums in a fixed tree order
-  __shared__ double red[2][DL_BLOCK];
-  red[0][threadIdx.x] = e[0];
-  red[1][threadIdx.x] = e[1];
-  __syncthreads();
-  for (int w = DL_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    m.part[blockIdx.x] = red[0][0];
-    m.part[m.nblk + blockIdx.x] = red[1][0];
-  }
+  milne_block_partials(m, e, blockIdx.x);
 }
 
 // error = (rms(u_pre - u_new) + dt rms(v_pre - v_new)) / 2 over all dofs
@@ -260,12 +154,9 @@ void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_
   else hipLaunchKernelGGL(k_milne_pred<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
 }
 
-void launch_milne_corr(const DevData &d, const MilneDev &m, double len_scale, bool so3,
-                       hipStream_t s) {
-  if (so3)
-    hipLaunchKernelGGL(k_milne_corr<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m, len_scale);
-  else
-    hipLaunchKernelGGL(k_milne_corr<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m, len_scale);
+void launch_milne_corr(const DevData &d, const MilneDev &m, bool so3, hipStream_t s) {
+  if (so3) hipLaunchKernelGGL(k_milne_corr<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
+  else hipLaunchKernelGGL(k_milne_corr<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, m);
 }
 
 void launch_milne_ctl(const MilneDev &m, const MilneCtl &mc, const int32_t *step, hipStream_t s) {

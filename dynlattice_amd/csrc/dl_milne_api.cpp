@@ -19,11 +19,18 @@ static int milne_buffers(dl_ctx *c, int64_t n_attempts) {
   int rc;
   if (!M.ready) {
     MilneDev &m = M.m;
-    m.nblk = (int32_t)((N + 255) / 256);
+    // partial sums: one per node block (node kernels), or one per strut
+    // block plus one per joint block (fused corrector)
+    const int64_t nb_node = (N + 255) / 256;
+    const int64_t nb_fused = (c->d.NL + 255) / 256 + (c->d.NJ + 255) / 256;
+    M.part_cap = std::max<int64_t>(nb_node, nb_fused);
+    m.u0 = c->d.u;
+    m.r0 = c->d.rot;
+    m.v0 = c->d.v;
     if ((rc = dalloc(c, &m.u1, 3 * N))) return rc;
     if ((rc = dalloc(c, &m.r1, 4 * N))) return rc;
     if ((rc = dalloc(c, &m.v1, 6 * N))) return rc;
-    if ((rc = dalloc(c, &m.part, 2 * (size_t)m.nblk))) return rc;
+    if ((rc = dalloc(c, &m.part, 2 * (size_t)M.part_cap))) return rc;
     if ((rc = dalloc(c, &m.dt, 1))) return rc;
     if ((rc = dalloc(c, &m.tacc, 1))) return rc;
     if ((rc = dalloc(c, &m.acc, 1))) return rc;
@@ -46,12 +53,25 @@ static int milne_buffers(dl_ctx *c, int64_t n_attempts) {
 // the start-state evaluation, after contact/strut/joint of the predicted one, end
 #define MILNE_EVENTS 9
 
-// one attempt, kernels only (capturable); ev: MILNE_EVENTS events or null
+// one attempt, kernels only (capturable); ev: MILNE_EVENTS events or null.
+// The predictor and corrector run inside the strut / joint kernels of the
+// two force evaluations (the nodal force stays in registers); with ImpactBC
+// groups, whose accelerations need the group sums first, they run as node
+// kernels over fout.
 static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
   const DevData &d = c->d;
   MilneDev m = c->milne.m;
   hipStream_t s = c->stream;
   const StepParams sp{0.0, 1, 0};
+  const bool fused = c->imp.ngroups == 0;
+  m.ls = mc.len_scale;
+  if (fused) {
+    m.jofs = (int32_t)((d.NL + 255) / 256);
+    m.nblk = m.jofs + (int32_t)((d.NJ + 255) / 256);
+  } else {
+    m.jofs = 0;
+    m.nblk = (int32_t)((d.N + 255) / 256);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
   // boundary values at the end time of the attempt; the clock moves on accept
   if (c->pd.nprog > 0) {
@@ -65,17 +85,18 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
     if (rc) return rc;
   }
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
-  launch_strut_pp(d, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, s);
+  launch_strut_pp(d, sp, c->pp, fused ? MODE_MPRED : MODE_FORCE, c->so3, c->mat_kind, c->contact,
+                  s, &m);
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
-  launch_joint(d, sp, MODE_FORCE, c->so3, s);
-  // ImpactBC: the groups' reactions at the start state set their accelerations
-  if (c->imp.ngroups > 0) {
+  launch_joint(d, sp, fused ? MODE_MPRED : MODE_FORCE, c->so3, s, &m);
+  if (!fused) {
+    // ImpactBC: the groups' reactions at the start state set their accelerations
     launch_impact_sum(c->imp, d, s);
     m.imp_ready = c->imp.ready;
     m.imp_dof = c->imp.dof;
   }
   if (ev) HIPCHK(hipEventRecord(ev[3], s));
-  launch_milne_pred(d, m, c->so3, s);
+  if (!fused) launch_milne_pred(d, m, c->so3, s);
   if (ev) HIPCHK(hipEventRecord(ev[4], s));
   // forces at the predicted state
   DevData dp = d;
@@ -87,11 +108,12 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
     if (rc) return rc;
   }
   if (ev) HIPCHK(hipEventRecord(ev[5], s));
-  launch_strut_pp(dp, sp, c->pp, MODE_FORCE, c->so3, c->mat_kind, c->contact, s);
+  launch_strut_pp(dp, sp, c->pp, fused ? MODE_MCORR : MODE_FORCE, c->so3, c->mat_kind, c->contact,
+                  s, &m);
   if (ev) HIPCHK(hipEventRecord(ev[6], s));
-  launch_joint(dp, sp, MODE_FORCE, c->so3, s);
+  launch_joint(dp, sp, fused ? MODE_MCORR : MODE_FORCE, c->so3, s, &m);
   if (ev) HIPCHK(hipEventRecord(ev[7], s));
-  launch_milne_corr(d, m, mc.len_scale, c->so3, s);
+  if (!fused) launch_milne_corr(d, m, c->so3, s);
   launch_milne_ctl(m, mc, c->step_ctr, s);
   launch_milne_restore(d, m, c->so3, s);
   // the material commits the predicted-state update (the last evaluation);
