@@ -146,6 +146,21 @@ DL_HD void elem_kinematics(const double Xa[3], const double ua[3], const double 
   }
 }
 
+// Initial material strain (initStrain_, SpecialCosseratRodModel.cpp:536-569):
+// the strain of the reference configuration, u = 0 and node rotations at
+// the identity.  The element kernels recompute it from the frames and
+// coordinates they load anyway instead of reading a stored copy (48 B per
+// element per evaluation); the INIT kernel stores the same bits for the
+// tables and the adjoint.
+DL_HD void elem_eps0(const double Xa[3], const double Xb[3], const Quat qNa, const Quat qNb,
+                     double eps0[6]) {
+  const double z[3] = {0.0, 0.0, 0.0};
+  const Quat id = {1.0, 0.0, 0.0, 0.0};
+  ElemKin k;
+  elem_kinematics(Xa, z, Xb, z, qmul(id, qNa), qmul(id, qNb), k);
+  for (int i = 0; i < 6; i++) eps0[i] = k.eps[i];
+}
+
 // internal nodal forces of the element from the material stress
 // fint[Idofs] += w XI sigma_spatial (SpecialCosseratRodModel.cpp:945, Line3D.cpp:141-162)
 DL_HD void elem_forces(const ElemKin &k, const double sm[6], double fa[6], double fb[6]) {

@@ -128,17 +128,13 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
   const int64_t E = d.E, N = d.N;
   const int32_t na = d.en0[e], nb = d.en1[e];
   double Xa[3], Xb[3];
-  const double z[3] = {0.0, 0.0, 0.0};
   for (int i = 0; i < 3; i++) { Xa[i] = d.X0[i * N + na]; Xb[i] = d.X0[i * N + nb]; }
   Quat qa, qb;
   qa.w = d.qN[e]; qa.x = d.qN[E + e]; qa.y = d.qN[2 * E + e]; qa.z = d.qN[3 * E + e];
   qb.w = d.qN[4 * E + e]; qb.x = d.qN[5 * E + e]; qb.y = d.qN[6 * E + e]; qb.z = d.qN[7 * E + e];
-  const Quat id = {1.0, 0.0, 0.0, 0.0};
-  qa = qmul(id, qa);
-  qb = qmul(id, qb);
-  ElemKin k;
-  elem_kinematics(Xa, z, Xb, z, qa, qb, k);
-  for (int i = 0; i < 6; i++) d.eps0[i * E + e] = k.eps[i];
+  double eps0[6];
+  elem_eps0(Xa, Xb, qa, qb, eps0);
+  for (int i = 0; i < 6; i++) d.eps0[i * E + e] = eps0[i];
 }
 
 // ------------------------------------------------------------------
@@ -187,11 +183,12 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     Quat qNa, qNb;
     qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
     qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+    double strain[6], sm[6];
+    elem_eps0(Xa, Xb, qNa, qNb, strain);
     ElemKin k;
     elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), k);
-    double strain[6], sm[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - d.eps0[i * E + e];
+    for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - strain[i];
     if (!PLASTIC) {
 #pragma unroll
       for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
