@@ -374,6 +374,38 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       d.lane_ident = lane_e[L] == (L < E ? (int32_t)L : -1);
     uint8_t *pu;
     TRY(upload(c, &pu, eflag)); d.eflag = pu;
+    // end-b frames for k_strut: by shuffle from the next lane where it holds
+    // the same frame bit for bit, else the per-element array
+    {
+      std::vector<double> qe[4];
+      std::vector<int32_t> wave((size_t)(d.NL / 64), 0);
+      bool share = !(c->flags & DL_FRAMES_PER_ELEMENT);
+      int32_t ends = 0;
+      for (int64_t L = 0; L < d.NL && share; L++) {
+        if (L % 64 == 0) wave[L / 64] = ends;
+        const int32_t e = lane_e[L];
+        if (e < 0) continue;
+        if (eflag[e] & 2u) {
+          for (int k = 0; k < 4; k++) qe[k].push_back(qN[(4 + k) * E + e]);
+          ends++;
+          continue;
+        }
+        const int32_t e1 = (L + 1 < d.NL && (L + 1) % 64) ? lane_e[L + 1] : -1;
+        for (int k = 0; k < 4 && share; k++)
+          share = e1 >= 0 && std::memcmp(&qN[(4 + k) * E + e], &qN[k * E + e1], sizeof(double)) == 0;
+      }
+      d.qN_share = share ? 1 : 0;
+      d.nqe = share ? ends : 0;
+      d.qNe = nullptr;
+      d.qNe_wave = nullptr;
+      if (share) {
+        std::vector<double> flat;
+        for (int k = 0; k < 4; k++) flat.insert(flat.end(), qe[k].begin(), qe[k].end());
+        double *p;
+        TRY(upload(c, &p, flat)); d.qNe = p;
+        TRY(upload(c, &pi, wave)); d.qNe_wave = pi;
+      }
+    }
     d.NJ = NJ;
     d.nslot = nslot;
     TRY(upload(c, &pi, jnode)); d.jnode = pi;

@@ -48,6 +48,13 @@ struct DevData {
   const int32_t *sb;      // [E] slot of node b (chain end) if shared, else -1
   // elements ([c][E])
   const double *qN;     // 8: LambdaN at end a (4), end b (4)
+  // frames at end b in k_strut when qN_share: an inner element's end-b
+  // frame is the next lane's end-a frame (the strut's frame at their common
+  // node); only chain ends read theirs, from a compact array
+  int32_t qN_share;
+  int32_t nqe;             // chain ends
+  const double *qNe;       // [4][chain ends] end-b frame of each chain's last element
+  const int32_t *qNe_wave; // [NL / 64] chain ends in the wavefronts before
   double *eps0;         // 6
   // plastic
   const double *pw;     // [6][R] yield weights

@@ -163,9 +163,9 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
   uint32_t fl = 0;
   double diss = 0.0;
   if (e >= 0) {
+    fl = d.eflag[e];
     na = d.en0[e];
     nb = d.en1[e];
-    fl = d.eflag[e];
     const int32_t s = d.es[e];
     double C[6];
 #pragma unroll
@@ -180,9 +180,22 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     Quat qna, qnb;
     load_rot<SO3>(d, na, ra, qna);
     load_rot<SO3>(d, nb, rb, qnb);
+    // end-a frame; a lane that is not its chain's last takes its end-b frame
+    // from the next lane (same chain, so that lane is active here too)
     Quat qNa, qNb;
     qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
-    qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+    qNb.w = __shfl_down(qNa.w, 1, 64);
+    qNb.x = __shfl_down(qNa.x, 1, 64);
+    qNb.y = __shfl_down(qNa.y, 1, 64);
+    qNb.z = __shfl_down(qNa.z, 1, 64);
+    const uint64_t ends = __ballot(fl & 2u);
+    if (!d.qN_share) {
+      qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+    } else if (fl & 2u) {
+      const int lane = threadIdx.x & 63;
+      const int64_t ne = d.qNe_wave[L >> 6] + __popcll(ends & ((1ull << lane) - 1ull));
+      qNb.w = d.qNe[ne]; qNb.x = d.qNe[d.nqe + ne]; qNb.y = d.qNe[2 * d.nqe + ne]; qNb.z = d.qNe[3 * d.nqe + ne];
+    }
     double strain[6], sm[6];
     elem_eps0(Xa, Xb, qNa, qNb, strain);
     ElemKin k;

@@ -25,6 +25,7 @@ DL_ERR_STATE = -6
 DL_INIT, DL_GET_INT_VECTOR, DL_GET_MATRIX2, DL_COMMIT, DL_CANCEL = range(5)
 DL_LOAD_CASE_DEFAULT, DL_LOAD_CASE_OUTPUT = 0, 1
 DL_TRACK_DISSIPATION = 1
+DL_FRAMES_PER_ELEMENT = 2
 
 #: every symbol declared in include/dl_gpu.h
 EXPORTS = (

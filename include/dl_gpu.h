@@ -55,6 +55,10 @@ extern "C" {
 
 /* dl_options.flags */
 #define DL_TRACK_DISSIPATION 1u /* per-step plastic dissipation history */
+#define DL_FRAMES_PER_ELEMENT 2u /* the element kernel reads both end frames of
+                                  * every element instead of taking an inner
+                                  * element's end-b frame from its neighbour
+                                  * (same values; for testing) */
 
 typedef struct dl_ctx dl_ctx;
 

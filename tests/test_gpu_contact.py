@@ -74,13 +74,13 @@ def test_colliding_segments_leapfrog():
     assert rel_inf(d, r.disp.reshape(-1, 6)) <= 1e-8
 
 
-def _run_with_skin(p, skin, K, max_pairs=0):
+def _run_with_skin(p, skin, K, max_pairs=0, flags=0):
     import copy
 
     from dynlattice_amd.gpu import GpuModel
     q = copy.deepcopy(p)
     q.contact.skin = skin
-    g = GpuModel(q, flags=0, max_pairs=max_pairs)
+    g = GpuModel(q, flags=flags, max_pairs=max_pairs)
     if "top" in q.groups:
         g.set_reaction_group(q.groups["top"], 2)
     o = g.step(K)

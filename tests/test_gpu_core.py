@@ -183,6 +183,29 @@ def test_step_deterministic(gpu):
     assert np.array_equal(res[0], res[1])
 
 
+@pytest.mark.parametrize("m,so3", [(4, True), (3, True), (4, False)])
+def test_shared_end_frames_equal_per_element_frames(gpu, m, so3):
+    """k_strut takes an inner element's end-b frame from the next lane (the
+    strut's frame at their common node) and reads only chain ends' frames;
+    with DL_FRAMES_PER_ELEMENT it reads both frames of every element.  Same
+    frames, so the runs agree bit for bit: 4-element struts (identity lane
+    packing) and 3-element struts (padded wavefronts), LeapFrog with
+    plasticity and contact, and MilneDevice attempts."""
+    from dynlattice_amd.gpu import DL_FRAMES_PER_ELEMENT, GpuModel, milne_opts
+    p = small_bcc(m=m, plastic=True, contact=True, exponent=2.0)
+    p.so3 = so3
+    p.velo0[p.groups["top"], 2] = -20.0
+    res = []
+    for flags in (0, DL_FRAMES_PER_ELEMENT):
+        g = GpuModel(p, flags=flags)
+        g.step(150)
+        g.milne(40, milne_opts(p.dt, 1e-4))
+        res.append(g.get_state() + (g.plastic_active(),))
+        g.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
 def test_commit_energy_matches_oracle(gpu):
     p = small_bcc(plastic=True)
     r, g = _pair(p)
