@@ -206,6 +206,28 @@ def test_shared_end_frames_equal_per_element_frames(gpu, m, so3):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("so3", [True, False])
+def test_static_rollup_closed_form(gpu, so3):
+    """The reference's static test 1 (two full turns under an end moment
+    8 pi): at the closed-form static solution the device's internal force is
+    zero at the inner nodes and balances the end moment, as the oracle's."""
+    import math
+
+    from dynlattice_amd.gpu import GpuModel
+    from tests.test_oracle_element import rollup_problem, rollup_state
+    p = rollup_problem(so3)
+    d = rollup_state(p)
+    g = GpuModel(p)
+    g.set_state(d, np.zeros_like(d))
+    f = g.int_vector()
+    g.close()
+    M = 8 * math.pi
+    assert np.allclose(f[1:-1], 0.0, atol=1e-12 * M)
+    assert np.allclose(f[-1], [0, 0, 0, 0, 0, M], atol=1e-12 * M)
+    assert np.allclose(f[0], [0, 0, 0, 0, 0, -M], atol=1e-12 * M)
+    assert rel_inf(f, RefModel(p).int_vector(d)) <= 1e-11
+
+
 def test_commit_energy_matches_oracle(gpu):
     p = small_bcc(plastic=True)
     r, g = _pair(p)

@@ -136,6 +136,52 @@ def test_pure_shear_of_one_element():
     assert np.abs(f[0, :3] + n).max() < 1e-9 * np.abs(n).max()
 
 
+def rollup_problem(so3):
+    """The reference's static test 1 (tests/beam/test1.pro, Simo & Vu-Quoc
+    Ex. 7.1): a rod along y, L = 1, five elements, EA = GA = EI = GJ = 2
+    (young = shear_modulus = 2, area = area_moment = polar_moment = 1,
+    shear_correction = 1), clamped at y = 0, end moment 8 pi about z."""
+    p = L.straight_rod(5, 1.0, direction=(0.0, 1.0, 0.0))
+    one = np.ones(1)
+    p.sections = L.Sections(area=one, I1=one.copy(), I2=one.copy(), J=one.copy(),
+                            shear_factor=one.copy(), young=2 * one, shear_modulus=2 * one,
+                            density=one.copy())
+    p.so3 = so3
+    return p
+
+
+def rollup_state(p):
+    """The static solution of 2-node elements under a uniform moment M:
+    every element bends by the same angle D = M L0 / EI and carries no force,
+    i.e. node k rotated by k D about z, chords of length L0 along the mean
+    rotation of their ends.  M = 8 pi: D = 4 pi / 5, two full turns, the tip
+    back at the root (tests/beam/test1.py: displacement (0, -1, 0, 0, 0,
+    4 pi), response 8 pi)."""
+    n, L0, D = p.n_nodes, 0.2, 4 * math.pi / 5
+    x = np.zeros((n, 3))
+    for k in range(n - 1):
+        a = (k + 0.5) * D
+        x[k + 1] = x[k] + L0 * np.array([-math.sin(a), math.cos(a), 0.0])
+    d = np.zeros((n, 6))
+    d[:, :3] = x - p.coords
+    d[:, 5] = D * np.arange(n)
+    return d
+
+
+@pytest.mark.parametrize("so3", [True, False])
+def test_static_rollup_closed_form(so3):
+    p = rollup_problem(so3)
+    d = rollup_state(p)
+    assert np.allclose(d[-1], [0.0, -1.0, 0.0, 0.0, 0.0, 4 * math.pi], atol=1e-12)
+    f = RefModel(p).int_vector(d)
+    M = 8 * math.pi
+    # the internal force balances the end moment: zero at the inner nodes,
+    # M at the tip, the reaction -M at the clamp
+    assert np.allclose(f[1:-1], 0.0, atol=1e-12 * M)
+    assert np.allclose(f[-1], [0, 0, 0, 0, 0, M], atol=1e-12 * M)
+    assert np.allclose(f[0], [0, 0, 0, 0, 0, -M], atol=1e-12 * M)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_force_and_moment_balance(seed):
     p = small_bcc()
