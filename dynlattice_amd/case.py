@@ -5,7 +5,9 @@ output the reference's validators read.
 (:mod:`.geo`) and builds the :class:`~.lattice.Problem` the models of the
 case describe:
 
-* ``Input``: GMSHInput (file, order 1, onelab constants) and GroupInput node
+* ``Input``: GMSHInput (file, order 1 or 2 -- a second-order mesh becomes
+  2-node elements on GMSH's second-order node set, see :mod:`.geo` --,
+  onelab constants) and GroupInput node
   groups (``xtype``/``ytype``/``ztype`` min | max | mid, ``xval``, ``nodes``,
   ``all``; GroupInputModule.cpp:167-330), plus the built-in group ``all``;
 * ``model.model``: the Lattice rod mesh with its ElasticRod material
@@ -16,13 +18,23 @@ case describe:
   (initial displacements and velocities), Mass (point masses), ImpactBC
   (group-reaction-driven accelerations) and None;
   time dependence becomes device programs (:class:`~.lattice.Program`);
+  a later Dirichlet model's constraint on a dof replaces an earlier one
+  (JIVE's Constraints keep one constraint per dof, the last added: a
+  library behaviour outside the reference's sources);
 * ``Solver.integrator``: MilneDevice or LeapFrog with deltaTime, precision,
   min/maxDtime, stepSaftey, increase/decreaseFactor, lengthScale; the
   SO(3) update when ``dofs_SO3`` is given (ExplicitModule.cpp:94-106);
 * ``control.runWhile`` ("t <= T", "t < T", "i < N");
 * ``Output``: CSVOutput modules of the state vectors with saveWhen
   (CSVDataPrinter.cpp: header "time,label,<dof>[<node>]...", one row per
-  sample).
+  sample); Sample modules (JIVE's SampleModule: one row of dataSets
+  expressions per sample, joined by the separator, after the optional
+  header; sampleWhen, default every step) over the variables t, i,
+  deltaTime, runtime, the energies kineticEnergy, potentialEnergy,
+  dissipatedEnergy and mass, and GroupOutput's group variables
+  <group>.disp.<dof> and <group>.velo.<dof> (group averages) and
+  <group>.resp.<dof> (group sums of the internal force, output load case;
+  GroupOutputModule.cpp:119-160).
 
 :func:`run` integrates the case on the GPU (GpuModel) and writes the CSV
 files.  The MilneDevice precision default of the reference is JIVE's
@@ -55,8 +67,11 @@ class CaseError(ValueError):
 class Output:
     name: str
     file: str
-    label: str          # vector label in the CSV ("disp" or "velo")
+    label: str          # vector label in the CSV ("disp" or "velo"); "" for a Sample module
     save_when: str      # expression of t, i, deltaTime
+    data_sets: list = field(default_factory=list)  # Sample: expressions, one column each
+    separator: str = " "
+    header: str | None = None
 
 
 @dataclass
@@ -192,8 +207,8 @@ def _apply_models(props, p, dofidx, mdl_root="model.model"):
                 q = len(p.programs) - 1
         own = pf if force else pa
         cur = own[nodes, dof]
-        if np.any((cur != -2) & (cur != q)):
-            raise CaseError("a dof takes values from two different time dependences")
+        if force and np.any((cur != -2) & (cur != q)):
+            raise CaseError("a dof takes forces from two different time dependences")
         own[nodes, dof] = q
         if force:
             np.add.at(p.fext[:, dof], nodes, factor)
@@ -324,6 +339,13 @@ def load(pro_path: str, case_name: str | None = None) -> Case:
     out = P.get(props, "Output", {}) or {}
     for m in out.get("modules", []) or []:
         spec = out.get(m, {}) or {}
+        if spec.get("type") == "Sample":
+            ds = spec.get("dataSets", [])
+            ds = ds if isinstance(ds, list) else [ds]
+            outs.append(Output(m, spec["file"], "", spec.get("sampleWhen", "1"),
+                               data_sets=[str(x) for x in ds],
+                               separator=spec.get("separator", " "), header=spec.get("header")))
+            continue
         if spec.get("type") != "CSVOutput":
             continue
         vecs = spec.get("vectors", ["state = disp"])
@@ -358,32 +380,102 @@ def write_csv(path: str, dof_names, n_nodes: int, rows):
             f.write(repr(float(t)) + "," + label + "," + ",".join(repr(float(x)) for x in vec) + "\n")
 
 
+_NAME = re.compile(r"[A-Za-z_][A-Za-z0-9_]*(?:\.[A-Za-z_][A-Za-z0-9_]*)*")
+
+
+def _sample_value(expr: str, t: float, i: int, var) -> float:
+    """value of a Sample data set: names other than t, i, PI and functions
+    are looked up with var(name) and substituted, then the expression is
+    evaluated by the program compiler"""
+    from .gpu import program_eval
+
+    def rep(m):
+        v = var(m.group(0))
+        return m.group(0) if v is None else f"({float(v)!r})"
+    return program_eval(_NAME.sub(rep, expr), t, float(i))
+
+
+def write_sample(path: str, header, separator: str, rows):
+    """SampleModule layout: the header line if any, then one row per sample"""
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wt") as f:
+        if header:
+            f.write(header + "\n")
+        for r in rows:
+            f.write(separator.join(repr(float(x)) for x in r) + "\n")
+
+
 def run(case: Case, out_dir: str = ".", precision: float | None = None, chunk: int = 1,
-        device: int = 0):
+        device: int = 0, progress=None, progress_every: float = 10.0):
     """Integrates the case on the GPU while runWhile holds; samples every
-    CSVOutput whenever its saveWhen holds after a taken step (the
+    output whenever its saveWhen / sampleWhen holds after a taken step (the
     reference's output modules run after each accepted step) and writes the
-    files under out_dir.  Returns {output name: path}, the final time and
-    the number of steps taken."""
+    files under out_dir.  progress(t, steps), if given, is called about
+    every progress_every seconds of wall time.  Returns {output name: path},
+    the final time and the number of steps taken."""
+    import time as _time
+
     from .gpu import GpuModel, milne_opts
     p = case.problem
     g = GpuModel(p, device=device)
     rows = {o.name: [] for o in case.outputs}
+    dof_of = {n: k for k, n in enumerate(case.dof_names)}
+    t_start = _time.perf_counter()
 
     def sample(t, i, dt_next):
-        d = v = None
+        cache = {}
+
+        def state():
+            if "state" not in cache:
+                cache["state"] = g.get_state()
+            return cache["state"]
+
+        def var(name):
+            if name in ("t", "i", "PI"):
+                return None
+            if name == "deltaTime":
+                return dt_next
+            if name == "runtime":
+                return _time.perf_counter() - t_start
+            if name in ("kineticEnergy", "potentialEnergy", "dissipatedEnergy", "mass"):
+                if "en" not in cache:
+                    cache["en"] = g.energies()
+                key = {"kineticEnergy": "kinetic", "potentialEnergy": "potential",
+                       "dissipatedEnergy": "dissipated", "mass": "mass"}[name]
+                return cache["en"][key]
+            parts = name.split(".")
+            if len(parts) == 3 and parts[0] in p.groups and parts[2] in dof_of:
+                nodes, k = p.groups[parts[0]], dof_of[parts[2]]
+                if parts[1] in ("disp", "velo"):
+                    d, v = state()
+                    a = d if parts[1] == "disp" else v
+                    return float(a[nodes, k].sum() / len(nodes))
+                if parts[1] == "resp":
+                    if "resp" not in cache:
+                        cache["resp"] = g.int_vector(inelastic=False)
+                    return float(cache["resp"][nodes, k].sum())
+            return None
+
         for o in case.outputs:
-            if _cond(o.save_when, t, i, dt_next):
-                if d is None:
-                    d, v = g.get_state()
+            if not _cond(o.save_when, t, i, dt_next):
+                continue
+            if o.label:
+                d, v = state()
                 rows[o.name].append((t, o.label, (d if o.label == "disp" else v).reshape(-1)))
+            else:
+                rows[o.name].append([_sample_value(e, t, i, var) for e in o.data_sets])
 
     t, i = 0.0, 0
     m = case.milne
     prec = m["precision"] if precision is None else precision
     dt = case.dt
+    t_report = _time.perf_counter()
     try:
         while _cond(case.run_while, t, i, dt):
+            if progress is not None and _time.perf_counter() - t_report > progress_every:
+                progress(t, i)
+                t_report = _time.perf_counter()
             if case.integrator == "LeapFrog":
                 g.step(1, dt, out=False)
                 t += dt
@@ -407,6 +499,9 @@ def run(case: Case, out_dir: str = ".", precision: float | None = None, chunk: i
     paths = {}
     for o in case.outputs:
         path = os.path.join(out_dir, P._subst(o.file, {"CASE_NAME": case.name}))
-        write_csv(path, case.dof_names, p.n_nodes, rows[o.name])
+        if o.label:
+            write_csv(path, case.dof_names, p.n_nodes, rows[o.name])
+        else:
+            write_sample(path, o.header, o.separator, rows[o.name])
         paths[o.name] = path
     return paths, t, i

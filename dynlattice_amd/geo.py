@@ -8,7 +8,7 @@ name through Input.input.onelab), assignments ``a = <expr>;``,
 ``Point(i) = {x, y, z, lc};``, ``Line(i) = {p, q};``, // and /* */ comments;
 expressions in JEM/GMSH arithmetic syntax.
 
-Meshing (first order only): every line is split into n equal elements with
+Meshing: every line is split into n equal elements with
 n = floor(L / lc + 0.99) (at least 1), lc the points' characteristic length
 (equal at both ends), GMSH's rule for uniform sizes (GMSH 4.9
 meshGEdge.cpp; a third-party library not in the reference's sources:
@@ -18,6 +18,13 @@ GMSHInputModule stores it: the points by tag (0-based JIVE index =
 position), then every line's interior nodes in line-tag order, from its
 first point to its second.  Each line is one strut (element group
 beam_<k>, k = 1, 2, ... in line order).
+
+Second order (Input.input.order = 2) gives GMSH's 3-node line elements:
+the n elements' end and midside nodes, 2n + 1 equally spaced nodes per
+line.  The element here is the 2-node one, so a second-order mesh becomes
+2n two-node elements on that same node set.  The interior nodes are
+numbered along the line (GMSH lists the midside nodes after the end
+nodes; no reference test reads an interior node of a second-order mesh).
 """
 from __future__ import annotations
 
@@ -82,8 +89,8 @@ def read(path: str, onelab: dict | None = None):
 
 def mesh(points: dict, lines: list, order: int = 1):
     """coords [N, 3], elem_nodes [E, 2], elem_strut [E], strut_names"""
-    if order != 1:
-        raise GeoError("only first-order (2-node) line elements are supported")
+    if order not in (1, 2):
+        raise GeoError("line elements of order 1 or 2 only")
     tags = sorted(points)
     index = {t: i for i, t in enumerate(tags)}
     coords = [points[t][:3] for t in tags]
@@ -95,7 +102,7 @@ def mesh(points: dict, lines: list, order: int = 1):
         lp, lq = points[p][3], points[q][3]
         if lp <= 0 or abs(lp - lq) > 1e-12 * max(lp, lq):
             raise GeoError(f"Line({tag}): equal positive characteristic lengths required")
-        n = max(1, int(np.linalg.norm(b - a) / lp + 0.99))
+        n = max(1, int(np.linalg.norm(b - a) / lp + 0.99)) * order
         chain = [index[p]]
         for j in range(1, n):
             coords.append(tuple(a + (b - a) * (j / n)))

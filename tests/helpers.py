@@ -61,3 +61,60 @@ def random_segments(n_struts=60, m=2, box=0.05, length=0.02, radius=1.5e-3, seed
     p.disp0[:, :3] = (spread - 1.0) * (strut_mid - c)
     p.dt = L.stable_dt(p, 0.5)
     return p
+
+
+# Comparison metrics of the reference's transient validators, restated from
+# tests/metrics.py:8-26 (interp_on_reference, relative_L2).
+def interp_on_reference(x_sim, y_sim, x_ref):
+    """linear interpolation of (x_sim, y_sim), sorted by x, at x_ref"""
+    order = np.argsort(x_sim)
+    return np.interp(x_ref, x_sim[order], y_sim[order])
+
+
+def relative_L2(y_sim, y_ref):
+    """||y_sim - y_ref|| / ||y_ref|| (the largest |y_ref| when the norm is
+    essentially zero)"""
+    denom = np.linalg.norm(y_ref)
+    if denom < 1e-12:
+        denom = max(np.max(np.abs(y_ref)), 1e-12)
+    return float(np.linalg.norm(y_sim - y_ref) / denom)
+
+
+def transient_test1_errors(sim, golden):
+    """tests/transient/test1.py's comparison with Simo & Vu-Quoc 1988 (Ex.
+    5.1, the spin-up of a flexible beam): sim rows are (free node dx .. rz,
+    hub rz, t); returns the relative L2 errors of the hub angle psi, the
+    rotational deviation psi - rz(tip) in degrees, and the tip's axial and
+    tangential deviations from the rigid shadow beam, each at the
+    reference's abscissae."""
+    import pandas as pd
+
+    def ref(name, csv=False):
+        path = f"{golden}/transient_test1_ref_{name}.csv"
+        if csv:
+            return np.loadtxt(path, delimiter=",")
+        return pd.read_csv(path, delimiter=";", decimal=",").values
+    bc, ang, u1, u2 = ref("BC", True), ref("angle"), ref("u1"), ref("u2")
+    t = sim[:, -1]
+    psi = np.unwrap(sim[:, -2])
+    px, py = 10 + sim[:, 0], sim[:, 1]
+    rz = np.unwrap(sim[:, 5])
+    delta = np.stack([px - 10 * np.cos(psi), py - 10 * np.sin(psi)], -1)
+    out = np.stack([np.cos(psi), np.sin(psi)], -1)
+    tan = np.stack([-np.sin(psi), np.cos(psi)], -1)
+    dev_rz = np.rad2deg(psi - rz)
+    dev_u1 = (delta * out).sum(axis=1)
+    dev_u2 = (delta * tan).sum(axis=1)
+    return {"psi": relative_L2(interp_on_reference(t, psi, bc[:, 0]), bc[:, 1]),
+            "rz": relative_L2(interp_on_reference(t, dev_rz, ang[:, 0]), ang[:, 1]),
+            "u1": relative_L2(interp_on_reference(t, dev_u1, u1[:, 0]), u1[:, 1]),
+            "u2": relative_L2(interp_on_reference(t, dev_u2, u2[:, 0]), u2[:, 1])}
+
+
+def energy_rise(time, load, e_tot):
+    """tests/transient/test3.py's energy criterion: the largest relative rise
+    of the total energy between samples while the load is zero (-inf when
+    it never rises)"""
+    d = np.diff(e_tot) / e_tot[1:]
+    rise = (d > 0) & (load[1:] == 0)
+    return float(d[rise].max()) if rise.any() else float("-inf")
