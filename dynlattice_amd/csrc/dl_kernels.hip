@@ -364,29 +364,13 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 // ------------------------------------------------------------------
 // joint kernel: shared nodes
 // ------------------------------------------------------------------
-template <int MODE, bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m) {
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (MODE == MODE_MCORR) {
-    // every thread joins the block's error partial sums
-    double em[2] = {0.0, 0.0};
-    if (j < d.NJ) {
-      const int32_t n = d.jnode[j];
-      double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int32_t t = d.jptr[j]; t < d.jptr[j + 1]; t++)
-#pragma unroll
-        for (int i = 0; i < 6; i++) f[i] += d.sbuf[i * (int64_t)d.nslot + t];
-      milne_corr_node<SO3>(d, m, n, f, em);
-    }
-    milne_block_partials(m, em, m.jofs + blockIdx.x);
-    return;
-  }
-  if (j >= d.NJ) return;
-  const int32_t n = d.jnode[j];
+// internal force of shared node j: its chains' slots summed in slot order,
+// four slots per iteration with their loads in flight together (the
+// additions keep the order, so the sum is the same bits as one by one)
+__device__ __forceinline__ void joint_force(const DevData &d, int32_t j, double f[6]) {
   const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
-  double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  // four slots per iteration with their loads in flight together; the
-  // additions keep the slot order, so the sum is the same bits as one by one
+#pragma unroll
+  for (int i = 0; i < 6; i++) f[i] = 0.0;
   for (int32_t t = t0; t < t1; t += 4) {
     double g[4][6];
 #pragma unroll
@@ -401,6 +385,26 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, Mi
       for (int i = 0; i < 6; i++)
         if (t + q < t1) f[i] += g[q][i];
   }
+}
+
+template <int MODE, bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE == MODE_MCORR) {
+    // every thread joins the block's error partial sums
+    double em[2] = {0.0, 0.0};
+    if (j < d.NJ) {
+      double f[6];
+      joint_force(d, j, f);
+      milne_corr_node<SO3>(d, m, d.jnode[j], f, em);
+    }
+    milne_block_partials(m, em, m.jofs + blockIdx.x);
+    return;
+  }
+  if (j >= d.NJ) return;
+  const int32_t n = d.jnode[j];
+  double f[6];
+  joint_force(d, j, f);
   if (MODE == MODE_FORCE) {
 #pragma unroll
     for (int i = 0; i < 6; i++) d.fout[i * d.N + n] = f[i];
