@@ -1,6 +1,6 @@
 """GPU parity of the element force, mass, gyro and LeapFrog step against the oracle.
 
-Tolerances (BASELINE north_star): nodal internal forces within 1e-11
+Tolerances: nodal internal forces within 1e-11
 relative (infinity norm over the vector) per evaluation; nodal displacements
 within 1e-8 relative after the stated horizon; plastic active sets
 bit-exact except points within 1e-12 of the yield surface.
