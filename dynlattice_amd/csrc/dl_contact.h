@@ -20,6 +20,7 @@ struct ContactDev {
   const double *crad;   // [R] contact radius
   const int32_t *nsptr; // [N+1] node -> struts CSR
   const int32_t *nslist;
+  const uint8_t *ejoint; // [E] bit 0: node a belongs to more than one strut, bit 1: node b
   double kS, kN;
   // grid
   double org[3];

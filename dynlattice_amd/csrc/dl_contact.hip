@@ -188,10 +188,11 @@ __global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const in
   GRID_STRIDE(i, n) out[i] += bsum[i / 1024];
 }
 
-// cell-ordered element record: AABB, element, strut, cell (one 64-byte line)
+// cell-ordered element record: AABB, element, strut, cell, joint bits of
+// its nodes (ContactDev::ejoint; one 64-byte line)
 struct __align__(64) CRec {
   double lo[3], hi[3];
-  int32_t e, s, cell, pad;
+  int32_t e, s, cell, jn;
 };
 
 // element ids into their cells (order within a cell set by the atomics)
@@ -229,7 +230,7 @@ __device__ __forceinline__ void place_one(const ContactDev &c, int64_t e) {
   r.e = (int32_t)e;
   r.s = c.es[e];
   r.cell = cell;
-  r.pad = 0;
+  r.jn = c.ejoint[e];
   ((CRec *)c.rec)[pos] = r;
   // compact prefilter record: bounds rounded outwards to float, strut id
   float4 lo, hi;
@@ -329,8 +330,10 @@ __device__ __forceinline__ void scan_candidates(const ContactDev &c, int64_t k, 
           if (!ov) continue;
           const int32_t Y = yr.e, sY = yr.s;
           const int32_t ya = c.en0[Y], yb = c.en1[Y];
-          if (node_in_strut(c, xa, sY) || node_in_strut(c, xb, sY) || node_in_strut(c, ya, sX) ||
-              node_in_strut(c, yb, sX))
+          // a node of one strut only is in no other strut (and Y's strut is
+          // not X's): the node-set walks are needed at joint nodes alone
+          if (((x.jn & 1) && node_in_strut(c, xa, sY)) || ((x.jn & 2) && node_in_strut(c, xb, sY)) ||
+              ((yr.jn & 1) && node_in_strut(c, ya, sX)) || ((yr.jn & 2) && node_in_strut(c, yb, sX)))
             continue;
           fn(Y, sY, ya, yb, sX < sY);
         }
@@ -748,6 +751,15 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = up(cs, &pd, crad))) return rc; c.crad = pd;
   if ((rc = up(cs, &pi, nsptr))) return rc; c.nsptr = pi;
   if ((rc = up(cs, &pi, nslist))) return rc; c.nslist = pi;
+  {
+    std::vector<uint8_t> ej(E);
+    for (int64_t e = 0; e < E; e++)
+      ej[e] = (uint8_t)((nsptr[e0[e] + 1] - nsptr[e0[e]] > 1 ? 1 : 0) |
+                        (nsptr[e1[e] + 1] - nsptr[e1[e]] > 1 ? 2 : 0));
+    uint8_t *pu;
+    if ((rc = up(cs, &pu, ej))) return rc;
+    c.ejoint = pu;
+  }
   cs.nblk_e = (int32_t)nb(E);
   cs.nscan_blk = (std::max<int64_t>(ncell, E) + 1023) / 1024 + 1;
   if ((rc = calloc_dev(cs, &c.extbits, 1))) return rc;
