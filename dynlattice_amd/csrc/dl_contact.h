@@ -32,7 +32,10 @@ struct ContactDev {
   int32_t *cell;        // [E]
   int32_t *rank;        // [E]
   int32_t *ccount;      // [ncell]
-  int32_t *cstart;      // [ncell+1]
+  int32_t *cstart;      // [ncell+1] chunk-local exclusive scan of the counts (1024 cells a
+                        // chunk), the total at [ncell]; cell k starts at
+                        // cstart[k] + cbsum[k / 1024] (cell_start)
+  int32_t *cbsum;       // [ncell / 1024 + 1] exclusive scan of the chunk totals
   int32_t *bsum;        // scan block sums
   void *rec;            // [E] cell-ordered 64-byte records (box, element, strut, cell)
   int32_t *tmp;         // [E] element ids in cell order, unsorted within a cell

@@ -6,7 +6,8 @@
 //   kc_cells     cell size h = max(h0, 1.001 * largest extent); integer cell of
 //                each AABB's min corner (clamped to the grid), per-cell counts
 //                with integer atomics
-//   scan         exclusive scan of the counts (3 launches)
+//   scan         exclusive scan of the counts: chunk-local scans and the
+//                chunk offsets (2 launches); a cell's start is their sum
 //   kc_scatter   element ids into their cells; kc_place writes each element's
 //                record at its stable (id-ordered) position in the cell
 //   kc_narrow    per element X: every Y in the 27 neighbour cells that the
@@ -188,6 +189,11 @@ __global__ __launch_bounds__(CB) void kc_scan3(int32_t *out, int64_t n, const in
   GRID_STRIDE(i, n) out[i] += bsum[i / 1024];
 }
 
+// first record of cell k in cell order (k = ncell: the number of records)
+__device__ __forceinline__ int32_t cell_start(const ContactDev &c, int64_t k) {
+  return k == c.ncell ? c.cstart[k] : c.cstart[k] + c.cbsum[k >> 10];
+}
+
 // cell-ordered element record: AABB, element, strut, cell, joint bits of
 // its nodes (ContactDev::ejoint; one 64-byte line)
 struct __align__(64) CRec {
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *ga
   }
   GRID_STRIDE(e, c.E) {
     const int32_t cell = c.cell[e];
-    c.tmp[c.cstart[cell] + c.rank[e]] = (int32_t)e;
+    c.tmp[cell_start(c, cell) + c.rank[e]] = (int32_t)e;
     c.ccount[cell] = 0;
   }
 }
@@ -219,7 +225,7 @@ __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *ga
 __device__ __forceinline__ void place_one(const ContactDev &c, int64_t e) {
   const int64_t E = c.E;
   const int32_t cell = c.cell[e];
-  const int32_t b = c.cstart[cell], n = c.cstart[cell + 1] - b;
+  const int32_t b = cell_start(c, cell), n = cell_start(c, cell + 1) - b;
   int32_t pos = b;
   for (int32_t j = 0; j < n; j++) pos += (c.tmp[b + j] < (int32_t)e) ? 1 : 0;
   CRec r;
@@ -301,9 +307,9 @@ __device__ __forceinline__ void scan_candidates(const ContactDev &c, int64_t k, 
       const int32_t x0 = cx > 0 ? cx - 1 : 0;
       const int32_t x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
       const int32_t row = (z * c.dim[1] + y) * c.dim[0];
-      const int32_t j1 = c.cstart[row + x1 + 1];
+      const int32_t j1 = cell_start(c, row + x1 + 1);
       const float4 *fr = (const float4 *)c.frec;
-      for (int32_t j0 = c.cstart[row + x0]; j0 < j1; j0 += 4) {
+      for (int32_t j0 = cell_start(c, row + x0); j0 < j1; j0 += 4) {
         // four independent prefilter loads in flight
         float4 flo[4], fhi[4];
 #pragma unroll
@@ -770,6 +776,7 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = calloc_dev(cs, &c.ccount, ncell))) return rc;
   CHK(hipMemset(c.ccount, 0, sizeof(int32_t) * ncell)); // kc_cells expects zero counts
   if ((rc = calloc_dev(cs, &c.cstart, ncell + 1))) return rc;
+  if ((rc = calloc_dev(cs, &c.cbsum, (ncell + 1023) / 1024 + 1))) return rc;
   if ((rc = calloc_dev(cs, &c.bsum, cs.nscan_blk + 1))) return rc;
   if ((rc = calloc_dev(cs, (double **)&c.rec, 8 * (size_t)E))) return rc;
   if ((rc = calloc_dev(cs, &c.tmp, E))) return rc;
@@ -833,7 +840,11 @@ static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double
   const unsigned ge = gs(c.E);
   hipLaunchKernelGGL(kc_boxes, dim3(ge), dim3(CB), 0, s, c, d, infl, gate);
   hipLaunchKernelGGL(kc_cells, dim3(ge), dim3(CB), 0, s, c, gate);
-  scan_i32(cs, c.ccount, c.ncell, c.cstart, s, gate);
+  // chunk-local scan and the chunk offsets; readers add the two (cell_start)
+  const int64_t nchunk = (c.ncell + 1023) / 1024;
+  hipLaunchKernelGGL(kc_scan1, dim3((unsigned)std::min<int64_t>(nchunk, GS_CAP)), dim3(CB), 0, s,
+                     (const int32_t *)c.ccount, c.ncell, c.cstart, c.cbsum, gate);
+  hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.cbsum, nchunk, c.cstart + c.ncell, gate);
   hipLaunchKernelGGL(kc_scatter, dim3(ge), dim3(CB), 0, s, c, gate);
   hipLaunchKernelGGL(kc_place, dim3(ge), dim3(CB), 0, s, c, gate);
 }
