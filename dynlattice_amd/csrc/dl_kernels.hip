@@ -46,12 +46,37 @@ __device__ __forceinline__ void load_rot(const DevData &d, int32_t n, double r[4
 // impact kernel updates them once the group's reaction is known
 // (impact_pass, with free_mask clearing the impact dof before the first
 // output).
+// The node's own data, loaded ahead of its force where registers allow
+// (the joint kernel: its loads overlap the slot loads).
+struct NodeData {
+  double v[6], u[3], rr[4], J0[6], mt, gm;
+  int32_t b;
+};
+
+template <bool SO3>
+__device__ __forceinline__ void load_node_data(const DevData &d, int32_t n, NodeData &nd) {
+  const int64_t N = d.N;
+  nd.b = d.bc[n];
+#pragma unroll
+  for (int c = 0; c < 6; c++) nd.v[c] = d.v[c * N + n];
+#pragma unroll
+  for (int c = 0; c < 3; c++) nd.u[c] = d.u[c * N + n];
+#pragma unroll
+  for (int c = 0; c < (SO3 ? 4 : 3); c++) nd.rr[c] = d.rot[c * N + n];
+  if (!SO3) nd.rr[3] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; c++) nd.J0[c] = d.J0[c * N + n];
+  nd.mt = d.mt[n];
+  nd.gm = SO3 ? d.gm[n] : 0.0;
+}
+
 template <bool SO3>
 __device__ __forceinline__ void node_update(const DevData &d, const StepParams &sp, int32_t n,
                                             double f[6], unsigned free_mask = 0u,
-                                            bool impact_pass = false) {
+                                            bool impact_pass = false,
+                                            const NodeData *pre = nullptr) {
   const int64_t N = d.N;
-  const int32_t b = d.bc[n];
+  const int32_t b = pre ? pre->b : d.bc[n];
   if (!impact_pass && b >= 0 && d.bc_imp && d.bc_imp[b] >= 0) {
 #pragma unroll
     for (int c = 0; c < 6; c++) d.fout[c * N + n] = f[c];
@@ -73,14 +98,28 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
   }
   double v[6], u[3], rr[4];
   Quat q;
-  // compiler barrier: reload the rotation instead of reusing the element
-  // part's copy (which would stay live in registers)
-  asm volatile("" ::: "memory");
-  load_rot<SO3>(d, n, rr, q);
+  if (pre) {
 #pragma unroll
-  for (int c = 0; c < 6; c++) v[c] = d.v[c * N + n];
+    for (int c = 0; c < 4; c++) rr[c] = pre->rr[c];
+    if (SO3) {
+      q.w = rr[0]; q.x = rr[1]; q.y = rr[2]; q.z = rr[3];
+    } else {
+      q = qexp(rr[0], rr[1], rr[2]);
+    }
 #pragma unroll
-  for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
+    for (int c = 0; c < 6; c++) v[c] = pre->v[c];
+#pragma unroll
+    for (int c = 0; c < 3; c++) u[c] = pre->u[c];
+  } else {
+    // compiler barrier: reload the rotation instead of reusing the element
+    // part's copy (which would stay live in registers)
+    asm volatile("" ::: "memory");
+    load_rot<SO3>(d, n, rr, q);
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = d.v[c * N + n];
+#pragma unroll
+    for (int c = 0; c < 3; c++) u[c] = d.u[c * N + n];
+  }
   if (d.tape) {
     // adjoint tape row: u(3) rot(4) v(6) f(6), component-major
     double *t = d.tape;
@@ -93,12 +132,13 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
 #pragma unroll
     for (int c = 0; c < 6; c++) t[(13 + c) * N + n] = f[c];
   }
-  const double mt = d.mt[n];
+  const double mt = pre ? pre->mt : d.mt[n];
   double J0[6];
 #pragma unroll
-  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  for (int c = 0; c < 6; c++) J0[c] = pre ? pre->J0[c] : d.J0[c * N + n];
   const double dt = d.ndt ? d.ndt[n] : sp.dt;
-  node_advance<SO3>(f, q, u, v, rr, mt, J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, dt);
+  node_advance<SO3>(f, q, u, v, rr, mt, J0, pre ? pre->gm : (SO3 ? d.gm[n] : 0.0), mask, ac, fe,
+                    dt);
 #pragma unroll
   for (int c = 0; c < 6; c++) d.v[c * N + n] = v[c];
 #pragma unroll
@@ -403,6 +443,8 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, Mi
   }
   if (j >= d.NJ) return;
   const int32_t n = d.jnode[j];
+  NodeData nd;
+  if (MODE == MODE_STEP) load_node_data<SO3>(d, n, nd);
   double f[6];
   joint_force(d, j, f);
   if (MODE == MODE_FORCE) {
@@ -414,7 +456,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, Mi
     milne_pred_node<SO3>(d, m, n, f);
     return;
   }
-  node_update<SO3>(d, sp, n, f);
+  node_update<SO3>(d, sp, n, f, 0u, false, &nd);
 }
 
 // ------------------------------------------------------------------
