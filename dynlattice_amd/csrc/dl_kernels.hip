@@ -405,22 +405,24 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 // joint kernel: shared nodes
 // ------------------------------------------------------------------
 // internal force of shared node j: its chains' slots summed in slot order,
-// four slots per iteration with their loads in flight together (the
-// additions keep the order, so the sum is the same bits as one by one)
+// eight slots per iteration (a BCC joint's eight struts in one round) with
+// their loads in flight together; the additions keep the order, so the sum
+// is the same bits as one by one.  The joint kernel has few wavefronts, so
+// the registers this takes cost no occupancy.
 __device__ __forceinline__ void joint_force(const DevData &d, int32_t j, double f[6]) {
   const int32_t t0 = d.jptr[j], t1 = d.jptr[j + 1];
 #pragma unroll
   for (int i = 0; i < 6; i++) f[i] = 0.0;
-  for (int32_t t = t0; t < t1; t += 4) {
-    double g[4][6];
+  for (int32_t t = t0; t < t1; t += 8) {
+    double g[8][6];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < 8; q++) {
       const int64_t tq = (t + q < t1) ? t + q : t;
 #pragma unroll
       for (int i = 0; i < 6; i++) g[q][i] = (t + q < t1) ? d.sbuf[i * (int64_t)d.nslot + tq] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < 8; q++)
 #pragma unroll
       for (int i = 0; i < 6; i++)
         if (t + q < t1) f[i] += g[q][i];
