@@ -526,80 +526,134 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
   }
 }
 
-// contact forces of the step from the lists, one thread per element (id
-// order): exact boxes of X and of each listed Y, then the pair as in
-// kc_narrow; per-element force rows (cidx[X] = X) and per-step counts
+// contact forces of the step from the lists: exact boxes of X and of each
+// listed Y, then the pair as in kc_narrow; the element's force row
+// (cidx[X] = X, or -1 without a closed pair) and the per-step counts
+__device__ __forceinline__ int32_t vnarrow_elem(const ContactDev &c, const DevData &d, int64_t X,
+                                                int32_t t1, unsigned long long &nact,
+                                                unsigned long long &ncand) {
+  const int64_t N = c.N, E = c.E;
+  const int32_t sX = c.es[X];
+  const double rX = c.crad[sX];
+  double pX[2][3], xlo[3], xhi[3];
+  elem_pos(d, c.en0[X], c.en1[X], pX);
+  elem_box(d, N, c.en0[X], c.en1[X], rX, xlo, xhi);
+  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
+  bool any = false;
+  const int32_t *lst = c.vlist + X * c.vk;
+  for (int32_t t = 0; t < t1; t++) {
+    const int32_t Y = lst[t];
+    const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
+    const double rY = c.crad[sY];
+    double ylo[3], yhi[3];
+    elem_box(d, N, ya, yb, rY, ylo, yhi);
+    bool ov = true;
+    for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
+    if (!ov) continue;
+    const bool XisA = sX < sY;
+    if (XisA) ncand++;
+    double pY[2][3];
+    elem_pos(d, ya, yb, pY);
+    const double R = rX + rY;
+    double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+    const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                             : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+    if (closed) {
+      any = true;
+      for (int i = 0; i < 3; i++) {
+        fa[i] += g0[i];
+        fb[i] += g1[i];
+      }
+      if (XisA) nact++;
+    }
+  }
+  if (!any) return -1;
+  for (int i = 0; i < 3; i++) {
+    c.fcs[i * E + X] = fa[i];
+    c.fcs[(3 + i) * E + X] = fb[i];
+  }
+  return (int32_t)X;
+}
+
+// Sixteen elements per thread: their list lengths arrive in four vector
+// loads and their cidx entries are cleared in four stores, and the grid is
+// small enough to be resident at once.  The elements that have a list go to
+// a queue of their wavefront, which its 64 lanes then share, so the pair
+// work is spread evenly whether few lists are filled (a lattice before it
+// buckles: about one memory round trip per step) or many.  The counts are
+// integers, summed per wavefront (any order gives the same sum) with one
+// atomic per wavefront.
+#define VN_PER 16
 __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
                                                  unsigned long long *cnt_act,
                                                  unsigned long long *cnt_cand,
                                                  const int32_t *step) {
-  const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
+  __shared__ int32_t qe[CB / 64][64 * VN_PER], qt[CB / 64][64 * VN_PER];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t T = (int64_t)blockIdx.x * CB + threadIdx.x;
   // the lists are current (list_refresh ran before): clear the rebuild flag;
   // the step's node updates may set it again
-  if (X == 0) *c.gate = 0;
-  unsigned long long nact = 0, ncand = 0;
-  if (X < c.E) {
-    const int64_t N = c.N, E = c.E;
-    const int32_t sX = c.es[X];
-    const double rX = c.crad[sX];
-    double pX[2][3], xlo[3], xhi[3];
-    elem_pos(d, c.en0[X], c.en1[X], pX);
-    elem_box(d, N, c.en0[X], c.en1[X], rX, xlo, xhi);
-    double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
-    bool any = false;
-    const int32_t *lst = c.vlist + X * c.vk;
-    const int32_t t1 = c.vcount[X];
-    for (int32_t t = 0; t < t1; t++) {
-      const int32_t Y = lst[t];
-      const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
-      const double rY = c.crad[sY];
-      double ylo[3], yhi[3];
-      elem_box(d, N, ya, yb, rY, ylo, yhi);
-      bool ov = true;
-      for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
-      if (!ov) continue;
-      const bool XisA = sX < sY;
-      if (XisA) ncand++;
-      double pY[2][3];
-      elem_pos(d, ya, yb, pY);
-      const double R = rX + rY;
-      double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
-      const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                               : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
-      if (closed) {
-        any = true;
-        for (int i = 0; i < 3; i++) {
-          fa[i] += g0[i];
-          fb[i] += g1[i];
-        }
-        if (XisA) nact++;
-      }
+  if (T == 0) *c.gate = 0;
+  const int64_t X0 = T * VN_PER;
+  int32_t t[VN_PER];
+  unsigned nz = 0u;
+  if (X0 + VN_PER <= c.E) {
+    // X0 is a multiple of 16 and the arrays start 256-byte aligned
+    const int4 *vc = reinterpret_cast<const int4 *>(c.vcount + X0);
+#pragma unroll
+    for (int q = 0; q < VN_PER / 4; q++) {
+      const int4 v = vc[q];
+      t[4 * q] = v.x; t[4 * q + 1] = v.y; t[4 * q + 2] = v.z; t[4 * q + 3] = v.w;
     }
-    if (any) {
-      c.cidx[X] = (int32_t)X;
-      for (int i = 0; i < 3; i++) {
-        c.fcs[i * E + X] = fa[i];
-        c.fcs[(3 + i) * E + X] = fb[i];
+    int4 *co = reinterpret_cast<int4 *>(c.cidx + X0);
+#pragma unroll
+    for (int q = 0; q < VN_PER / 4; q++) co[q] = make_int4(-1, -1, -1, -1);
+  } else {
+#pragma unroll
+    for (int k = 0; k < VN_PER; k++) {
+      t[k] = 0;
+      if (X0 + k < c.E) {
+        t[k] = c.vcount[X0 + k];
+        c.cidx[X0 + k] = -1;
       }
-    } else {
-      c.cidx[X] = -1;
     }
   }
-  __shared__ unsigned long long ra[CB], rc[CB];
-  ra[threadIdx.x] = nact;
-  rc[threadIdx.x] = ncand;
+#pragma unroll
+  for (int k = 0; k < VN_PER; k++) nz |= (t[k] > 0 ? 1u : 0u) << k;
+  // the wavefront's queue, in element order: exclusive prefix of the lanes'
+  // counts
+  const int cnt = __popc(nz);
+  int pre = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(pre, o, 64);
+    if (lane >= o) pre += v;
+  }
+  const int total = __shfl(pre, 63, 64);
+  pre -= cnt;
+#pragma unroll
+  for (int k = 0; k < VN_PER; k++)
+    if (nz & (1u << k)) {
+      qe[wv][pre] = (int32_t)(X0 + k);
+      qt[wv][pre] = t[k];
+      pre++;
+    }
+  // queue visible to the wavefront, and the cleared cidx entries stored
+  // before the ones written below
   __syncthreads();
-  for (int w = CB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      ra[threadIdx.x] += ra[threadIdx.x + w];
-      rc[threadIdx.x] += rc[threadIdx.x + w];
-    }
-    __syncthreads();
+  unsigned long long nact = 0, ncand = 0;
+  for (int i = lane; i < total; i += 64) {
+    const int64_t X = qe[wv][i];
+    const int32_t r = vnarrow_elem(c, d, X, qt[wv][i], nact, ncand);
+    if (r >= 0) c.cidx[X] = r;
   }
-  if (threadIdx.x == 0 && cnt_act) {
+  for (int o = 32; o > 0; o >>= 1) {
+    nact += __shfl_xor(nact, o, 64);
+    ncand += __shfl_xor(ncand, o, 64);
+  }
+  if (lane == 0 && cnt_act) {
     const int64_t st = *step;
-    if (ra[0]) atomicAdd(cnt_act + st, ra[0]);
-    if (rc[0]) atomicAdd(cnt_cand + st, rc[0]);
+    if (nact) atomicAdd(cnt_act + st, nact);
+    if (ncand) atomicAdd(cnt_cand + st, ncand);
   }
 }
 
@@ -941,7 +995,8 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
   }
   // list rebuild, gated on the trigger (no-ops while the lists are valid)
   list_refresh(cs, d, s);
-  hipLaunchKernelGGL(kc_vnarrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d,
+  const int64_t nthr = (c.E + VN_PER - 1) / VN_PER;
+  hipLaunchKernelGGL(kc_vnarrow, dim3((unsigned)((nthr + CB - 1) / CB)), dim3(CB), 0, s, c, d,
                      (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
   CHK(hipGetLastError());
   return DL_OK;

@@ -506,19 +506,36 @@ __device__ __forceinline__ void step_end_sum(int g, const double *rbuf, const in
     in = dbuf;
     n = ndb;
   }
+  // thread t sums in[t], in[t + 1024], ... in that order, four loads in
+  // flight at a time
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) s += in[i];
+  int64_t i = threadIdx.x;
+  for (; i + 3 * 1024 < n; i += 4 * 1024) {
+    const double a0 = in[i], a1 = in[i + 1024], a2 = in[i + 2048], a3 = in[i + 3072];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; i < n; i += 1024) s += in[i];
+  // tree: red[t] += red[t + w] for w = 512 .. 1; the levels below 64 run in
+  // the first wavefront as shuffles (same pairs, same order)
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
+  for (int w = 512; w >= 64; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
+  double r = 0.0;
+  if (threadIdx.x < 64) {
+    r = red[threadIdx.x];
+    for (int w = 32; w > 0; w >>= 1) r += __shfl_down(r, w, 64);
+  }
   if (threadIdx.x == 0) {
     if (g < ngroups) {
-      hist_react[st * ngroups + g] = red[0];
+      hist_react[st * ngroups + g] = r;
     } else {
-      const double t = *diss_total + red[0];
+      const double t = *diss_total + r;
       *diss_total = t;
       hist_diss[st] = t;
     }
