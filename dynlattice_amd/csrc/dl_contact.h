@@ -21,6 +21,8 @@ struct ContactDev {
   const int32_t *nsptr; // [N+1] node -> struts CSR
   const int32_t *nslist;
   const uint8_t *ejoint; // [E] bit 0: node a belongs to more than one strut, bit 1: node b
+  const int32_t *sjptr; // [R+1] strut -> its joint nodes (nodes of more than one strut) CSR
+  const int32_t *sjlist;
   double kS, kN;
   // grid
   double org[3];

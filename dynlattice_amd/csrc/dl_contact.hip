@@ -297,6 +297,34 @@ __device__ __forceinline__ void scan_candidates(const ContactDev &c, int64_t k, 
   const int32_t cy = (x.cell / c.dim[0]) % c.dim[1];
   const int32_t cz = x.cell / (c.dim[0] * c.dim[1]);
   const float4 xfl = ((const float4 *)c.frec)[2 * k], xfh = ((const float4 *)c.frec)[2 * k + 1];
+  // X's side of the strut filter in registers, loaded once: the struts at
+  // X's joint nodes and the joint nodes of X's strut (the lists of a BCC
+  // joint and strut fit; longer ones are walked in memory).  A node of one
+  // strut only is in no other strut, and Y's strut is never X's here, so
+  // only joint nodes need the test.
+  constexpr int KS = 8, KJ = 4;
+  int32_t nla, nlb, nlj;
+  int4 la0, la1, lb0, lb1, lj0;
+  {
+    const int32_t a0 = (x.jn & 1) ? c.nsptr[xa] : 0, a1 = (x.jn & 1) ? c.nsptr[xa + 1] : 0;
+    const int32_t b0 = (x.jn & 2) ? c.nsptr[xb] : 0, b1 = (x.jn & 2) ? c.nsptr[xb + 1] : 0;
+    const int32_t j0 = c.sjptr[sX], j1 = c.sjptr[sX + 1];
+    nla = a1 - a0;
+    nlb = b1 - b0;
+    nlj = j1 - j0;
+    auto ld = [&](const int32_t *l, int32_t b, int32_t n, int q) { return q < n ? l[b + q] : -1; };
+    la0 = make_int4(ld(c.nslist, a0, nla, 0), ld(c.nslist, a0, nla, 1), ld(c.nslist, a0, nla, 2),
+                    ld(c.nslist, a0, nla, 3));
+    la1 = make_int4(ld(c.nslist, a0, nla, 4), ld(c.nslist, a0, nla, 5), ld(c.nslist, a0, nla, 6),
+                    ld(c.nslist, a0, nla, 7));
+    lb0 = make_int4(ld(c.nslist, b0, nlb, 0), ld(c.nslist, b0, nlb, 1), ld(c.nslist, b0, nlb, 2),
+                    ld(c.nslist, b0, nlb, 3));
+    lb1 = make_int4(ld(c.nslist, b0, nlb, 4), ld(c.nslist, b0, nlb, 5), ld(c.nslist, b0, nlb, 6),
+                    ld(c.nslist, b0, nlb, 7));
+    lj0 = make_int4(ld(c.sjlist, j0, nlj, 0), ld(c.sjlist, j0, nlj, 1), ld(c.sjlist, j0, nlj, 2),
+                    ld(c.sjlist, j0, nlj, 3));
+  }
+  auto eq4 = [](const int4 a, int32_t v) { return (a.x == v) | (a.y == v) | (a.z == v) | (a.w == v); };
   for (int dz = -1; dz <= 1; dz++) {
     const int32_t z = cz + dz;
     if (z < 0 || z >= c.dim[2]) continue;
@@ -330,17 +358,27 @@ __device__ __forceinline__ void scan_candidates(const ContactDev &c, int64_t k, 
         while (hit) {
           const int q = __builtin_ctz(hit);
           hit &= hit - 1u;
+          // Y and its strut from the prefilter record, so that Y's nodes
+          // load together with its full record
+          int32_t Y = __float_as_int(fhi[0].w), sY = __float_as_int(flo[0].w);
+#pragma unroll
+          for (int p = 1; p < 4; p++) { // select, not a register-array index
+            Y = q == p ? __float_as_int(fhi[p].w) : Y;
+            sY = q == p ? __float_as_int(flo[p].w) : sY;
+          }
+          const int32_t ya = c.en0[Y], yb = c.en1[Y];
           const CRec yr = rec[j0 + q];
           bool ov = true;
           for (int i = 0; i < 3; i++) ov = ov && (x.hi[i] >= yr.lo[i]) && (yr.hi[i] >= x.lo[i]);
           if (!ov) continue;
-          const int32_t Y = yr.e, sY = yr.s;
-          const int32_t ya = c.en0[Y], yb = c.en1[Y];
-          // a node of one strut only is in no other strut (and Y's strut is
-          // not X's): the node-set walks are needed at joint nodes alone
-          if (((x.jn & 1) && node_in_strut(c, xa, sY)) || ((x.jn & 2) && node_in_strut(c, xb, sY)) ||
-              ((yr.jn & 1) && node_in_strut(c, ya, sX)) || ((yr.jn & 2) && node_in_strut(c, yb, sX)))
-            continue;
+          // the strut filter
+          bool ex = false;
+          if (x.jn & 1) ex = nla > KS ? node_in_strut(c, xa, sY) : (eq4(la0, sY) | eq4(la1, sY));
+          if (!ex && (x.jn & 2))
+            ex = nlb > KS ? node_in_strut(c, xb, sY) : (eq4(lb0, sY) | eq4(lb1, sY));
+          if (!ex && (yr.jn & 1)) ex = nlj > KJ ? node_in_strut(c, ya, sX) : eq4(lj0, ya);
+          if (!ex && (yr.jn & 2)) ex = nlj > KJ ? node_in_strut(c, yb, sX) : eq4(lj0, yb);
+          if (ex) continue;
           fn(Y, sY, ya, yb, sX < sY);
         }
       }
@@ -819,6 +857,21 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     uint8_t *pu;
     if ((rc = up(cs, &pu, ej))) return rc;
     c.ejoint = pu;
+    // strut -> joint nodes, in node order
+    std::vector<int32_t> sjptr((size_t)R + 1, 0), sjlist;
+    for (size_t k = 0; k < key.size(); k++) {
+      const int64_t n = key[k] >> 32;
+      if (nsptr[n + 1] - nsptr[n] > 1) sjptr[(key[k] & 0xffffffff) + 1]++;
+    }
+    for (int32_t s = 0; s < R; s++) sjptr[s + 1] += sjptr[s];
+    sjlist.resize(sjptr[R]);
+    std::vector<int32_t> fill(sjptr.begin(), sjptr.end() - 1);
+    for (size_t k = 0; k < key.size(); k++) {
+      const int64_t n = key[k] >> 32;
+      if (nsptr[n + 1] - nsptr[n] > 1) sjlist[fill[key[k] & 0xffffffff]++] = (int32_t)n;
+    }
+    if ((rc = up(cs, &pi, sjptr))) return rc; c.sjptr = pi;
+    if ((rc = up(cs, &pi, sjlist))) return rc; c.sjlist = pi;
   }
   cs.nblk_e = (int32_t)nb(E);
   cs.nscan_blk = (std::max<int64_t>(ncell, E) + 1023) / 1024 + 1;
