@@ -524,7 +524,9 @@ __global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, Adj
 //
 // Built from the broadphase with boxes inflated by the skin s: every
 // partner Y of X whose inflated box overlaps X's (plus the static filters:
-// other strut, no shared node with the other strut, not blacklisted).  While
+// other strut, no node in the other strut; pairs blacklisted at INIT are
+// kept but marked, so that they count as candidates as in the cell scan and
+// are never evaluated).  While
 // no node has moved by more than the trigger distance (< s) since the build,
 // two boxes that overlap now overlapped inflated by s at the build, so the
 // list holds every current candidate; the step then tests only the listed
@@ -533,6 +535,10 @@ __global__ __launch_bounds__(CB) void kc_adj_narrow(ContactDev c, DevData d, Adj
 // summed in an order that does not depend on the cell layout or on when the
 // lists were built.
 // ------------------------------------------------------------------
+// A blacklisted partner stays in the list, stored as ~Y (negative): it is
+// counted as a candidate like in the cell scan but never evaluated.
+__device__ __forceinline__ int32_t list_partner(int32_t v) { return v < 0 ? ~v : v; }
+
 __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
   GRID_STRIDE(k, c.E) {
@@ -542,15 +548,15 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     int32_t *lst = c.vlist + (int64_t)X * c.vk;
     const int32_t cap = c.vk;
     scan_candidates(c, k, x, [&](int32_t Y, int32_t, int32_t, int32_t, bool XisA) {
-      if (blacklisted(c, XisA ? X : Y, XisA ? Y : X)) return;
+      const bool bl = blacklisted(c, XisA ? X : Y, XisA ? Y : X);
       if (n < cap) {
         // insertion by partner id
         int32_t i = n;
-        while (i > 0 && lst[i - 1] > Y) {
+        while (i > 0 && list_partner(lst[i - 1]) > Y) {
           lst[i] = lst[i - 1];
           i--;
         }
-        lst[i] = Y;
+        lst[i] = bl ? ~Y : Y;
       }
       n++;
     });
@@ -580,7 +586,7 @@ __device__ __forceinline__ int32_t vnarrow_elem(const ContactDev &c, const DevDa
   bool any = false;
   const int32_t *lst = c.vlist + X * c.vk;
   for (int32_t t = 0; t < t1; t++) {
-    const int32_t Y = lst[t];
+    const int32_t Y = list_partner(lst[t]);
     const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
     const double rY = c.crad[sY];
     double ylo[3], yhi[3];
@@ -590,6 +596,7 @@ __device__ __forceinline__ int32_t vnarrow_elem(const ContactDev &c, const DevDa
     if (!ov) continue;
     const bool XisA = sX < sY;
     if (XisA) ncand++;
+    if (lst[t] < 0) continue; // blacklisted at INIT
     double pY[2][3];
     elem_pos(d, ya, yb, pY);
     const double R = rX + rY;
@@ -717,6 +724,7 @@ __global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, Ad
   const int32_t t1 = c.vcount[X];
   for (int32_t t = 0; t < t1; t++) {
     const int32_t Y = lst[t];
+    if (Y < 0) continue; // blacklisted at INIT
     const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
     const double rY = c.crad[sY];
     double ylo[3], yhi[3];

@@ -118,3 +118,61 @@ def energy_rise(time, load, e_tot):
     d = np.diff(e_tot) / e_tot[1:]
     rise = (d > 0) & (load[1:] == 0)
     return float(d[rise].max()) if rise.any() else float("-inf")
+
+
+def hub_struts(seed=41, n_outer=12, m=3, radius=2.5e-3, spine_m=12, n_branch=6, kS=1e6,
+               kN=2e6):
+    """Struts with crowded joints: a hub shared by n_outer struts (more than
+    eight struts at one node), a ring of struts between the outer ends, and
+    a spine of spine_m elements with n_branch struts leaving from its
+    interior nodes (a strut with more than four joint nodes).  Exercises the
+    contact filter "no node of X in Y's strut" at joints of every size."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n_outer, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    centre = np.array([0.02, 0.02, 0.02])
+    outer = centre + 0.015 * d
+    coords = [centre] + list(outer)
+    en, es = [], []
+
+    def strut(i, j, k, n_el):
+        a, b = coords[i], coords[j]
+        prev = i
+        for t in range(1, n_el):
+            coords.append(a + (b - a) * t / n_el)
+            en.append((prev, len(coords) - 1))
+            es.append(k)
+            prev = len(coords) - 1
+        en.append((prev, j))
+        es.append(k)
+
+    k = 0
+    for i in range(n_outer):
+        strut(0, 1 + i, k, m)
+        k += 1
+    for i in range(0, n_outer, 2):  # ring pieces between outer ends
+        strut(1 + i, 1 + (i + 1) % n_outer, k, m)
+        k += 1
+    # spine through the hub region with branches from its interior nodes
+    a = len(coords)
+    coords.append(centre + np.array([-0.018, 0.004, -0.003]))
+    b = len(coords)
+    coords.append(centre + np.array([0.018, -0.003, 0.004]))
+    first = len(coords)
+    strut(a, b, k, spine_m)
+    k += 1
+    inner = list(range(first, first + spine_m - 1))
+    for q, n in enumerate(inner[1::2][:n_branch]):
+        coords.append(coords[n] + 0.01 * rng.normal(size=3))
+        strut(n, len(coords) - 1, k, m)
+        k += 1
+    coords = np.array(coords)
+    en = np.array(en, dtype=np.int32)
+    es = np.array(es, dtype=np.int32)
+    r = np.full(k, radius)
+    p = L.Problem(coords=coords, elem_nodes=en, elem_strut=es, n_struts=k,
+                  sections=L.Sections.circle(r * 0.5, 2e9, 0.3, 1200.0))
+    p.contact = L.Contact(radius=r, penalty_sts=kS, penalty_nts=kN)
+    p.finalize()
+    p.dt = L.stable_dt(p, 0.5)
+    return p

@@ -126,3 +126,44 @@ def test_step_candidate_list_capacity_is_checked():
     with pytest.raises(DLError) as e:
         _run_with_skin(p, 0.0, 5, max_pairs=4)
     assert e.value.code == DL_ERR_CAPACITY
+
+
+def test_crowded_joints_pair_sets_and_lists():
+    """Joints shared by twelve struts and a strut with five joint nodes (the
+    strut filter's lists longer than the scan keeps in registers): candidate
+    and active sets equal the oracle's, forces within 1e-11, and the step's
+    candidate lists find the full broadphase's pairs every step."""
+    from dynlattice_amd.gpu import GpuModel
+    from tests.helpers import hub_struts
+    p = hub_struts()
+    r = RefModel(p)
+    g = GpuModel(p)
+    rng = np.random.default_rng(41)
+    d = np.zeros((p.n_nodes, 6))
+    d[:, :3] = rng.normal(size=(p.n_nodes, 3)) * 1e-4
+    d[:, 3:] = rng.normal(size=(p.n_nodes, 3)) * 1e-3
+    g.set_state(d)
+    fr = r.int_vector(d)
+    fg = g.int_vector()
+    ca, cb = r.contact_candidates(d)
+    aa, ab = r.contact_active()
+    ga, gb = g.contact_candidates()
+    gaa, gab = g.contact_pairs()
+    g.close()
+    print("candidates", len(ca), "active", len(aa))
+    assert len(ca) > 100 and len(aa) > 0
+    assert _sets(ga, gb) == _sets(ca, cb)
+    assert _sets(gaa, gab) == _sets(aa, ab)
+    assert rel_inf(fg, fr) <= 1e-11
+    # the struts pulled towards the hub collide while stepping
+    K = 200
+    c = p.coords.mean(0)
+    p.velo0[:, :3] = -(p.coords - c) * (0.3 / (K * p.dt))
+    d0, v0, h0, c0 = _run_with_skin(p, -1.0, K)
+    d1, v1, h1, c1 = _run_with_skin(p, 0.5 * float(p.contact.radius.max()), K, 400 * p.n_elems)
+    out = RefModel(p).leapfrog(K)
+    print("active per step (max)", h0["n_contacts"].max(), "candidates", c0)
+    assert h0["n_contacts"].max() > 5
+    assert np.array_equal(h0["n_contacts"], out["n_contacts"])
+    assert np.array_equal(h1["n_contacts"], h0["n_contacts"]) and c1 == c0
+    assert rel_inf(d1, d0) <= 1e-8
