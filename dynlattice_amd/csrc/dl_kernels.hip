@@ -1,13 +1,15 @@
 // dl_kernels.hip -- gfx950 kernels of the explicit rod-lattice step (FP64).
 //
 // One explicit step is two launches (plus contact when enabled):
-//   k_strut  one thread per strut.  Walks the strut's element chain, computes
-//            every element's internal force (kinematics, strains, material
-//            return map, XI^T sigma), sums the two element contributions of
-//            each private (single-strut) node in registers and immediately
-//            applies the node update (mass, gyro, constraints, central
-//            difference, SO(3) composition).  Contributions to shared
-//            (joint) nodes go to a slot buffer.
+//   k_strut  one lane per element; a strut's elements (a chain of at most
+//            64) sit in consecutive lanes of one wavefront.  Computes the
+//            element's internal force (kinematics, strains, material return
+//            map, XI^T sigma), sums the two element contributions of each
+//            private (single-strut) node in registers (the left neighbour's
+//            by a lane shuffle) and immediately applies the node update
+//            (mass, gyro, constraints, central difference, SO(3)
+//            composition).  Contributions to shared (joint) nodes go to a
+//            slot buffer.
 //   k_joint  one thread per shared node.  Sums its slots in a fixed order
 //            (CSR, strut order) and applies the same node update.
 // Assembly is therefore a deterministic gather: no floating-point atomics.
