@@ -46,9 +46,8 @@ struct ContactDev {
   const unsigned long long *black;
   int64_t nblack;
   // per-element contact forces
-  int32_t *cidx;        // [E] slot or -1
-  double *fcs;          // [6][E]
-  int32_t *slot_ctr;    // [1]
+  unsigned long long *cbits; // [(E+63)/64] bit X: element X has a closed pair this evaluation
+  double *fcs;          // [6][E] force rows by element
   // listing
   int32_t *lcount;      // [E]
   int32_t *loff;        // [E+1]

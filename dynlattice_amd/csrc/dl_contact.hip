@@ -207,11 +207,8 @@ struct __align__(64) CRec {
 __global__ __launch_bounds__(CB) void kc_scatter(ContactDev c, const int32_t *gate) {
   GATED(gate);
   // the extent maximum has been used (kc_cells): reset it for the next
-  // broadphase, and the force-slot counter of kc_narrow
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *c.extbits = 0ull;
-    c.slot_ctr[0] = 0;
-  }
+  // broadphase
+  if (blockIdx.x == 0 && threadIdx.x == 0) *c.extbits = 0ull;
   GRID_STRIDE(e, c.E) {
     const int32_t cell = c.cell[e];
     c.tmp[cell_start(c, cell) + c.rank[e]] = (int32_t)e;
@@ -442,18 +439,14 @@ __global__ __launch_bounds__(CB) void kc_narrow(ContactDev c, DevData d, int kin
         }
       }
     });
-    if (MODE == 0) {
-      if (any) {
-        const int32_t slot = atomicAdd(c.slot_ctr, 1);
-        c.cidx[X] = slot;
-        const int64_t E = c.E;
-        for (int i = 0; i < 3; i++) {
-          c.fcs[i * E + slot] = fa[i];
-          c.fcs[(3 + i) * E + slot] = fb[i];
-        }
-      } else {
-        c.cidx[X] = -1;
+    if (MODE == 0 && any) {
+      // the element's force row and its bit (cleared by kc_clear_bits)
+      const int64_t E = c.E;
+      for (int i = 0; i < 3; i++) {
+        c.fcs[i * E + X] = fa[i];
+        c.fcs[(3 + i) * E + X] = fb[i];
       }
+      atomicOr(c.cbits + (X >> 6), 1ull << (X & 63));
     }
     if (MODE == 1) c.lcount[k] = nlist;
   }
@@ -571,11 +564,11 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 }
 
 // contact forces of the step from the lists: exact boxes of X and of each
-// listed Y, then the pair as in kc_narrow; the element's force row
-// (cidx[X] = X, or -1 without a closed pair) and the per-step counts
-__device__ __forceinline__ int32_t vnarrow_elem(const ContactDev &c, const DevData &d, int64_t X,
-                                                int32_t t1, unsigned long long &nact,
-                                                unsigned long long &ncand) {
+// listed Y, then the pair as in kc_narrow; the element's force row (returns
+// whether X has a closed pair) and the per-step counts
+__device__ __forceinline__ bool vnarrow_elem(const ContactDev &c, const DevData &d, int64_t X,
+                                             int32_t t1, unsigned long long &nact,
+                                             unsigned long long &ncand) {
   const int64_t N = c.N, E = c.E;
   const int32_t sX = c.es[X];
   const double rX = c.crad[sX];
@@ -612,56 +605,49 @@ __device__ __forceinline__ int32_t vnarrow_elem(const ContactDev &c, const DevDa
       if (XisA) nact++;
     }
   }
-  if (!any) return -1;
+  if (!any) return false;
   for (int i = 0; i < 3; i++) {
     c.fcs[i * E + X] = fa[i];
     c.fcs[(3 + i) * E + X] = fb[i];
   }
-  return (int32_t)X;
+  return true;
 }
 
 // Sixteen elements per thread: their list lengths arrive in four vector
-// loads and their cidx entries are cleared in four stores, and the grid is
-// small enough to be resident at once.  The elements that have a list go to
-// a queue of their wavefront, which its 64 lanes then share, so the pair
-// work is spread evenly whether few lists are filled (a lattice before it
-// buckles: about one memory round trip per step) or many.  The counts are
-// integers, summed per wavefront (any order gives the same sum) with one
-// atomic per wavefront.
+// loads, and the grid is small enough to be resident at once.  The elements
+// that have a list go to a queue of their wavefront, which its 64 lanes then
+// share, so the pair work is spread evenly whether few lists are filled (a
+// lattice before it buckles: about one memory round trip per step) or many.
+// The wavefront's 1024 elements own 16 words of the contact bit mask, built
+// in LDS and stored whole.  The counts are integers, summed per wavefront
+// (any order gives the same sum) with one atomic per wavefront.
 #define VN_PER 16
 __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
                                                  unsigned long long *cnt_act,
                                                  unsigned long long *cnt_cand,
                                                  const int32_t *step) {
   __shared__ int32_t qe[CB / 64][64 * VN_PER], qt[CB / 64][64 * VN_PER];
+  __shared__ unsigned long long wb[CB / 64][VN_PER];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t T = (int64_t)blockIdx.x * CB + threadIdx.x;
   // the lists are current (list_refresh ran before): clear the rebuild flag;
   // the step's node updates may set it again
   if (T == 0) *c.gate = 0;
+  if (lane < VN_PER) wb[wv][lane] = 0ull;
   const int64_t X0 = T * VN_PER;
   int32_t t[VN_PER];
   unsigned nz = 0u;
   if (X0 + VN_PER <= c.E) {
-    // X0 is a multiple of 16 and the arrays start 256-byte aligned
+    // X0 is a multiple of 16 and the array starts 256-byte aligned
     const int4 *vc = reinterpret_cast<const int4 *>(c.vcount + X0);
 #pragma unroll
     for (int q = 0; q < VN_PER / 4; q++) {
       const int4 v = vc[q];
       t[4 * q] = v.x; t[4 * q + 1] = v.y; t[4 * q + 2] = v.z; t[4 * q + 3] = v.w;
     }
-    int4 *co = reinterpret_cast<int4 *>(c.cidx + X0);
-#pragma unroll
-    for (int q = 0; q < VN_PER / 4; q++) co[q] = make_int4(-1, -1, -1, -1);
   } else {
 #pragma unroll
-    for (int k = 0; k < VN_PER; k++) {
-      t[k] = 0;
-      if (X0 + k < c.E) {
-        t[k] = c.vcount[X0 + k];
-        c.cidx[X0 + k] = -1;
-      }
-    }
+    for (int k = 0; k < VN_PER; k++) t[k] = X0 + k < c.E ? c.vcount[X0 + k] : 0;
   }
 #pragma unroll
   for (int k = 0; k < VN_PER; k++) nz |= (t[k] > 0 ? 1u : 0u) << k;
@@ -682,14 +668,18 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
       qt[wv][pre] = t[k];
       pre++;
     }
-  // queue visible to the wavefront, and the cleared cidx entries stored
-  // before the ones written below
-  __syncthreads();
+  __syncthreads(); // queue and cleared mask words visible
   unsigned long long nact = 0, ncand = 0;
+  const int64_t xw = (T - lane) * VN_PER; // the wavefront's first element
   for (int i = lane; i < total; i += 64) {
     const int64_t X = qe[wv][i];
-    const int32_t r = vnarrow_elem(c, d, X, qt[wv][i], nact, ncand);
-    if (r >= 0) c.cidx[X] = r;
+    if (vnarrow_elem(c, d, X, qt[wv][i], nact, ncand))
+      atomicOr(&wb[wv][(X - xw) >> 6], 1ull << (X & 63));
+  }
+  __syncthreads();
+  {
+    const int64_t w = (xw >> 6) + lane;
+    if (lane < VN_PER && w < (c.E + 63) / 64) c.cbits[w] = wb[wv][lane];
   }
   for (int o = 32; o > 0; o >>= 1) {
     nact += __shfl_xor(nact, o, 64);
@@ -700,6 +690,12 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
     if (nact) atomicAdd(cnt_act + st, nact);
     if (ncand) atomicAdd(cnt_cand + st, ncand);
   }
+}
+
+// clears the contact bit mask before a cell-scan evaluation sets its bits
+__global__ __launch_bounds__(CB) void kc_clear_bits(unsigned long long *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x;
+  if (i < n) p[i] = 0ull;
 }
 
 // Adjoint of the contact forces from the lists (see kc_adj_narrow), one
@@ -896,17 +892,17 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
   if ((rc = calloc_dev(cs, (double **)&c.rec, 8 * (size_t)E))) return rc;
   if ((rc = calloc_dev(cs, &c.tmp, E))) return rc;
   if ((rc = calloc_dev(cs, (double **)&c.frec, 4 * (size_t)E))) return rc;
-  if ((rc = calloc_dev(cs, &c.cidx, E))) return rc;
+  if ((rc = calloc_dev(cs, &c.cbits, (E + 63) / 64))) return rc;
   if ((rc = calloc_dev(cs, &c.fcs, 6 * E))) return rc;
-  if ((rc = calloc_dev(cs, &c.slot_ctr, 1))) return rc;
   if ((rc = calloc_dev(cs, &c.lcount, E))) return rc;
   if ((rc = calloc_dev(cs, &c.loff, E + 1))) return rc;
   c.black = nullptr;
   c.nblack = 0;
   c.la = c.lb = nullptr;
   c.lcap = 0;
-  CHK(hipMemset(c.cidx, 0xff, sizeof(int32_t) * E)); // -1: no contact before the first evaluation
-  d.cidx = c.cidx;
+  // no contact before the first evaluation
+  CHK(hipMemset(c.cbits, 0, sizeof(unsigned long long) * ((E + 63) / 64)));
+  d.cbits = c.cbits;
   d.fcs = c.fcs;
   // candidate lists of the step: skin 0 = half the largest radius, < 0 = off
   c.skin = co->skin < 0.0 ? 0.0 : (co->skin > 0.0 ? co->skin : 0.5 * rmax);
@@ -949,6 +945,11 @@ static void scan_i32(ContactState &cs, const int32_t *in, int64_t n, int32_t *ou
 
 // broadphase up to sorted cell lists, boxes inflated by infl beyond the
 // contact radii; gated (see GATED) when gate is not null
+static void clear_bits(const ContactDev &c, hipStream_t s) {
+  const int64_t n = (c.E + 63) / 64;
+  hipLaunchKernelGGL(kc_clear_bits, dim3((unsigned)((n + CB - 1) / CB)), dim3(CB), 0, s, c.cbits, n);
+}
+
 static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double infl = 0.0,
                        const int32_t *gate = nullptr) {
   ContactDev &c = cs.c;
@@ -1027,6 +1028,7 @@ int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s) {
 int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   broadphase(cs, d, s);
+  clear_bits(c, s);
   hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
                      nullptr, nullptr, nullptr);
   CHK(hipGetLastError());
@@ -1049,6 +1051,7 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
   if (!(c.skin > 0.0)) {
     // no lists: full broadphase and cell scan every step
     broadphase(cs, d, s);
+    clear_bits(c, s);
     hipLaunchKernelGGL(kc_narrow<0>, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, (int)LIST_ACTIVE,
                        (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
     CHK(hipGetLastError());

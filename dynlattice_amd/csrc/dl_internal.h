@@ -77,8 +77,8 @@ struct DevData {
   // outputs
   double *fout;         // [6][N] (action path)
   // contact: per-element translational contact force on (node a, node b)
-  const int32_t *cidx;  // [E] slot into fcs or -1
-  const double *fcs;    // [6][E] (a: 0..2, b: 3..5)
+  const unsigned long long *cbits; // [(E+63)/64] bit e: element e has a contact force row
+  const double *fcs;    // [6][E] (a: 0..2, b: 3..5), row e valid where its bit is set
   double *rbuf;         // reaction contributions [n_react]
   double *dbuf;         // [R] per-strut dissipation increment
   int32_t *err;         // device error flag

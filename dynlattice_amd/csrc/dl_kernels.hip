@@ -333,12 +333,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     elem_forces(k, sm, fa, fb);
     if (CONTACT) {
       // rod-rod contact force on this element's nodes (RodContactModel.cpp:729)
-      const int32_t ci = d.cidx[e];
-      if (ci >= 0) {
+      if ((d.cbits[e >> 6] >> (e & 63)) & 1ull) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-          fa[i] += d.fcs[i * E + ci];
-          fb[i] += d.fcs[(3 + i) * E + ci];
+          fa[i] += d.fcs[i * E + e];
+          fb[i] += d.fcs[(3 + i) * E + e];
         }
       }
     }
