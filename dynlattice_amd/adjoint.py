@@ -49,11 +49,12 @@ class ImpactSimulation:
     set of section data (SO(3) update, elastic material)."""
 
     def __init__(self, problem: L.Problem, nsteps: int, checkpoint_every: int = 100,
-                 group: str = "top", dof: int = 2, device: int = 0):
+                 group: str = "top", dof: int = 2, device: int = 0, max_pairs: int = 0):
         self.p = problem
         self.nsteps = int(nsteps)
         self.every = int(checkpoint_every)
-        self.model = GpuModel(problem, device=device, init=False)
+        # max_pairs: contact candidate-list capacity (0: the library default)
+        self.model = GpuModel(problem, device=device, init=False, max_pairs=max_pairs)
         self.model.set_reaction_group(problem.groups[group], dof)
 
     def run(self, sections: dict, contact_radius=None) -> np.ndarray:

@@ -38,9 +38,9 @@ def _no_contact_case():
     return p
 
 
-def _sim(p, nsteps, every):
+def _sim(p, nsteps, every, max_pairs=0):
     from dynlattice_amd.adjoint import ImpactSimulation
-    return ImpactSimulation(p, nsteps, every)
+    return ImpactSimulation(p, nsteps, every, max_pairs=max_pairs)
 
 
 def test_checkpointed_forward_is_the_explicit_step():
@@ -80,9 +80,9 @@ def test_backward_requires_a_forward_run():
     sim.close()
 
 
-def _fd_check(p, nsteps, every, picks, contact, rtol):
+def _fd_check(p, nsteps, every, picks, contact, rtol, max_pairs=0):
     rng = np.random.default_rng(3)
-    sim = _sim(p, nsteps, every)
+    sim = _sim(p, nsteps, every, max_pairs)
     base = _sections(p)
     cr = p.contact.radius.copy() if contact else None
     r = sim.run(base, cr)
@@ -153,3 +153,21 @@ def test_radius_gradient_matches_oracle_finite_differences():
     err = np.abs(ad - fd) / np.abs(fd)
     assert err.max() <= 1e-4, (int(np.argmax(err)), float(err.max()))
     sim.close()
+
+
+def test_gradients_with_init_blacklist_and_crowded_joints():
+    """Struts around a 12-strut joint with pairs blacklisted at INIT (kept in
+    the candidate lists as marked entries, which the pair VJPs skip like the
+    forward pass), pulled towards the joint so that they collide: section and
+    contact-radius gradients of the joint's reaction against central
+    differences of the GPU forward."""
+    from tests.helpers import hub_struts
+    p = hub_struts()
+    K = 200
+    c = p.coords.mean(0)
+    p.velo0[:, :3] = -(p.coords - c) * (0.3 / (K * p.dt))
+    p.groups["top"] = np.array([0], dtype=np.int32)
+    picks = [("area", 3), ("young", 18), ("density", 20), ("contact_radius", 5),
+             ("contact_radius", 18)]
+    assert _fd_check(p, K, 50, picks, contact=True, rtol=1e-4,
+                     max_pairs=400 * p.n_elems) == len(picks)
