@@ -494,51 +494,75 @@ __global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) 
 // fixed tree order (deterministic), written to history row *step
 // ------------------------------------------------------------------
 // one sum (reaction group g < ngroups, else the dissipation partials)
-__device__ __forceinline__ void step_end_sum(int g, const double *rbuf, const int32_t *rgptr,
-                                             int32_t ngroups, const double *dbuf, int64_t ndb,
-                                             double *diss_total, double *hist_react,
-                                             double *hist_diss, int64_t st, double *red) {
-  const double *in;
-  int64_t n;
-  if (g < ngroups) {
-    in = rbuf + rgptr[g];
-    n = rgptr[g + 1] - rgptr[g];
-  } else {
-    in = dbuf;
-    n = ndb;
+// Sums g0 .. g0 + P - 1 (those below nsums) side by side: their loads are
+// in flight together and they share the tree's barriers.  Each sum keeps
+// its own fixed order.
+template <int P>
+__device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *rbuf,
+                                              const int32_t *rgptr, int32_t ngroups,
+                                              const double *dbuf, int64_t ndb,
+                                              double *diss_total, double *hist_react,
+                                              double *hist_diss, int64_t st,
+                                              double (*red)[1024]) {
+  const double *in[P];
+  int64_t n[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    const int g = g0 + q;
+    if (g >= nsums) {
+      in[q] = dbuf;
+      n[q] = 0;
+    } else if (g < ngroups) {
+      in[q] = rbuf + rgptr[g];
+      n[q] = rgptr[g + 1] - rgptr[g];
+    } else {
+      in[q] = dbuf;
+      n[q] = ndb;
+    }
   }
-  // thread t sums in[t], in[t + 1024], ... in that order, four loads in
-  // flight at a time
-  double s = 0.0;
-  int64_t i = threadIdx.x;
-  for (; i + 3 * 1024 < n; i += 4 * 1024) {
-    const double a0 = in[i], a1 = in[i + 1024], a2 = in[i + 2048], a3 = in[i + 3072];
-    s += a0;
-    s += a1;
-    s += a2;
-    s += a3;
+  // thread t sums in[t], in[t + 1024], ... in that order, four loads per
+  // sum in flight at a time
+  double s[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    s[q] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * 1024 < n[q]; i += 4 * 1024) {
+      const double a0 = in[q][i], a1 = in[q][i + 1024], a2 = in[q][i + 2048],
+                   a3 = in[q][i + 3072];
+      s[q] += a0;
+      s[q] += a1;
+      s[q] += a2;
+      s[q] += a3;
+    }
+    for (; i < n[q]; i += 1024) s[q] += in[q][i];
+    red[q][threadIdx.x] = s[q];
   }
-  for (; i < n; i += 1024) s += in[i];
   // tree: red[t] += red[t + w] for w = 512 .. 1; the levels below 64 run in
   // the first wavefront as shuffles (same pairs, same order)
-  red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 512; w >= 64; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < P; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    }
     __syncthreads();
   }
-  double r = 0.0;
   if (threadIdx.x < 64) {
-    r = red[threadIdx.x];
-    for (int w = 32; w > 0; w >>= 1) r += __shfl_down(r, w, 64);
-  }
-  if (threadIdx.x == 0) {
-    if (g < ngroups) {
-      hist_react[st * ngroups + g] = r;
-    } else {
-      const double t = *diss_total + r;
-      *diss_total = t;
-      hist_diss[st] = t;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      double r = red[q][threadIdx.x];
+      for (int w = 32; w > 0; w >>= 1) r += __shfl_down(r, w, 64);
+      const int g = g0 + q;
+      if (threadIdx.x == 0 && g < nsums) {
+        if (g < ngroups) {
+          hist_react[st * ngroups + g] = r;
+        } else {
+          const double t = *diss_total + r;
+          *diss_total = t;
+          hist_diss[st] = t;
+        }
+      }
     }
   }
   __syncthreads();
@@ -549,22 +573,23 @@ __global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int
                                                    int32_t ngroups, const double *dbuf, int64_t ndb,
                                                    double *diss_total, double *hist_react,
                                                    double *hist_diss, const int32_t *step) {
-  __shared__ double red[1024];
-  step_end_sum(blockIdx.x, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss,
-               *step, red);
+  __shared__ double red[1][1024];
+  step_end_sums<1>(blockIdx.x, blockIdx.x + 1, rbuf, rgptr, ngroups, dbuf, ndb, diss_total,
+                   hist_react, hist_diss, *step, red);
 }
 
-// few sums: one block does them in turn and then moves the step counter
+// few sums: one block does them two at a time and then moves the step counter
 // (saves the counter's own launch every step)
 __global__ __launch_bounds__(1024) void k_step_end1(const double *rbuf, const int32_t *rgptr,
                                                     int32_t ngroups, int32_t nsums,
                                                     const double *dbuf, int64_t ndb,
                                                     double *diss_total, double *hist_react,
                                                     double *hist_diss, int32_t *step) {
-  __shared__ double red[1024];
+  __shared__ double red[2][1024];
   const int64_t st = *step;
-  for (int g = 0; g < nsums; g++)
-    step_end_sum(g, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss, st, red);
+  for (int g = 0; g < nsums; g += 2)
+    step_end_sums<2>(g, nsums, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss,
+                     st, red);
   if (threadIdx.x == 0) *step = (int32_t)(st + 1);
 }
 
