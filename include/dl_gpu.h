@@ -120,7 +120,9 @@ typedef struct {
   double penalty_nts;
   const double *radius; /* [n_struts] */
   int64_t max_pairs;    /* step candidate-list capacity (partners summed over the
-                           elements; max_pairs / n_elems per element); 0 = 48 * n_elems */
+                           elements; max_pairs / n_elems per element, pairs
+                           blacklisted at INIT included); 0 = 48 * n_elems;
+                           overflow is DL_ERR_CAPACITY */
   double skin;          /* candidate-list skin distance of the step: 0 = half the
                            largest radius, < 0 = full broadphase every step */
 } dl_contact;
