@@ -21,7 +21,7 @@
 // Each pair is evaluated once from each side; each side accumulates only its
 // own nodes' forces, in a fixed order, so no floating-point atomics are used
 // and the result is bitwise deterministic.  The strut kernel adds the
-// per-element forces to its nodal sums.
+// per-element force rows (flagged by one bit per element) to its nodal sums.
 //
 // The explicit step does not run this per step: it keeps per-element
 // candidate lists built by the same broadphase with boxes inflated by a skin
