@@ -57,6 +57,18 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def yield_family(p):
+    """the yield condition's family: power-sum of the weighted resultants"""
+    if p.plastic is None:
+        return None
+    e = set(float(x) for x in p.plastic.exponent)
+    if e == {1.0}:
+        return "sum-abs resultants (p=1)"
+    if e == {2.0}:
+        return "quadratic resultant interaction (p=2)"
+    return "power-sum resultants (p=" + ",".join(f"{x:g}" for x in p.plastic.exponent) + ")"
+
+
 def private_node_count(p):
     """nodes held by exactly one strut-chain position (updated inside k_strut)"""
     # chain positions: element starts + last node of every strut
@@ -94,7 +106,10 @@ def cpu_baseline(p, budget_s, steps_gpu, omp):
             "kind": "port",
             "sample": f"oracle (oracle/ref_cpu.c, gcc -O3 -ffp-contract=off, {build}, "
                       f"{cores} thread(s)) on the same {p.name} lattice, {k} LeapFrog steps "
-                      f"after 1 warm-up step, {dt:.1f} s"}
+                      f"after 1 warm-up step, {dt:.1f} s"
+                      + ("; contact broadphase on the oracle's uniform grid (candidate sets "
+                         "equal to the reference's all-pairs loops, which cannot run at this "
+                         "size)" if p.contact is not None else "")}
 
 
 PROFILE = os.path.join(ROOT, "profiles", "bcc40_elastoplastic_contact.md")
@@ -474,7 +489,8 @@ def main():
         "config": {"workload": p.name, "elements": p.n_elems, "nodes": p.n_nodes,
                    "struts": p.n_struts, "dt": p.dt,
                    "material": "elastoplastic" if p.plastic is not None else "elastic",
-                   "contact": p.contact is not None, "so3": bool(p.so3)},
+                   "contact": p.contact is not None, "so3": bool(p.so3),
+                   "yield": yield_family(p)},
         "roofline": roof,
         "graph_steps": graph_steps,
         "phase_ms": {k: round(v, 5) for k, v in phases.items()},

@@ -460,12 +460,15 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   if (contact && contact->enabled) {
     c->contact = true;
     // the allocator outlives dl_create (pair lists grow later): capture the context by value
-    TRY(contact_create(c->cs, d, mesh, contact, [c](void **p, size_t bytes) {
-      hipError_t he = hipMalloc(p, bytes ? bytes : 1);
-      if (he != hipSuccess) return fail(DL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(he));
-      c->allocs.push_back(*p);
-      return (int)DL_OK;
-    }));
+    TRY(contact_create(
+        c->cs, d, mesh, contact,
+        [c](void **p, size_t bytes) {
+          hipError_t he = hipMalloc(p, bytes ? bytes : 1);
+          if (he != hipSuccess) return fail(DL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(he));
+          c->allocs.push_back(*p);
+          return (int)DL_OK;
+        },
+        [c](void *p) { dfree(c, p); }));
   }
 #undef TRY
   *out = c;
@@ -528,24 +531,45 @@ extern "C" int dl_set_sections(dl_ctx *c, const dl_sections *sec, const double *
 
 extern "C" int dl_sync(dl_ctx *c) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return DL_OK;
+  hipSetDevice(c->device);
+  return check_err(c);
 }
 
+// reads and clears the device error flag (after a stream synchronisation);
+// a capacity overflow also marks the candidate lists stale, so the next
+// step rebuilds them (and reports the overflow again if it persists)
 int check_err(dl_ctx *c) {
   int32_t e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->d.err, sizeof e, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (e == DL_ERR_PLASTIC_ITER) {
     HIPCHK(hipMemsetAsync(c->d.err, 0, sizeof(int32_t), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return fail(DL_ERR_PLASTIC_ITER, "Too many iterations in plasticity loop");
   }
   if (e == DL_ERR_CAPACITY) {
     HIPCHK(hipMemsetAsync(c->d.err, 0, sizeof(int32_t), c->stream));
+    if (c->contact) contact_invalidate(c->cs, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
     return fail(DL_ERR_CAPACITY, "contact candidate lists exceed their capacity "
                                  "(dl_contact.max_pairs); the step results are invalid");
   }
   if (e) return fail(e, "device error");
+  return DL_OK;
+}
+
+// the context's staging buffer for DOF-layout conversions (grown, never
+// shrunk): GET_MATRIX2, set_state and get_state allocate nothing per call
+static int scratch(dl_ctx *c, size_t n_doubles, double **p) {
+  if (c->stage_cap < n_doubles) {
+    dfree(c, c->stage);
+    c->stage = nullptr;
+    c->stage_cap = 0;
+    int rc = dalloc(c, &c->stage, n_doubles);
+    if (rc) return rc;
+    c->stage_cap = n_doubles;
+  }
+  *p = c->stage;
   return DL_OK;
 }
 
@@ -558,7 +582,8 @@ extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, i
   c->adj.valid = false;
   hipSetDevice(c->device);
   double *tmp = nullptr;
-  HIPCHK(hipMalloc((void **)&tmp, 2 * ndof * sizeof(double)));
+  int rc = scratch(c, 2 * ndof, &tmp);
+  if (rc) return rc;
   hipError_t e1 = hipSuccess, e2 = hipSuccess;
   if (disp) e1 = hipMemcpyAsync(tmp, disp, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (velo) e2 = hipMemcpyAsync(tmp + ndof, velo, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -566,7 +591,6 @@ extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, i
     launch_state_io(c->d, c->so3, false, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
   if (c->contact) contact_invalidate(c->cs, c->stream);
   hipError_t e3 = hipStreamSynchronize(c->stream);
-  hipFree(tmp);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
     return fail(DL_ERR_HIP, "dl_set_state copy failed");
   return DL_OK;
@@ -577,13 +601,13 @@ extern "C" int dl_get_state(dl_ctx *c, double *disp, double *velo, int64_t ndof)
   if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
   hipSetDevice(c->device);
   double *tmp = nullptr;
-  HIPCHK(hipMalloc((void **)&tmp, 2 * ndof * sizeof(double)));
+  int rc = scratch(c, 2 * ndof, &tmp);
+  if (rc) return rc;
   launch_state_io(c->d, c->so3, true, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
   hipError_t e1 = hipSuccess, e2 = hipSuccess;
   if (disp) e1 = hipMemcpyAsync(disp, tmp, ndof * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (velo) e2 = hipMemcpyAsync(velo, tmp + ndof, ndof * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   hipError_t e3 = hipStreamSynchronize(c->stream);
-  hipFree(tmp);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
     return fail(DL_ERR_HIP, "dl_get_state copy failed");
   return DL_OK;
@@ -647,6 +671,12 @@ static int upload_bcs(dl_ctx *c, const uint8_t *cmask, const double *cval, const
   double *pa, *pf;
   int32_t *pr;
   int rc;
+  for (const void *q : {(const void *)c->d.bc_mask, (const void *)c->d.bc_a, (const void *)c->d.bc_f,
+                        (const void *)c->d.bc_rslot, (const void *)c->d.bc_imp})
+    dfree(c, (void *)q);
+  c->d.bc_mask = nullptr;
+  c->d.bc_a = c->d.bc_f = nullptr;
+  c->d.bc_rslot = c->d.bc_imp = nullptr;
   if ((rc = upload(c, &pm, mask))) return rc;
   if ((rc = upload(c, &pa, a))) return rc;
   if ((rc = upload(c, &pf, f))) return rc;
@@ -825,6 +855,9 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   const int64_t N = c->d.N, E = c->d.E;
   switch (action) {
   case DL_INIT: {
+    // the blacklist's pointer and count live in the captured step graphs:
+    // re-capture after every INIT
+    c->gen++;
     launch_init_strain(c->d, c->stream);
     // no output has run yet: ImpactBC dofs start free
     if (c->imp.ready) launch_set_i32(c->imp.ready, 0, c->stream);
@@ -862,12 +895,12 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   case DL_GET_MATRIX2: {
     if (!p || !p->matrix2) return fail(DL_ERR_INPUT, "params.matrix2 is required");
     double *blk = nullptr;
-    HIPCHK(hipMalloc((void **)&blk, 36 * N * sizeof(double)));
+    int rc = scratch(c, 36 * N, &blk);
+    if (rc) return rc;
     launch_mass_blocks(c->d, c->so3, p->lumped != 0, blk, c->stream);
     std::vector<double> h(36 * N);
     hipError_t e1 = hipMemcpyAsync(h.data(), blk, 36 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     hipError_t e2 = hipStreamSynchronize(c->stream);
-    hipFree(blk);
     if (e1 != hipSuccess || e2 != hipSuccess) return fail(DL_ERR_HIP, "GET_MATRIX2 copy failed");
     for (int64_t i = 0; i < 36 * N; i++) p->matrix2[i] += h[i];
     return DL_OK;

@@ -71,13 +71,19 @@ struct ContactState {
   int64_t nscan_blk = 0;
   std::vector<int32_t> last_a, last_b;     // active pairs of the last listed evaluation
   std::vector<int32_t> cand_a, cand_b;     // candidates of the last listed evaluation
+  int64_t black_cap = 0;                   // blacklist buffer capacity (keys)
   std::function<int(void **, size_t)> alloc;
+  std::function<void(void *)> release;     // frees a buffer from alloc
 };
 
 using DevAlloc = std::function<int(void **, size_t)>;
+using DevFree = std::function<void(void *)>;
 
 int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_contact *co,
-                   const DevAlloc &alloc);
+                   const DevAlloc &alloc, const DevFree &release);
+// pairs in contact at the current state, stored in one device buffer that
+// is reused (and grown) by every INIT; the caller re-captures any step graph
+// since the kernels take the blacklist's pointer and count by value
 int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s);
 // evaluation at the current state for the action path (also lists pairs)
 int contact_force(ContactState &cs, const DevData &d, hipStream_t s);

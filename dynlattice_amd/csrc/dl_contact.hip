@@ -631,8 +631,10 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t T = (int64_t)blockIdx.x * CB + threadIdx.x;
   // the lists are current (list_refresh ran before): clear the rebuild flag;
-  // the step's node updates may set it again
-  if (T == 0) *c.gate = 0;
+  // the step's node updates may set it again.  Lists truncated by an overflow
+  // stay stale, so every later step rebuilds them and reports the overflow
+  // again instead of running on the truncated lists.
+  if (T == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0;
   if (lane < VN_PER) wb[wv][lane] = 0ull;
   const int64_t X0 = T * VN_PER;
   int32_t t[VN_PER];
@@ -702,7 +704,7 @@ __global__ __launch_bounds__(CB) void kc_clear_bits(unsigned long long *p, int64
 // thread per element in id order.
 __global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, AdjDev a) {
   const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (X == 0) *c.gate = 0; // lists current for this state (see kc_vnarrow)
+  if (X == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0; // lists current (see kc_vnarrow)
   if (X >= c.E) return;
   const int64_t N = c.N, E = c.E;
   const int32_t sX = c.es[X], xa = c.en0[X], xb = c.en1[X];
@@ -773,8 +775,9 @@ static int up(ContactState &cs, T **p, const std::vector<T> &h) {
 }
 
 int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_contact *co,
-                   const DevAlloc &alloc) {
+                   const DevAlloc &alloc, const DevFree &release) {
   cs.alloc = alloc;
+  cs.release = release;
   ContactDev &c = cs.c;
   const int64_t E = mesh->n_elems, N = mesh->n_nodes;
   const int32_t R = mesh->n_struts;
@@ -977,6 +980,10 @@ static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kin
   if (total > c.lcap) {
     int rc;
     const int64_t cap = std::max<int64_t>(total, 1024);
+    cs.release(c.la);
+    cs.release(c.lb);
+    c.la = c.lb = nullptr;
+    c.lcap = 0;
     if ((rc = calloc_dev(cs, &c.la, cap))) return rc;
     if ((rc = calloc_dev(cs, &c.lb, cap))) return rc;
     c.lcap = cap;
@@ -1016,9 +1023,20 @@ int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s) {
     keys[i] = (lo << 32) | hi;
   }
   std::sort(keys.begin(), keys.end());
-  unsigned long long *pk;
-  if ((rc = up(cs, &pk, keys))) return rc;
-  c.black = pk;
+  // one buffer for the life of the context, grown when an INIT finds more
+  // pairs than it holds
+  if ((int64_t)keys.size() > cs.black_cap || !c.black) {
+    cs.release((void *)c.black);
+    c.black = nullptr;
+    cs.black_cap = 0;
+    unsigned long long *pk;
+    if ((rc = calloc_dev(cs, &pk, std::max<size_t>(keys.size(), 64)))) return rc;
+    c.black = pk;
+    cs.black_cap = (int64_t)std::max<size_t>(keys.size(), 64);
+  }
+  if (!keys.empty())
+    CHK(hipMemcpy((void *)c.black, keys.data(), sizeof(unsigned long long) * keys.size(),
+                  hipMemcpyHostToDevice));
   c.nblack = (int64_t)keys.size();
   cs.last_a.clear();
   cs.last_b.clear();

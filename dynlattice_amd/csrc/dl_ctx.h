@@ -129,6 +129,9 @@ struct dl_ctx {
   int32_t strut_blocks = 0;
   // action path energy accumulators
   double *ediss_acc = nullptr, *epot_e = nullptr, *ediss_inc = nullptr;
+  // staging buffer of the DOF-layout conversions (GET_MATRIX2, set/get_state)
+  double *stage = nullptr;
+  size_t stage_cap = 0;
   // timing
   bool timing = false;
   double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0, t_node = 0;

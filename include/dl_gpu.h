@@ -216,7 +216,13 @@ int dl_set_graph(dl_ctx *ctx, int32_t steps_per_graph);
  * ExplicitModule::commit, nsteps times with fixed dt (maxDtime = minDtime =
  * deltaTime), with the intended store of u, v (LeapFrogModule.cpp:44-84,
  * ExplicitModule.cpp:259-419).  Runs asynchronously on the context's stream;
- * out (may be NULL) is filled after a stream synchronisation. */
+ * out (may be NULL) is filled after a stream synchronisation.  Errors the
+ * kernels raise (DL_ERR_PLASTIC_ITER, DL_ERR_CAPACITY) are kept in a device
+ * flag and returned by the first call that synchronises and reads it: this
+ * call when out is not NULL, else a later dl_explicit_step with out,
+ * dl_milne_step, dl_sync or an action.  After DL_ERR_CAPACITY the candidate
+ * lists stay marked stale, so every later step rebuilds them (and raises the
+ * error again while they still overflow). */
 int dl_explicit_step(dl_ctx *ctx, int64_t nsteps, double dt, dl_step_out *out);
 
 /* Per-step history of the last dl_explicit_step call: reaction, dissipation,
@@ -389,7 +395,8 @@ int dl_get_table(dl_ctx *ctx, const char *name, double *out, int64_t n_rows, int
  * potentialEnergy and dissipatedEnergy).  Any pointer may be NULL. */
 int dl_get_energies(dl_ctx *ctx, double *e_kin, double *e_pot, double *e_diss, double *mass);
 
-/* Wait for the context's stream. */
+/* Wait for the context's stream; returns (and clears) a pending device
+ * error of the kernels run since the last check. */
 int dl_sync(dl_ctx *ctx);
 
 /* Message of the last failed call on this thread. */

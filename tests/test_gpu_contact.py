@@ -167,3 +167,62 @@ def test_crowded_joints_pair_sets_and_lists():
     assert np.array_equal(h0["n_contacts"], out["n_contacts"])
     assert np.array_equal(h1["n_contacts"], h0["n_contacts"]) and c1 == c0
     assert rel_inf(d1, d0) <= 1e-8
+
+
+def test_capacity_overflow_reported_on_every_call():
+    """Truncated candidate lists are never used silently: after an overflow
+    the lists stay stale, so the next call rebuilds them and raises again."""
+    import copy
+
+    from dynlattice_amd.gpu import DL_ERR_CAPACITY, DLError, GpuModel
+    p, K = _cases()[0]
+    q = copy.deepcopy(p)
+    q.contact.skin = 0.0
+    g = GpuModel(q, max_pairs=4)
+    for _ in range(3):
+        with pytest.raises(DLError) as e:
+            g.step(5)
+        assert e.value.code == DL_ERR_CAPACITY
+    g.close()
+
+
+def test_reinit_under_graph_replay_uses_the_new_blacklist():
+    """INIT after a change of contact radius refreshes the blacklist of a
+    context that replays captured step graphs: the run equals, bit for bit, a
+    fresh context with the new radius and direct launches."""
+    import copy
+
+    from dynlattice_amd.gpu import GpuModel
+    p = random_segments(seed=21, spread=1.0)  # INIT in the packed state
+    p.contact.skin = 0.5 * float(p.contact.radius.max())
+    K = 16
+    g = GpuModel(p)
+    g.set_graph(8)
+    g.step(K)
+    d1, v1 = g.get_state()
+    r2 = 1.3 * p.contact.radius
+    g.set_sections(contact_radius=r2)
+    g.set_state(d1, v1)
+    g.init()
+    g.step(K)
+    da, va = g.get_state()
+    g.close()
+    q = copy.deepcopy(p)
+    q.contact.radius = r2.copy()
+    f = GpuModel(q, init=False)
+    f.set_state(d1, v1)
+    f.init()
+    f.step(K)
+    db, vb = f.get_state()
+    f.close()
+    # the radius change matters: INIT at d1 blacklists more pairs at the
+    # larger radius, so a stale blacklist would change the run
+    nb = []
+    for pr in (p, q):
+        ref = RefModel(pr, init=False)
+        ref.disp[:] = d1.reshape(-1)
+        ref.init()
+        nb.append(len(ref.blacklist()[0]))
+    print("blacklisted pairs at d1: radius r", nb[0], "radius 1.3 r", nb[1])
+    assert nb[1] > nb[0]
+    assert np.array_equal(da, db) and np.array_equal(va, vb)
