@@ -89,7 +89,7 @@ __global__ __launch_bounds__(AB) void k_adj_node(DevData d, AdjDev a, double dt_
   NodeVJP r;
   node_advance_vjp(f, q, v, d.mt[n], J0, d.gm[n], mask, ac, fe, dt, ubn, qbn, vbn, r);
   // the recorded reaction is the internal force of the constrained dof
-  const double rw = (rs >= 0) ? a.rbar[a.slot_group[rs]] : 0.0;
+  const double rw = (rs >= 0 && a.rbar) ? a.rbar[a.slot_group[rs]] : 0.0;
 #pragma unroll
   for (int c = 0; c < 6; c++) r.f[c] += (c == d.react_dof) ? rw : 0.0;
 #pragma unroll
@@ -145,7 +145,23 @@ __global__ __launch_bounds__(AB) void k_adj_elem(DevData d, AdjDev a) {
   for (int i = 0; i < 6; i++) a.Cbar[i * E + e] += r.C[i];
 }
 
+// the closing reaction row's weight as the adjoint of the nodal internal
+// force at the final state: rbar of the node's group on the reaction dof
+__global__ __launch_bounds__(AB) void k_adj_seed(DevData d, AdjDev a) {
+  const int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x;
+  if (n >= d.N) return;
+  const int32_t b = d.bc[n];
+  const int32_t rs = b >= 0 ? d.bc_rslot[b] : -1;
+  const double rw = rs >= 0 ? a.rbar[a.slot_group[rs]] : 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; c++) a.fbar[c * d.N + n] = (c == d.react_dof) ? rw : 0.0;
+}
+
 static unsigned nblk_a(int64_t n) { return (unsigned)((n + AB - 1) / AB); }
+
+void launch_adj_seed(const DevData &d, const AdjDev &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_seed, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a);
+}
 
 void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_node, dim3(nblk_a(d.N)), dim3(AB), 0, s, d, a, dt, (int)gather);

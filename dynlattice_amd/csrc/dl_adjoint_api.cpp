@@ -71,6 +71,7 @@ extern "C" int dl_adjoint_forward(dl_ctx *c, int64_t nsteps, double dt, int32_t 
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, true))) return rc;
     if ((rc = advance_steps(c, std::min<int64_t>(every, nsteps - seg * every), sp))) return rc;
   }
+  if ((rc = launch_reaction_tail(c))) return rc;
   if ((rc = state_copy(c, A.final_state, true))) return rc;
   HIPCHK(hipGetLastError());
   c->hist_n = nsteps;
@@ -118,7 +119,7 @@ static int adjoint_buffers(dl_ctx *c) {
   // reaction slot -> group (the groups may have changed since the last call)
   std::vector<int32_t> sg(std::max<int64_t>(c->n_react, 1), 0);
   for (int32_t g = 0; g < c->n_groups; g++)
-    for (int32_t k = c->rgptr_h[g]; k < c->rgptr_h[g + 1]; k++) sg[k] = g;
+    for (int32_t k = c->rn_ptr_h[g]; k < c->rn_ptr_h[g + 1]; k++) sg[k] = g;
   dfree(c, A.slot_group);
   A.slot_group = nullptr;
   if ((rc = upload(c, &A.slot_group, sg))) return rc;
@@ -252,6 +253,21 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
   a.nptr = A.nptr; a.nlist = A.nlist; a.slot_group = A.slot_group;
   const StepParams sp{A.dt, 1, 0};
   const int64_t nseg = (A.nsteps + A.every - 1) / A.every;
+  // Row t of the reaction history is the GroupOutput sum at the state after
+  // step t, i.e. the internal force step t + 1 starts from; the last row is
+  // the closing evaluation at the final state.  Its weight enters first:
+  // the internal force's VJP at the final state (elements and contact) into
+  // the element-end buffers, which the last step's node kernel gathers.
+  a.rbar = A.rbar + (A.nsteps - 1) * ng;
+  a.tape = A.final_state; // rows 0..6: u, rotation (the tape's layout)
+  launch_adj_seed(c->d, a, s);
+  launch_adj_elem(c->d, a, s);
+  if (c->contact) {
+    contact_invalidate(c->cs, s);
+    DevData dn = c->d;
+    dn.u = A.final_state;
+    if ((rc = contact_adjoint(c->cs, dn, a, s))) return rc;
+  }
   for (int64_t seg = nseg - 1; seg >= 0; seg--) {
     const int64_t t0 = seg * A.every, len = std::min<int64_t>(A.every, A.nsteps - t0);
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, false))) return rc;
@@ -265,11 +281,12 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
       launch_joint(dj, sp, MODE_STEP, true, s);
     }
     // reverse it; the node kernel first gathers the element ends of the
-    // step reversed before (none before the last step)
+    // step reversed before (before the last step: the closing evaluation)
     for (int64_t j = len - 1; j >= 0; j--) {
       a.tape = A.tape + j * 19 * N;
-      a.rbar = A.rbar + (t0 + j) * ng;
-      launch_adj_node(c->d, a, A.dt, t0 + j < A.nsteps - 1, s);
+      // the start-state force of step t is row t - 1's reaction
+      a.rbar = (t0 + j >= 1) ? A.rbar + (t0 + j - 1) * ng : nullptr;
+      launch_adj_node(c->d, a, A.dt, true, s);
       launch_adj_elem(c->d, a, s);
       if (c->contact) {
         DevData dn = c->d;

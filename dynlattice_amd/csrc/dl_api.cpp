@@ -728,18 +728,55 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
   std::sort(srt.begin(), srt.end());
   if (std::adjacent_find(srt.begin(), srt.end()) != srt.end())
     return fail(DL_ERR_INPUT, "a node may belong to one reaction group only");
+  hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
   c->d.react_dof = dof;
   c->n_react = n;
   c->n_groups = n_groups;
-  double *rb;
-  int rc = dalloc(c, &rb, (size_t)std::max<int64_t>(n, 1));
-  if (rc) return rc;
-  c->d.rbuf = rb;
-  HIPCHK(hipMemset(rb, 0, sizeof(double) * std::max<int64_t>(n, 1)));
   std::vector<int32_t> gp(group_ptr, group_ptr + n_groups + 1);
   if (n_groups == 0) gp.assign(1, 0);
-  if ((rc = upload(c, &c->rgptr, gp))) return rc;
-  c->rgptr_h = gp;
+  c->rn_ptr_h = gp;
+  // GroupOutput entries: every (element, end) at a node of group g, the
+  // group's nodes in their order, a node's elements in id order
+  const MeshHost &m = c->mesh;
+  std::vector<int32_t> nptr(m.N + 1, 0), nlist(2 * m.E);
+  for (int64_t e = 0; e < m.E; e++) {
+    nptr[m.en[2 * e] + 1]++;
+    nptr[m.en[2 * e + 1] + 1]++;
+  }
+  for (int64_t k = 0; k < m.N; k++) nptr[k + 1] += nptr[k];
+  {
+    std::vector<int32_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t e = 0; e < m.E; e++)
+      for (int end = 0; end < 2; end++) nlist[fill[m.en[2 * e + end]]++] = (int32_t)(2 * e + end);
+  }
+  std::vector<int32_t> rent, egp(n_groups + 1, 0);
+  for (int32_t g = 0; g < n_groups; g++) {
+    for (int32_t j = gp[g]; j < gp[g + 1]; j++) {
+      const int32_t x = h.react[j];
+      for (int32_t t = nptr[x]; t < nptr[x + 1]; t++) rent.push_back(nlist[t]);
+    }
+    if (rent.size() >= (size_t)INT32_MAX) return fail(DL_ERR_INPUT, "reaction groups too large");
+    egp[g + 1] = (int32_t)rent.size();
+  }
+  if (n_groups == 0) egp.assign(1, 0);
+  dfree(c, c->d.rbuf);
+  dfree(c, (void *)c->d.rent);
+  dfree(c, c->rgptr);
+  c->d.rbuf = nullptr;
+  c->d.rent = nullptr;
+  c->rgptr = nullptr;
+  c->d.nrent = (int64_t)rent.size();
+  int rc;
+  int32_t *pe;
+  if ((rc = upload(c, &pe, rent))) return rc;
+  c->d.rent = pe;
+  double *rb;
+  if ((rc = dalloc(c, &rb, (size_t)std::max<int64_t>(c->d.nrent, 1)))) return rc;
+  c->d.rbuf = rb;
+  HIPCHK(hipMemset(rb, 0, sizeof(double) * std::max<int64_t>(c->d.nrent, 1)));
+  if ((rc = upload(c, &c->rgptr, egp))) return rc;
+  c->rgptr_h = egp;
   c->hist_cap = 0; // history rows are n_groups wide: reallocate on the next step call
   c->gen++;
   return upload_bcs(c, h.cmask.empty() ? nullptr : h.cmask.data(), h.cval.empty() ? nullptr : h.cval.data(),
@@ -964,6 +1001,7 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   double v;
   if (ph == "step") v = c->t_step;
   else if (ph == "contact") v = c->t_contact;
+  else if (ph == "react") v = c->t_react;
   else if (ph == "strut") v = c->t_strut;
   else if (ph == "joint") v = c->t_joint;
   else if (ph == "node") v = c->t_node;
@@ -972,6 +1010,10 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   if (count) *count = c->t_count;
   return DL_OK;
 }
+
+// events per timed step: start, after contact, after the reaction entries,
+// after the strut kernel, after the joint (and impact) kernels
+#define STEP_EVENTS 5
 
 // one explicit step: kernels only (capturable into a hipGraph)
 int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
@@ -983,14 +1025,29 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
     if (rc) return rc;
   }
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
-  launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
+  // the previous step's GroupOutput row, at this step's start state
+  launch_react(c->d, c->so3, c->plastic, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+  launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
+  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
   launch_impact_step(c->d, sp, c->imp, c->so3, c->stream);
-  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));
   launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
                   c->hist_react, c->hist_diss, c->step_ctr, c->stream);
   launch_clock_advance(c->pd, sp.dt, c->stream);
+  return DL_OK;
+}
+
+int launch_reaction_tail(dl_ctx *c) {
+  if (c->d.nrent == 0) return DL_OK;
+  if (c->contact) {
+    int rc = contact_step(c->cs, c->d, c->stream, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  launch_react(c->d, c->so3, c->plastic, c->contact, c->stream);
+  launch_step_end(c->d, c->rgptr, c->n_groups, false, 0, nullptr, c->hist_react, c->hist_diss,
+                  c->step_ctr, c->stream, true);
   return DL_OK;
 }
 
@@ -1064,30 +1121,34 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
   }
   StepParams sp{dt, 1, c->track_diss ? 1 : 0};
   if (c->timing) {
-    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 1);
+    const int K = STEP_EVENTS;
+    std::vector<hipEvent_t> ev(K * (size_t)nsteps + 1);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (int64_t st = 0; st < nsteps; st++) {
-      int rc = launch_one_step(c, sp, &ev[4 * st]);
+      int rc = launch_one_step(c, sp, &ev[K * st]);
       if (rc) return rc;
     }
-    HIPCHK(hipEventRecord(ev[4 * nsteps], c->stream));
+    HIPCHK(hipEventRecord(ev[K * nsteps], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->t_step = c->t_contact = c->t_strut = c->t_joint = c->t_node = 0.0;
+    c->t_step = c->t_contact = c->t_react = c->t_strut = c->t_joint = c->t_node = 0.0;
     for (int64_t st = 0; st < nsteps; st++) {
-      float a, b, cc, dd;
-      HIPCHK(hipEventElapsedTime(&a, ev[4 * st], ev[4 * st + 1]));
-      HIPCHK(hipEventElapsedTime(&b, ev[4 * st + 1], ev[4 * st + 2]));
-      HIPCHK(hipEventElapsedTime(&cc, ev[4 * st + 2], ev[4 * st + 3]));
-      HIPCHK(hipEventElapsedTime(&dd, ev[4 * st], ev[4 * st + 4]));
-      c->t_contact += a;
-      c->t_strut += b;
-      c->t_joint += cc;
-      c->t_step += dd;
+      float t[STEP_EVENTS], all;
+      for (int i = 0; i < K; i++) HIPCHK(hipEventElapsedTime(&t[i], ev[K * st + i], ev[K * st + i + 1]));
+      HIPCHK(hipEventElapsedTime(&all, ev[K * st], ev[K * st + K]));
+      c->t_contact += t[0];
+      c->t_react += t[1];
+      c->t_strut += t[2];
+      c->t_joint += t[3];
+      c->t_step += all;
     }
     c->t_count = nsteps;
     for (auto &e : ev) hipEventDestroy(e);
   } else {
     int rc = advance_steps(c, nsteps, sp);
+    if (rc) return rc;
+  }
+  if (nsteps > 0) {
+    int rc = launch_reaction_tail(c);
     if (rc) return rc;
   }
   HIPCHK(hipGetLastError());

@@ -108,10 +108,13 @@ struct dl_ctx {
   // host copies needed by actions
   std::vector<double> w_e;        // Gauss weight (L0) per element
   std::vector<int32_t> elem_perm; // identity (elements are stored in input order)
-  // reaction groups: slots [rgptr[g], rgptr[g+1]) of rbuf belong to group g
+  // reaction groups: entries [rgptr[g], rgptr[g+1]) of rbuf belong to group
+  // g; the group's nodes are slots [rn_ptr_h[g], rn_ptr_h[g+1]) of the
+  // boundary records' reaction slots
   int64_t n_react = 0;
   int32_t n_groups = 0;
   int32_t *rgptr = nullptr;
+  std::vector<int32_t> rn_ptr_h;
   int32_t *step_ctr = nullptr;   // device step counter (history row)
   double *ndt = nullptr;          // per-node dt
   // captured step graphs
@@ -134,7 +137,7 @@ struct dl_ctx {
   size_t stage_cap = 0;
   // timing
   bool timing = false;
-  double t_step = 0, t_contact = 0, t_strut = 0, t_joint = 0, t_node = 0;
+  double t_step = 0, t_contact = 0, t_react = 0, t_strut = 0, t_joint = 0, t_node = 0;
   int64_t t_count = 0;
   // contact
   ContactState cs{};
@@ -176,8 +179,11 @@ static inline int upload(dl_ctx *c, T **p, const std::vector<T> &h) {
   return DL_OK;
 }
 
-// one explicit step (contact, strut chains, joints, step end)
+// one explicit step (contact, reaction entries, strut chains, joints, step end)
 int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev);
+// the call's closing GroupOutput evaluation: contact search and reaction
+// entries at the state the call ends in, summed into the last history row
+int launch_reaction_tail(dl_ctx *c);
 // zeroes the per-step histories for nsteps rows and rewinds the step counter
 int prepare_history(dl_ctx *c, int64_t nsteps);
 // nsteps further steps (captured graph when configured), continuing the step counter

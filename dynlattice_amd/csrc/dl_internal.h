@@ -30,7 +30,7 @@ struct DevData {
   const uint32_t *bc_mask; // bit i: dof i constrained
   const double *bc_a;      // [6][nbc] prescribed acceleration
   const double *bc_f;      // [6][nbc] external force
-  const int32_t *bc_rslot; // reaction slot or -1
+  const int32_t *bc_rslot; // reaction-group node slot or -1 (the adjoint's objective weight)
   const int32_t *bc_imp;   // impact group of the record or -1 (null: no impact groups)
   int32_t nbc;
   int32_t react_dof;
@@ -79,7 +79,12 @@ struct DevData {
   // contact: per-element translational contact force on (node a, node b)
   const unsigned long long *cbits; // [(E+63)/64] bit e: element e has a contact force row
   const double *fcs;    // [6][E] (a: 0..2, b: 3..5), row e valid where its bit is set
-  double *rbuf;         // reaction contributions [n_react]
+  // GroupOutput reaction: entries (element << 1 | end) at the reaction
+  // groups' nodes, group by group; rbuf holds each entry's share of the
+  // group dof of the output-load-case internal force
+  const int32_t *rent;  // [nrent]
+  int64_t nrent;
+  double *rbuf;         // [nrent]
   double *dbuf;         // [R] per-strut dissipation increment
   int32_t *err;         // device error flag
   // adjoint tape row of the current step ([19][N]: u 3, rot 4, v 6, f 6) or null
@@ -195,9 +200,17 @@ void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s,
                   const MilneDev *m = nullptr);
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
+// step end: the reaction groups' sums of rbuf into history row *step - 1
+// (the reaction of the previous step, evaluated at this step's start state;
+// none for *step == 0), the dissipation increment into row *step, then
+// *step += 1.  tail: only the reaction row *step - 1 (the call's closing
+// evaluation), the counter stays.
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
-                     int32_t *step, hipStream_t s);
+                     int32_t *step, hipStream_t s, bool tail = false);
+// GroupOutput reaction entries at the current state (after the contact pass
+// of this state): rbuf[k] = the dof share of entry k
+void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s);
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
@@ -206,6 +219,9 @@ void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, doub
 // adjoint (dl_adjoint.hip): [gather of the element ends +] node update, element force
 void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s);
 void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
+// objective weight of the call's closing reaction row: fbar = rbar at the
+// reaction groups' dofs, zero elsewhere
+void launch_adj_seed(const DevData &d, const AdjDev &a, hipStream_t s);
 // MilneDevice (dl_milne.hip): predictor and corrector node passes over the
 // internal force in d.fout, step-size control, and the accept/reject commit
 void launch_milne_pred(const DevData &d, const MilneDev &m, bool so3, hipStream_t s);

@@ -95,8 +95,6 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
       ac[c] = d.bc_a[c * nb + b];
       fe[c] = d.bc_f[c * nb + b];
     }
-    const int32_t rs = d.bc_rslot[b];
-    if (rs >= 0) d.rbuf[rs] = f[d.react_dof];
   }
   double v[6], u[3], rr[4];
   Quat q;
@@ -490,19 +488,72 @@ __global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) 
 }
 
 // ------------------------------------------------------------------
-// end of step: per-group reaction sums and the dissipation increment, in a
-// fixed tree order (deterministic), written to history row *step
+// GroupOutput reaction (GroupOutputModule.cpp:100-116,162): the model's
+// internal force with LOAD_CASE "output" at the current state -- the rod
+// elements with the elastic predictor from the committed plastic state
+// (SpecialCosseratRodModel.cpp:762, getStress with inelastic = false) and
+// the contact forces of this state's contact search (RodContactModel.cpp:
+// 153-166; the default update condition searches again) -- summed over a
+// node group's dof.  One thread per entry (element, end) at a group node:
+// the entry's share of the dof goes to rbuf, and the step-end kernel sums
+// each group's entries in a fixed order.  It runs right after the contact
+// pass of a state (the next step's start, or the closing pass of a call),
+// so the contact rows it adds are this state's.
+// ------------------------------------------------------------------
+template <bool SO3, bool PLASTIC, bool CONTACT>
+__global__ __launch_bounds__(DL_BLOCK) void k_react(DevData d) {
+  const int64_t k = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
+  if (k >= d.nrent) return;
+  const int32_t code = d.rent[k], e = code >> 1, end = code & 1;
+  const int64_t N = d.N, E = d.E, R = d.R;
+  const int32_t na = d.en0[e], nb = d.en1[e], s = d.es[e];
+  double Xa[3], ua[3], Xb[3], ub[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    Xa[i] = d.X0[i * N + na]; ua[i] = d.u[i * N + na];
+    Xb[i] = d.X0[i * N + nb]; ub[i] = d.u[i * N + nb];
+  }
+  double ra[4], rb[4];
+  Quat qna, qnb, qNa, qNb;
+  load_rot<SO3>(d, na, ra, qna);
+  load_rot<SO3>(d, nb, rb, qnb);
+  qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
+  qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
+  double strain[6], sm[6], fa[6], fb[6];
+  elem_eps0(Xa, Xb, qNa, qNb, strain);
+  ElemKin kin;
+  elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), kin);
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    strain[i] = kin.eps[i] - strain[i];
+    if (PLASTIC) strain[i] = strain[i] - d.ep[i * E + e];
+    sm[i] = d.C[i * R + s] * strain[i];
+  }
+  elem_forces(kin, sm, fa, fb);
+  const int dof = d.react_dof;
+  double v = end ? fb[dof] : fa[dof];
+  if (CONTACT && dof < 3 && ((d.cbits[e >> 6] >> (e & 63)) & 1ull))
+    v += d.fcs[(3 * end + dof) * E + e];
+  d.rbuf[k] = v;
+}
+
+// ------------------------------------------------------------------
+// end of step: per-group reaction sums (the reaction of the state the step
+// started from, i.e. the previous step's GroupOutput row) and the
+// dissipation increment, in a fixed tree order (deterministic)
 // ------------------------------------------------------------------
 // one sum (reaction group g < ngroups, else the dissipation partials)
 // Sums g0 .. g0 + P - 1 (those below nsums) side by side: their loads are
 // in flight together and they share the tree's barriers.  Each sum keeps
 // its own fixed order.
+// rrow: history row of the reaction sums (< 0: not written); drow: row of
+// the dissipation sum
 template <int P>
 __device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *rbuf,
                                               const int32_t *rgptr, int32_t ngroups,
                                               const double *dbuf, int64_t ndb,
                                               double *diss_total, double *hist_react,
-                                              double *hist_diss, int64_t st,
+                                              double *hist_diss, int64_t rrow, int64_t drow,
                                               double (*red)[1024]) {
   const double *in[P];
   int64_t n[P];
@@ -556,11 +607,11 @@ __device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *r
       const int g = g0 + q;
       if (threadIdx.x == 0 && g < nsums) {
         if (g < ngroups) {
-          hist_react[st * ngroups + g] = r;
+          if (rrow >= 0) hist_react[rrow * ngroups + g] = r;
         } else {
           const double t = *diss_total + r;
           *diss_total = t;
-          hist_diss[st] = t;
+          hist_diss[drow] = t;
         }
       }
     }
@@ -574,23 +625,25 @@ __global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int
                                                    double *diss_total, double *hist_react,
                                                    double *hist_diss, const int32_t *step) {
   __shared__ double red[1][1024];
+  const int64_t st = *step;
   step_end_sums<1>(blockIdx.x, blockIdx.x + 1, rbuf, rgptr, ngroups, dbuf, ndb, diss_total,
-                   hist_react, hist_diss, *step, red);
+                   hist_react, hist_diss, st - 1, st, red);
 }
 
 // few sums: one block does them two at a time and then moves the step counter
-// (saves the counter's own launch every step)
+// (saves the counter's own launch every step); tail: the reaction row only
 __global__ __launch_bounds__(1024) void k_step_end1(const double *rbuf, const int32_t *rgptr,
                                                     int32_t ngroups, int32_t nsums,
                                                     const double *dbuf, int64_t ndb,
                                                     double *diss_total, double *hist_react,
-                                                    double *hist_diss, int32_t *step) {
+                                                    double *hist_diss, int32_t *step, int tail) {
   __shared__ double red[2][1024];
   const int64_t st = *step;
   for (int g = 0; g < nsums; g += 2)
     step_end_sums<2>(g, nsums, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss,
-                     st, red);
-  if (threadIdx.x == 0) *step = (int32_t)(st + 1);
+                     st - 1, st, red);
+  __syncthreads();
+  if (threadIdx.x == 0 && !tail) *step = (int32_t)(st + 1);
 }
 
 __global__ void k_step_inc(int32_t *step) {
@@ -868,18 +921,38 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
 
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
-                     int32_t *step, hipStream_t s) {
+                     int32_t *step, hipStream_t s, bool tail) {
+  if (tail) diss = false;
   const int nb = ngroups + (diss ? 1 : 0);
+  if (tail && nb == 0) return;
   if (nb <= 8) {
     hipLaunchKernelGGL(k_step_end1, dim3(1), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
                        ngroups, nb, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
-                       step);
+                       step, (int)tail);
     return;
   }
   hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
                      ngroups, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
                      (const int32_t *)step);
-  hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step);
+  if (!tail) hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step);
+}
+
+template <bool SO3, bool PLASTIC>
+static void lr(const DevData &d, bool contact, hipStream_t s) {
+  const dim3 g(nblk(d.nrent)), b(DL_BLOCK);
+  if (contact) hipLaunchKernelGGL((k_react<SO3, PLASTIC, true>), g, b, 0, s, d);
+  else hipLaunchKernelGGL((k_react<SO3, PLASTIC, false>), g, b, 0, s, d);
+}
+
+void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s) {
+  if (d.nrent == 0) return;
+  if (so3) {
+    if (plastic) lr<true, true>(d, contact, s);
+    else lr<true, false>(d, contact, s);
+  } else {
+    if (plastic) lr<false, true>(d, contact, s);
+    else lr<false, false>(d, contact, s);
+  }
 }
 
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {

@@ -84,6 +84,9 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
     int rc = contact_step(c->cs, d, s, c->hist_nc, c->hist_ncand, c->step_ctr);
     if (rc) return rc;
   }
+  // the previous attempt's GroupOutput row, at this attempt's start state
+  // (timed with the contact phase)
+  launch_react(d, c->so3, c->plastic, c->contact, s);
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
   launch_strut_pp(d, sp, c->pp, fused ? MODE_MPRED : MODE_FORCE, c->so3, c->mat_kind, c->contact,
                   s, &m);
@@ -213,6 +216,7 @@ extern "C" int dl_milne_step(dl_ctx *c, int64_t n_attempts, const dl_milne_opts 
                     o->len_scale, 6 * c->d.N};
   if ((rc = c->timing ? timed_attempts(c, n_attempts, mc) : milne_attempts(c, n_attempts, mc)))
     return rc;
+  if (n_attempts > 0 && (rc = launch_reaction_tail(c))) return rc;
   HIPCHK(hipGetLastError());
   c->hist_n = n_attempts;
   if ((rc = check_err(c))) return rc;

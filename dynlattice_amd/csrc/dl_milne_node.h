@@ -8,11 +8,10 @@
 #include "dl_internal.h"
 
 // boundary record of node n: constrained dofs, prescribed acceleration,
-// external force; the reaction slot receives f[react_dof].  An ImpactBC
-// dof is free until the first accepted attempt (no group reaction yet).
+// external force.  An ImpactBC dof is free until the first accepted attempt
+// (no group reaction yet).
 __device__ __forceinline__ void milne_node_bc(const DevData &d, const MilneDev &m, int32_t n,
-                                        const double f[6], unsigned &mask, double ac[6],
-                                        double fe[6], bool reaction) {
+                                              unsigned &mask, double ac[6], double fe[6]) {
   const int32_t b = d.bc[n];
   mask = 0u;
 #pragma unroll
@@ -26,8 +25,6 @@ __device__ __forceinline__ void milne_node_bc(const DevData &d, const MilneDev &
     ac[c] = d.bc_a[c * nb + b];
     fe[c] = d.bc_f[c * nb + b];
   }
-  const int32_t rs = d.bc_rslot[b];
-  if (reaction && rs >= 0) d.rbuf[rs] = f[d.react_dof];
 }
 
 template <bool SO3>
@@ -74,14 +71,14 @@ __device__ __forceinline__ void list_trigger(const DevData &d, int64_t n, const 
 
 // predictor (MilneDeviceModule.cpp:64-96) at node n with internal force f
 // at the start state (m.u0, m.r0, m.v0): the predicted state into (u1, r1,
-// v1); the reaction slot takes f
+// v1)
 template <bool SO3>
 __device__ __forceinline__ void milne_pred_node(const DevData &d, const MilneDev &m, int64_t n,
                                                 double f[6]) {
   const int64_t N = d.N;
   double ac[6], fe[6];
   unsigned mask;
-  milne_node_bc(d, m, (int32_t)n, f, mask, ac, fe, true);
+  milne_node_bc(d, m, (int32_t)n, mask, ac, fe);
   double u[3], r[4], v[6];
   Quat q;
   load_node<SO3>(m.u0, m.r0, m.v0, N, n, u, r, v, q);
@@ -108,7 +105,7 @@ __device__ __forceinline__ void milne_corr_node(const DevData &d, const MilneDev
   const double dt = *m.dt, ls = m.ls, rs = SO3 ? 1.0 : m.ls;
   double ac[6], fe[6];
   unsigned mask;
-  milne_node_bc(d, m, (int32_t)n, f, mask, ac, fe, false);
+  milne_node_bc(d, m, (int32_t)n, mask, ac, fe);
   double r[4];
   Quat q;
   if (SO3) {

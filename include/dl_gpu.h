@@ -147,7 +147,7 @@ typedef struct {
 
 /* Output of dl_explicit_step: values after the last step. */
 typedef struct {
-  double reaction;      /* reaction group sum at the last step's start state */
+  double reaction;      /* GroupOutput sum of the reaction group after the last step */
   double e_diss;        /* accumulated plastic dissipation (DL_TRACK_DISSIPATION) */
   int64_t n_contacts;   /* active contact pairs in the last step */
   int64_t n_candidates; /* broadphase candidates in the last step */
@@ -180,14 +180,23 @@ int dl_set_bcs(dl_ctx *ctx, const uint8_t *cmask, const double *cval, const doub
                int64_t ndof);
 
 /* Replaces: GroupOutputModule's resp sum over a node group's DOF
- * (GroupOutputModule.cpp:100-116,162), recorded once per step. */
+ * (GroupOutputModule.cpp:100-116,162), recorded once per step: after the
+ * step, with LOAD_CASE "output", the internal force at the new state -- the
+ * rod elements' elastic predictor from the committed plastic state
+ * (SpecialCosseratRodModel.cpp:762) plus the contact forces of a contact
+ * search at that state (RodContactModel.cpp:153-166) -- summed over the
+ * group's nodes on dof.  The sum of step k is evaluated right after the
+ * contact pass of step k + 1 (the same state); the last step's by a closing
+ * contact pass and evaluation at the end of the call. */
 int dl_set_reaction_group(dl_ctx *ctx, const int32_t *nodes, int64_t n, int32_t dof);
 
 /* Replaces: ImpactModel (ImpactBC, ImpactModel.cpp:74-104): every node of
  * group g gets on dof dofs[g] the prescribed acceleration -R_g / weights[g],
  * R_g the group's internal-force sum on that dof at the step's start state
- * (the "resp" the reference's GroupOutput computed after the previous step).
- * Until the first step (LeapFrog) or accepted attempt (MilneDevice) after
+ * (the "resp" the reference's GroupOutput computed after the previous step;
+ * that is the output load case, which for an element yielding in the step
+ * differs from the step's own evaluation used here -- identical for the
+ * elastic material).  Until the first step (LeapFrog) or accepted attempt (MilneDevice) after
  * INIT there is no such output and the dof stays free, as in the reference.
  * nodes[group_ptr[g] .. group_ptr[g+1]); a node belongs to one group at
  * most, and the dof must not be constrained by dl_set_bcs.  n_groups = 0
@@ -225,8 +234,9 @@ int dl_set_graph(dl_ctx *ctx, int32_t steps_per_graph);
  * error again while they still overflow). */
 int dl_explicit_step(dl_ctx *ctx, int64_t nsteps, double dt, dl_step_out *out);
 
-/* Per-step history of the last dl_explicit_step call: reaction, dissipation,
- * active contact pairs; n = number of steps recorded. */
+/* Per-step history of the last dl_explicit_step call: GroupOutput reaction
+ * after the step, dissipation after the step, active contact pairs of the
+ * step's force evaluation (its start state); n = number of steps recorded. */
 int dl_get_history(dl_ctx *ctx, double *reaction, double *e_diss, int64_t *n_contacts,
                    int64_t cap, int64_t *n);
 /* All reaction groups of the last dl_explicit_step call, [steps][n_groups]. */
@@ -271,15 +281,17 @@ typedef struct {
 
 /* Adjoint design gradients (no counterpart in the reference, which has no
  * sensitivities).  dl_adjoint_forward runs nsteps explicit steps like
- * dl_explicit_step (same histories; reaction[nsteps][n_groups] returned when
- * not NULL) and keeps a checkpoint of the state every checkpoint_every
- * steps.  dl_adjoint_backward then takes the adjoint of an objective with
+ * dl_explicit_step (same histories; the GroupOutput reactions
+ * reaction[nsteps][n_groups] returned when not NULL) and keeps a checkpoint
+ * of the state every checkpoint_every steps.  dl_adjoint_backward then takes the adjoint of an objective with
  * respect to that reaction history, reaction_bar[nsteps][n_groups] (e.g.
  * dt * v_plate for the absorbed energy, a one-hot row for the peak), and
  * returns the gradient of the objective with respect to the section data by
  * a reverse sweep: every segment is recomputed from its checkpoint, then
  * reversed through hand-written adjoint kernels of the node update, the
- * element internal force (elastic material) and the contact pair forces.
+ * element internal force (elastic material) and the contact pair forces;
+ * the last row's weight enters through the internal force's adjoint at the
+ * final state.
  * SO(3) rotation update and elastic material only.  The forward run's final
  * state is restored afterwards; backward may be called again with another
  * reaction_bar until the state, boundary data or sections change. */
@@ -318,8 +330,10 @@ typedef struct {
  * accept (the material commits its predicted-state update and the state
  * moves to the corrected one) or reject, and the next step size -- all on the
  * device, without a host round trip (dl_set_graph replays attempts from a
- * captured graph).  Histories as dl_explicit_step, one row per attempt
- * (reaction and contact pairs at the attempt's start state).  attempts (may
+ * captured graph).  Histories as dl_explicit_step, one row per attempt: the
+ * GroupOutput reaction at the state the attempt leaves (the accepted state,
+ * or the start state a rejected attempt restores) and the contact pairs at
+ * the attempt's start state.  attempts (may
  * be NULL): host [n_attempts][4] = dt, error, accepted (0/1), next dt.
  * Elastic and elastoplastic material, contact, both rotation updates; not
  * with per-node time steps (dl_set_node_dt). */

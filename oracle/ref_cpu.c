@@ -1812,8 +1812,77 @@ int ref_commit(ref_model *m, const double *disp, double *e_pot, double *e_diss) 
   return REF_OK;
 }
 
+/* GroupOutputModule::run (GroupOutputModule.cpp:100-116,162) after a step:
+ * LOAD_CASE "output", the model's GET_INT_VECTOR at the current state, the
+ * group's dof summed.  The rod elements use the elastic predictor from the
+ * committed plastic state (SpecialCosseratRodModel.cpp:762 passes
+ * inelastic = false to getStress, ElastoPlasticRodMaterial.cpp:198-269 then
+ * skips the return map); the contact model searches the contacts again at
+ * this state (RodContactModel.cpp:153-166, the update condition defaults to
+ * true, :84-88) and adds their forces.  Only elements with a node in the
+ * group reach the group's dofs, so only those are evaluated; the material
+ * state is not written (the reference overwrites its current state here,
+ * but the next step's evaluation overwrites it again before anything reads
+ * it). */
+static int output_reaction(ref_model *m, const double *disp, const int32_t *group,
+                           int64_t group_n, int32_t dof, double *r) {
+  const int64_t N = m->N;
+  double *f = (double *)calloc(6 * N, sizeof(double));
+  char *in = (char *)calloc(N, 1);
+  for (int64_t k = 0; k < group_n; k++) in[group[k]] = 1;
+  for (int32_t s = 0; s < m->R; s++) {
+    const strut_t *S = &m->st[s];
+    for (int ie = 0; ie < S->n_el; ie++) {
+      const int64_t e = S->elems[ie];
+      const int32_t a = m->en[2 * e], b = m->en[2 * e + 1];
+      if (!in[a] && !in[b]) continue;
+      double c[2][3], u[2][3], L[2][9], strain[6], Lip[9], phiP[3], g[2], w, sm[6], ss[6];
+      elem_gather(m, e, disp, c, u, L);
+      elem_strains(c, u, L, m->strain0 + 6 * e, strain, Lip, phiP, g, &w);
+      const double *K = m->K + 6 * s;
+      if (m->pl.model == REF_ELASTIC) {
+        elastic_stress(K, strain, sm);
+      } else {
+        double el[6];
+        for (int i = 0; i < 6; i++) el[i] = strain[i] - m->old_ep[6 * e + i];
+        elastic_stress(K, el, sm);
+      }
+      mv3(Lip, sm, ss);
+      mv3(Lip, sm + 3, ss + 3);
+      double SK[9];
+      skew3(phiP, SK);
+      for (int k = 0; k < 2; k++) {
+        double rr[6];
+        for (int i = 0; i < 3; i++) rr[i] = g[k] * ss[i];
+        for (int i = 0; i < 3; i++) {
+          double t = (-1. * 0.5 * SK[3 * i + 0]) * ss[0];
+          t += (-1. * 0.5 * SK[3 * i + 1]) * ss[1];
+          t += (-1. * 0.5 * SK[3 * i + 2]) * ss[2];
+          t += g[k] * ss[3 + i];
+          rr[3 + i] = t;
+        }
+        const int32_t n = m->en[2 * e + k];
+        for (int i = 0; i < 6; i++) f[6 * n + i] += w * rr[i];
+      }
+    }
+  }
+  if (m->co.enabled) ref_contact_force(m, disp, f);
+  double sum = 0.;
+  for (int64_t k = 0; k < group_n; k++) sum += f[6 * (int64_t)group[k] + dof];
+  *r = sum;
+  free(f);
+  free(in);
+  return REF_OK;
+}
+
+int ref_output_reaction(ref_model *m, const double *disp, const int32_t *group, int64_t group_n,
+                        int32_t dof, double *r) {
+  return output_reaction(m, disp, group, group_n, dof, r);
+}
+
 /* ExplicitModule::advance/commit + LeapFrogModule::solve with the intended
- * store (LeapFrogModule.cpp:44-84; ExplicitModule.cpp:259-419). */
+ * store (LeapFrogModule.cpp:44-84; ExplicitModule.cpp:259-419).  react[st]:
+ * the GroupOutput sum after step st (output_reaction at the new state). */
 int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t nsteps,
                  const int32_t *group, int64_t group_n, int32_t group_dof, double *react,
                  int commit_energy, double *energies, double *unused, int64_t *n_contacts) {
@@ -1844,11 +1913,6 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
     int64_t nc = 0;
     if (m->co.enabled) nc = ref_contact_force(m, disp, fint);
     if (n_contacts) n_contacts[st] = nc;
-    if (react) {
-      double r = 0.;
-      for (int64_t k = 0; k < group_n; k++) r += fint[6 * (int64_t)group[k] + group_dof];
-      react[st] = r;
-    }
     if (m->imp_n) impact_constraints(m, fint);
     /* fres = fext - fint; getAcce */
     REF_PARALLEL_FOR
@@ -1904,6 +1968,8 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
     }
     if (m->n_prog) prog_commit(m, dt, scale);
     m->imp_ready = 1;
+    /* output modules after the solver's commit */
+    if (react) output_reaction(m, disp, group, group_n, group_dof, react + st);
   }
   free(blocks); free(fint); free(fg); free(acc);
   return rc;
@@ -2023,8 +2089,10 @@ static void milne_solve(ref_model *m, const double *blocks, const double *fint, 
  *   then dt = max(min(s dt_opt, incr dt, dt_max), decr dt, dt_min), the
  *   material state of the last evaluation is committed and the state moves
  *   to (u_new, v_new); otherwise dt = max(s dt_opt, decr dt, dt_min).
- * Per attempt: out[6*k] = dt used, err, accepted, reaction at the start
- * state (group sum of f_int), contact pairs at the start state, new dt. */
+ * Per attempt: out[6*k] = dt used, err, accepted, the GroupOutput sum after
+ * the attempt (output_reaction at the accepted state, or at the start
+ * state a rejected attempt leaves in place), contact pairs at the start
+ * state, new dt. */
 int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
               int64_t n_attempts, const int32_t *group, int64_t group_n, int32_t group_dof,
               double *out) {
@@ -2049,8 +2117,6 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
     if ((rc = milne_forces(m, disp, velo, blocks, fint, fg, &nc))) break;
     if (m->imp_n) impact_constraints(m, fint);
     milne_solve(m, blocks, fint, fg, acc);
-    double r = 0.;
-    for (int64_t g = 0; g < group_n; g++) r += fint[6 * (int64_t)group[g] + group_dof];
     /* predictor (ABupdate: dy = dt f) */
     for (int64_t i = 0; i < nd; i++) dy[i] = dt * acc[i];
     update_vec(m, velo, dy, v_pre, 0);
@@ -2089,6 +2155,8 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
       if (t < o->dt_min) t = o->dt_min;
       dt_next = t;
     }
+    double r = 0.;
+    if (group_n > 0) output_reaction(m, disp, group, group_n, group_dof, &r);
     if (out) {
       out[6 * k] = dt;
       out[6 * k + 1] = err;

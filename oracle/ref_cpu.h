@@ -114,10 +114,17 @@ int ref_mass_blocks(ref_model *m, const double *disp, double *blocks, int lumped
 int ref_set_bcs(ref_model *m, const uint8_t *cmask, const double *cval,
                 const double *fext);
 
+/* GroupOutput's reaction (GroupOutputModule.cpp:100-116,162) at the state
+ * disp: the sum over group nodes of fint[6*node+dof] with the output load
+ * case (elastic predictor from the committed plastic state, contact forces
+ * of a fresh search at disp). */
+int ref_output_reaction(ref_model *m, const double *disp, const int32_t *group, int64_t group_n,
+                        int32_t dof, double *r);
+
 /* LeapFrog step with store (ExplicitModule + LeapFrogModule), nsteps times at
- * fixed dt.  disp/velo updated in place.  Reaction group: per step the sum of
- * fint[6*node+dof] (elements + contact, evaluated at the step's start state)
- * over group nodes is written to react[step] (may be NULL).  If
+ * fixed dt.  disp/velo updated in place.  Reaction group: after each step,
+ * GroupOutput's sum (ref_output_reaction at the new state) is written to
+ * react[step] (may be NULL).  If
  * commit_energy != 0, energies[3*step + (0,1,2)] = (E_pot, E_diss, E_kin)
  * after each step's COMMIT. n_contacts[step] = active contact pairs. */
 int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt,

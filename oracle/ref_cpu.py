@@ -90,6 +90,7 @@ def lib(omp=False):
     L.ref_leapfrog.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int64, ip32, C.c_int64,
                                C.c_int32, dp, C.c_int, dp, dp, ip64]
     L.ref_commit.argtypes = [C.c_void_p, dp, dp, dp]
+    L.ref_output_reaction.argtypes = [C.c_void_p, dp, ip32, C.c_int64, C.c_int32, dp]
     L.ref_milne.argtypes = [C.c_void_p, dp, dp, C.POINTER(MilneOpts), C.c_int64, ip32,
                             C.c_int64, C.c_int32, dp]
     L.ref_set_programs.argtypes = [C.c_void_p, C.c_int, ip32, dp, C.POINTER(C.c_char_p), ip32]
@@ -244,6 +245,16 @@ class RefModel:
         self._check(self.L.ref_get_int_vector(self.h, _p(d), _p(v), _p(f), int(inelastic),
                                               int(gyro), int(contact and self.p.contact is not None)))
         return f.reshape(-1, 6)
+
+    def output_reaction(self, group, dof=2, disp=None):
+        """GroupOutput's reaction at disp (default: the current state): the
+        group dof's sum of the output-load-case internal force"""
+        d = self.disp if disp is None else _c(disp).reshape(-1)
+        g = _c(group, np.int32)
+        r = C.c_double()
+        self._check(self.L.ref_output_reaction(self.h, _p(d), _p(g, ip32), int(g.shape[0]),
+                                               int(dof), C.byref(r)))
+        return r.value
 
     def elem_fint(self, e, disp=None, inelastic=True):
         d = self.disp if disp is None else _c(disp).reshape(-1)
