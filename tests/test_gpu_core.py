@@ -260,3 +260,32 @@ def test_plastic_iteration_cap_is_an_error(gpu):
     with pytest.raises(DLError) as ei:
         g.int_vector()
     assert ei.value.code == DL_ERR_PLASTIC_ITER
+
+
+@pytest.mark.parametrize("contact", [False, True])
+def test_groupoutput_reaction_plastic_crush(gpu, contact):
+    """The reaction history is GroupOutput's (output load case at the state
+    after each step, elastic predictor from the committed plastic state,
+    contact forces of that state): equal to the oracle's within 1e-9 on a
+    yielding crush; a run split into three calls (whose last rows come from
+    the closing evaluation) equals one call bit for bit."""
+    from dynlattice_amd.gpu import GpuModel
+    p = small_bcc(plastic=True, exponent=2.0, contact=contact)
+    p.velo0[p.groups["top"], 2] = -40.0
+    K = 300
+    out = RefModel(p).leapfrog(K, group=p.groups["top"], group_dof=2)
+    runs = []
+    for parts in ((K,), (100, 37, 163)):
+        g = GpuModel(p)
+        g.set_reaction_group(p.groups["top"], 2)
+        rows = []
+        for n in parts:
+            g.step(n)
+            rows.append(g.history(n)["reaction"])
+        runs.append((np.concatenate(rows), g.plastic_active()))
+        g.close()
+    h, act = runs[0]
+    print("contact", contact, "reaction rel", rel_inf(h, out["reaction"]), "active", act.sum())
+    assert act.sum() > 0, "case must yield"
+    assert rel_inf(h, out["reaction"]) <= 1e-9
+    assert np.array_equal(runs[1][0], h)
