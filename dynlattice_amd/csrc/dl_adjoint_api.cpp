@@ -43,6 +43,8 @@ static int adjoint_supported(dl_ctx *c) {
   if (c->plastic) return fail(DL_ERR_STATE, "the adjoint supports the elastic material only");
   if (c->pd.nprog > 0) return fail(DL_ERR_STATE, "the adjoint does not support time programs");
   if (c->imp.ngroups > 0) return fail(DL_ERR_STATE, "the adjoint does not support impact groups");
+  if (c->n_mirrored > 0)
+    return fail(DL_ERR_STATE, "the adjoint does not support -I strut frames (struts along -e3)");
   return DL_OK;
 }
 

@@ -305,16 +305,39 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     c->mat.weights = nullptr;
   }
   TRY(store_sections(c, sec));
-  // per (element, end) frame quaternions and Gauss weights
+  // per (element, end) frame quaternions and Gauss weights.  The -I frame
+  // of a strut along -e3 (SpecialCosseratRodModel.cpp:651-654) is no
+  // rotation: such elements keep identity quaternions and flag bit 2; an
+  // element with one -I end and one rotation end (a curved strut passing
+  // through -e3) has a reflection as its relative rotation, which no
+  // quaternion route can follow
   std::vector<double> qN(8 * E), wE(E);
+  auto minus_eye = [](const double *L) {
+    for (int i = 0; i < 9; i++)
+      if (L[i] != ((i % 4 == 0) ? -1.0 : 0.0)) return false;
+    return true;
+  };
+  int64_t mixed = -1;
   walk_elements(c->mesh, [&](int32_t, int64_t e, int32_t a, int32_t b, const double *La,
                              const double *Lb) {
-    const Quat qa = mat2quat(La), qb = mat2quat(Lb);
+    const bool ma = minus_eye(La), mb = minus_eye(Lb);
+    if (ma != mb && mixed < 0) mixed = e;
+    const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const Quat qa = mat2quat(ma ? I : La), qb = mat2quat(mb ? I : Lb);
+    if (ma && mb) {
+      eflag[e] |= 4u;
+      c->n_mirrored++;
+    }
     qN[0 * E + e] = qa.w; qN[1 * E + e] = qa.x; qN[2 * E + e] = qa.y; qN[3 * E + e] = qa.z;
     qN[4 * E + e] = qb.w; qN[5 * E + e] = qb.x; qN[6 * E + e] = qb.y; qN[7 * E + e] = qb.z;
     wE[e] = 2. * elem_jac(c->mesh.X.data(), a, b);
   });
   c->w_e = wE;
+  if (mixed >= 0) {
+    dl_destroy(c);
+    return fail(DL_ERR_INPUT, "element " + std::to_string(mixed) +
+                                  " joins a -I frame (tangent along -e3) and a rotation frame");
+  }
 
   // upload
   DevData &d = c->d;

@@ -101,6 +101,7 @@ struct dl_ctx {
   bool contact = false;
   bool initialized = false;
   bool track_diss = false;
+  int64_t n_mirrored = 0; // elements with -I end frames (struts along -e3)
   uint32_t flags = 0;
   PlasticParams pp{};
   DevData d{};

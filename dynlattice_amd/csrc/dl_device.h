@@ -146,6 +146,17 @@ DL_HD void elem_kinematics(const double Xa[3], const double ua[3], const double 
   }
 }
 
+// A strut along -e3 gets the frame -I from the reference's construction
+// ("180 deg turn == point mirroring", SpecialCosseratRodModel.cpp:651-654),
+// which is no rotation.  Its elements keep identity frame quaternions and a
+// flag; with Lambda = -R the kinematics give Gamma = Lambda^T phi' = -R^T phi'
+// and spatial resultants Lambda sigma = -R sigma, while K = unskew(Lambda^T
+// Lambda') is unchanged -- exactly the negated R and Gamma below.
+DL_HD void elem_mirror(ElemKin &k) {
+  for (int i = 0; i < 9; i++) k.R[i] = -k.R[i];
+  for (int i = 0; i < 3; i++) k.eps[i] = -k.eps[i];
+}
+
 // Initial material strain (initStrain_, SpecialCosseratRodModel.cpp:536-569):
 // the strain of the reference configuration, u = 0 and node rotations at
 // the identity.  The element kernels recompute it from the frames and
@@ -153,12 +164,12 @@ DL_HD void elem_kinematics(const double Xa[3], const double ua[3], const double 
 // element per evaluation); the INIT kernel stores the same bits for the
 // tables and the adjoint.
 DL_HD void elem_eps0(const double Xa[3], const double Xb[3], const Quat qNa, const Quat qNb,
-                     double eps0[6]) {
+                     double eps0[6], bool mirror = false) {
   const double z[3] = {0.0, 0.0, 0.0};
   const Quat id = {1.0, 0.0, 0.0, 0.0};
   ElemKin k;
   elem_kinematics(Xa, z, Xb, z, qmul(id, qNa), qmul(id, qNb), k);
-  for (int i = 0; i < 6; i++) eps0[i] = k.eps[i];
+  for (int i = 0; i < 6; i++) eps0[i] = (mirror && i < 3) ? -k.eps[i] : k.eps[i];
 }
 
 // internal nodal forces of the element from the material stress

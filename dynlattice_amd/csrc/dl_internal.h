@@ -43,7 +43,8 @@ struct DevData {
   const int32_t *en0;     // [E] node a
   const int32_t *en1;     // [E] node b
   const int32_t *es;      // [E] strut
-  const uint8_t *eflag;   // [E] bit 0: first element of its chain, bit 1: last
+  const uint8_t *eflag;   // [E] bit 0: first element of its chain, bit 1: last,
+                          //     bit 2: end frames -I (strut along -e3; elem_mirror)
   const int32_t *sa;      // [E] slot of node a if shared, else -1
   const int32_t *sb;      // [E] slot of node b (chain end) if shared, else -1
   // elements ([c][E])

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
   qa.w = d.qN[e]; qa.x = d.qN[E + e]; qa.y = d.qN[2 * E + e]; qa.z = d.qN[3 * E + e];
   qb.w = d.qN[4 * E + e]; qb.x = d.qN[5 * E + e]; qb.y = d.qN[6 * E + e]; qb.z = d.qN[7 * E + e];
   double eps0[6];
-  elem_eps0(Xa, Xb, qa, qb, eps0);
+  elem_eps0(Xa, Xb, qa, qb, eps0, (d.eflag[e] & 4u) != 0);
   for (int i = 0; i < 6; i++) d.eps0[i * E + e] = eps0[i];
 }
 
@@ -240,6 +240,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     elem_eps0(Xa, Xb, qNa, qNb, strain);
     ElemKin k;
     elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), k);
+    if (fl & 4u) { // frames -I (strut along -e3)
+      elem_mirror(k);
+#pragma unroll
+      for (int i = 0; i < 3; i++) strain[i] = -strain[i];
+    }
 #pragma unroll
     for (int i = 0; i < 6; i++) strain[i] = k.eps[i] - strain[i];
     if (!PLASTIC) {
@@ -520,9 +525,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_react(DevData d) {
   qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
   qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
   double strain[6], sm[6], fa[6], fb[6];
-  elem_eps0(Xa, Xb, qNa, qNb, strain);
+  const bool mir = (d.eflag[e] & 4u) != 0;
+  elem_eps0(Xa, Xb, qNa, qNb, strain, mir);
   ElemKin kin;
   elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), kin);
+  if (mir) elem_mirror(kin);
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     strain[i] = kin.eps[i] - strain[i];

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_elem_table(DevData d, PlasticParam
   qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
   ElemKin k;
   elem_kinematics(Xa, ua, Xb, ub, qmul(node_rot<SO3>(d, na), qNa), qmul(node_rot<SO3>(d, nb), qNb), k);
+  if (d.eflag[e] & 4u) elem_mirror(k); // frames -I (strut along -e3)
   double strain[6], C[6], sm[6], ep[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = 0; i < 6; i++) {
     strain[i] = k.eps[i] - d.eps0[i * E + e];

@@ -39,6 +39,18 @@
 #endif
 #define REF_IS_TINY(x) (fabs(x) < DBL_MIN)
 
+/* Planted mistakes: the tests that pin the contact forces and the
+ * gyroscopic term build this file once per mistake k with -DREF_PLANT=k
+ * (into a temporary directory) and show that their check fails on it.
+ * Every build of the oracle itself has REF_PLANT 0.
+ *   1 STS force sign flipped        2 STS shape weights of A swapped
+ *   3 NTS penalty doubled           4 NTS master weights swapped
+ *   5 gyroscopic term sign flipped  6 gyroscopic term once per node
+ *                                     instead of once per rod model */
+#ifndef REF_PLANT
+#define REF_PLANT 0
+#endif
+
 /* ------------------------------------------------------------------ */
 /* small dense helpers (row-major 3x3)                                 */
 /* ------------------------------------------------------------------ */
@@ -1075,9 +1087,17 @@ static int contact_sts(const double pA[2][3], const double pB[2][3], double uA, 
   if (dist > R) return 0;
   if (!f) return 1;
   double n[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
+#if REF_PLANT == 2
+  const double NA[2] = {0.5 * (1. + uA), 0.5 * (1. - uA)};
+#else
   const double NA[2] = {0.5 * (1. - uA), 0.5 * (1. + uA)};
+#endif
   const double NB[2] = {0.5 * (1. - uB), 0.5 * (1. + uB)};
+#if REF_PLANT == 1
+  const double sc = kap * (R - dist);
+#else
   const double sc = kap * (dist - R);
+#endif
   for (int i = 0; i < 3; i++) {
     f[0 + i] += sc * ((-1. * NA[0]) * n[i]);
     f[3 + i] += sc * ((-1. * NA[1]) * n[i]);
@@ -1099,8 +1119,16 @@ static int contact_nts(const double s[3], const double pM[2][3], double uM, doub
   if (!f) return 1;
   double n[3];
   for (int i = 0; i < 3; i++) n[i] = (s[i] - xM[i]) / dist;
+#if REF_PLANT == 3
+  const double sc = 2. * kap * (dist - R);
+#else
   const double sc = kap * (dist - R);
+#endif
+#if REF_PLANT == 4
+  const double a = -.5 * (1. + uM), b = -.5 * (1. - uM);
+#else
   const double a = -.5 * (1. - uM), b = -.5 * (1. + uM);
+#endif
   for (int i = 0; i < 3; i++) {
     f[i] += sc * n[i];
     f[3 + i] += sc * (a * n[i]);
@@ -1710,7 +1738,7 @@ static int int_vector(ref_model *m, const double *disp, const double *velo, doub
         for (int i = 0; i < 6; i++) fint[6 * n + i] += f[6 * k + i];
       }
     }
-    if (fgyro) { /* assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): per rod, its nodes */
+    if (fgyro && REF_PLANT != 6) { /* assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): per rod, its nodes */
       for (int k = 0; k < S->n_nd; k++) {
         const int32_t n = S->nodes[k];
         const double *B = blocks + 36 * n;
@@ -1723,10 +1751,25 @@ static int int_vector(ref_model *m, const double *disp, const double *velo, doub
         }
         skew3(v + 3, Sw);
         mv3(Sw, t, g);
-        for (int i = 0; i < 3; i++) fgyro[6 * n + 3 + i] += g[i];
+        for (int i = 0; i < 3; i++) fgyro[6 * n + 3 + i] += (REF_PLANT == 5 ? -1. : 1.) * g[i];
       }
     }
   }
+#if REF_PLANT == 6
+  if (fgyro)
+    for (int64_t n = 0; n < m->N; n++) {
+      const double *B = blocks + 36 * n, *v = velo + 6 * n;
+      double t[3], Sw[9], g[3];
+      for (int i = 0; i < 3; i++) {
+        double a = B[6 * (3 + i)] * v[0];
+        for (int j = 1; j < 6; j++) a += B[6 * (3 + i) + j] * v[j];
+        t[i] = a;
+      }
+      skew3(v + 3, Sw);
+      mv3(Sw, t, g);
+      for (int i = 0; i < 3; i++) fgyro[6 * n + 3 + i] += g[i];
+    }
+#endif
   return REF_OK;
 }
 #endif

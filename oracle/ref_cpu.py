@@ -69,13 +69,25 @@ def build(omp=False):
     subprocess.check_call(["make", "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
 
 
-def lib(omp=False):
-    name = "libref_cpu_omp.so" if omp else "libref_cpu.so"
+def build_planted(k, outdir):
+    """the oracle built with planted mistake k (REF_PLANT in ref_cpu.c) as
+    outdir/libref_cpu_plant<k>.so, for the tests that show a pin fails on a
+    wrong oracle"""
+    out = os.path.join(outdir, f"libref_cpu_plant{k}.so")
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-std=c99", "-shared",
+                           f"-DREF_PLANT={int(k)}", "-o", out, os.path.join(_HERE, "ref_cpu.c"),
+                           "-lm"])
+    return out
+
+
+def lib(omp=False, path=None):
+    name = path or ("libref_cpu_omp.so" if omp else "libref_cpu.so")
     if name in _LIBS:
         return _LIBS[name]
-    path = os.path.join(_HERE, name)
-    if not os.path.exists(path):
-        build()
+    if path is None:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            build()
     L = C.CDLL(path)
     L.ref_create.restype = C.c_void_p
     L.ref_create.argtypes = [C.POINTER(_Mesh), C.POINTER(_Sections), C.POINTER(_Plastic),
@@ -158,8 +170,8 @@ def expr_eval(expr, t=0.0, i=0.0, omp=False):
 class RefModel:
     """Oracle instance for one Problem (see dynlattice_amd.lattice.Problem)."""
 
-    def __init__(self, p, omp=False, grid=False, init=True):
-        self.L = lib(omp)
+    def __init__(self, p, omp=False, grid=False, init=True, libpath=None):
+        self.L = lib(omp, libpath)
         self.p = p
         self._keep = []
         k = self._keep.append
