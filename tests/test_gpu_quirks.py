@@ -114,12 +114,17 @@ def test_q5_minus_e3_struts_match_oracle(so3):
     for name, kind in (("strain", 0), ("mat_strain", 1), ("stress", 2), ("mat_stress", 3)):
         tr = r.elem_table(name, d)
         assert rel_inf(g.table(name), tr) <= 1e-11, name
-    # a short LeapFrog run from rest (the -I struts resist twist with the
-    # wrong sign, so the horizon stays short)
+    # a short LeapFrog run from the same state: the -I struts resist twist
+    # and bending with the wrong sign, so their highest modes grow by about
+    # e per step in both the oracle and the device, and only a few steps
+    # stay well conditioned
     g2 = GpuModel(p)
+    g2.set_state(d, v)
     r2 = RefModel(p)
-    r2.leapfrog(60)
-    g2.step(60)
+    r2.disp[:] = d.reshape(-1)
+    r2.velo[:] = v.reshape(-1)
+    r2.leapfrog(8)
+    g2.step(8)
     dg, _ = g2.get_state()
     assert rel_inf(dg, r2.disp.reshape(-1, 6)) <= 1e-8
     g.close()
