@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,33 @@
 #include "dl_setup.h"
 
 static thread_local std::string g_err;
+
+// ------------------------------------------------------------------
+// scratch of the allocation-free step (dl_explicit_step_noalloc): the step
+// counter and the per-step history rows live in this array instead of the
+// context's history buffers.  Each GPU holds its own copy; its device
+// address is looked up once per device (dl_create, with the device
+// current) and only that address is used from the host.
+// ------------------------------------------------------------------
+static __device__ __align__(256) double scratch[DL_NOALLOC_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch_dev; // per device ordinal
+
+static int scratch_address(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return DL_ERR_HIP;
+    }
+    g_scratch_dev[device] = (double *)p;
+  }
+  *out = g_scratch_dev[device];
+  return DL_OK;
+}
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -195,6 +223,13 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     if (he != hipSuccess) {
       delete c;
       return fail(DL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(he));
+    }
+  }
+  {
+    const int rc0 = scratch_address(c->device, &c->scratch);
+    if (rc0) {
+      delete c;
+      return rc0;
     }
   }
   if (opt && opt->stream) {
@@ -583,7 +618,7 @@ int check_err(dl_ctx *c) {
 
 // the context's staging buffer for DOF-layout conversions (grown, never
 // shrunk): GET_MATRIX2, set_state and get_state allocate nothing per call
-static int scratch(dl_ctx *c, size_t n_doubles, double **p) {
+static int stage_buffer(dl_ctx *c, size_t n_doubles, double **p) {
   if (c->stage_cap < n_doubles) {
     dfree(c, c->stage);
     c->stage = nullptr;
@@ -605,7 +640,7 @@ extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, i
   c->adj.valid = false;
   hipSetDevice(c->device);
   double *tmp = nullptr;
-  int rc = scratch(c, 2 * ndof, &tmp);
+  int rc = stage_buffer(c, 2 * ndof, &tmp);
   if (rc) return rc;
   hipError_t e1 = hipSuccess, e2 = hipSuccess;
   if (disp) e1 = hipMemcpyAsync(tmp, disp, ndof * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -624,7 +659,7 @@ extern "C" int dl_get_state(dl_ctx *c, double *disp, double *velo, int64_t ndof)
   if (ndof != 6 * c->d.N) return fail(DL_ERR_INPUT, "ndof must be 6 * n_nodes");
   hipSetDevice(c->device);
   double *tmp = nullptr;
-  int rc = scratch(c, 2 * ndof, &tmp);
+  int rc = stage_buffer(c, 2 * ndof, &tmp);
   if (rc) return rc;
   launch_state_io(c->d, c->so3, true, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
   hipError_t e1 = hipSuccess, e2 = hipSuccess;
@@ -955,7 +990,7 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   case DL_GET_MATRIX2: {
     if (!p || !p->matrix2) return fail(DL_ERR_INPUT, "params.matrix2 is required");
     double *blk = nullptr;
-    int rc = scratch(c, 36 * N, &blk);
+    int rc = stage_buffer(c, 36 * N, &blk);
     if (rc) return rc;
     launch_mass_blocks(c->d, c->so3, p->lumped != 0, blk, c->stream);
     std::vector<double> h(36 * N);
@@ -1190,6 +1225,59 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
       HIPCHK(hipMemcpy(&out->n_candidates, c->hist_ncand + nsteps - 1, sizeof(int64_t), hipMemcpyDeviceToHost));
     }
     HIPCHK(hipMemcpy(&out->e_diss, c->diss_total, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return DL_OK;
+}
+
+extern "C" int dl_explicit_step_noalloc(dl_ctx *c, int64_t nsteps, double dt, double *reaction,
+                                        double *e_diss, int64_t *n_contacts) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (!c->initialized) return fail(DL_ERR_STATE, "explicit step before INIT");
+  if (nsteps < 0 || !(dt > 0.0)) return fail(DL_ERR_INPUT, "nsteps >= 0 and dt > 0 required");
+  if (!c->scratch) return fail(DL_ERR_STATE, "no scratch address for this device");
+  const int64_t ng = std::max<int32_t>(c->n_groups, 1);
+  // layout: [0] step counter, then reaction rows [nsteps][ng], dissipation
+  // [nsteps], active pairs [nsteps] and candidates [nsteps] (int64)
+  const int64_t need = 1 + nsteps * (ng + 3);
+  if (need > DL_NOALLOC_SCRATCH_DOUBLES)
+    return fail(DL_ERR_INPUT, "dl_explicit_step_noalloc: nsteps * (n_groups + 3) + 1 = " +
+                                  std::to_string(need) + " exceeds the scratch of " +
+                                  std::to_string(DL_NOALLOC_SCRATCH_DOUBLES) + " doubles");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  double *base = c->scratch;
+  int32_t *ctr = (int32_t *)base;
+  double *hr = base + 1, *hd = hr + nsteps * ng;
+  int64_t *hn = (int64_t *)(hd + nsteps), *hc = hn + nsteps;
+  // the array keeps the last call's rows: clear what this call reads back
+  launch_zero_f64(base, need, c->stream);
+  double *keep_r = c->hist_react, *keep_d = c->hist_diss;
+  int64_t *keep_n = c->hist_nc, *keep_c = c->hist_ncand;
+  int32_t *keep_s = c->step_ctr;
+  c->hist_react = hr;
+  c->hist_diss = hd;
+  c->hist_nc = hn;
+  c->hist_ncand = hc;
+  c->step_ctr = ctr;
+  StepParams sp{dt, 1, c->track_diss ? 1 : 0};
+  int rc = DL_OK;
+  for (int64_t st = 0; st < nsteps && !rc; st++) rc = launch_one_step(c, sp, nullptr);
+  if (!rc && nsteps > 0) rc = launch_reaction_tail(c);
+  c->hist_react = keep_r;
+  c->hist_diss = keep_d;
+  c->hist_nc = keep_n;
+  c->hist_ncand = keep_c;
+  c->step_ctr = keep_s;
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  if ((rc = check_err(c))) return rc;
+  if (nsteps > 0) {
+    if (reaction) HIPCHK(hipMemcpy(reaction, hr, nsteps * ng * sizeof(double), hipMemcpyDeviceToHost));
+    if (e_diss) {
+      if (c->track_diss) HIPCHK(hipMemcpy(e_diss, hd, nsteps * sizeof(double), hipMemcpyDeviceToHost));
+      else std::fill(e_diss, e_diss + nsteps, 0.0);
+    }
+    if (n_contacts) HIPCHK(hipMemcpy(n_contacts, hn, nsteps * sizeof(int64_t), hipMemcpyDeviceToHost));
   }
   return DL_OK;
 }

@@ -133,6 +133,8 @@ struct dl_ctx {
   int32_t strut_blocks = 0;
   // action path energy accumulators
   double *ediss_acc = nullptr, *epot_e = nullptr, *ediss_inc = nullptr;
+  // device address of this GPU's scratch array (dl_explicit_step_noalloc)
+  double *scratch = nullptr;
   // staging buffer of the DOF-layout conversions (GET_MATRIX2, set/get_state)
   double *stage = nullptr;
   size_t stage_cap = 0;

@@ -213,6 +213,8 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
 // of this state): rbuf[k] = the dof share of entry k
 void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s);
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
+// p[0 .. n) = 0 by a kernel (no memset node when a caller captures the stream)
+void launch_zero_f64(double *p, int64_t n, hipStream_t s);
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);
 void launch_mass_blocks(const DevData &d, bool so3, bool lumped, double *blocks, hipStream_t s);
 void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, double *velo,

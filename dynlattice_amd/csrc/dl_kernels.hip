@@ -18,6 +18,8 @@
 // then updates every DOF (ExplicitModule.cpp:339-392, LeapFrogModule.cpp:44-84).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dl_internal.h"
 #include "dl_milne_node.h"
 
@@ -661,6 +663,12 @@ __global__ void k_set_i32(int32_t *p, int32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+__global__ __launch_bounds__(DL_BLOCK) void k_zero_f64(double *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * DL_BLOCK)
+    p[i] = 0.0;
+}
+
 // ------------------------------------------------------------------
 // boundary programs: the scale of every program at (clock time + dt, steps
 // taken), then every program-driven record value = its base x its scale.
@@ -964,6 +972,12 @@ void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStr
 
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, p, v);
+}
+
+void launch_zero_f64(double *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + DL_BLOCK - 1) / DL_BLOCK, 1024);
+  hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(DL_BLOCK), 0, s, p, n);
 }
 
 void launch_bc_program(const ProgDev &P, double dt, const double *dtp, hipStream_t s) {

@@ -36,8 +36,10 @@ EXPORTS = (
     "dl_set_reaction_groups", "dl_set_node_dt", "dl_set_graph", "dl_get_group_history",
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
-    "dl_get_table", "dl_get_energies", "dl_set_impact_groups",
+    "dl_get_table", "dl_get_energies", "dl_set_impact_groups", "dl_explicit_step_noalloc",
 )
+#: DL_NOALLOC_SCRATCH_DOUBLES of include/dl_gpu.h
+NOALLOC_SCRATCH_DOUBLES = 1 << 18
 ELEM_TABLES = ("strain", "mat_strain", "stress", "mat_stress")
 NODE_TABLES = ("potentialEnergy", "kineticEnergy", "mass")
 DL_PROG_CONST, DL_PROG_STEP, DL_PROG_EXPR = 0, 1, 2
@@ -160,6 +162,7 @@ def load():
     L.dl_set_bcs.argtypes = [C.c_void_p, u8p, dp, dp, C.c_int64]
     L.dl_set_reaction_group.argtypes = [C.c_void_p, ip32, C.c_int64, C.c_int32]
     L.dl_explicit_step.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(dl_step_out)]
+    L.dl_explicit_step_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, dp, dp, ip64]
     L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
     L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
     L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
@@ -366,6 +369,19 @@ class GpuModel:
         self._check(self.L.dl_explicit_step(self.h, int(nsteps), float(dt),
                                             C.byref(o) if out else None))
         return o
+
+    def step_noalloc(self, nsteps, dt=None):
+        """nsteps steps through the allocation-free entry point; returns its
+        histories (reaction [nsteps, n_groups], e_diss, n_contacts)"""
+        dt = self.p.dt if dt is None else dt
+        ng = max(getattr(self, "_ngroups", 1), 1)
+        r = np.zeros(max(nsteps, 1) * ng)
+        e = np.zeros(max(nsteps, 1))
+        c = np.zeros(max(nsteps, 1), np.int64)
+        self._check(self.L.dl_explicit_step_noalloc(self.h, int(nsteps), float(dt), _p(r), _p(e),
+                                                    _p(c, ip64)))
+        return {"reaction": r[: nsteps * ng].reshape(-1, ng), "e_diss": e[:nsteps],
+                "n_contacts": c[:nsteps]}
 
     def history(self, nsteps):
         r = np.zeros(nsteps)

@@ -234,6 +234,26 @@ int dl_set_graph(dl_ctx *ctx, int32_t steps_per_graph);
  * error again while they still overflow). */
 int dl_explicit_step(dl_ctx *ctx, int64_t nsteps, double dt, dl_step_out *out);
 
+/* Allocation-free variant of dl_explicit_step, for callers that cannot
+ * allocate device memory (code that runs under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same steps and the same histories as dl_explicit_step, with
+ * direct launches (no internal graph); the step counter and the history
+ * rows live in a fixed per-GPU scratch array of DL_NOALLOC_SCRATCH_DOUBLES
+ * doubles (2 MiB, held by every GPU the library runs on until the process
+ * ends) instead of the context's history buffers, so no hipMalloc, hipFree
+ * or allocating library call happens.  nsteps * (n_groups + 3) + 1 must not
+ * exceed DL_NOALLOC_SCRATCH_DOUBLES (else DL_ERR_INPUT).  The rows are copied
+ * out before returning (after a stream synchronisation): reaction
+ * [nsteps][max(n_groups, 1)], e_diss [nsteps], n_contacts [nsteps], each may
+ * be NULL.  dl_get_history keeps describing the last dl_explicit_step call.
+ * All contexts on one GPU share its scratch array: calls that overlap on one
+ * GPU (other threads, other streams) race and must be serialised by the
+ * caller. */
+#define DL_NOALLOC_SCRATCH_DOUBLES (1 << 18)
+int dl_explicit_step_noalloc(dl_ctx *ctx, int64_t nsteps, double dt, double *reaction,
+                             double *e_diss, int64_t *n_contacts);
+
 /* Per-step history of the last dl_explicit_step call: GroupOutput reaction
  * after the step, dissipation after the step, active contact pairs of the
  * step's force evaluation (its start state); n = number of steps recorded. */
