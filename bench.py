@@ -86,13 +86,25 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(p, budget_s, steps_gpu, omp):
+def cpu_baseline(p, budget_s, steps_gpu, omp, model=None):
     """The oracle (C restatement, oracle/ref_cpu.c) on this host, bounded
     sample: the same lattice, K steps with K chosen so the timed part takes
     about budget_s.  omp: the all-cores build (struts and pairs threaded
-    with OpenMP) on cpu_threads() threads; else the serial build."""
+    with OpenMP) on cpu_threads() threads; else the serial build.  model: a
+    GpuModel whose current state (displacements, velocities, plastic state)
+    the oracle starts from, i.e. the state the GPU steps were timed at."""
     from oracle.ref_cpu import RefModel
+    t_all = time.perf_counter()
     ref = RefModel(p, grid=True, omp=omp)
+    state = "the start of the run"
+    if model is not None:
+        d, v = model.get_state()
+        ref.disp[:] = d.reshape(-1)
+        ref.velo[:] = v.reshape(-1)
+        if p.plastic is not None:
+            ep, al = model.plastic_state()
+            ref.set_plastic_state(ep, al)
+        state = "the GPU's timed state"
     t0 = time.perf_counter()
     ref.leapfrog(1)
     t1 = time.perf_counter() - t0
@@ -103,10 +115,10 @@ def cpu_baseline(p, budget_s, steps_gpu, omp):
     cores = cpu_threads() if omp else 1
     build = "OpenMP over struts and contact pairs" if omp else "1 thread"
     return {"value": p.n_elems * k / dt, "unit": "element-GP updates/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "wall_s": round(time.perf_counter() - t_all, 2),
             "sample": f"oracle (oracle/ref_cpu.c, gcc -O3 -ffp-contract=off, {build}, "
-                      f"{cores} thread(s)) on the same {p.name} lattice, {k} LeapFrog steps "
-                      f"after 1 warm-up step, {dt:.1f} s"
+                      f"{cores} thread(s)) on the same {p.name} lattice from {state}, "
+                      f"{k} LeapFrog steps after 1 warm-up step, {dt:.1f} s"
                       + ("; contact broadphase on the oracle's uniform grid (candidate sets "
                          "equal to the reference's all-pairs loops, which cannot run at this "
                          "size)" if p.contact is not None else "")}
@@ -115,16 +127,21 @@ def cpu_baseline(p, budget_s, steps_gpu, omp):
 PROFILE = os.path.join(ROOT, "profiles", "bcc40_elastoplastic_contact.md")
 
 
-def profiled_traffic(workload):
-    """HBM bytes per k_strut launch measured by rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE, scaled by the copy calibration) and recorded in the profile
-    summary of this workload, or None."""
+def profiled_counters(workload):
+    """Figures of the k_strut launch that rocprofv3 --pmc runs measured and
+    the profile summary of this workload records: HBM bytes per launch
+    (FETCH_SIZE, WRITE_SIZE, scaled by the copy calibration) as "traffic",
+    FP64 operations per launch (the FP64 VALU instruction counters) as
+    "fp64_flops"; {} when there is no summary."""
+    out = {}
     if workload != "bcc-40x40x40-m4-elastoplastic-contact" or not os.path.exists(PROFILE):
-        return None
+        return out
     for line in open(PROFILE):
         if line.startswith("k_strut HBM bytes per launch:"):
-            return int(line.split(":")[1])
-    return None
+            out["traffic"] = int(line.split(":")[1])
+        if line.startswith("k_strut FP64 operations per launch:"):
+            out["fp64_flops"] = float(line.split(":")[1])
+    return out
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=5):
@@ -232,16 +249,23 @@ def sweep(args):
 
 
 def adjoint(args):
-    """Adjoint design gradient of the 20^3 BCC impact with contact: gradient
-    of J = absorbed energy + peak plate force with respect to all strut
-    radii, by the checkpointed forward run and the reverse sweep; reported as
-    (forward with checkpoints + reverse sweep) / plain forward time, with a
-    central finite-difference spot check of the GPU forward run."""
+    """Adjoint design gradient of the 20^3 BCC impact with contact (see
+    adjoint_measure), with a central finite-difference spot check."""
+    print(json.dumps(adjoint_measure(args.adjoint_n, args.adjoint_steps, args.checkpoint_every,
+                                     args.fd_struts)), flush=True)
+
+
+def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0):
+    """Adjoint design gradient of the n_cells^3 BCC impact with contact:
+    gradient of J = absorbed energy + peak plate force with respect to all
+    strut radii, by the checkpointed forward run and the reverse sweep;
+    reported as (forward with checkpoints + reverse sweep) / plain forward
+    time, with a central finite-difference spot check of the GPU forward run
+    on fd_struts struts."""
     from dynlattice_amd import lattice as L
     from dynlattice_amd.adjoint import ImpactSimulation
     from dynlattice_amd.gpu import SECTION_FIELDS
-    p = L.bcc_c5(args.adjoint_n)
-    n, every = args.adjoint_steps, args.checkpoint_every
+    p = L.bcc_c5(n_cells)
     print(f"adjoint: {p.name}: {p.n_elems} elements, {p.n_struts} struts, {n} steps, "
           f"checkpoint every {every}", file=sys.stderr, flush=True)
     sim = ImpactSimulation(p, n, every)
@@ -280,7 +304,7 @@ def adjoint(args):
             g["J"] * 2 * pi * r0 ** 3 + g["contact_radius"])
     # finite-difference spot check on the GPU forward run
     fd = []
-    for s in np.random.default_rng(5).choice(p.n_struts, args.fd_struts, replace=False):
+    for s in np.random.default_rng(5).choice(p.n_struts, fd_struts, replace=False):
         h = 1e-6 * r0[s]
         Js = []
         for sg in (1, -1):
@@ -299,7 +323,7 @@ def adjoint(args):
                       "steps": n, "checkpoint_every": every, "design_variables": p.n_struts},
            "t_forward_plain_s": t_plain, "t_forward_checkpointed_s": t_fwd,
            "t_reverse_s": t_bwd, "objective": J0, "fd_check": fd}
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def milne(args):
@@ -380,6 +404,105 @@ def milne(args):
     print(json.dumps(res), flush=True)
 
 
+def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0, acc=None):
+    """Runs the crush (top plate at its prescribed velocity) through the given
+    nominal strains and times a window of `window` steps at each: wall time
+    per step, kernel phases (HIP events), plastic-active fraction, active
+    contact pairs per step, candidate-list rebuilds.  With cpu_budget > 0 the
+    all-cores oracle is timed at the same state (state and plastic state
+    copied over) on a sample of about cpu_budget seconds.  start: steps the
+    model has taken; acc: {"steps", "wall_s"} accumulating the ordinary
+    step calls (not the event-timed passes)."""
+    H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
+    v = float(np.abs(p.velo0[p.groups["top"], 2]).max())
+    out = []
+    done = start
+    acc = {} if acc is None else acc
+
+    def stepped(k, wall):
+        acc["steps"] = acc.get("steps", 0) + k
+        acc["wall_s"] = acc.get("wall_s", 0.0) + wall
+
+    for eps in strains:
+        target = int(round(eps * H / v / p.dt))
+        while done < target:
+            k = min(4000, target - done)
+            t0 = time.perf_counter()
+            g.step(k)  # with an output: device errors surface per chunk
+            stepped(k, time.perf_counter() - t0)
+            done += k
+        g.sync()
+        t0 = time.perf_counter()
+        o = g.step(window)
+        wall = time.perf_counter() - t0
+        stepped(window, wall)
+        h = g.history(window)
+        act = g.plastic_active()
+        g.set_timing(True)
+        g.step(window, out=False)
+        g.sync()
+        ph = {}
+        for name in ("step", "contact", "refresh", "react", "strut", "joint"):
+            ms, cnt = g.phase_time(name)
+            ph[name] = round(ms / max(cnt, 1), 5)
+        g.set_timing(False)
+        done += 2 * window
+        w = {"strain": eps, "step": done, "ms_per_step": 1e3 * wall / window,
+             "value": p.n_elems * window / wall, "phase_ms": ph,
+             "plastic_active_fraction": float(act.mean()),
+             "active_pairs_per_step": float(h["n_contacts"].mean()),
+             "rebuilds_per_window": int(o.n_rebuilds),
+             "candidates_last_step": int(o.n_candidates),
+             "list_entries": int(g.contact_list_stats()[0]) if p.contact is not None else 0,
+             "dissipation_J": float(h["e_diss"][-1])}
+        if cpu_budget > 0:
+            from oracle.ref_cpu import RefModel
+            t_cpu = time.perf_counter()
+            d, vv = g.get_state()
+            ep, al = g.plastic_state()
+            ref = RefModel(p, grid=True, omp=True)
+            ref.disp[:] = d.reshape(-1)
+            ref.velo[:] = vv.reshape(-1)
+            ref.set_plastic_state(ep, al)
+            t1 = time.perf_counter()
+            ref.leapfrog(1)
+            t1 = time.perf_counter() - t1
+            k = int(max(1, min(window, cpu_budget / max(t1, 1e-9))))
+            t1 = time.perf_counter()
+            ref.leapfrog(k)
+            t1 = time.perf_counter() - t1
+            w["cpu_baseline"] = {"value": p.n_elems * k / t1, "unit": "element-GP updates/s",
+                                 "cores": cpu_threads(), "kind": "port",
+                                 "sample": f"oracle, all cores, {k} steps from this state"}
+            w["speedup_vs_cpu"] = w["value"] / w["cpu_baseline"]["value"]
+            w["cpu_s"] = time.perf_counter() - t_cpu
+        print(f"bench: strain {eps:.2f} (step {done}): {w['ms_per_step']:.4f} ms/step, "
+              f"plastic {w['plastic_active_fraction']:.3f}, pairs {w['active_pairs_per_step']:.0f}, "
+              f"rebuilds {w['rebuilds_per_window']}", file=log, flush=True)
+        out.append(w)
+    return out
+
+
+def crush(args):
+    """The crush through 10 %, 20 % and 40 % nominal strain with a timed window at each"""
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
+    p = workload(args.workload)
+    if args.max_iter and p.plastic is not None:
+        p.plastic.max_iter = args.max_iter
+    print(f"crush: {p.name}: {p.n_elems} elements, dt {p.dt:.3e}", file=sys.stderr, flush=True)
+    g = GpuModel(p, device=0, flags=DL_TRACK_DISSIPATION)
+    g.set_reaction_group(p.groups["bottom"], 2)
+    g.set_graph(10)
+    strains = [float(x) for x in args.strains.split(",")]
+    res = {"metric": "element-GP updates/s per explicit step (FP64), crush windows",
+           "config": {"workload": p.name, "elements": p.n_elems, "dt": p.dt,
+                      "yield": yield_family(p), "window_steps": args.steps},
+           "windows": crush_windows(g, p, strains, args.steps,
+                                    0.0 if args.no_cpu else args.cpu_budget)}
+    g.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sweep", action="store_true",
@@ -396,6 +519,11 @@ def main():
     ap.add_argument("--milne-precision", type=float, default=1e-4)
     ap.add_argument("--milne-graph", type=int, default=10,
                     help="attempts per captured graph (0: direct launches)")
+    ap.add_argument("--crush", action="store_true",
+                    help="the crush through --strains with a timed window at each")
+    ap.add_argument("--strains", default="0.1,0.2,0.4")
+    ap.add_argument("--max-iter", type=int, default=0,
+                    help="return-map iteration cap (the material's maxIter; 0: the workload's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,6 +534,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the per-kernel event timing pass")
+    ap.add_argument("--no-crush", action="store_true",
+                    help="time the steps at the start of the run instead of in the crush")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward time ratio (20^3 lattice, 5000 steps)")
+    ap.add_argument("--window", type=int, default=200, help="steps per crush window")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -418,6 +551,9 @@ def main():
         return
     if args.milne:
         milne(args)
+        return
+    if args.crush:
+        crush(args)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus > 1 is not supported "
@@ -435,23 +571,36 @@ def main():
         g.set_reaction_group(p.groups["bottom"], 2)
     from dynlattice_amd.batch import autotune_graph
     graph_steps, tune = autotune_graph(g, candidates=(0, 10), steps=10)
+    crushing = p.contact is not None and p.plastic is not None and not args.no_crush
+    windows = []
+    acc = {}
+    if crushing:
+        # the timed steps sit in the crush where it works: 20 % nominal strain
+        windows += crush_windows(g, p, [0.1, 0.2], args.window,
+                                 0.0 if args.no_cpu else args.cpu_budget / 2, acc=acc)
+    t0 = time.perf_counter()
     g.step(args.warmup, out=False)
     g.sync()
+    t_w = time.perf_counter() - t0
     t0 = time.perf_counter()
     o = g.step(args.steps, out=False)
     g.sync()
     wall = time.perf_counter() - t0
     o = g.step(0)  # error check only
+    acc["steps"] = acc.get("steps", 0) + args.warmup + args.steps
+    acc["wall_s"] = acc.get("wall_s", 0.0) + t_w + wall
     value = p.n_elems * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
+    h_act = g.plastic_active().mean() if p.plastic is not None else 0.0
 
     roof = None
+    step_roof = None
     phases = {}
     if not args.no_timing:
         g.set_timing(True)
         g.step(args.steps, out=False)
         g.sync()
-        for ph in ("step", "contact", "strut", "joint"):
+        for ph in ("step", "contact", "refresh", "react", "strut", "joint"):
             ms, cnt = g.phase_time(ph)
             phases[ph] = ms / max(cnt, 1)
         g.set_timing(False)
@@ -460,7 +609,8 @@ def main():
         bytes_strut = b_elem * p.n_elems + B_NODE * npriv
         t_strut = phases["strut"] * 1e-3
         achieved = bytes_strut / t_strut / 1e9
-        traffic = profiled_traffic(p.name)
+        prof = profiled_counters(p.name)
+        traffic = prof.get("traffic")
         copy_gbs = copy_bandwidth()
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -472,6 +622,17 @@ def main():
             actual = traffic / t_strut / 1e9
             roof["actual_GBs"] = round(actual, 1)
             roof["actual_frac_of_device_copy"] = round(actual / copy_gbs, 4)
+        if prof.get("fp64_flops"):
+            tf = prof["fp64_flops"] / t_strut / 1e12
+            roof["fp64_TFLOPs"] = round(tf, 3)
+            roof["fp64_frac"] = round(tf / FP64_PEAK_TFLOPS, 4)
+        # the whole step against the same bound: every element's and node's
+        # algorithmic bytes (SURVEY figures) over the measured step time
+        b_step = b_elem * p.n_elems + B_NODE * p.n_nodes
+        a_step = b_step / (ms_per_step * 1e-3) / 1e9
+        step_roof = {"bound": "hbm", "achieved": round(a_step, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(a_step / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_step": b_step}
 
     res = {
         "metric": "element-GP updates/s per explicit step (FP64)",
@@ -490,18 +651,41 @@ def main():
                    "struts": p.n_struts, "dt": p.dt,
                    "material": "elastoplastic" if p.plastic is not None else "elastic",
                    "contact": p.contact is not None, "so3": bool(p.so3),
-                   "yield": yield_family(p)},
+                   "yield": yield_family(p),
+                   "state": ("crush at 20 % nominal strain" if crushing else "start of the run")},
         "roofline": roof,
+        "step_roofline": step_roof,
         "graph_steps": graph_steps,
         "phase_ms": {k: round(v, 5) for k, v in phases.items()},
+        "plastic_active_fraction": float(h_act),
     }
     if not args.no_cpu:
-        cb = cpu_baseline(p, args.cpu_budget, args.steps, omp=True)
-        c1 = cpu_baseline(p, args.cpu_budget, args.steps, omp=False)
+        cb = cpu_baseline(p, args.cpu_budget, args.steps, omp=True, model=g)
+        c1 = cpu_baseline(p, args.cpu_budget, args.steps, omp=False, model=g)
         res["cpu_baseline"] = cb
         res["speedup_vs_cpu"] = value / cb["value"]
         res["cpu_baseline_serial"] = c1
         res["speedup_vs_cpu_serial"] = value / c1["value"]
+    if crushing:
+        try:
+            start = windows[-1]["step"] + args.warmup + (1 if args.no_timing else 2) * args.steps
+            windows += crush_windows(g, p, [0.4], args.window,
+                                     0.0 if args.no_cpu else args.cpu_budget / 2, start=start,
+                                     acc=acc)
+            # the crush from 0 to 40 % strain: the ordinary step calls (list
+            # rebuilds and windows included, event-timed passes excluded)
+            res["whole_crush"] = {"steps": acc["steps"], "wall_s": round(acc["wall_s"], 3),
+                                  "value": p.n_elems * acc["steps"] / acc["wall_s"],
+                                  "last_step": windows[-1]["step"]}
+        except Exception as e:  # report, keep the line
+            res["crush_error"] = str(e)
+        res["crush_windows"] = windows
+    g.close()
+    if not args.no_adjoint:
+        a = adjoint_measure(20, 5000, 100, 0)
+        res["adjoint_forward_ratio"] = round(a["value"], 3)
+        res["adjoint"] = {k: a[k] for k in ("config", "t_forward_plain_s",
+                                            "t_forward_checkpointed_s", "t_reverse_s")}
     print(json.dumps(res), flush=True)
 
 

@@ -1060,6 +1060,7 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   if (ph == "step") v = c->t_step;
   else if (ph == "contact") v = c->t_contact;
   else if (ph == "react") v = c->t_react;
+  else if (ph == "refresh") v = c->t_refresh;
   else if (ph == "strut") v = c->t_strut;
   else if (ph == "joint") v = c->t_joint;
   else if (ph == "node") v = c->t_node;
@@ -1069,9 +1070,10 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
   return DL_OK;
 }
 
-// events per timed step: start, after contact, after the reaction entries,
-// after the strut kernel, after the joint (and impact) kernels
-#define STEP_EVENTS 5
+// events per timed step: start, after the candidate-list refresh, after
+// contact, after the reaction entries, after the strut kernel, after the
+// joint (and impact) kernels
+#define STEP_EVENTS 6
 
 // one explicit step: kernels only (capturable into a hipGraph)
 int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
@@ -1079,18 +1081,20 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   // boundary values at the end time of the step (after advance)
   launch_bc_program(c->pd, sp.dt, nullptr, c->stream);
   if (c->contact) {
-    int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc, c->hist_ncand, c->step_ctr);
+    int rc = contact_step(c->cs, c->d, c->stream, c->hist_nc, c->hist_ncand, c->step_ctr,
+                          ev ? ev[1] : nullptr);
     if (rc) return rc;
   }
-  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+  if (ev && !(c->contact && c->cs.c.skin > 0.0)) HIPCHK(hipEventRecord(ev[1], c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
   // the previous step's GroupOutput row, at this step's start state
   launch_react(c->d, c->so3, c->plastic, c->contact, c->stream);
-  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
-  launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
+  if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));
   launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
   launch_impact_step(c->d, sp, c->imp, c->so3, c->stream);
-  if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[5], c->stream));
   launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
                   c->hist_react, c->hist_diss, c->step_ctr, c->stream);
   launch_clock_advance(c->pd, sp.dt, c->stream);
@@ -1178,6 +1182,11 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     if (rc) return rc;
   }
   StepParams sp{dt, 1, c->track_diss ? 1 : 0};
+  int64_t builds0 = 0;
+  if (out && c->contact) {
+    int rc = contact_rebuilds(c->cs, c->stream, &builds0);
+    if (rc) return rc;
+  }
   if (c->timing) {
     const int K = STEP_EVENTS;
     std::vector<hipEvent_t> ev(K * (size_t)nsteps + 1);
@@ -1188,15 +1197,16 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     }
     HIPCHK(hipEventRecord(ev[K * nsteps], c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->t_step = c->t_contact = c->t_react = c->t_strut = c->t_joint = c->t_node = 0.0;
+    c->t_step = c->t_contact = c->t_refresh = c->t_react = c->t_strut = c->t_joint = c->t_node = 0.0;
     for (int64_t st = 0; st < nsteps; st++) {
       float t[STEP_EVENTS], all;
       for (int i = 0; i < K; i++) HIPCHK(hipEventElapsedTime(&t[i], ev[K * st + i], ev[K * st + i + 1]));
       HIPCHK(hipEventElapsedTime(&all, ev[K * st], ev[K * st + K]));
-      c->t_contact += t[0];
-      c->t_react += t[1];
-      c->t_strut += t[2];
-      c->t_joint += t[3];
+      c->t_refresh += t[0];
+      c->t_contact += t[0] + t[1];
+      c->t_react += t[2];
+      c->t_strut += t[3];
+      c->t_joint += t[4];
       c->t_step += all;
     }
     c->t_count = nsteps;
@@ -1215,6 +1225,12 @@ extern "C" int dl_explicit_step(dl_ctx *c, int64_t nsteps, double dt, dl_step_ou
     int rc = check_err(c);
     if (rc) return rc;
     out->steps_done = nsteps;
+    out->n_rebuilds = 0;
+    if (c->contact) {
+      int64_t builds1 = 0;
+      if ((rc = contact_rebuilds(c->cs, c->stream, &builds1))) return rc;
+      out->n_rebuilds = builds1 - builds0;
+    }
     out->reaction = 0.0;
     out->e_diss = 0.0;
     out->n_contacts = 0;
@@ -1357,6 +1373,18 @@ extern "C" int dl_contact_pairs(dl_ctx *c, int32_t *a, int32_t *b, int64_t cap, 
     return DL_OK;
   }
   return contact_read_pairs(c->cs, c->stream, true, a, b, cap, n);
+}
+
+extern "C" int dl_contact_list_stats(dl_ctx *c, int64_t *entries, int64_t *builds) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  if (entries) *entries = 0;
+  if (builds) *builds = 0;
+  if (!c->contact) return DL_OK;
+  hipSetDevice(c->device);
+  int rc;
+  if (entries && (rc = contact_list_entries(c->cs, c->stream, entries))) return rc;
+  if (builds && (rc = contact_rebuilds(c->cs, c->stream, builds))) return rc;
+  return DL_OK;
 }
 
 extern "C" int dl_contact_candidates(dl_ctx *c, int32_t *a, int32_t *b, int64_t cap, int64_t *n) {

@@ -63,6 +63,7 @@ struct ContactDev {
   int32_t *vlist;       // [E][vk] partner ids
   int32_t vk;           // list capacity per element
   int32_t *err;         // device error flag (capacity)
+  unsigned long long *nbuild; // [1] list builds so far
 };
 
 struct ContactState {
@@ -90,12 +91,16 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s);
 // evaluation inside the explicit step: per-element forces + counts added to
 // n_active[*step], n_cand[*step] (kernels only: safe inside a captured graph)
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand, const int32_t *step);
+                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh = nullptr);
+// candidate-list entries summed over the elements (after a stream sync)
+int contact_list_entries(ContactState &cs, hipStream_t s, int64_t *n);
 // adjoint of the contact forces at the state d (the taped start state of a
 // step): broadphase, then per closed pair the VJP into a.ebuf and a.crbar
 int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s);
 // marks the candidate lists stale (the state was set from outside the step)
 void contact_invalidate(ContactState &cs, hipStream_t s);
+// candidate-list builds since the context was created (after a stream sync)
+int contact_rebuilds(ContactState &cs, hipStream_t s, int64_t *n);
 // replaces the per-strut contact radii (host array of n_struts values)
 int contact_set_radius(ContactState &cs, const double *r, hipStream_t s);
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,

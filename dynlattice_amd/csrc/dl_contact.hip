@@ -534,6 +534,7 @@ __device__ __forceinline__ int32_t list_partner(int32_t v) { return v < 0 ? ~v :
 
 __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(c.nbuild, 1ull);
   GRID_STRIDE(k, c.E) {
     const CRec x = ((const CRec *)c.rec)[k];
     const int32_t X = x.e;
@@ -919,6 +920,8 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     if (vk > INT32_MAX) return dl_set_error(DL_ERR_INPUT, "max_pairs too large");
     c.vk = (int32_t)vk;
     if ((rc = calloc_dev(cs, &c.gate, 1))) return rc;
+    if ((rc = calloc_dev(cs, &c.nbuild, 1))) return rc;
+    CHK(hipMemset(c.nbuild, 0, sizeof(unsigned long long)));
     if ((rc = calloc_dev(cs, &c.uref, 3 * N))) return rc;
     if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
     if ((rc = calloc_dev(cs, &c.vlist, (size_t)E * c.vk))) return rc;
@@ -1064,7 +1067,7 @@ static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
 }
 
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand, const int32_t *step) {
+                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh) {
   ContactDev &c = cs.c;
   if (!(c.skin > 0.0)) {
     // no lists: full broadphase and cell scan every step
@@ -1077,6 +1080,7 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
   }
   // list rebuild, gated on the trigger (no-ops while the lists are valid)
   list_refresh(cs, d, s);
+  if (after_refresh) CHK(hipEventRecord(after_refresh, s));
   const int64_t nthr = (c.E + VN_PER - 1) / VN_PER;
   hipLaunchKernelGGL(kc_vnarrow, dim3((unsigned)((nthr + CB - 1) / CB)), dim3(CB), 0, s, c, d,
                      (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
@@ -1087,6 +1091,28 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
 void contact_invalidate(ContactState &cs, hipStream_t s) {
   if (cs.c.skin > 0.0) hipLaunchKernelGGL(kc_fill_i32, dim3(1), dim3(CB), 0, s, cs.c.gate, 1, 1,
                                           (const int32_t *)nullptr);
+}
+
+int contact_list_entries(ContactState &cs, hipStream_t s, int64_t *n) {
+  *n = 0;
+  if (!(cs.c.skin > 0.0)) return DL_OK;
+  std::vector<int32_t> h(cs.c.E);
+  CHK(hipStreamSynchronize(s));
+  CHK(hipMemcpy(h.data(), cs.c.vcount, sizeof(int32_t) * cs.c.E, hipMemcpyDeviceToHost));
+  int64_t t = 0;
+  for (int32_t v : h) t += v;
+  *n = t;
+  return DL_OK;
+}
+
+int contact_rebuilds(ContactState &cs, hipStream_t s, int64_t *n) {
+  *n = 0;
+  if (!(cs.c.skin > 0.0)) return DL_OK;
+  unsigned long long v = 0;
+  CHK(hipMemcpyAsync(&v, cs.c.nbuild, sizeof v, hipMemcpyDeviceToHost, s));
+  CHK(hipStreamSynchronize(s));
+  *n = (int64_t)v;
+  return DL_OK;
 }
 
 int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a, int32_t *b,

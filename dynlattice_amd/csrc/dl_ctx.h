@@ -140,7 +140,7 @@ struct dl_ctx {
   size_t stage_cap = 0;
   // timing
   bool timing = false;
-  double t_step = 0, t_contact = 0, t_react = 0, t_strut = 0, t_joint = 0, t_node = 0;
+  double t_step = 0, t_contact = 0, t_refresh = 0, t_react = 0, t_strut = 0, t_joint = 0, t_node = 0;
   int64_t t_count = 0;
   // contact
   ContactState cs{};

@@ -37,6 +37,7 @@ EXPORTS = (
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
     "dl_get_table", "dl_get_energies", "dl_set_impact_groups", "dl_explicit_step_noalloc",
+    "dl_contact_list_stats",
 )
 #: DL_NOALLOC_SCRATCH_DOUBLES of include/dl_gpu.h
 NOALLOC_SCRATCH_DOUBLES = 1 << 18
@@ -102,7 +103,7 @@ class dl_section_grads(C.Structure):
 
 class dl_step_out(C.Structure):
     _fields_ = [("reaction", C.c_double), ("e_diss", C.c_double), ("n_contacts", C.c_int64),
-                ("n_candidates", C.c_int64), ("steps_done", C.c_int64)]
+                ("n_candidates", C.c_int64), ("steps_done", C.c_int64), ("n_rebuilds", C.c_int64)]
 
 
 class dl_milne_opts(C.Structure):
@@ -163,6 +164,7 @@ def load():
     L.dl_set_reaction_group.argtypes = [C.c_void_p, ip32, C.c_int64, C.c_int32]
     L.dl_explicit_step.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(dl_step_out)]
     L.dl_explicit_step_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, dp, dp, ip64]
+    L.dl_contact_list_stats.argtypes = [C.c_void_p, ip64, ip64]
     L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
     L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
     L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
@@ -494,6 +496,12 @@ class GpuModel:
         b = np.zeros(max(k, 1), np.int32)
         self._check(fn(self.h, _p(a, ip32), _p(b, ip32), k, C.byref(n)))
         return a[:k], b[:k]
+
+    def contact_list_stats(self):
+        """(partners listed over all elements, list builds so far)"""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.L.dl_contact_list_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def contact_pairs(self):
         return self._pairs(self.L.dl_contact_pairs)

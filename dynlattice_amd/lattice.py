@@ -359,7 +359,11 @@ def bcc_c3(n: int = 40, contact: bool = True) -> Problem:
     (lognormal, mean 0.4 mm), seed 3, E = 70 GPa, nu = 0.3, rho = 2700,
     yield 250 MPa (quadratic interaction of the resultants, p = 2), kinematic
     hardening H = C/100,
-    penalty contact 1e6 N/m; top plate at -20 m/s."""
+    penalty contact 1e6 N/m; top plate at -20 m/s.  dt = 0.25 x the axial
+    CFL bound: at 0.5 x CFL the SO(3) explicit update of the reference
+    algorithm diverges near 35 % nominal strain (nodal spins grow to 1e6
+    rad/s and the return map stops converging; the oracle agrees with the
+    device there), at 0.25 x CFL the crush runs to 40 %."""
     m, cell = 4, 10e-3
     R = 8 * n ** 3
     r = 0.4e-3 * np.exp(0.1 * normal(3, R) - 0.005)
@@ -368,7 +372,7 @@ def bcc_c3(n: int = 40, contact: bool = True) -> Problem:
     if contact:
         p.contact = Contact(radius=r.copy(), penalty_sts=1e6, penalty_nts=1e6)
     crush_bcs(p, -20.0)
-    p.dt = stable_dt(p, 0.5)
+    p.dt = stable_dt(p, 0.25)
     p.name = f"bcc-{n}x{n}x{n}-m4-elastoplastic-contact" if contact else \
         f"bcc-{n}x{n}x{n}-m4-elastoplastic"
     return p

@@ -152,6 +152,7 @@ typedef struct {
   int64_t n_contacts;   /* active contact pairs in the last step */
   int64_t n_candidates; /* broadphase candidates in the last step */
   int64_t steps_done;
+  int64_t n_rebuilds;   /* contact candidate-list rebuilds during the call */
 } dl_step_out;
 
 /* Replaces: model tree construction by factories (ModelFactory "Lattice" ->
@@ -267,6 +268,11 @@ int dl_get_group_history(dl_ctx *ctx, double *reaction, int64_t cap_steps, int64
 int dl_contact_pairs(dl_ctx *ctx, int32_t *a, int32_t *b, int64_t cap, int64_t *n);
 /* Broadphase candidates of the last force evaluation, sorted by (a, b). */
 int dl_contact_candidates(dl_ctx *ctx, int32_t *a, int32_t *b, int64_t cap, int64_t *n);
+/* The step's candidate lists: partners listed, summed over the elements
+ * (compare with dl_contact.max_pairs, the capacity), and list builds since
+ * the context was created.  Either pointer may be NULL; zeros without
+ * lists. */
+int dl_contact_list_stats(dl_ctx *ctx, int64_t *entries, int64_t *builds);
 
 /* Replaces: the plastic-strain table of ElastoPlasticRodMaterial::getTable
  * (ElastoPlasticRodMaterial.cpp:318-340): per element the plastic-correction
@@ -277,10 +283,11 @@ int dl_plastic_state(dl_ctx *ctx, double *eps_p, double *alpha, int64_t n_elems)
 
 /* Device timing of the kernels of the last dl_explicit_step (or
  * dl_milne_step) call, measured with HIP events on the context's stream:
- * total milliseconds of the named phase ("step", "contact", "strut",
- * "joint"; for MilneDevice both force evaluations of an attempt, and "node":
- * predictor, corrector, step control and commit) and the number of steps
- * (attempts). */
+ * total milliseconds of the named phase ("step", "contact" -- of which
+ * "refresh" is the candidate-list rebuild chain --, "react" (the reaction
+ * groups' entries), "strut", "joint"; for MilneDevice both force
+ * evaluations of an attempt, and "node": predictor, corrector, step control
+ * and commit) and the number of steps (attempts). */
 int dl_phase_time(dl_ctx *ctx, const char *phase, double *ms, int64_t *count);
 int dl_set_timing(dl_ctx *ctx, int32_t enabled);
 

@@ -2277,6 +2277,21 @@ int ref_plastic_state(ref_model *m, double *eps_p, double *alpha, double *dflow,
   return REF_OK;
 }
 
+/* sets the committed (and current) plastic strain [E][6] and hardening
+ * parameters [E][7], e.g. to continue from a state another run reached */
+int ref_set_plastic_state(ref_model *m, const double *eps_p, const double *alpha) {
+  const int64_t E = m->E;
+  if (eps_p) {
+    memcpy(m->old_ep, eps_p, 6 * E * sizeof(double));
+    memcpy(m->curr_ep, eps_p, 6 * E * sizeof(double));
+  }
+  if (alpha) {
+    memcpy(m->old_al, alpha, 7 * E * sizeof(double));
+    memcpy(m->curr_al, alpha, 7 * E * sizeof(double));
+  }
+  return REF_OK;
+}
+
 int ref_curr_strains(ref_model *m, double *strain) {
   memcpy(strain, m->curr_strain, 6 * m->E * sizeof(double));
   return REF_OK;

@@ -204,6 +204,7 @@ int64_t ref_blacklist(ref_model *m, int32_t *a, int32_t *b, int64_t cap);
 
 /* Plastic state read-out: per element eps_p[6], alpha[7] (iso, kin...),
  * delta_flow, and the 'plastic active' flag of the last material call. */
+int ref_set_plastic_state(ref_model *m, const double *eps_p, const double *alpha);
 int ref_plastic_state(ref_model *m, double *eps_p, double *alpha,
                       double *dflow, uint8_t *active);
 /* Material strains of the last call (currStrains_), [6*n_elems]. */

@@ -122,6 +122,7 @@ def lib(omp=False, path=None):
     L.ref_blacklist.restype = C.c_int64
     L.ref_blacklist.argtypes = [C.c_void_p, ip32, ip32, C.c_int64]
     L.ref_plastic_state.argtypes = [C.c_void_p, dp, dp, dp, u8p]
+    L.ref_set_plastic_state.argtypes = [C.c_void_p, dp, dp]
     L.ref_curr_strains.argtypes = [C.c_void_p, dp]
     L.ref_frames.argtypes = [C.c_void_p, dp, dp]
     L.ref_set_grid.argtypes = [C.c_void_p, C.c_int]
@@ -396,6 +397,12 @@ class RefModel:
         b = np.zeros(max(n, 1), np.int32)
         self.L.ref_blacklist(self.h, _p(a, ip32), _p(b, ip32), n)
         return a[:n], b[:n]
+
+    def set_plastic_state(self, eps_p, alpha):
+        """committed plastic strain [E, 6] and hardening parameters [E, 7]"""
+        ep = _c(eps_p).reshape(-1)
+        al = _c(alpha).reshape(-1)
+        self._check(self.L.ref_set_plastic_state(self.h, _p(ep), _p(al)))
 
     def plastic_state(self):
         E = self.p.n_elems
