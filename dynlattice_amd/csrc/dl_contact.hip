@@ -26,7 +26,8 @@
 // The explicit step does not run this per step: it keeps per-element
 // candidate lists built by the same broadphase with boxes inflated by a skin
 // (kc_vbuild) and rebuilds them only when a node has moved by nearly the
-// skin (kc_vnarrow evaluates the listed pairs; see the list section below).
+// skin (kc_vmark and kc_vsum evaluate the listed pairs; see the list section
+// below).
 // The action path (GET_INT_VECTOR, pair listings, blacklist) and the adjoint
 // without lists use the cell scan directly.
 #include <hip/hip_runtime.h>
@@ -245,8 +246,10 @@ __device__ __forceinline__ void place_one(const ContactDev &c, int64_t e) {
   ((float4 *)c.frec)[2 * pos + 1] = hi;
 }
 
-__global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate) {
+// zero: a counter the list build that follows appends to (or null)
+__global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate, int32_t *zero) {
   GATED(gate);
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   GRID_STRIDE(e, c.E) place_one(c, e);
 }
 
@@ -556,6 +559,7 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     });
     if (n > cap) *c.err = DL_ERR_CAPACITY;
     c.vcount[X] = n < cap ? n : cap;
+    if (n > 0) c.vlisted[atomicAdd(c.nlisted, 1)] = X; // order is irrelevant (kc_vmark)
   }
   // the lists are built: remember the displacement they were built at (no
   // kernel here reads uref; the node updates that follow compare against it)
@@ -564,125 +568,95 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
   }
 }
 
-// contact forces of the step from the lists: exact boxes of X and of each
-// listed Y, then the pair as in kc_narrow; the element's force row (returns
-// whether X has a closed pair) and the per-step counts
-__device__ __forceinline__ bool vnarrow_elem(const ContactDev &c, const DevData &d, int64_t X,
-                                             int32_t t1, unsigned long long &nact,
-                                             unsigned long long &ncand) {
-  const int64_t N = c.N, E = c.E;
-  const int32_t sX = c.es[X];
-  const double rX = c.crad[sX];
-  double pX[2][3], xlo[3], xhi[3];
-  elem_pos(d, c.en0[X], c.en1[X], pX);
-  elem_box(d, N, c.en0[X], c.en1[X], rX, xlo, xhi);
-  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
-  bool any = false;
-  const int32_t *lst = c.vlist + X * c.vk;
-  for (int32_t t = 0; t < t1; t++) {
-    const int32_t Y = list_partner(lst[t]);
-    const int32_t sY = c.es[Y], ya = c.en0[Y], yb = c.en1[Y];
-    const double rY = c.crad[sY];
-    double ylo[3], yhi[3];
-    elem_box(d, N, ya, yb, rY, ylo, yhi);
-    bool ov = true;
-    for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
-    if (!ov) continue;
-    const bool XisA = sX < sY;
-    if (XisA) ncand++;
-    if (lst[t] < 0) continue; // blacklisted at INIT
-    double pY[2][3];
-    elem_pos(d, ya, yb, pY);
-    const double R = rX + rY;
-    double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
-    const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                             : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
-    if (closed) {
-      any = true;
-      for (int i = 0; i < 3; i++) {
-        fa[i] += g0[i];
-        fb[i] += g1[i];
-      }
-      if (XisA) nact++;
-    }
-  }
-  if (!any) return false;
-  for (int i = 0; i < 3; i++) {
-    c.fcs[i * E + X] = fa[i];
-    c.fcs[(3 + i) * E + X] = fb[i];
-  }
-  return true;
-}
-
-// Sixteen elements per thread: their list lengths arrive in four vector
-// loads, and the grid is small enough to be resident at once.  The elements
-// that have a list go to a queue of their wavefront, which its 64 lanes then
-// share, so the pair work is spread evenly whether few lists are filled (a
-// lattice before it buckles: about one memory round trip per step) or many.
-// The wavefront's 1024 elements own 16 words of the contact bit mask, built
-// in LDS and stored whole.  The counts are integers, summed per wavefront
-// (any order gives the same sum) with one atomic per wavefront.
-#define VN_PER 16
-__global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
-                                                 unsigned long long *cnt_act,
-                                                 unsigned long long *cnt_cand,
-                                                 const int32_t *step) {
-  __shared__ int32_t qe[CB / 64][64 * VN_PER], qt[CB / 64][64 * VN_PER];
-  __shared__ unsigned long long wb[CB / 64][VN_PER];
+// ------------------------------------------------------------------
+// The step's pair pass from the lists, in two kernels.
+//   kc_vmark  parallel over list entries: the exact boxes of X and of the
+//             listed Y, the candidate and active counts, and for every
+//             closed pair its bit in vmark and X's bit in cnew.  A
+//             wavefront takes 64 listed elements and spreads their entries
+//             over its lanes, so the work follows the number of entries and
+//             not the longest list.
+//   kc_vsum   per element with a closed pair (16 elements per lane, the
+//             wavefront owning 1024 elements = 16 mask words): its closed
+//             pairs recomputed and summed in list order, i.e. by partner id
+//             -- the operations and the order of one serial pass over the
+//             list, so the force does not depend on the cell layout, on when
+//             the lists were built or on the split into two kernels.  It
+//             writes the element's force row, stores the wavefront's contact
+//             mask words whole (k_strut reads them) and clears the pair and
+//             element bits it used, so both arrays are zero between steps.
+// ------------------------------------------------------------------
+#define VM_GRID 2048
+__global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
+                                               unsigned long long *cnt_act,
+                                               unsigned long long *cnt_cand,
+                                               const int32_t *step) {
+  __shared__ int32_t sx[CB / 64][64], spre[CB / 64][65];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t T = (int64_t)blockIdx.x * CB + threadIdx.x;
-  // the lists are current (list_refresh ran before): clear the rebuild flag;
-  // the step's node updates may set it again.  Lists truncated by an overflow
-  // stay stale, so every later step rebuilds them and reports the overflow
-  // again instead of running on the truncated lists.
-  if (T == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0;
-  if (lane < VN_PER) wb[wv][lane] = 0ull;
-  const int64_t X0 = T * VN_PER;
-  int32_t t[VN_PER];
-  unsigned nz = 0u;
-  if (X0 + VN_PER <= c.E) {
-    // X0 is a multiple of 16 and the array starts 256-byte aligned
-    const int4 *vc = reinterpret_cast<const int4 *>(c.vcount + X0);
-#pragma unroll
-    for (int q = 0; q < VN_PER / 4; q++) {
-      const int4 v = vc[q];
-      t[4 * q] = v.x; t[4 * q + 1] = v.y; t[4 * q + 2] = v.z; t[4 * q + 3] = v.w;
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < VN_PER; k++) t[k] = X0 + k < c.E ? c.vcount[X0 + k] : 0;
-  }
-#pragma unroll
-  for (int k = 0; k < VN_PER; k++) nz |= (t[k] > 0 ? 1u : 0u) << k;
-  // the wavefront's queue, in element order: exclusive prefix of the lanes'
-  // counts
-  const int cnt = __popc(nz);
-  int pre = cnt;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(pre, o, 64);
-    if (lane >= o) pre += v;
-  }
-  const int total = __shfl(pre, 63, 64);
-  pre -= cnt;
-#pragma unroll
-  for (int k = 0; k < VN_PER; k++)
-    if (nz & (1u << k)) {
-      qe[wv][pre] = (int32_t)(X0 + k);
-      qt[wv][pre] = t[k];
-      pre++;
-    }
-  __syncthreads(); // queue and cleared mask words visible
+  const int64_t N = c.N;
+  const int32_t nl = *c.nlisted;
+  const int64_t nwave = ((int64_t)nl + 63) / 64;
   unsigned long long nact = 0, ncand = 0;
-  const int64_t xw = (T - lane) * VN_PER; // the wavefront's first element
-  for (int i = lane; i < total; i += 64) {
-    const int64_t X = qe[wv][i];
-    if (vnarrow_elem(c, d, X, qt[wv][i], nact, ncand))
-      atomicOr(&wb[wv][(X - xw) >> 6], 1ull << (X & 63));
-  }
-  __syncthreads();
-  {
-    const int64_t w = (xw >> 6) + lane;
-    if (lane < VN_PER && w < (c.E + 63) / 64) c.cbits[w] = wb[wv][lane];
+  // rounds of CB / 64 wavefronts per block; the round count is the same for
+  // every wavefront of the block, so the barriers below are uniform
+  for (int64_t b = blockIdx.x; b * (CB / 64) < nwave; b += gridDim.x) {
+    const int64_t w = b * (CB / 64) + wv;
+    const int64_t j = w * 64 + lane;
+    const int32_t X = j < nl ? c.vlisted[j] : 0;
+    const int32_t cnt = j < nl ? c.vcount[X] : 0;
+    // inclusive prefix of the counts over the wavefront
+    int32_t pre = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(pre, o, 64);
+      if (lane >= o) pre += v;
+    }
+    sx[wv][lane] = X;
+    spre[wv][lane + 1] = pre;
+    if (lane == 0) spre[wv][0] = 0;
+    const int32_t total = __shfl(pre, 63, 64);
+    __syncthreads();
+    for (int32_t q0 = 0; q0 < total; q0 += 64) {
+      const int32_t q = q0 + lane;
+      if (q < total) {
+        // owner: the last element whose exclusive prefix is <= q
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (spre[wv][mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        const int32_t Xq = sx[wv][lo];
+        const int32_t t = q - spre[wv][lo];
+        const int32_t code = c.vlist[(int64_t)Xq * c.vk + t];
+        const int32_t Y = list_partner(code);
+        const int32_t sX = c.es[Xq], sY = c.es[Y];
+        const double rX = c.crad[sX], rY = c.crad[sY];
+        double xlo[3], xhi[3], ylo[3], yhi[3];
+        elem_box(d, N, c.en0[Xq], c.en1[Xq], rX, xlo, xhi);
+        elem_box(d, N, c.en0[Y], c.en1[Y], rY, ylo, yhi);
+        bool ov = true;
+        for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
+        if (ov) {
+          const bool XisA = sX < sY;
+          if (XisA) ncand++;
+          if (code >= 0) { // not blacklisted at INIT
+            double pX[2][3], pY[2][3], g0[3], g1[3];
+            elem_pos(d, c.en0[Xq], c.en1[Xq], pX);
+            elem_pos(d, c.en0[Y], c.en1[Y], pY);
+            const double R = rX + rY;
+            const bool closed = XisA ? cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                                     : cm_narrowphase<false, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+            if (closed) {
+              const int64_t bit = (int64_t)Xq * c.vkb + t;
+              atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
+              atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
+              if (XisA) nact++;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads(); // the wavefront's LDS rows are reused by its next round
   }
   for (int o = 32; o > 0; o >>= 1) {
     nact += __shfl_xor(nact, o, 64);
@@ -692,6 +666,79 @@ __global__ __launch_bounds__(CB) void kc_vnarrow(ContactDev c, DevData d,
     const int64_t st = *step;
     if (nact) atomicAdd(cnt_act + st, nact);
     if (ncand) atomicAdd(cnt_cand + st, ncand);
+  }
+}
+
+// the force row of X from its closed listed pairs, in list order (the pairs
+// kc_vmark marked; every marked pair is closed, so no test is repeated)
+__device__ __forceinline__ void vsum_elem(const ContactDev &c, const DevData &d, int64_t X) {
+  const int64_t E = c.E;
+  const int32_t sX = c.es[X];
+  const double rX = c.crad[sX];
+  double pX[2][3];
+  elem_pos(d, c.en0[X], c.en1[X], pX);
+  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
+  const int32_t *lst = c.vlist + X * c.vk;
+  uint32_t *mk = c.vmark + X * (c.vkb / 32);
+  const int32_t t1 = c.vcount[X];
+  for (int32_t w0 = 0; w0 < t1; w0 += 32) {
+    uint32_t m = mk[w0 >> 5];
+    if (!m) continue;
+    mk[w0 >> 5] = 0u;
+    while (m) {
+      const int t = w0 + __ffs(m) - 1;
+      m &= m - 1u;
+      const int32_t Y = lst[t];
+      const int32_t sY = c.es[Y];
+      double pY[2][3], g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+      elem_pos(d, c.en0[Y], c.en1[Y], pY);
+      const double R = rX + c.crad[sY];
+      if (sX < sY) cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1);
+      else cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+      for (int i = 0; i < 3; i++) {
+        fa[i] += g0[i];
+        fb[i] += g1[i];
+      }
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    c.fcs[i * E + X] = fa[i];
+    c.fcs[(3 + i) * E + X] = fb[i];
+  }
+}
+
+#define VN_PER 16
+__global__ __launch_bounds__(CB) void kc_vsum(ContactDev c, DevData d) {
+  __shared__ int32_t qe[CB / 64][64 * VN_PER];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t T = (int64_t)blockIdx.x * CB + threadIdx.x;
+  // the lists are current (list_refresh ran before): clear the rebuild flag;
+  // the step's node updates may set it again.  Lists truncated by an overflow
+  // stay stale, so every later step rebuilds them and reports the overflow
+  // again instead of running on the truncated lists.
+  if (T == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0;
+  const int64_t xw = (T - lane) * VN_PER; // the wavefront's first element
+  const int64_t nw = (c.E + 63) / 64;
+  // the wavefront's 16 mask words: lane k < 16 holds word k
+  unsigned long long word = 0ull;
+  const int64_t wi = (xw >> 6) + lane;
+  if (lane < VN_PER && wi < nw) word = c.cnew[wi];
+  // queue of the elements with a closed pair, in element order
+  const int cnt = lane < VN_PER ? __popcll(word) : 0;
+  int pre = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(pre, o, 64);
+    if (lane >= o) pre += v;
+  }
+  const int total = __shfl(pre, 63, 64);
+  pre -= cnt;
+  for (unsigned long long m = word; m; m &= m - 1ull)
+    qe[wv][pre++] = (int32_t)((wi << 6) + __ffsll((long long)m) - 1);
+  __syncthreads();
+  for (int i = lane; i < total; i += 64) vsum_elem(c, d, qe[wv][i]);
+  if (lane < VN_PER && wi < nw) {
+    c.cbits[wi] = word;
+    c.cnew[wi] = 0ull;
   }
 }
 
@@ -705,7 +752,7 @@ __global__ __launch_bounds__(CB) void kc_clear_bits(unsigned long long *p, int64
 // thread per element in id order.
 __global__ __launch_bounds__(CB) void kc_adj_vnarrow(ContactDev c, DevData d, AdjDev a) {
   const int64_t X = (int64_t)blockIdx.x * CB + threadIdx.x;
-  if (X == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0; // lists current (see kc_vnarrow)
+  if (X == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0; // lists current (see kc_vsum)
   if (X >= c.E) return;
   const int64_t N = c.N, E = c.E;
   const int32_t sX = c.es[X], xa = c.en0[X], xb = c.en1[X];
@@ -925,6 +972,15 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     if ((rc = calloc_dev(cs, &c.uref, 3 * N))) return rc;
     if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
     if ((rc = calloc_dev(cs, &c.vlist, (size_t)E * c.vk))) return rc;
+    c.vkb = (c.vk + 31) & ~31;
+    if ((rc = calloc_dev(cs, &c.vlisted, E))) return rc;
+    if ((rc = calloc_dev(cs, &c.nlisted, 1))) return rc;
+    if ((rc = calloc_dev(cs, &c.vmark, (size_t)E * (c.vkb / 32)))) return rc;
+    if ((rc = calloc_dev(cs, &c.cnew, (E + 63) / 64))) return rc;
+    CHK(hipMemset(c.nlisted, 0, sizeof(int32_t)));
+    CHK(hipMemset(c.vmark, 0, sizeof(uint32_t) * (size_t)E * (c.vkb / 32)));
+    CHK(hipMemset(c.cnew, 0, sizeof(unsigned long long) * ((E + 63) / 64)));
+    CHK(hipMemset(c.vcount, 0, sizeof(int32_t) * E));
     const int32_t one = 1; // lists are built at the first step
     CHK(hipMemcpy(c.gate, &one, sizeof one, hipMemcpyHostToDevice));
     d.uref = c.uref;
@@ -957,7 +1013,7 @@ static void clear_bits(const ContactDev &c, hipStream_t s) {
 }
 
 static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double infl = 0.0,
-                       const int32_t *gate = nullptr) {
+                       const int32_t *gate = nullptr, int32_t *zero = nullptr) {
   ContactDev &c = cs.c;
   const unsigned ge = gs(c.E);
   hipLaunchKernelGGL(kc_boxes, dim3(ge), dim3(CB), 0, s, c, d, infl, gate);
@@ -968,7 +1024,7 @@ static void broadphase(ContactState &cs, const DevData &d, hipStream_t s, double
                      (const int32_t *)c.ccount, c.ncell, c.cstart, c.cbsum, gate);
   hipLaunchKernelGGL(kc_scan2, dim3(1), dim3(1024), 0, s, c.cbsum, nchunk, c.cstart + c.ncell, gate);
   hipLaunchKernelGGL(kc_scatter, dim3(ge), dim3(CB), 0, s, c, gate);
-  hipLaunchKernelGGL(kc_place, dim3(ge), dim3(CB), 0, s, c, gate);
+  hipLaunchKernelGGL(kc_place, dim3(ge), dim3(CB), 0, s, c, gate, zero);
 }
 
 static int list_pairs(ContactState &cs, const DevData &d, hipStream_t s, int kind,
@@ -1062,7 +1118,7 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
 static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   const int32_t *g = c.gate;
-  broadphase(cs, d, s, c.skin, g);
+  broadphase(cs, d, s, c.skin, g, c.nlisted);
   hipLaunchKernelGGL(kc_vbuild, dim3(gs(std::max(c.E, c.N))), dim3(CB), 0, s, c, d, g);
 }
 
@@ -1082,8 +1138,10 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
   list_refresh(cs, d, s);
   if (after_refresh) CHK(hipEventRecord(after_refresh, s));
   const int64_t nthr = (c.E + VN_PER - 1) / VN_PER;
-  hipLaunchKernelGGL(kc_vnarrow, dim3((unsigned)((nthr + CB - 1) / CB)), dim3(CB), 0, s, c, d,
-                     (unsigned long long *)n_active, (unsigned long long *)n_cand, step);
+  const unsigned gm = (unsigned)std::min<int64_t>(VM_GRID, (c.E + CB - 1) / CB);
+  hipLaunchKernelGGL(kc_vmark, dim3(gm), dim3(CB), 0, s, c, d, (unsigned long long *)n_active,
+                     (unsigned long long *)n_cand, step);
+  hipLaunchKernelGGL(kc_vsum, dim3((unsigned)((nthr + CB - 1) / CB)), dim3(CB), 0, s, c, d);
   CHK(hipGetLastError());
   return DL_OK;
 }
