@@ -404,7 +404,20 @@ def milne(args):
     print(json.dumps(res), flush=True)
 
 
-def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0, acc=None):
+def profiler_region(on):
+    """Starts (on) or stops the collection of a rocprofv3 run with
+    --selected-regions (roctxProfilerResume / roctxProfilerPause); a no-op
+    without the profiler."""
+    import ctypes as C
+    try:
+        lib = C.CDLL("librocprofiler-sdk-roctx.so")
+    except OSError:
+        return
+    (lib.roctxProfilerResume if on else lib.roctxProfilerPause)(C.c_uint64(0))
+
+
+def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0, acc=None,
+                  profile=False):
     """Runs the crush (top plate at its prescribed velocity) through the given
     nominal strains and times a window of `window` steps at each: wall time
     per step, kernel phases (HIP events), plastic-active fraction, active
@@ -412,7 +425,8 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
     all-cores oracle is timed at the same state (state and plastic state
     copied over) on a sample of about cpu_budget seconds.  start: steps the
     model has taken; acc: {"steps", "wall_s"} accumulating the ordinary
-    step calls (not the event-timed passes)."""
+    step calls (not the event-timed passes).  profile: the windows' timed
+    steps are the regions a rocprofv3 --selected-regions run collects."""
     H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
     v = float(np.abs(p.velo0[p.groups["top"], 2]).max())
     out = []
@@ -432,9 +446,13 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
             stepped(k, time.perf_counter() - t0)
             done += k
         g.sync()
+        if profile:
+            profiler_region(True)
         t0 = time.perf_counter()
         o = g.step(window)
         wall = time.perf_counter() - t0
+        if profile:
+            profiler_region(False)
         stepped(window, wall)
         h = g.history(window)
         act = g.plastic_active()
@@ -492,13 +510,34 @@ def crush(args):
     print(f"crush: {p.name}: {p.n_elems} elements, dt {p.dt:.3e}", file=sys.stderr, flush=True)
     g = GpuModel(p, device=0, flags=DL_TRACK_DISSIPATION)
     g.set_reaction_group(p.groups["bottom"], 2)
-    g.set_graph(10)
+    # direct launches under a --selected-regions profile: that profiler mode
+    # fails inside captured-graph launches
+    g.set_graph(0 if args.profile_windows else 10)
     strains = [float(x) for x in args.strains.split(",")]
+    if args.from_state:
+        # a short run from a saved crush state (e.g. under rocprofv3)
+        z = np.load(args.from_state)
+        g.set_state(z["disp"], z["velo"])
+        g.set_plastic_state(z["eps_p"], z["alpha"])
+        g.step(args.warmup)
+        t0 = time.perf_counter()
+        o = g.step(args.steps)
+        wall = time.perf_counter() - t0
+        print(json.dumps({"from_state": args.from_state, "steps": args.steps,
+                          "ms_per_step": 1e3 * wall / args.steps,
+                          "active_pairs_last_step": int(o.n_contacts)}), flush=True)
+        g.close()
+        return
     res = {"metric": "element-GP updates/s per explicit step (FP64), crush windows",
            "config": {"workload": p.name, "elements": p.n_elems, "dt": p.dt,
                       "yield": yield_family(p), "window_steps": args.steps},
            "windows": crush_windows(g, p, strains, args.steps,
-                                    0.0 if args.no_cpu else args.cpu_budget)}
+                                    0.0 if args.no_cpu else args.cpu_budget,
+                                    profile=args.profile_windows)}
+    if args.save_state:
+        d, v = g.get_state()
+        ep, al = g.plastic_state()
+        np.savez(args.save_state, disp=d, velo=v, eps_p=ep, alpha=al)
     g.close()
     print(json.dumps(res), flush=True)
 
@@ -522,6 +561,13 @@ def main():
     ap.add_argument("--crush", action="store_true",
                     help="the crush through --strains with a timed window at each")
     ap.add_argument("--strains", default="0.1,0.2,0.4")
+    ap.add_argument("--save-state", default="",
+                    help="with --crush: after the last window, save the state (npz) here")
+    ap.add_argument("--from-state", default="",
+                    help="start from a state saved by --save-state (then time --steps steps)")
+    ap.add_argument("--profile-windows", action="store_true",
+                    help="with --crush: only the windows' timed steps are collected by a "
+                         "rocprofv3 --selected-regions run")
     ap.add_argument("--max-iter", type=int, default=0,
                     help="return-map iteration cap (the material's maxIter; 0: the workload's)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -275,7 +275,40 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   std::vector<int32_t> occ(N, 0);
   for (int64_t g = 0; g < NG; g++)
     for (int64_t k = 0; k <= gptr[g + 1] - gptr[g]; k++) occ[gnode(g, k)]++;
-  // slots for shared nodes in (chain, position) order, i.e. strut order
+  // wavefront packing: a chain never straddles two 64-lane wavefronts
+  std::vector<int32_t> lane_e, elane(E, -1);
+  for (int64_t g = 0; g < NG; g++) {
+    const int64_t len = gptr[g + 1] - gptr[g];
+    if ((int64_t)(lane_e.size() % 64) + len > 64)
+      while (lane_e.size() % 64) lane_e.push_back(-1);
+    for (int64_t k = 0; k < len; k++) {
+      elane[gptr[g] + k] = (int32_t)lane_e.size();
+      lane_e.push_back((int32_t)(gptr[g] + k));
+    }
+  }
+  while (lane_e.size() % 256) lane_e.push_back(-1);
+  // In-wavefront joints: a shared node all of whose chain positions are
+  // node a of an element (no chain end) in one wavefront -- a BCC cell
+  // centre's eight struts -- is summed by shuffles in k_strut, in lane order
+  // (= chain order, the order of its slots), and updated by its lowest lane.
+  // Its elements carry sa = -2 - (that lane within the wavefront); it gets no
+  // slots and no joint-kernel thread.
+  std::vector<int32_t> jwave(N, -2), jlow(N, 64);  // -2 unseen, -1 not eligible
+  if (!(c->flags & DL_JOINTS_IN_MEMORY))
+    for (int64_t g = 0; g < NG; g++) {
+      const int64_t len = gptr[g + 1] - gptr[g];
+      for (int64_t k = 0; k <= len; k++) {
+        const int32_t n = gnode(g, k);
+        if (occ[n] < 2 || jwave[n] == -1) continue;
+        if (k == len) { jwave[n] = -1; continue; } // a chain end
+        const int32_t L = elane[gptr[g] + k];
+        if (jwave[n] == -2) jwave[n] = L / 64;
+        else if (jwave[n] != L / 64) { jwave[n] = -1; continue; }
+        jlow[n] = std::min(jlow[n], L % 64);
+      }
+    }
+  auto in_wave = [&](int32_t n) { return occ[n] > 1 && jwave[n] >= 0; };
+  // slots for the other shared nodes in (chain, position) order, i.e. strut order
   std::vector<int32_t> sa(E, -1), sb(E, -1), jidx(N, -1), jnode;
   std::vector<uint8_t> eflag(E, 0);
   std::vector<std::pair<int32_t, int32_t>> slot_node; // (slot, node)
@@ -285,7 +318,9 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     for (int64_t k = 0; k <= len; k++) {
       const int32_t n = gnode(g, k);
       int32_t sl = -1;
-      if (occ[n] > 1) {
+      if (in_wave(n)) {
+        sl = -2 - jlow[n];
+      } else if (occ[n] > 1) {
         sl = nslot++;
         slot_node.push_back({sl, n});
       }
@@ -296,7 +331,7 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     eflag[gptr[g] + len - 1] |= 2u;
   }
   for (int64_t n = 0; n < N; n++)
-    if (occ[n] > 1) { jidx[n] = (int32_t)jnode.size(); jnode.push_back((int32_t)n); }
+    if (occ[n] > 1 && !in_wave(n)) { jidx[n] = (int32_t)jnode.size(); jnode.push_back((int32_t)n); }
   const int32_t NJ = (int32_t)jnode.size();
   // renumber the slots so that the contributions to one shared node are
   // contiguous (in chain order): the joint kernel then reads
@@ -311,18 +346,10 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       if (sa[e] >= 0) sa[e] = remap[sa[e]];
       if (sb[e] >= 0) sb[e] = remap[sb[e]];
     }
+    c->n_wave_joints = 0;
+    for (int64_t n = 0; n < N; n++) c->n_wave_joints += in_wave((int32_t)n) ? 1 : 0;
     for (int32_t t = 0; t < nslot; t++) jslot[t] = t;
   }
-  // wavefront packing: a chain never straddles two 64-lane wavefronts
-  std::vector<int32_t> lane_e;
-  for (int64_t g = 0; g < NG; g++) {
-    const int64_t len = gptr[g + 1] - gptr[g];
-    if ((int64_t)(lane_e.size() % 64) + len > 64)
-      while (lane_e.size() % 64) lane_e.push_back(-1);
-    for (int64_t k = 0; k < len; k++) lane_e.push_back((int32_t)(gptr[g] + k));
-  }
-  while (lane_e.size() % 256) lane_e.push_back(-1);
-
   // host copies of the mesh, sections and material (tables are rebuilt from
   // them when the sections change; the adjoint maps back through them)
   {
@@ -1362,6 +1389,31 @@ extern "C" int dl_plastic_state(dl_ctx *c, double *eps_p, double *alpha, int64_t
     HIPCHK(hipMemcpy(h.data(), c->d.al, 7 * E * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t e = 0; e < E; e++)
       for (int i = 0; i < 7; i++) alpha[7 * e + i] = h[i * E + e];
+  }
+  return DL_OK;
+}
+
+extern "C" int dl_set_plastic_state(dl_ctx *c, const double *eps_p, const double *alpha,
+                                    int64_t n_elems) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  const int64_t E = c->d.E;
+  if (n_elems != E) return fail(DL_ERR_INPUT, "n_elems mismatch");
+  if (!c->plastic) return fail(DL_ERR_STATE, "elastic material has no plastic state");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> h(7 * E);
+  if (eps_p) {
+    for (int64_t e = 0; e < E; e++)
+      for (int i = 0; i < 6; i++) h[i * E + e] = eps_p[6 * e + i];
+    HIPCHK(hipMemcpy(c->d.ep, h.data(), 6 * E * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d.ep_cur, h.data(), 6 * E * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (alpha) {
+    for (int64_t e = 0; e < E; e++)
+      for (int i = 0; i < 7; i++) h[i * E + e] = alpha[7 * e + i];
+    HIPCHK(hipMemcpy(c->d.al, h.data(), 7 * E * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d.al_cur, h.data(), 7 * E * sizeof(double), hipMemcpyHostToDevice));
   }
   return DL_OK;
 }

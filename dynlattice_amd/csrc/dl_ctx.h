@@ -102,6 +102,7 @@ struct dl_ctx {
   bool initialized = false;
   bool track_diss = false;
   int64_t n_mirrored = 0; // elements with -I end frames (struts along -e3)
+  int64_t n_wave_joints = 0; // shared nodes summed within one wavefront (k_strut)
   uint32_t flags = 0;
   PlasticParams pp{};
   DevData d{};

@@ -204,6 +204,11 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
   int32_t na = 0, nb = 0;
   uint32_t fl = 0;
   double diss = 0.0;
+  // LDS of the block: the plastic material's stash (21 doubles per lane),
+  // later the forces of the wavefront joints (6 per lane); each lane writes
+  // its own column
+  __shared__ double lds[PLASTIC ? 21 : 6][DL_BLOCK];
+  const int tx = threadIdx.x;
   if (e >= 0) {
     fl = d.eflag[e];
     na = d.en0[e];
@@ -271,8 +276,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
       // and the strain wait in LDS meanwhile: the return map's working set
       // then fits 168 VGPRs, i.e. 3 waves per SIMD instead of 2.  The
       // compiler barriers keep the values from being forwarded in registers.
-      __shared__ double stash[21][DL_BLOCK];
-      const int tx = threadIdx.x;
+      double (*stash)[DL_BLOCK] = lds;
 #pragma unroll
       for (int i = 0; i < 9; i++) stash[i][tx] = k.R[i];
 #pragma unroll
@@ -352,6 +356,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
   double f[6];
 #pragma unroll
   for (int i = 0; i < 6; i++) f[i] = __shfl_up(fb[i], 1, 64);
+  int32_t slot = -1;
   if (e >= 0) {
     if (fl & 1u) {
 #pragma unroll
@@ -360,10 +365,69 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
 #pragma unroll
       for (int i = 0; i < 6; i++) f[i] += fa[i];
     }
-    const int32_t slot = d.sa[e];
+    slot = d.sa[e];
+  }
+  // Joints whose slots are all node-a forces of this wavefront (a BCC cell
+  // centre's eight chains): every contributing lane puts its force in its
+  // LDS column, and the joint's lowest lane sums the columns of its
+  // contributors in lane order -- the joint's slot order, summed as the
+  // joint kernel sums slots -- and then updates the node.  The exchange
+  // stays within the wavefront, whose LDS writes and the reads after them
+  // execute in order.  (Columns 0..5 of the block's LDS; each lane writes
+  // its own.)
+  const int me = tx & 63;
+  bool own_a = slot == -1; // this lane updates node a
+  const uint64_t jm = __ballot(slot <= -2); // the wavefront's contributors
+  if (jm) {
+    const int own = -2 - slot;
+    if (slot <= -2) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) lds[i][tx] = f[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // each owner's contributors: one ballot per owner of the wavefront
+    uint64_t mine = 0ull;
+    for (uint64_t om = __ballot(slot <= -2 && own == me); om; om &= om - 1ull) {
+      const int o = __ffsll((long long)om) - 1;
+      const uint64_t mo = __ballot(slot <= -2 && own == o);
+      if (me == o) mine = mo;
+    }
+    if (mine) {
+      own_a = true;
+      const int base = tx - me; // the wavefront's first column
+      // the owner's own force is in its column too: start from zero, add in
+      // lane order, two columns' loads ahead of their additions
+#pragma unroll
+      for (int i = 0; i < 6; i++) f[i] = 0.0;
+      while (mine) {
+        const int c0 = base + __ffsll((long long)mine) - 1;
+        mine &= mine - 1ull;
+        const bool two = mine != 0ull;
+        const int c1 = two ? base + __ffsll((long long)mine) - 1 : c0;
+        if (two) mine &= mine - 1ull;
+        double v0[6], v1[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+          v0[i] = lds[i][c0];
+          v1[i] = lds[i][c1];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) f[i] += v0[i];
+        if (two) {
+#pragma unroll
+          for (int i = 0; i < 6; i++) f[i] += v1[i];
+        }
+      }
+    }
+  }
+  if (e >= 0) {
     if (slot >= 0) {
 #pragma unroll
       for (int i = 0; i < 6; i++) d.sbuf[(int64_t)i * d.nslot + slot] = f[i];
+    } else if (!own_a) {
+      // a joint of this wavefront summed by another lane
     } else if (MODE == MODE_FORCE) {
 #pragma unroll
       for (int i = 0; i < 6; i++) d.fout[i * N + na] = f[i];

@@ -26,6 +26,7 @@ DL_INIT, DL_GET_INT_VECTOR, DL_GET_MATRIX2, DL_COMMIT, DL_CANCEL = range(5)
 DL_LOAD_CASE_DEFAULT, DL_LOAD_CASE_OUTPUT = 0, 1
 DL_TRACK_DISSIPATION = 1
 DL_FRAMES_PER_ELEMENT = 2
+DL_JOINTS_IN_MEMORY = 4
 
 #: every symbol declared in include/dl_gpu.h
 EXPORTS = (
@@ -37,7 +38,7 @@ EXPORTS = (
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
     "dl_get_table", "dl_get_energies", "dl_set_impact_groups", "dl_explicit_step_noalloc",
-    "dl_contact_list_stats",
+    "dl_contact_list_stats", "dl_set_plastic_state",
 )
 #: DL_NOALLOC_SCRATCH_DOUBLES of include/dl_gpu.h
 NOALLOC_SCRATCH_DOUBLES = 1 << 18
@@ -165,6 +166,7 @@ def load():
     L.dl_explicit_step.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(dl_step_out)]
     L.dl_explicit_step_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, dp, dp, ip64]
     L.dl_contact_list_stats.argtypes = [C.c_void_p, ip64, ip64]
+    L.dl_set_plastic_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
     L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
     L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
     L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
@@ -487,6 +489,12 @@ class GpuModel:
         al = np.zeros(7 * E)
         self._check(self.L.dl_plastic_state(self.h, _p(ep), _p(al), E))
         return ep.reshape(E, 6), al.reshape(E, 7)
+
+    def set_plastic_state(self, eps_p, alpha):
+        """committed plastic strain [E, 6] and hardening parameters [E, 7]"""
+        ep = _c(eps_p).reshape(-1)
+        al = _c(alpha).reshape(-1)
+        self._check(self.L.dl_set_plastic_state(self.h, _p(ep), _p(al), self.p.n_elems))
 
     def _pairs(self, fn):
         n = C.c_int64()

@@ -59,6 +59,10 @@ extern "C" {
                                   * every element instead of taking an inner
                                   * element's end-b frame from its neighbour
                                   * (same values; for testing) */
+#define DL_JOINTS_IN_MEMORY 4u   /* every shared node goes through the slot
+                                  * buffer and the joint kernel, also those
+                                  * whose chains all sit in one wavefront
+                                  * (same bits; for testing) */
 
 typedef struct dl_ctx dl_ctx;
 
@@ -280,6 +284,11 @@ int dl_contact_list_stats(dl_ctx *ctx, int64_t *entries, int64_t *builds);
  * hardening parameters [7E] (iso, kin...) of the committed state. */
 int dl_plastic_active(dl_ctx *ctx, uint8_t *mask, int64_t n);
 int dl_plastic_state(dl_ctx *ctx, double *eps_p, double *alpha, int64_t n_elems);
+/* Sets the committed plastic state (the layout of dl_plastic_state), e.g. to
+ * continue a run from a saved state together with dl_set_state; the
+ * counterpart of the material's committed history (ElastoPlasticRodMaterial
+ * oldPlastStrains_ / oldHardParams_).  Either array may be NULL. */
+int dl_set_plastic_state(dl_ctx *ctx, const double *eps_p, const double *alpha, int64_t n_elems);
 
 /* Device timing of the kernels of the last dl_explicit_step (or
  * dl_milne_step) call, measured with HIP events on the context's stream:

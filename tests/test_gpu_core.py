@@ -206,6 +206,39 @@ def test_shared_end_frames_equal_per_element_frames(gpu, m, so3):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("m,so3", [(4, True), (3, True), (4, False)])
+def test_wavefront_joints_equal_joint_kernel(gpu, m, so3):
+    """A BCC cell centre's eight chains sit in one wavefront: k_strut sums
+    that joint by shuffles in slot order and updates it; with
+    DL_JOINTS_IN_MEMORY every joint goes through the slot buffer and the
+    joint kernel.  The same additions in the same order, so LeapFrog with
+    plasticity and contact, fixed-step MilneDevice attempts and the action
+    path's internal force agree bit for bit.  The MilneDevice error estimate
+    is a sum of per-block partials, and the joint's node moves to another
+    block, so with an adaptive step only the accept/reject sequence is equal
+    and the states agree to rounding."""
+    from dynlattice_amd.gpu import DL_JOINTS_IN_MEMORY, GpuModel, milne_opts
+    p = small_bcc(n=3, m=m, plastic=True, contact=True, exponent=2.0)
+    p.so3 = so3
+    p.velo0[p.groups["top"], 2] = -20.0
+    dtm = p.dt / 10
+    res, ada = [], []
+    for flags in (0, DL_JOINTS_IN_MEMORY):
+        g = GpuModel(p, flags=flags)
+        g.set_reaction_group(p.groups["bottom"], 2)
+        g.step(150)
+        h = g.history(150)["reaction"]
+        g.milne(40, milne_opts(dtm, 1e-4, dt_min=dtm, dt_max=dtm))
+        res.append(g.get_state() + (g.plastic_active(), g.int_vector(), h))
+        q = g.milne(40, milne_opts(dtm, 1e-4))
+        ada.append((g.get_state()[0], q["accepted"]))
+        g.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ada[0][1], ada[1][1])
+    assert rel_inf(ada[0][0], ada[1][0]) <= 1e-12
+
+
 @pytest.mark.parametrize("so3", [True, False])
 def test_static_rollup_closed_form(gpu, so3):
     """The reference's static test 1 (two full turns under an end moment
