@@ -104,3 +104,47 @@ def test_graph_replay_equals_direct(K):
         for k in h0:
             assert np.array_equal(h0[k], h1[k]), k
         assert a0 == a1 and b0 == b1
+
+
+def test_c4_sweep_of_256_instances():
+    """Configuration C4: the 256 seeded instances (n in 4..16 cells, 2..6
+    elements per strut, per-instance dt) in one context replaying captured
+    graphs, 100 steps: every instance equals its own context bit for bit
+    (state and reaction history), and the smallest, median and largest
+    instances match the oracle (displacements within 1e-8, bottom reaction
+    within 1e-9; in 100 steps the stress wave reaches the bottom face of the
+    smallest instances only, so the larger ones check a reaction history
+    that is still exactly zero)."""
+    from dynlattice_amd.gpu import GpuModel
+    probs = [L.c4_instance(i) for i in range(256)]
+    bt = BT.concat(probs)
+    g = BT.make_model(bt, graph_steps=10)
+    n = 100
+    g.step(n, out=False)
+    d, v = g.get_state()
+    h = g.group_history(n)
+    g.close()
+    sizes = np.array([p.n_elems for p in probs])
+    print("instances", len(probs), "elements", sizes.sum(), "range", sizes.min(), sizes.max())
+    for b, p in enumerate(probs):
+        q = _standalone(bt, b, probs)
+        s = GpuModel(q)
+        s.set_reaction_group(q.groups["bottom"], 2)
+        s.step(n, q.dt, out=False)
+        ds, vs = s.get_state()
+        hs = s.history(n)["reaction"]
+        s.close()
+        lo, hi = bt.node_off[b], bt.node_off[b + 1]
+        assert np.array_equal(d[lo:hi], ds), b
+        assert np.array_equal(v[lo:hi], vs), b
+        assert np.array_equal(h[:, b], hs), b
+    order = np.argsort(sizes, kind="stable")
+    for b in (order[0], order[len(order) // 2], order[-1]):
+        q = _standalone(bt, b, probs)
+        r = RefModel(q, omp=True)
+        o = r.leapfrog(n, group=q.groups["bottom"], group_dof=2)
+        lo, hi = bt.node_off[b], bt.node_off[b + 1]
+        print("instance", b, q.n_elems, "elements: disp", rel_inf(d[lo:hi], r.disp.reshape(-1, 6)),
+              "reaction", rel_inf(h[:, b], o["reaction"]))
+        assert rel_inf(d[lo:hi], r.disp.reshape(-1, 6)) <= 1e-8
+        assert rel_inf(h[:, b], o["reaction"]) <= 1e-9

@@ -84,3 +84,81 @@ def test_fullsize_steps_are_reproducible_and_graphs_equal_direct(c3):
     for k in (1, 2):
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(a, b)
+
+
+def test_mid_crush_state_matches_oracle(c3):
+    """The crush carried to 25 % nominal strain on the device (about 23,500
+    steps; plastic flow in about a quarter of the elements, contact pairs
+    closing), then 30 more steps on the device and on the oracle from that
+    state (positions, rotations, velocities and the committed plastic state
+    copied over): displacements and velocities within 1e-8, the GroupOutput
+    reaction within 1e-9, equal active contact pairs per step and equal
+    plastic active sets.  At that state the internal force is
+    self-equilibrated, a replay of the 30 steps from the saved state gives
+    the same bits, and the energy balances: the work of the top face
+    (trapezoidal sum of its reaction times v dt) equals the change of kinetic
+    energy plus the elastic energy plus the plastic dissipation within 1 %
+    (the rest is the penalty energy of the closed pairs and the integrator's
+    O(dt) energy error)."""
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
+    from oracle.ref_cpu import RefModel
+    from tests.conftest import rel_inf
+    p = c3
+    bot, top = p.groups["bottom"], p.groups["top"]
+    g = GpuModel(p, flags=DL_TRACK_DISSIPATION)
+    g.set_reaction_groups(np.concatenate([bot, top]), [0, len(bot), len(bot) + len(top)], 2)
+    g.set_graph(10)
+    e0 = g.energies()
+    v_top = float(p.velo0[top[0], 2])
+    target = int(round(0.25 * 0.4 / abs(v_top) / p.dt))
+    done = 0
+    r_top = [np.zeros(1)]  # f_int of the rest state
+    while done < target:
+        k = min(5000, target - done)
+        g.step(k)
+        r_top.append(g.group_history(k)[:, 1])
+        done += k
+    r_top = np.concatenate(r_top)
+    work = float(np.sum(0.5 * (r_top[1:] + r_top[:-1]))) * v_top * p.dt
+    e1 = g.energies()
+    stored = e1["kinetic"] - e0["kinetic"] + e1["potential"] + e1["dissipated"]
+    print("energy: work", work, "dE_kin", e1["kinetic"] - e0["kinetic"], "E_pot", e1["potential"],
+          "E_diss", e1["dissipated"], "residual", (work - stored) / work)
+    assert work > 0 and e1["dissipated"] > 0
+    assert abs(work - stored) <= 1e-2 * work
+    d0, v0 = g.get_state()
+    ep0, al0 = g.plastic_state()
+    f = g.int_vector()
+    scale = np.abs(f).sum(axis=0)
+    assert np.all(np.abs(f[:, :3].sum(axis=0)) <= 1e-12 * scale[:3])
+    x = p.coords + d0[:, :3]
+    mom = np.cross(x, f[:, :3]).sum(axis=0) + f[:, 3:].sum(axis=0)
+    mscale = np.abs(np.cross(x, f[:, :3])).sum(axis=0) + scale[3:]
+    assert np.all(np.abs(mom) <= 1e-12 * mscale)
+    K = 30
+    runs = []
+    for _ in range(2):
+        g.set_state(d0, v0)
+        g.set_plastic_state(ep0, al0)
+        g.step(K)
+        runs.append(g.get_state() + (g.group_history(K)[:, 0], g.history(K)["n_contacts"],
+                                     g.plastic_active()))
+    g.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+    r = RefModel(p, grid=True, omp=True)
+    r.disp[:] = d0.reshape(-1)
+    r.velo[:] = v0.reshape(-1)
+    r.set_plastic_state(ep0, al0)
+    o = r.leapfrog(K, group=bot, group_dof=2)
+    dg, vg, rg, ng, ag = runs[0]
+    dr, vr = r.disp.reshape(-1, 6), r.velo.reshape(-1, 6)
+    act_r = r.plastic_state()[3]
+    print("mid-crush: active pairs", ng.max(), "plastic", ag.mean(), "disp", rel_inf(dg, dr),
+          "velo", rel_inf(vg, vr), "reaction", rel_inf(rg, o["reaction"]))
+    assert ag.mean() > 0.05 and ng.max() > 0
+    assert rel_inf(dg, dr) <= 1e-8
+    assert rel_inf(vg, vr) <= 1e-8
+    assert rel_inf(rg, o["reaction"]) <= 1e-9
+    assert np.array_equal(ng, o["n_contacts"])
+    assert np.array_equal(ag.astype(bool), act_r.astype(bool))
