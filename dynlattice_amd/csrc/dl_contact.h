@@ -62,11 +62,12 @@ struct ContactDev {
   int32_t *vcount;      // [E] partners listed
   int32_t *vlist;       // [E][vk] partner ids
   int32_t vk;           // list capacity per element
-  // the step's pair pass: elements with a non-empty list (in build order),
-  // a bit per closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32,
-  // so an element owns whole words) and a bit per element with a closed pair
-  int32_t *vlisted;     // [E]
-  int32_t *nlisted;     // [1]
+  // the step's pair pass: every list entry's position X * vk + t (the
+  // entries of one element contiguous, elements in build order), a bit per
+  // closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32, so an
+  // element owns whole words) and a bit per element with a closed pair
+  uint32_t *ventry;     // [E * vk]
+  int32_t *nentries;    // [1]
   int32_t vkb;
   uint32_t *vmark;      // [E * vkb / 32]
   unsigned long long *cnew; // [(E+63)/64]

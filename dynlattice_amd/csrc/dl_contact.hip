@@ -559,7 +559,12 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     });
     if (n > cap) *c.err = DL_ERR_CAPACITY;
     c.vcount[X] = n < cap ? n : cap;
-    if (n > 0) c.vlisted[atomicAdd(c.nlisted, 1)] = X; // order is irrelevant (kc_vmark)
+    if (n > 0) {
+      // the entry table of kc_vmark (the order of the elements is irrelevant)
+      const int32_t m = n < cap ? n : cap;
+      const int32_t base = atomicAdd(c.nentries, m);
+      for (int32_t t = 0; t < m; t++) c.ventry[base + t] = (uint32_t)X * (uint32_t)cap + (uint32_t)t;
+    }
   }
   // the lists are built: remember the displacement they were built at (no
   // kernel here reads uref; the node updates that follow compare against it)
@@ -570,12 +575,13 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 
 // ------------------------------------------------------------------
 // The step's pair pass from the lists, in two kernels.
-//   kc_vmark  parallel over list entries: the exact boxes of X and of the
-//             listed Y, the candidate and active counts, and for every
-//             closed pair its bit in vmark and X's bit in cnew.  A
-//             wavefront takes 64 listed elements and spreads their entries
-//             over its lanes, so the work follows the number of entries and
-//             not the longest list.
+//   kc_vmark  parallel over list entries (the entry table written by the
+//             list build): the exact boxes of X and of the listed Y, the
+//             candidate and active counts, and for every closed pair its
+//             bit in vmark and X's bit in cnew.  A thread takes one entry
+//             at a time, so the work follows the number of entries and not
+//             the longest list, and a light load spreads over as many
+//             wavefronts as it has entries.
 //   kc_vsum   per element with a closed pair (16 elements per lane, the
 //             wavefront owning 1024 elements = 16 mask words): its closed
 //             pairs recomputed and summed in list order, i.e. by partner id
@@ -591,72 +597,42 @@ __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
                                                unsigned long long *cnt_act,
                                                unsigned long long *cnt_cand,
                                                const int32_t *step) {
-  __shared__ int32_t sx[CB / 64][64], spre[CB / 64][65];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int64_t N = c.N;
-  const int32_t nl = *c.nlisted;
-  const int64_t nwave = ((int64_t)nl + 63) / 64;
+  const int64_t ne = *c.nentries;
+  const uint32_t vk = (uint32_t)c.vk;
   unsigned long long nact = 0, ncand = 0;
-  // rounds of CB / 64 wavefronts per block; the round count is the same for
-  // every wavefront of the block, so the barriers below are uniform
-  for (int64_t b = blockIdx.x; b * (CB / 64) < nwave; b += gridDim.x) {
-    const int64_t w = b * (CB / 64) + wv;
-    const int64_t j = w * 64 + lane;
-    const int32_t X = j < nl ? c.vlisted[j] : 0;
-    const int32_t cnt = j < nl ? c.vcount[X] : 0;
-    // inclusive prefix of the counts over the wavefront
-    int32_t pre = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t v = __shfl_up(pre, o, 64);
-      if (lane >= o) pre += v;
-    }
-    sx[wv][lane] = X;
-    spre[wv][lane + 1] = pre;
-    if (lane == 0) spre[wv][0] = 0;
-    const int32_t total = __shfl(pre, 63, 64);
-    __syncthreads();
-    for (int32_t q0 = 0; q0 < total; q0 += 64) {
-      const int32_t q = q0 + lane;
-      if (q < total) {
-        // owner: the last element whose exclusive prefix is <= q
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (spre[wv][mid] <= q) lo = mid;
-          else hi = mid - 1;
-        }
-        const int32_t Xq = sx[wv][lo];
-        const int32_t t = q - spre[wv][lo];
-        const int32_t code = c.vlist[(int64_t)Xq * c.vk + t];
-        const int32_t Y = list_partner(code);
-        const int32_t sX = c.es[Xq], sY = c.es[Y];
-        const double rX = c.crad[sX], rY = c.crad[sY];
-        double xlo[3], xhi[3], ylo[3], yhi[3];
-        elem_box(d, N, c.en0[Xq], c.en1[Xq], rX, xlo, xhi);
-        elem_box(d, N, c.en0[Y], c.en1[Y], rY, ylo, yhi);
-        bool ov = true;
-        for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
-        if (ov) {
-          const bool XisA = sX < sY;
-          if (XisA) ncand++;
-          if (code >= 0) { // not blacklisted at INIT
-            double pX[2][3], pY[2][3], g0[3], g1[3];
-            elem_pos(d, c.en0[Xq], c.en1[Xq], pX);
-            elem_pos(d, c.en0[Y], c.en1[Y], pY);
-            const double R = rX + rY;
-            const bool closed = XisA ? cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                                     : cm_narrowphase<false, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
-            if (closed) {
-              const int64_t bit = (int64_t)Xq * c.vkb + t;
-              atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
-              atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
-              if (XisA) nact++;
-            }
-          }
+  for (int64_t q = (int64_t)blockIdx.x * CB + threadIdx.x; q < ne; q += (int64_t)gridDim.x * CB) {
+    const uint32_t pos = c.ventry[q];
+    const int32_t Xq = (int32_t)(pos / vk);
+    const int32_t t = (int32_t)(pos - (uint32_t)Xq * vk);
+    const int32_t code = c.vlist[pos];
+    const int32_t Y = list_partner(code);
+    const int32_t sX = c.es[Xq], sY = c.es[Y];
+    const double rX = c.crad[sX], rY = c.crad[sY];
+    double xlo[3], xhi[3], ylo[3], yhi[3];
+    elem_box(d, N, c.en0[Xq], c.en1[Xq], rX, xlo, xhi);
+    elem_box(d, N, c.en0[Y], c.en1[Y], rY, ylo, yhi);
+    bool ov = true;
+    for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
+    if (ov) {
+      const bool XisA = sX < sY;
+      if (XisA) ncand++;
+      if (code >= 0) { // not blacklisted at INIT
+        double pX[2][3], pY[2][3], g0[3], g1[3];
+        elem_pos(d, c.en0[Xq], c.en1[Xq], pX);
+        elem_pos(d, c.en0[Y], c.en1[Y], pY);
+        const double R = rX + rY;
+        const bool closed = XisA ? cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                                 : cm_narrowphase<false, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+        if (closed) {
+          const int64_t bit = (int64_t)Xq * c.vkb + t;
+          atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
+          atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
+          if (XisA) nact++;
         }
       }
     }
-    __syncthreads(); // the wavefront's LDS rows are reused by its next round
   }
   for (int o = 32; o > 0; o >>= 1) {
     nact += __shfl_xor(nact, o, 64);
@@ -973,11 +949,13 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
     if ((rc = calloc_dev(cs, &c.vlist, (size_t)E * c.vk))) return rc;
     c.vkb = (c.vk + 31) & ~31;
-    if ((rc = calloc_dev(cs, &c.vlisted, E))) return rc;
-    if ((rc = calloc_dev(cs, &c.nlisted, 1))) return rc;
+    if ((uint64_t)E * (uint64_t)c.vk > (uint64_t)UINT32_MAX)
+      return dl_set_error(DL_ERR_INPUT, "n_elems * (max_pairs / n_elems) must stay below 2^32");
+    if ((rc = calloc_dev(cs, &c.ventry, (size_t)E * c.vk))) return rc;
+    if ((rc = calloc_dev(cs, &c.nentries, 1))) return rc;
     if ((rc = calloc_dev(cs, &c.vmark, (size_t)E * (c.vkb / 32)))) return rc;
     if ((rc = calloc_dev(cs, &c.cnew, (E + 63) / 64))) return rc;
-    CHK(hipMemset(c.nlisted, 0, sizeof(int32_t)));
+    CHK(hipMemset(c.nentries, 0, sizeof(int32_t)));
     CHK(hipMemset(c.vmark, 0, sizeof(uint32_t) * (size_t)E * (c.vkb / 32)));
     CHK(hipMemset(c.cnew, 0, sizeof(unsigned long long) * ((E + 63) / 64)));
     CHK(hipMemset(c.vcount, 0, sizeof(int32_t) * E));
@@ -1118,7 +1096,7 @@ int contact_force(ContactState &cs, const DevData &d, hipStream_t s) {
 static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
   ContactDev &c = cs.c;
   const int32_t *g = c.gate;
-  broadphase(cs, d, s, c.skin, g, c.nlisted);
+  broadphase(cs, d, s, c.skin, g, c.nentries);
   hipLaunchKernelGGL(kc_vbuild, dim3(gs(std::max(c.E, c.N))), dim3(CB), 0, s, c, d, g);
 }
 
