@@ -459,6 +459,7 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       d.lane_ident = lane_e[L] == (L < E ? (int32_t)L : -1);
     uint8_t *pu;
     TRY(upload(c, &pu, eflag)); d.eflag = pu;
+    c->eflag_h = eflag;
     // end-b frames for k_strut: by shuffle from the next lane where it holds
     // the same frame bit for bit, else the per-element array
     {
@@ -540,6 +541,7 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   TRY(dalloc(c, &c->diss_total, 1));
   HIPCHK(hipMemset(c->diss_total, 0, sizeof(double)));
   d.rbuf = nullptr;
+  d.rslot_e = nullptr;
   d.react_dof = 2;
   // contact
   if (contact && contact->enabled) {
@@ -847,15 +849,35 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
   if (n_groups == 0) egp.assign(1, 0);
   dfree(c, c->d.rbuf);
   dfree(c, (void *)c->d.rent);
+  dfree(c, (void *)c->d.rslot_e);
   dfree(c, c->rgptr);
   c->d.rbuf = nullptr;
   c->d.rent = nullptr;
+  c->d.rslot_e = nullptr;
   c->rgptr = nullptr;
   c->d.nrent = (int64_t)rent.size();
   int rc;
   int32_t *pe;
   if ((rc = upload(c, &pe, rent))) return rc;
   c->d.rent = pe;
+  // the step writes the entries from k_strut: per element the entry of each
+  // end (or -1), and flag bit 3 on the elements that have one
+  {
+    std::vector<uint8_t> fl = c->eflag_h;
+    for (auto &x : fl) x &= (uint8_t)~8u;
+    if (!rent.empty()) {
+      std::vector<int32_t> rs(2 * (size_t)m.E, -1);
+      for (size_t k = 0; k < rent.size(); k++) {
+        const int32_t e = rent[k] >> 1, end = rent[k] & 1;
+        rs[(size_t)end * m.E + e] = (int32_t)k;
+        fl[e] |= 8u;
+      }
+      int32_t *pr;
+      if ((rc = upload(c, &pr, rs))) return rc;
+      c->d.rslot_e = pr;
+    }
+    HIPCHK(hipMemcpy((void *)c->d.eflag, fl.data(), fl.size(), hipMemcpyHostToDevice));
+  }
   double *rb;
   if ((rc = dalloc(c, &rb, (size_t)std::max<int64_t>(c->d.nrent, 1)))) return rc;
   c->d.rbuf = rb;
@@ -1114,8 +1136,8 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   }
   if (ev && !(c->contact && c->cs.c.skin > 0.0)) HIPCHK(hipEventRecord(ev[1], c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
-  // the previous step's GroupOutput row, at this step's start state
-  launch_react(c->d, c->so3, c->plastic, c->contact, c->stream);
+  // the previous step's GroupOutput row, at this step's start state: written
+  // by k_strut from the trial stress (see there)
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));

@@ -192,6 +192,26 @@ DL_HD void elem_forces(const ElemKin &k, const double sm[6], double fa[6], doubl
   }
 }
 
+// component DOF of elem_forces's fa and fb, by the same operations (the
+// reaction rows of the step, which need one component only)
+template <int DOF>
+DL_HD void elem_force_dof(const ElemKin &k, const double sm[6], double *va, double *vb) {
+  if (DOF < 3) {
+    constexpr int i = DOF < 3 ? DOF : 0;
+    const double n = k.R[3 * i] * sm[0] + k.R[3 * i + 1] * sm[1] + k.R[3 * i + 2] * sm[2];
+    *va = k.w * (k.g0 * n);
+    *vb = k.w * (k.g1 * n);
+  } else {
+    constexpr int i = DOF < 3 ? 0 : DOF - 3, i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+    const double n1 = k.R[3 * i1] * sm[0] + k.R[3 * i1 + 1] * sm[1] + k.R[3 * i1 + 2] * sm[2];
+    const double n2 = k.R[3 * i2] * sm[0] + k.R[3 * i2 + 1] * sm[1] + k.R[3 * i2 + 2] * sm[2];
+    const double m = k.R[3 * i] * sm[3] + k.R[3 * i + 1] * sm[4] + k.R[3 * i + 2] * sm[5];
+    const double c = k.phiP[i1] * n2 - k.phiP[i2] * n1;
+    *va = k.w * (-0.5 * c + k.g0 * m);
+    *vb = k.w * (-0.5 * c + k.g1 * m);
+  }
+}
+
 // ---------------------------------------------------------------------
 // elastoplastic rod material, power-sum yield family
 //   f = sum_i |w_i (s_i + ck_i q_i)|^p_i - Y (1 + ci q_0),  q = -H alpha

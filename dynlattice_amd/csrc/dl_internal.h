@@ -44,7 +44,8 @@ struct DevData {
   const int32_t *en1;     // [E] node b
   const int32_t *es;      // [E] strut
   const uint8_t *eflag;   // [E] bit 0: first element of its chain, bit 1: last,
-                          //     bit 2: end frames -I (strut along -e3; elem_mirror)
+                          //     bit 2: end frames -I (strut along -e3; elem_mirror),
+                          //     bit 3: an end at a reaction-group node (rslot_e)
   const int32_t *sa;      // [E] slot of node a if shared, else -1
   const int32_t *sb;      // [E] slot of node b (chain end) if shared, else -1
   // elements ([c][E])
@@ -86,6 +87,9 @@ struct DevData {
   const int32_t *rent;  // [nrent]
   int64_t nrent;
   double *rbuf;         // [nrent]
+  // entry of the element's end a ([0][E]) and end b ([1][E]) or -1: the
+  // step's k_strut writes the rows of the elements flagged in eflag bit 3
+  const int32_t *rslot_e;
   double *dbuf;         // [R] per-strut dissipation increment
   int32_t *err;         // device error flag
   // adjoint tape row of the current step ([19][N]: u 3, rot 4, v 6, f 6) or null

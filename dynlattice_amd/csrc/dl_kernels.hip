@@ -191,8 +191,39 @@ __global__ __launch_bounds__(DL_BLOCK) void k_init_strain(DevData d) {
 // slot and the joint kernel finishes it.
 // ------------------------------------------------------------------
 // MAT: 0 elastic; 1, 2, 3 elastoplastic with yield exponents all 1, all 2, general
+// GroupOutput rows of an element with an end at a reaction-group node
+// (GroupOutputModule.cpp:100-116, SpecialCosseratRodModel.cpp:762): the
+// output load case at the step's start state is the elastic predictor from
+// the committed plastic state -- the trial stress st of this step -- plus
+// the element's contact row; the same operations as k_react.
+template <bool CONTACT>
+__device__ __forceinline__ void react_rows(const DevData &d, const ElemKin &k, const double st[6],
+                                           int32_t e) {
+  const int64_t E = d.E;
+  const int dof = d.react_dof;
+  double va = 0.0, vb = 0.0;
+  switch (dof) {
+  case 0: elem_force_dof<0>(k, st, &va, &vb); break;
+  case 1: elem_force_dof<1>(k, st, &va, &vb); break;
+  case 2: elem_force_dof<2>(k, st, &va, &vb); break;
+  case 3: elem_force_dof<3>(k, st, &va, &vb); break;
+  case 4: elem_force_dof<4>(k, st, &va, &vb); break;
+  default: elem_force_dof<5>(k, st, &va, &vb); break;
+  }
+  const bool cb = CONTACT && dof < 3 && ((d.cbits[e >> 6] >> (e & 63)) & 1ull);
+  const int32_t ka = d.rslot_e[e], kb = d.rslot_e[E + e];
+  if (ka >= 0) {
+    if (cb) va += d.fcs[dof * E + e];
+    d.rbuf[ka] = va;
+  }
+  if (kb >= 0) {
+    if (cb) vb += d.fcs[(3 + dof) * E + e];
+    d.rbuf[kb] = vb;
+  }
+}
+
 template <int MODE, bool SO3, int MAT, bool CONTACT>
-__global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, PlasticParams pp,
+__global__ __launch_bounds__(DL_BLOCK, MAT == 3 ? 1 : 3) void k_strut(DevData d, StepParams sp, PlasticParams pp,
                                                     MilneDev m) {
   constexpr bool PLASTIC = MAT != 0;
   const int64_t L = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
@@ -257,6 +288,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
     if (!PLASTIC) {
 #pragma unroll
       for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
+      if (MODE == MODE_STEP && (fl & 8u)) react_rows<CONTACT>(d, k, sm, e);
       if (MODE != MODE_STEP) {
 #pragma unroll
         for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
@@ -299,6 +331,14 @@ __global__ __launch_bounds__(DL_BLOCK) void k_strut(DevData d, StepParams sp, Pl
       k.w = stash[18][tx];
       k.g0 = stash[19][tx];
       k.g1 = stash[20][tx];
+      if (MODE == MODE_STEP && (fl & 8u)) {
+        // the reaction rows from the trial stress: the committed plastic
+        // strain is still the one in memory here
+        double st[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) st[i] = C[i] * (strain[i] - d.ep[i * E + e]);
+        react_rows<CONTACT>(d, k, st, e);
+      }
       if (it < 0) *d.err = -3;
       if (MODE == MODE_STEP) {
         if (sp.track_diss) {
