@@ -342,13 +342,20 @@ def crush_bcs(p: Problem, v_top: float):
 def bcc_c2() -> Problem:
     """Elastic BCC impact: 10^3 cells of 10 mm, 4 elements per strut,
     r = 0.5 mm (1 + 0.05 N(0,1)), seed 2, E = 2 GPa, nu = 0.3, rho = 1200,
-    bottom clamped, top at -5 m/s."""
+    bottom clamped, top at -5 m/s, dt = 0.25 x the stable estimate.
+
+    20,000 steps at that dt take the top face to 42 % nominal strain.  At
+    0.5 x the estimate the same 20,000 steps would reach 84 %, and the
+    explicit SO(3) update (the oracle's as much as the device's) overflows
+    near step 13,500 (57 % strain) of this lattice without contact; at 0.25
+    a 1e-15 relative perturbation of the initial velocities stays below
+    3e-12 in the state and the reaction over the whole run."""
     n, m, cell = 10, 4, 10e-3
     R = 8 * n ** 3
     r = 0.5e-3 * (1.0 + 0.05 * normal(2, R))
     p = bcc_lattice(n, m, cell, r, 2e9, 0.3, 1200.0)
     crush_bcs(p, -5.0)
-    p.dt = stable_dt(p, 0.5)
+    p.dt = stable_dt(p, 0.25)
     p.name = "bcc-10x10x10-m4-elastic"
     return p
 
