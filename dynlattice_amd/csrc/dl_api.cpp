@@ -1141,11 +1141,23 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));
-  launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
-  launch_impact_step(c->d, sp, c->imp, c->so3, c->stream);
-  if (ev) HIPCHK(hipEventRecord(ev[5], c->stream));
-  launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
-                  c->hist_react, c->hist_diss, c->step_ctr, c->stream);
+  // the end-of-step sums ride in the joint kernel's launch (one block more)
+  // unless impact groups update nodes in between, there are no shared nodes
+  // or the sums need several blocks
+  const bool fuse = c->d.NJ > 0 && c->imp.ngroups == 0 &&
+                    step_end_fusable(c->n_groups, c->track_diss);
+  if (fuse) {
+    const StepEnd se = make_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks,
+                                     c->diss_total, c->hist_react, c->hist_diss, c->step_ctr);
+    launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream, nullptr, &se);
+    if (ev) HIPCHK(hipEventRecord(ev[5], c->stream));
+  } else {
+    launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream);
+    launch_impact_step(c->d, sp, c->imp, c->so3, c->stream);
+    if (ev) HIPCHK(hipEventRecord(ev[5], c->stream));
+    launch_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks, c->diss_total,
+                    c->hist_react, c->hist_diss, c->step_ctr, c->stream);
+  }
   launch_clock_advance(c->pd, sp.dt, c->stream);
   return DL_OK;
 }

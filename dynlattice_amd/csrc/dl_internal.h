@@ -168,6 +168,22 @@ struct ImpDev {
   int32_t nbc;
 };
 
+// end-of-step sums of an explicit step (k_step_end1, or one more block of
+// the step's k_joint launch): the reaction groups' rows and the dissipation
+struct StepEnd {
+  const double *rbuf;
+  const int32_t *rgptr;
+  int32_t ngroups;
+  int32_t nsums;        // ngroups (+ 1 with the dissipation sum)
+  const double *dbuf;
+  int64_t ndb;
+  double *diss_total;
+  double *hist_react;
+  double *hist_diss;
+  int32_t *step;
+  int32_t tail;         // reaction row only, the counter stays
+};
+
 // MilneDevice attempts: predicted then corrected state, error partials and
 // the step-size control state, all on the device
 struct MilneDev {
@@ -203,7 +219,7 @@ void launch_init_strain(const DevData &d, hipStream_t s);
 void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams &pp, int mode,
                      bool so3, int mat, bool contact, hipStream_t s, const MilneDev *m = nullptr);
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s,
-                  const MilneDev *m = nullptr);
+                  const MilneDev *m = nullptr, const StepEnd *se = nullptr);
 void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 // step end: the reaction groups' sums of rbuf into history row *step - 1
 // (the reaction of the previous step, evaluated at this step's start state;
@@ -213,6 +229,12 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s, bool tail = false);
+// the sums of launch_step_end for the one-block case (<= 8 sums), to hand to
+// launch_joint; false where they need their own launch
+bool step_end_fusable(int32_t ngroups, bool diss);
+StepEnd make_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
+                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
+                      int32_t *step, bool tail = false);
 // GroupOutput reaction entries at the current state (after the contact pass
 // of this state): rbuf[k] = the dof share of entry k
 void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s);

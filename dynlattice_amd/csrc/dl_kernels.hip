@@ -539,8 +539,128 @@ __device__ __forceinline__ void joint_force(const DevData &d, int32_t j, double 
   }
 }
 
-template <int MODE, bool SO3>
-__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m) {
+// ------------------------------------------------------------------
+// end of step: per-group reaction sums (the reaction of the state the step
+// started from, i.e. the previous step's GroupOutput row) and the
+// dissipation increment, in a fixed tree order (deterministic)
+// ------------------------------------------------------------------
+// one sum (reaction group g < ngroups, else the dissipation partials)
+// Sums g0 .. g0 + P - 1 (those below nsums) side by side: their loads are
+// in flight together and they share the tree's barriers.  Each sum keeps
+// its own fixed order.
+// rrow: history row of the reaction sums (< 0: not written); drow: row of
+// the dissipation sum
+#define SE_NT 256 // threads of a step-end block
+template <int P>
+__device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *rbuf,
+                                              const int32_t *rgptr, int32_t ngroups,
+                                              const double *dbuf, int64_t ndb,
+                                              double *diss_total, double *hist_react,
+                                              double *hist_diss, int64_t rrow, int64_t drow,
+                                              double (*red)[SE_NT]) {
+  const double *in[P];
+  int64_t n[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    const int g = g0 + q;
+    if (g >= nsums) {
+      in[q] = dbuf;
+      n[q] = 0;
+    } else if (g < ngroups) {
+      in[q] = rbuf + rgptr[g];
+      n[q] = rgptr[g + 1] - rgptr[g];
+    } else {
+      in[q] = dbuf;
+      n[q] = ndb;
+    }
+  }
+  // thread t sums in[t], in[t + SE_NT], ... in that order, four loads per
+  // sum in flight at a time
+  double s[P];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    s[q] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * SE_NT < n[q]; i += 4 * SE_NT) {
+      const double a0 = in[q][i], a1 = in[q][i + SE_NT], a2 = in[q][i + 2 * SE_NT],
+                   a3 = in[q][i + 3 * SE_NT];
+      s[q] += a0;
+      s[q] += a1;
+      s[q] += a2;
+      s[q] += a3;
+    }
+    for (; i < n[q]; i += SE_NT) s[q] += in[q][i];
+    red[q][threadIdx.x] = s[q];
+  }
+  // tree: red[t] += red[t + w] for w = SE_NT / 2 .. 1; the levels below 64
+  // run in the first wavefront as shuffles (same pairs, same order)
+  __syncthreads();
+  for (int w = SE_NT / 2; w >= 64; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < P; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      double r = red[q][threadIdx.x];
+      for (int w = 32; w > 0; w >>= 1) r += __shfl_down(r, w, 64);
+      const int g = g0 + q;
+      if (threadIdx.x == 0 && g < nsums) {
+        if (g < ngroups) {
+          if (rrow >= 0) hist_react[rrow * ngroups + g] = r;
+        } else {
+          const double t = *diss_total + r;
+          *diss_total = t;
+          hist_diss[drow] = t;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// several sums, one block each (the step counter moves in k_step_inc)
+__global__ __launch_bounds__(SE_NT) void k_step_end(const double *rbuf, const int32_t *rgptr,
+                                                    int32_t ngroups, const double *dbuf, int64_t ndb,
+                                                    double *diss_total, double *hist_react,
+                                                    double *hist_diss, const int32_t *step) {
+  __shared__ double red[1][SE_NT];
+  const int64_t st = *step;
+  step_end_sums<1>(blockIdx.x, blockIdx.x + 1, rbuf, rgptr, ngroups, dbuf, ndb, diss_total,
+                   hist_react, hist_diss, st - 1, st, red);
+}
+
+// few sums: one block does them two at a time and then moves the step
+// counter; tail: the reaction row only.  Run by k_step_end1 or by the last
+// block of the step's k_joint launch (no other block of it reads the counter)
+__device__ __forceinline__ void step_end_block(const StepEnd &se, double (*red)[SE_NT]) {
+  const int64_t st = *se.step;
+  for (int g = 0; g < se.nsums; g += 2)
+    step_end_sums<2>(g, se.nsums, se.rbuf, se.rgptr, se.ngroups, se.dbuf, se.ndb, se.diss_total,
+                     se.hist_react, se.hist_diss, st - 1, st, red);
+  __syncthreads();
+  if (threadIdx.x == 0 && !se.tail) *se.step = (int32_t)(st + 1);
+}
+
+__global__ __launch_bounds__(SE_NT) void k_step_end1(StepEnd se) {
+  __shared__ double red[2][SE_NT];
+  step_end_block(se, red);
+}
+
+template <int MODE, bool SO3, bool SE>
+__global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m, StepEnd se) {
+  static_assert(DL_BLOCK == SE_NT, "a step-end block is a k_joint block");
+  if constexpr (SE) {
+    // the last block: the step's end sums (their inputs come from k_strut)
+    if (blockIdx.x == gridDim.x - 1) {
+      __shared__ double red[2][SE_NT];
+      step_end_block(se, red);
+      return;
+    }
+  }
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (MODE == MODE_MCORR) {
     // every thread joins the block's error partial sums
@@ -648,115 +768,6 @@ __global__ __launch_bounds__(DL_BLOCK) void k_react(DevData d) {
   if (CONTACT && dof < 3 && ((d.cbits[e >> 6] >> (e & 63)) & 1ull))
     v += d.fcs[(3 * end + dof) * E + e];
   d.rbuf[k] = v;
-}
-
-// ------------------------------------------------------------------
-// end of step: per-group reaction sums (the reaction of the state the step
-// started from, i.e. the previous step's GroupOutput row) and the
-// dissipation increment, in a fixed tree order (deterministic)
-// ------------------------------------------------------------------
-// one sum (reaction group g < ngroups, else the dissipation partials)
-// Sums g0 .. g0 + P - 1 (those below nsums) side by side: their loads are
-// in flight together and they share the tree's barriers.  Each sum keeps
-// its own fixed order.
-// rrow: history row of the reaction sums (< 0: not written); drow: row of
-// the dissipation sum
-template <int P>
-__device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *rbuf,
-                                              const int32_t *rgptr, int32_t ngroups,
-                                              const double *dbuf, int64_t ndb,
-                                              double *diss_total, double *hist_react,
-                                              double *hist_diss, int64_t rrow, int64_t drow,
-                                              double (*red)[1024]) {
-  const double *in[P];
-  int64_t n[P];
-#pragma unroll
-  for (int q = 0; q < P; q++) {
-    const int g = g0 + q;
-    if (g >= nsums) {
-      in[q] = dbuf;
-      n[q] = 0;
-    } else if (g < ngroups) {
-      in[q] = rbuf + rgptr[g];
-      n[q] = rgptr[g + 1] - rgptr[g];
-    } else {
-      in[q] = dbuf;
-      n[q] = ndb;
-    }
-  }
-  // thread t sums in[t], in[t + 1024], ... in that order, four loads per
-  // sum in flight at a time
-  double s[P];
-#pragma unroll
-  for (int q = 0; q < P; q++) {
-    s[q] = 0.0;
-    int64_t i = threadIdx.x;
-    for (; i + 3 * 1024 < n[q]; i += 4 * 1024) {
-      const double a0 = in[q][i], a1 = in[q][i + 1024], a2 = in[q][i + 2048],
-                   a3 = in[q][i + 3072];
-      s[q] += a0;
-      s[q] += a1;
-      s[q] += a2;
-      s[q] += a3;
-    }
-    for (; i < n[q]; i += 1024) s[q] += in[q][i];
-    red[q][threadIdx.x] = s[q];
-  }
-  // tree: red[t] += red[t + w] for w = 512 .. 1; the levels below 64 run in
-  // the first wavefront as shuffles (same pairs, same order)
-  __syncthreads();
-  for (int w = 512; w >= 64; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-#pragma unroll
-      for (int q = 0; q < P; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 64) {
-#pragma unroll
-    for (int q = 0; q < P; q++) {
-      double r = red[q][threadIdx.x];
-      for (int w = 32; w > 0; w >>= 1) r += __shfl_down(r, w, 64);
-      const int g = g0 + q;
-      if (threadIdx.x == 0 && g < nsums) {
-        if (g < ngroups) {
-          if (rrow >= 0) hist_react[rrow * ngroups + g] = r;
-        } else {
-          const double t = *diss_total + r;
-          *diss_total = t;
-          hist_diss[drow] = t;
-        }
-      }
-    }
-  }
-  __syncthreads();
-}
-
-// several sums, one block each (the step counter moves in k_step_inc)
-__global__ __launch_bounds__(1024) void k_step_end(const double *rbuf, const int32_t *rgptr,
-                                                   int32_t ngroups, const double *dbuf, int64_t ndb,
-                                                   double *diss_total, double *hist_react,
-                                                   double *hist_diss, const int32_t *step) {
-  __shared__ double red[1][1024];
-  const int64_t st = *step;
-  step_end_sums<1>(blockIdx.x, blockIdx.x + 1, rbuf, rgptr, ngroups, dbuf, ndb, diss_total,
-                   hist_react, hist_diss, st - 1, st, red);
-}
-
-// few sums: one block does them two at a time and then moves the step counter
-// (saves the counter's own launch every step); tail: the reaction row only
-__global__ __launch_bounds__(1024) void k_step_end1(const double *rbuf, const int32_t *rgptr,
-                                                    int32_t ngroups, int32_t nsums,
-                                                    const double *dbuf, int64_t ndb,
-                                                    double *diss_total, double *hist_react,
-                                                    double *hist_diss, int32_t *step, int tail) {
-  __shared__ double red[2][1024];
-  const int64_t st = *step;
-  for (int g = 0; g < nsums; g += 2)
-    step_end_sums<2>(g, nsums, rbuf, rgptr, ngroups, dbuf, ndb, diss_total, hist_react, hist_diss,
-                     st - 1, st, red);
-  __syncthreads();
-  if (threadIdx.x == 0 && !tail) *step = (int32_t)(st + 1);
 }
 
 __global__ void k_step_inc(int32_t *step) {
@@ -1016,14 +1027,28 @@ void launch_strut_pp(const DevData &d, const StepParams &sp, const PlasticParams
 template <int MODE>
 static void lj(const DevData &d, const StepParams &sp, const MilneDev &m, bool so3, hipStream_t s) {
   const dim3 g(nblk(d.NJ)), b(DL_BLOCK);
-  if (so3) hipLaunchKernelGGL((k_joint<MODE, true>), g, b, 0, s, d, sp, m);
-  else hipLaunchKernelGGL((k_joint<MODE, false>), g, b, 0, s, d, sp, m);
+  const StepEnd se{};
+  if (so3) hipLaunchKernelGGL((k_joint<MODE, true, false>), g, b, 0, s, d, sp, m, se);
+  else hipLaunchKernelGGL((k_joint<MODE, false, false>), g, b, 0, s, d, sp, m, se);
+}
+
+// the step's joint kernel with the end-of-step sums as its last block
+static void lj_end(const DevData &d, const StepParams &sp, bool so3, const StepEnd &se,
+                   hipStream_t s) {
+  const dim3 g(nblk(d.NJ) + 1), b(DL_BLOCK);
+  const MilneDev m{};
+  if (so3) hipLaunchKernelGGL((k_joint<MODE_STEP, true, true>), g, b, 0, s, d, sp, m, se);
+  else hipLaunchKernelGGL((k_joint<MODE_STEP, false, true>), g, b, 0, s, d, sp, m, se);
 }
 
 void launch_joint(const DevData &d, const StepParams &sp, int mode, bool so3, hipStream_t s,
-                  const MilneDev *m) {
+                  const MilneDev *m, const StepEnd *se) {
   if (d.NJ == 0) return;
   const MilneDev mm = m ? *m : MilneDev{};
+  if (se && mode == MODE_STEP) {
+    lj_end(d, sp, so3, *se, s);
+    return;
+  }
   switch (mode) {
   case MODE_STEP: lj<MODE_STEP>(d, sp, mm, so3, s); break;
   case MODE_FORCE: lj<MODE_FORCE>(d, sp, mm, so3, s); break;
@@ -1038,19 +1063,29 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
   else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, 1);
 }
 
+bool step_end_fusable(int32_t ngroups, bool diss) { return ngroups + (diss ? 1 : 0) <= 8; }
+
+StepEnd make_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
+                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
+                      int32_t *step, bool tail) {
+  if (tail) diss = false;
+  return StepEnd{(const double *)d.rbuf, rgptr, ngroups, ngroups + (diss ? 1 : 0),
+                 (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss, step, tail ? 1 : 0};
+}
+
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s, bool tail) {
   if (tail) diss = false;
   const int nb = ngroups + (diss ? 1 : 0);
   if (tail && nb == 0) return;
-  if (nb <= 8) {
-    hipLaunchKernelGGL(k_step_end1, dim3(1), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
-                       ngroups, nb, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
-                       step, (int)tail);
+  if (step_end_fusable(ngroups, diss)) {
+    const StepEnd se = make_step_end(d, rgptr, ngroups, diss, ndb, diss_total, hist_react,
+                                     hist_diss, step, tail);
+    hipLaunchKernelGGL(k_step_end1, dim3(1), dim3(SE_NT), 0, s, se);
     return;
   }
-  hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(1024), 0, s, (const double *)d.rbuf, rgptr,
+  hipLaunchKernelGGL(k_step_end, dim3(nb), dim3(SE_NT), 0, s, (const double *)d.rbuf, rgptr,
                      ngroups, (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss,
                      (const int32_t *)step);
   if (!tail) hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step);
