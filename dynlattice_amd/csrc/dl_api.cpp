@@ -459,7 +459,6 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
       d.lane_ident = lane_e[L] == (L < E ? (int32_t)L : -1);
     uint8_t *pu;
     TRY(upload(c, &pu, eflag)); d.eflag = pu;
-    c->eflag_h = eflag;
     // end-b frames for k_strut: by shuffle from the next lane where it holds
     // the same frame bit for bit, else the per-element array
     {
@@ -505,10 +504,11 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
     TRY(upload(c, &d.ecur, z6e));
     TRY(upload(c, &d.eo, z6e));
     if (plastic) {
-      TRY(upload(c, &d.ep, z6e));
-      TRY(upload(c, &d.ep_cur, z6e));
-      TRY(upload(c, &d.al, z7e));
-      TRY(upload(c, &d.al_cur, z7e));
+      std::vector<double> z12e(12 * E, 0.0), z1e(E, 0.0);
+      TRY(upload(c, &d.pst, z12e));
+      TRY(upload(c, &d.pst_cur, z12e));
+      TRY(upload(c, &d.aiso, z1e));
+      TRY(upload(c, &d.aiso_cur, z1e));
       std::vector<double> zE(E, 0.0);
       TRY(upload(c, &d.dflow, zE));
       std::vector<uint8_t> zb(E, 0);
@@ -863,7 +863,9 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
   // the step writes the entries from k_strut: per element the entry of each
   // end (or -1), and flag bit 3 on the elements that have one
   {
-    std::vector<uint8_t> fl = c->eflag_h;
+    // the device flags (bit 4 is kept: the kernels set it)
+    std::vector<uint8_t> fl(m.E);
+    HIPCHK(hipMemcpy(fl.data(), c->d.eflag, m.E, hipMemcpyDeviceToHost));
     for (auto &x : fl) x &= (uint8_t)~8u;
     if (!rent.empty()) {
       std::vector<int32_t> rs(2 * (size_t)m.E, -1);
@@ -876,7 +878,7 @@ extern "C" int dl_set_reaction_groups(dl_ctx *c, const int32_t *nodes, const int
       if ((rc = upload(c, &pr, rs))) return rc;
       c->d.rslot_e = pr;
     }
-    HIPCHK(hipMemcpy((void *)c->d.eflag, fl.data(), fl.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d.eflag, fl.data(), fl.size(), hipMemcpyHostToDevice));
   }
   double *rb;
   if ((rc = dalloc(c, &rb, (size_t)std::max<int64_t>(c->d.nrent, 1)))) return rc;
@@ -1080,8 +1082,8 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
   }
   case DL_CANCEL: {
     if (c->plastic) {
-      HIPCHK(hipMemcpyAsync(c->d.ep_cur, c->d.ep, 6 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->d.al_cur, c->d.al, 7 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->d.pst_cur, c->d.pst, 12 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->d.aiso_cur, c->d.aiso, E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemsetAsync(c->d.dflow, 0, E * sizeof(double), c->stream));
     }
     HIPCHK(hipMemcpyAsync(c->d.ecur, c->d.eo, 6 * E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1413,16 +1415,18 @@ extern "C" int dl_plastic_state(dl_ctx *c, double *eps_p, double *alpha, int64_t
     if (alpha) std::fill(alpha, alpha + 7 * E, 0.0);
     return DL_OK;
   }
-  std::vector<double> h(7 * E);
-  if (eps_p) {
-    HIPCHK(hipMemcpy(h.data(), c->d.ep, 6 * E * sizeof(double), hipMemcpyDeviceToHost));
-    for (int64_t e = 0; e < E; e++)
-      for (int i = 0; i < 6; i++) eps_p[6 * e + i] = h[i * E + e];
-  }
-  if (alpha) {
-    HIPCHK(hipMemcpy(h.data(), c->d.al, 7 * E * sizeof(double), hipMemcpyDeviceToHost));
-    for (int64_t e = 0; e < E; e++)
-      for (int i = 0; i < 7; i++) alpha[7 * e + i] = h[i * E + e];
+  // element records: ep (6), alpha_1..6 (6); alpha_0 apart.  A record
+  // whose eflag bit 4 is clear was never written and is zero.
+  std::vector<double> h(12 * E), a0(E);
+  HIPCHK(hipMemcpy(h.data(), c->d.pst, 12 * E * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(a0.data(), c->d.aiso, E * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < E; e++) {
+    if (eps_p)
+      for (int i = 0; i < 6; i++) eps_p[6 * e + i] = h[12 * e + i];
+    if (alpha) {
+      alpha[7 * e] = a0[e];
+      for (int i = 0; i < 6; i++) alpha[7 * e + 1 + i] = h[12 * e + 6 + i];
+    }
   }
   return DL_OK;
 }
@@ -1436,19 +1440,29 @@ extern "C" int dl_set_plastic_state(dl_ctx *c, const double *eps_p, const double
   hipSetDevice(c->device);
   c->adj.valid = false;
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<double> h(7 * E);
-  if (eps_p) {
-    for (int64_t e = 0; e < E; e++)
-      for (int i = 0; i < 6; i++) h[i * E + e] = eps_p[6 * e + i];
-    HIPCHK(hipMemcpy(c->d.ep, h.data(), 6 * E * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d.ep_cur, h.data(), 6 * E * sizeof(double), hipMemcpyHostToDevice));
+  // merge into the device records (a NULL part stays), then flag the
+  // elements whose record is nonzero (eflag bit 4) and clear the others
+  std::vector<double> h(12 * E), a0(E);
+  std::vector<uint8_t> fl(E);
+  HIPCHK(hipMemcpy(h.data(), c->d.pst, 12 * E * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(a0.data(), c->d.aiso, E * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(fl.data(), c->d.eflag, E, hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < E; e++) {
+    if (eps_p)
+      for (int i = 0; i < 6; i++) h[12 * e + i] = eps_p[6 * e + i];
+    if (alpha) {
+      a0[e] = alpha[7 * e];
+      for (int i = 0; i < 6; i++) h[12 * e + 6 + i] = alpha[7 * e + 1 + i];
+    }
+    bool nz = a0[e] != 0.0;
+    for (int i = 0; i < 12; i++) nz = nz || h[12 * e + i] != 0.0;
+    fl[e] = nz ? (uint8_t)(fl[e] | 16u) : (uint8_t)(fl[e] & ~16u);
   }
-  if (alpha) {
-    for (int64_t e = 0; e < E; e++)
-      for (int i = 0; i < 7; i++) h[i * E + e] = alpha[7 * e + i];
-    HIPCHK(hipMemcpy(c->d.al, h.data(), 7 * E * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d.al_cur, h.data(), 7 * E * sizeof(double), hipMemcpyHostToDevice));
-  }
+  HIPCHK(hipMemcpy(c->d.pst, h.data(), 12 * E * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d.pst_cur, h.data(), 12 * E * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d.aiso, a0.data(), E * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d.aiso_cur, a0.data(), E * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d.eflag, fl.data(), E, hipMemcpyHostToDevice));
   return DL_OK;
 }
 

@@ -152,7 +152,6 @@ struct dl_ctx {
   std::vector<double> pl_weights; // [R][6] yield weights (elastoplastic)
   std::vector<double> crad;       // [R] contact radius
   std::vector<int32_t> rgptr_h;   // host copy of the reaction group pointer
-  std::vector<uint8_t> eflag_h;   // host copy of d.eflag
   // adjoint
   AdjState adj;
   // MilneDevice

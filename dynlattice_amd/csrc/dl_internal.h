@@ -43,9 +43,12 @@ struct DevData {
   const int32_t *en0;     // [E] node a
   const int32_t *en1;     // [E] node b
   const int32_t *es;      // [E] strut
-  const uint8_t *eflag;   // [E] bit 0: first element of its chain, bit 1: last,
+  uint8_t *eflag;         // [E] bit 0: first element of its chain, bit 1: last,
                           //     bit 2: end frames -I (strut along -e3; elem_mirror),
-                          //     bit 3: an end at a reaction-group node (rslot_e)
+                          //     bit 3: an end at a reaction-group node (rslot_e),
+                          //     bit 4: committed plastic state possibly nonzero
+                          //     (set by every writer of pst; where it is clear the
+                          //     record is zero and is not read)
   const int32_t *sa;      // [E] slot of node a if shared, else -1
   const int32_t *sb;      // [E] slot of node b (chain end) if shared, else -1
   // elements ([c][E])
@@ -61,14 +64,19 @@ struct DevData {
   // plastic
   const double *pw;     // [6][R] yield weights
   const double *Hk;     // [6][R] kinematic hardening diag
-  double *ep;           // [6][E] committed plastic strain
-  double *al;           // [7][E] committed hardening parameters
-  double *ep_cur;       // [6][E] (action path)
-  double *al_cur;       // [7][E]
+  // element-major records: an element's plastic state is one 96-byte
+  // record, read only where eflag bit 4 is set and written only where the
+  // return map corrected it, so yielding elements scattered over the mesh
+  // touch whole lines instead of a part of one line per component
+  double *pst;          // [E][12] committed plastic strain (6), kinematic
+                        //         hardening parameters alpha_1..6 (6)
+  double *aiso;         // [E] committed isotropic hardening parameter alpha_0
+  double *pst_cur;      // [E][12] trial state (action path, Milne)
+  double *aiso_cur;     // [E]
   double *dflow;        // [E]
   uint8_t *active;      // [E]
-  double *eo;           // [6][E] committed total strain (dissipation)
-  double *ecur;         // [6][E] last material strain (action path)
+  double *eo;           // [E][6] committed total strain (dissipation)
+  double *ecur;         // [E][6] last material strain (action path)
   // shared nodes
   int32_t NJ;
   const int32_t *jnode; // [NJ]
@@ -167,6 +175,52 @@ struct ImpDev {
   double *bc_a;          // record accelerations written ([6][nbc])
   int32_t nbc;
 };
+
+// element-major plastic records (DevData::pst, eo, ecur)
+#define PST_FLAG 16u // eflag bit 4
+__device__ __forceinline__ void pst_load(const double *pst, const double *aiso, int64_t e, bool iso,
+                                         double ep[6], double al[7]) {
+  const double2 *p = (const double2 *)(pst + 12 * e);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double2 a = p[k], b = p[3 + k];
+    ep[2 * k] = a.x;
+    ep[2 * k + 1] = a.y;
+    al[1 + 2 * k] = b.x;
+    al[2 + 2 * k] = b.y;
+  }
+  al[0] = iso ? aiso[e] : 0.0;
+}
+__device__ __forceinline__ void pst_zero(double ep[6], double al[7]) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) ep[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 7; i++) al[i] = 0.0;
+}
+__device__ __forceinline__ void pst_store(double *pst, double *aiso, int64_t e, bool iso,
+                                          const double ep[6], const double al[7]) {
+  double2 *p = (double2 *)(pst + 12 * e);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    p[k] = make_double2(ep[2 * k], ep[2 * k + 1]);
+    p[3 + k] = make_double2(al[1 + 2 * k], al[2 + 2 * k]);
+  }
+  if (iso) aiso[e] = al[0];
+}
+__device__ __forceinline__ void rec6_load(const double *a, int64_t e, double v[6]) {
+  const double2 *p = (const double2 *)(a + 6 * e);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double2 x = p[k];
+    v[2 * k] = x.x;
+    v[2 * k + 1] = x.y;
+  }
+}
+__device__ __forceinline__ void rec6_store(double *a, int64_t e, const double v[6]) {
+  double2 *p = (double2 *)(a + 6 * e);
+#pragma unroll
+  for (int k = 0; k < 3; k++) p[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
 
 // end-of-step sums of an explicit step (k_step_end1, or one more block of
 // the step's k_joint launch): the reaction groups' rows and the dissipation

@@ -289,20 +289,16 @@ __global__ __launch_bounds__(DL_BLOCK, MAT == 3 ? 1 : 3) void k_strut(DevData d,
 #pragma unroll
       for (int i = 0; i < 6; i++) sm[i] = C[i] * strain[i];
       if (MODE == MODE_STEP && (fl & 8u)) react_rows<CONTACT>(d, k, sm, e);
-      if (MODE != MODE_STEP) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
-      }
+      if (MODE != MODE_STEP) rec6_store(d.ecur, e, strain);
     } else {
       double pw[6], Hk[6];
 #pragma unroll
       for (int i = 0; i < 6; i++) { pw[i] = d.pw[i * R + s]; Hk[i] = kin_modulus(pp, i, C[i]); }
+      // committed plastic state: its record only where it may be nonzero;
+      // the isotropic parameter only when in use
       double ep[6], al[7];
-#pragma unroll
-      for (int i = 0; i < 6; i++) ep[i] = d.ep[i * E + e];
-      al[0] = pp.has_iso ? d.al[e] : 0.0; // isotropic parameter only when in use
-#pragma unroll
-      for (int i = 1; i < 7; i++) al[i] = d.al[i * E + e];
+      if (fl & PST_FLAG) pst_load(d.pst, d.aiso, e, pp.has_iso, ep, al);
+      else pst_zero(ep, al);
       double dfl;
       // The kinematics needed after the return map (Lambda, phi', weights)
       // and the strain wait in LDS meanwhile: the return map's working set
@@ -334,9 +330,13 @@ __global__ __launch_bounds__(DL_BLOCK, MAT == 3 ? 1 : 3) void k_strut(DevData d,
       if (MODE == MODE_STEP && (fl & 8u)) {
         // the reaction rows from the trial stress: the committed plastic
         // strain is still the one in memory here
-        double st[6];
+        double st[6], epc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (fl & PST_FLAG) {
 #pragma unroll
-        for (int i = 0; i < 6; i++) st[i] = C[i] * (strain[i] - d.ep[i * E + e]);
+          for (int i = 0; i < 6; i++) epc[i] = d.pst[12 * (int64_t)e + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) st[i] = C[i] * (strain[i] - epc[i]);
         react_rows<CONTACT>(d, k, st, e);
       }
       if (it < 0) *d.err = -3;
@@ -344,37 +344,32 @@ __global__ __launch_bounds__(DL_BLOCK, MAT == 3 ? 1 : 3) void k_strut(DevData d,
         if (sp.track_diss) {
           // applyDeform dissipation (ElastoPlasticRodMaterial.cpp:285-296)
           if (fabs(dfl) > 2.220446049250313e-16) {
+            double eo[6], epo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            rec6_load(d.eo, e, eo);
+            if (fl & PST_FLAG) { // committed plastic strain (updated below)
+#pragma unroll
+              for (int i = 0; i < 6; i++) epo[i] = d.pst[12 * (int64_t)e + i];
+            }
             double t = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-              const double epo = d.ep[i * E + e]; // committed plastic strain (updated below)
-              const double so = C[i] * (d.eo[i * E + e] - epo);
-              t += ((so + sm[i]) / 2.0) * (ep[i] - epo);
+              const double so = C[i] * (eo[i] - epo[i]);
+              t += ((so + sm[i]) / 2.0) * (ep[i] - epo[i]);
             }
             diss = k.w * t;
           }
-#pragma unroll
-          for (int i = 0; i < 6; i++) d.eo[i * E + e] = strain[i];
+          rec6_store(d.eo, e, strain);
         }
         if (it > 0) {
-#pragma unroll
-          for (int i = 0; i < 6; i++) d.ep[i * E + e] = ep[i];
-          if (pp.has_iso) d.al[e] = al[0];
-#pragma unroll
-          for (int i = 1; i < 7; i++) d.al[i * E + e] = al[i];
+          pst_store(d.pst, d.aiso, e, pp.has_iso, ep, al);
+          if (!(fl & PST_FLAG)) d.eflag[e] = (uint8_t)(fl | PST_FLAG);
         }
       } else {
         // trial plastic state: written only where the return map corrected
         // it (elsewhere it equals the committed state, and COMMIT reads the
         // committed one through the active flag)
-        if (it > 0) {
-#pragma unroll
-          for (int i = 0; i < 6; i++) d.ep_cur[i * E + e] = ep[i];
-#pragma unroll
-          for (int i = 0; i < 7; i++) d.al_cur[i * E + e] = al[i];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) d.ecur[i * E + e] = strain[i];
+        if (it > 0) pst_store(d.pst_cur, d.aiso_cur, e, true, ep, al);
+        rec6_store(d.ecur, e, strain);
         d.dflow[e] = dfl;
       }
       d.active[e] = (uint8_t)(it > 0);
@@ -751,7 +746,9 @@ __global__ __launch_bounds__(DL_BLOCK) void k_react(DevData d) {
   qNa.w = d.qN[e]; qNa.x = d.qN[E + e]; qNa.y = d.qN[2 * E + e]; qNa.z = d.qN[3 * E + e];
   qNb.w = d.qN[4 * E + e]; qNb.x = d.qN[5 * E + e]; qNb.y = d.qN[6 * E + e]; qNb.z = d.qN[7 * E + e];
   double strain[6], sm[6], fa[6], fb[6];
-  const bool mir = (d.eflag[e] & 4u) != 0;
+  const uint8_t fl = d.eflag[e];
+  const bool mir = (fl & 4u) != 0;
+  const bool pst = PLASTIC && (fl & PST_FLAG);
   elem_eps0(Xa, Xb, qNa, qNb, strain, mir);
   ElemKin kin;
   elem_kinematics(Xa, ua, Xb, ub, qmul(qna, qNa), qmul(qnb, qNb), kin);
@@ -759,7 +756,7 @@ __global__ __launch_bounds__(DL_BLOCK) void k_react(DevData d) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     strain[i] = kin.eps[i] - strain[i];
-    if (PLASTIC) strain[i] = strain[i] - d.ep[i * E + e];
+    if (PLASTIC) strain[i] = strain[i] - (pst ? d.pst[12 * (int64_t)e + i] : 0.0);
     sm[i] = d.C[i * R + s] * strain[i];
   }
   elem_forces(kin, sm, fa, fb);
@@ -872,37 +869,44 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
   const int64_t R = d.R;
   double C[6];
   for (int i = 0; i < 6; i++) C[i] = d.C[i * R + lo];
+  double ec[6];
+  rec6_load(d.ecur, e, ec);
   if (!PLASTIC) {
     double t = 0.0;
     for (int i = 0; i < 6; i++) {
-      const double c = d.ecur[i * E + e];
+      const double c = ec[i];
       t += c * (C[i] * c);
     }
     epot[e] = 0.5 * t;
     ediss[e] = 0.0;
-    for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+    rec6_store(d.eo, e, ec);
     return;
   }
-  // the trial plastic state is in ep_cur/al_cur only where the last
-  // evaluation corrected it (active); elsewhere it is the committed one
+  // the trial plastic state is in pst_cur only where the last evaluation
+  // corrected it (active); elsewhere it is the committed one
   const bool act = d.active[e] != 0;
+  const uint8_t fl = d.eflag[e];
+  double eo[6], epo[6], alo[7], epc[6], alc[7];
+  rec6_load(d.eo, e, eo);
+  if (fl & PST_FLAG) pst_load(d.pst, d.aiso, e, true, epo, alo);
+  else pst_zero(epo, alo);
+  if (act) pst_load(d.pst_cur, d.aiso_cur, e, true, epc, alc);
   double t = 0.0, ds = 0.0;
   for (int i = 0; i < 6; i++) {
-    const double epo = d.ep[i * E + e];
-    const double epc = act ? d.ep_cur[i * E + e] : epo;
-    const double oel = d.eo[i * E + e] - epo;
-    const double cel = d.ecur[i * E + e] - epc;
-    const double dep = epc - epo;
+    const double epn = act ? epc[i] : epo[i];
+    const double oel = eo[i] - epo[i];
+    const double cel = ec[i] - epn;
+    const double dep = epn - epo[i];
     const double os = C[i] * oel, cs = C[i] * cel;
     t += cel * cs;
     ds += ((os + cs) / 2.0) * dep;
   }
   epot[e] = 0.5 * t;
   ediss[e] = (fabs(d.dflow[e]) > 2.220446049250313e-16) ? ds : 0.0;
-  for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+  rec6_store(d.eo, e, ec);
   if (act) {
-    for (int i = 0; i < 6; i++) d.ep[i * E + e] = d.ep_cur[i * E + e];
-    for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
+    pst_store(d.pst, d.aiso, e, true, epc, alc);
+    if (!(fl & PST_FLAG)) d.eflag[e] = (uint8_t)(fl | PST_FLAG);
   }
   d.dflow[e] = 0.0;
 }

@@ -131,13 +131,15 @@ __global__ __launch_bounds__(DL_BLOCK) void k_milne_commit_pl(DevData d, const i
   const int64_t e = (int64_t)blockIdx.x * DL_BLOCK + threadIdx.x;
   if (e >= d.E) return;
   const int64_t E = d.E;
-#pragma unroll
-  for (int i = 0; i < 6; i++) d.eo[i * E + e] = d.ecur[i * E + e];
+  double ec[6];
+  rec6_load(d.ecur, e, ec);
+  rec6_store(d.eo, e, ec);
   if (d.active[e]) {
-#pragma unroll
-    for (int i = 0; i < 6; i++) d.ep[i * E + e] = d.ep_cur[i * E + e];
-#pragma unroll
-    for (int i = 0; i < 7; i++) d.al[i * E + e] = d.al_cur[i * E + e];
+    double ep[6], al[7];
+    pst_load(d.pst_cur, d.aiso_cur, e, true, ep, al);
+    pst_store(d.pst, d.aiso, e, true, ep, al);
+    const uint8_t fl = d.eflag[e];
+    if (!(fl & PST_FLAG)) d.eflag[e] = (uint8_t)(fl | PST_FLAG);
   }
 }
 

@@ -66,10 +66,9 @@ __global__ __launch_bounds__(DL_BLOCK) void k_elem_table(DevData d, PlasticParam
     for (int i = 0; i < 6; i++) {
       pw[i] = d.pw[i * R + s];
       Hk[i] = kin_modulus(pp, i, C[i]);
-      ep[i] = d.ep[i * E + e];
     }
-    al[0] = pp.has_iso ? d.al[e] : 0.0;
-    for (int i = 1; i < 7; i++) al[i] = d.al[i * E + e];
+    if (d.eflag[e] & PST_FLAG) pst_load(d.pst, d.aiso, e, pp.has_iso, ep, al);
+    else pst_zero(ep, al);
     if (return_map<MAT == 3 ? 0 : MAT>(pp, C, pw, Hk, strain, ep, al, sm, &dfl, 1) < 0) *d.err = -3;
   }
   if (epot) {
