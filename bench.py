@@ -191,37 +191,55 @@ def calibrate_copy(nbytes=1 << 30):
 
 
 def sweep(args):
-    """Batched design sweep (256 seeded elastoplastic BCC lattices, per-instance
-    dt) in one context replayed from a captured graph, against the same 256
-    instances stepped one after another in their own contexts."""
+    """Configuration C4: the 256 seeded elastoplastic BCC lattices (per-instance
+    dt, each crushed to 20 % nominal strain in its 10,000 steps) as one
+    batched context whose steps replay captured hipGraphs, with the schedule
+    (instance order x graph length) chosen by batch.autotune; the whole
+    10,000-step sweep is timed from the initial state.  Against it: the same
+    instances one after another in their own contexts (a sample of steps
+    each), and the oracle on the median instance (all cores, one core)."""
     from dynlattice_amd import batch as BT
     from dynlattice_amd import lattice as L
-    from dynlattice_amd.gpu import GpuModel
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
     n_inst = args.sweep_instances
-    steps = args.steps
+    nsteps = args.sweep_steps
     probs = [L.c4_instance(i) for i in range(n_inst)]
-    bt = BT.concat(probs)
-    print(f"sweep: {n_inst} instances, {bt.problem.n_elems} elements, {bt.problem.n_nodes} nodes",
-          file=sys.stderr, flush=True)
-    g = BT.make_model(bt)
-    best, tune = BT.autotune_graph(g, candidates=(0, 8, 32), steps=16)
-    g.step(args.warmup, out=False)
-    g.sync()
+    flags = DL_TRACK_DISSIPATION
+    E = sum(p.n_elems for p in probs)
+    print(f"sweep: {n_inst} instances, {E} elements", file=sys.stderr, flush=True)
+    order, K, tune = BT.autotune(probs, steps=args.sweep_tune_steps, flags=flags, log=sys.stderr)
+    perm = BT.instance_order(probs, order)
+    bt = BT.concat([probs[i] for i in perm])
+    g = BT.make_model(bt, flags=flags, graph_steps=K)
     t0 = time.perf_counter()
-    g.step(steps, out=False)
+    g.step(nsteps, out=False)
     g.sync()
     t_batch = time.perf_counter() - t0
-    g.step(0)
+    hist = g.group_history(nsteps)  # (dl_sync above raised any device error)
+    # per-step kernel phases of the batched step (event-timed, direct launches)
+    g.set_graph(0)
+    g.set_timing(True)
+    g.step(20, out=False)
+    g.sync()
+    phases = {}
+    for ph in ("step", "strut", "joint"):
+        ms, cnt = g.phase_time(ph)
+        phases[ph] = ms / max(cnt, 1)
+    g.set_timing(False)
     g.close()
+    npriv = private_node_count(bt.problem)
+    b_strut = (B_ELEM + B_ELEM_PLASTIC) * E + B_NODE * npriv
+    a_strut = b_strut / (phases["strut"] * 1e-3) / 1e9
     # the same instances one by one: their own contexts, direct launches, all
     # on ONE stream so that no two instances overlap on the device
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     stream = C.c_void_p()
     assert hip.hipStreamCreateWithFlags(C.byref(stream), 1) == 0
+    k_seq = args.sweep_seq_steps
     singles = []
     for p in probs:
-        m = GpuModel(p, stream=stream)
+        m = GpuModel(p, stream=stream, flags=flags)
         m.set_reaction_group(p.groups["bottom"], 2)
         m.step(args.warmup, p.dt, out=False)
         singles.append(m)
@@ -229,22 +247,59 @@ def sweep(args):
         m.sync()
     t0 = time.perf_counter()
     for m, p in zip(singles, probs):
-        m.step(steps, p.dt, out=False)
+        m.step(k_seq, p.dt, out=False)
     for m in singles:
         m.sync()
     t_seq = time.perf_counter() - t0
     for m in singles:
         m.close()
     hip.hipStreamDestroy(stream)
-    E = bt.problem.n_elems
+    batched = E * nsteps / t_batch
+    sequential = E * k_seq / t_seq
     res = {"metric": "batched design-sweep element-GP updates/s (FP64)",
-           "value": E * steps / t_batch, "unit": "element-GP updates/s", "n_gpus": 1,
-           "steps": steps, "warmup": args.warmup, "higher_is_better": True,
+           "value": batched, "unit": "element-GP updates/s", "n_gpus": 1,
+           "steps": nsteps, "warmup": 0, "ms_per_step": 1e3 * t_batch / nsteps,
+           "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (seeded BCC lattice generator per instance)",
            "config": {"workload": f"sweep-{n_inst}-bcc-elastoplastic", "instances": n_inst,
                       "elements": E, "nodes": bt.problem.n_nodes,
-                      "dt_range": [float(bt.dts.min()), float(bt.dts.max())]},
-           "graph_steps": best, "autotune_s_per_step": tune,
-           "sequential_value": E * steps / t_seq, "batched_over_sequential": t_seq / t_batch}
+                      "dt_range": [float(bt.dts.min()), float(bt.dts.max())],
+                      "steps_per_instance": nsteps, "nominal_strain": L.C4_STRAIN,
+                      "yield": yield_family(bt.problem)},
+           "schedule": {"instance_order": order, "graph_steps": K,
+                        "autotune_ms_per_step": {f"{o}/{k}": round(1e3 * t, 5)
+                                                 for (o, k), t in tune.items()}},
+           "wall_s": t_batch,
+           "phase_ms": {k: round(v, 5) for k, v in phases.items()},
+           "roofline": {"bound": "hbm", "kernel": "k_strut", "achieved": round(a_strut, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(a_strut / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": b_strut,
+                        "avg_launch_ms": round(phases["strut"], 5)},
+           "reaction_peak_range_N": [float(np.abs(hist).max(0).min()),
+                                     float(np.abs(hist).max(0).max())],
+           "sequential_value": sequential, "sequential_steps_per_instance": k_seq,
+           "batched_over_sequential": batched / sequential}
+    if not args.no_cpu:
+        from oracle.ref_cpu import RefModel
+        b = int(np.argsort([p.n_elems for p in probs])[n_inst // 2])
+        q = probs[b]
+        for omp, key in ((True, "cpu_baseline"), (False, "cpu_baseline_serial")):
+            ref = RefModel(q, omp=omp)
+            t1 = time.perf_counter()
+            ref.leapfrog(1)
+            t1 = time.perf_counter() - t1
+            k = int(max(1, min(nsteps, args.cpu_budget / max(t1, 1e-9))))
+            t1 = time.perf_counter()
+            ref.leapfrog(k)
+            t1 = time.perf_counter() - t1
+            cores = cpu_threads() if omp else 1
+            res[key] = {"value": q.n_elems * k / t1, "unit": "element-GP updates/s",
+                        "cores": cores, "kind": "port",
+                        "sample": f"oracle (oracle/ref_cpu.c, {cores} thread(s)) on the median "
+                                  f"instance {q.name} ({q.n_elems} elements), {k} LeapFrog "
+                                  f"steps after 1, {t1:.1f} s"}
+        res["speedup_vs_cpu"] = batched / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
 
 
@@ -547,6 +602,11 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="batched design sweep vs sequential single-instance runs")
     ap.add_argument("--sweep-instances", type=int, default=256)
+    ap.add_argument("--sweep-steps", type=int, default=10_000,
+                    help="steps of the timed sweep (every instance; C4: 10,000)")
+    ap.add_argument("--sweep-tune-steps", type=int, default=50)
+    ap.add_argument("--sweep-seq-steps", type=int, default=200,
+                    help="steps per instance of the one-by-one comparison")
     ap.add_argument("--adjoint", action="store_true",
                     help="adjoint gradient of the 20^3 impact: adjoint/forward time ratio")
     ap.add_argument("--adjoint-n", type=int, default=20)

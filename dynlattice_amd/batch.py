@@ -9,8 +9,9 @@ whose mesh is the disjoint union of all instances:
 * one reaction group per instance (dl_set_reaction_groups);
 * instances are translated apart along x, so that even with contact enabled
   no two instances can touch;
-* the step sequence is replayed from a captured hipGraph (dl_set_graph); the
-  graph length is chosen by :func:`autotune_graph`.
+* the step sequence is replayed from captured hipGraphs (dl_set_graph); the
+  graph length and the order of the instances in the union are chosen by
+  :func:`autotune` on the real workload.
 
 Every instance computes exactly what it would compute alone: the kernels
 never mix data of two instances (nodes, elements and struts are disjoint).
@@ -118,6 +119,56 @@ def autotune_graph(model, candidates=(0, 8, 32, 128), steps=256, dt=None):
     best = min(times, key=times.get)
     model.set_graph(best)
     return best, times
+
+
+# orders of the instances in the union that autotune tries: as given, by
+# decreasing size, and grouped by elements per strut (chains of one length
+# pack the same way into 64-lane wavefronts) and then by size
+ORDERS = ("given", "size-descending", "chain-length")
+
+
+def instance_order(problems: List[L.Problem], order: str) -> np.ndarray:
+    n = len(problems)
+    if order == "given":
+        return np.arange(n)
+    size = np.array([p.n_elems for p in problems])
+    if order == "size-descending":
+        return np.argsort(-size, kind="stable")
+    if order == "chain-length":
+        per = np.array([p.n_elems // max(p.n_struts, 1) for p in problems])
+        return np.lexsort((-size, per))
+    raise ValueError(f"unknown order {order}")
+
+
+def autotune(problems: List[L.Problem], orders=ORDERS, graph_lengths=(0, 10, 100), steps=100,
+             flags=0, log=None):
+    """Schedule of a batched sweep, chosen by measurement: for every order of
+    the instances in the union (a different mesh numbering, hence lane
+    packing, memory layout and block -> instance mapping) and every
+    captured-graph length, ``steps`` steps after one warm-up pass of the same
+    length (which also captures the graph).  The instances advance during
+    tuning, from their initial state for every order.  Returns
+    (order, graph length, {(order, length): seconds per step})."""
+    times = {}
+    for order in orders:
+        perm = instance_order(problems, order)
+        bt = concat([problems[i] for i in perm])
+        g = make_model(bt, flags=flags)
+        for k in graph_lengths:
+            g.set_graph(k)
+            g.step(max(steps, k), out=False)
+            g.sync()
+            t0 = time.perf_counter()
+            g.step(max(steps, k), out=False)
+            g.sync()
+            times[(order, k)] = (time.perf_counter() - t0) / max(steps, k)
+            if log:
+                print(f"autotune: order {order}, graph {k}: "
+                      f"{1e3 * times[(order, k)]:.4f} ms/step", file=log, flush=True)
+        g.sync()  # raises a pending device error
+        g.close()
+    best = min(times, key=times.get)
+    return best[0], best[1], times
 
 
 def c4_batch(n_instances=256):

@@ -536,6 +536,8 @@ extern "C" int dl_create(const dl_mesh *mesh, const dl_sections *sec, const dl_m
   TRY(dalloc(c, &d.fout, 6 * N));
   TRY(dalloc(c, &d.err, 1));
   HIPCHK(hipMemset(d.err, 0, sizeof(int32_t)));
+  TRY(dalloc(c, &c->end_done, 1));
+  HIPCHK(hipMemset(c->end_done, 0, sizeof(int32_t)));
   c->strut_blocks = (int32_t)(d.NL / 256);
   TRY(dalloc(c, &d.dbuf, (size_t)c->strut_blocks));
   TRY(dalloc(c, &c->diss_total, 1));
@@ -1143,14 +1145,14 @@ int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   launch_strut_pp(c->d, sp, c->pp, MODE_STEP, c->so3, c->mat_kind, c->contact, c->stream);
   if (ev) HIPCHK(hipEventRecord(ev[4], c->stream));
-  // the end-of-step sums ride in the joint kernel's launch (one block more)
-  // unless impact groups update nodes in between, there are no shared nodes
-  // or the sums need several blocks
-  const bool fuse = c->d.NJ > 0 && c->imp.ngroups == 0 &&
-                    step_end_fusable(c->n_groups, c->track_diss);
+  // the end-of-step sums ride in the joint kernel's launch (one block more,
+  // or one per sum beyond 8) unless impact groups update nodes in between or
+  // there are no shared nodes
+  const bool fuse = c->d.NJ > 0 && c->imp.ngroups == 0;
   if (fuse) {
     const StepEnd se = make_step_end(c->d, c->rgptr, c->n_groups, c->track_diss, c->strut_blocks,
-                                     c->diss_total, c->hist_react, c->hist_diss, c->step_ctr);
+                                     c->diss_total, c->hist_react, c->hist_diss, c->step_ctr,
+                                     c->end_done);
     launch_joint(c->d, sp, MODE_STEP, c->so3, c->stream, nullptr, &se);
     if (ev) HIPCHK(hipEventRecord(ev[5], c->stream));
   } else {

@@ -118,6 +118,7 @@ struct dl_ctx {
   int32_t *rgptr = nullptr;
   std::vector<int32_t> rn_ptr_h;
   int32_t *step_ctr = nullptr;   // device step counter (history row)
+  int32_t *end_done = nullptr;   // blocks of the end-of-step sums finished (zero between steps)
   double *ndt = nullptr;          // per-node dt
   // captured step graphs
   int32_t graph_steps = 0;

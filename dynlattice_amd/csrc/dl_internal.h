@@ -236,6 +236,10 @@ struct StepEnd {
   double *hist_diss;
   int32_t *step;
   int32_t tail;         // reaction row only, the counter stays
+  // blocks: 1 (up to 8 sums, two at a time) or one per sum; with several,
+  // the last block to finish (counted in *done) moves the step counter
+  int32_t nblocks;
+  int32_t *done;
 };
 
 // MilneDevice attempts: predicted then corrected state, error partials and
@@ -283,12 +287,11 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s);
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                      int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
                      int32_t *step, hipStream_t s, bool tail = false);
-// the sums of launch_step_end for the one-block case (<= 8 sums), to hand to
-// launch_joint; false where they need their own launch
-bool step_end_fusable(int32_t ngroups, bool diss);
+// the sums of launch_step_end as blocks to hand to launch_joint (done: a
+// device counter, zero between steps, for more than 8 sums)
 StepEnd make_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                       int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
-                      int32_t *step, bool tail = false);
+                      int32_t *step, int32_t *done, bool tail = false);
 // GroupOutput reaction entries at the current state (after the contact pass
 // of this state): rbuf[k] = the dof share of entry k
 void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s);

@@ -640,6 +640,18 @@ __device__ __forceinline__ void step_end_block(const StepEnd &se, double (*red)[
   if (threadIdx.x == 0 && !se.tail) *se.step = (int32_t)(st + 1);
 }
 
+// one sum per block (more than 8 sums); the last block to finish moves the
+// counter -- every other block has read it by then
+__device__ __forceinline__ void step_end_one(const StepEnd &se, int g, double (*red)[SE_NT]) {
+  const int64_t st = *se.step;
+  step_end_sums<1>(g, g + 1, se.rbuf, se.rgptr, se.ngroups, se.dbuf, se.ndb, se.diss_total,
+                   se.hist_react, se.hist_diss, st - 1, st, red);
+  if (threadIdx.x == 0 && atomicAdd(se.done, 1) == se.nblocks - 1) {
+    *se.done = 0;
+    if (!se.tail) *se.step = (int32_t)(st + 1);
+  }
+}
+
 __global__ __launch_bounds__(SE_NT) void k_step_end1(StepEnd se) {
   __shared__ double red[2][SE_NT];
   step_end_block(se, red);
@@ -649,10 +661,13 @@ template <int MODE, bool SO3, bool SE>
 __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m, StepEnd se) {
   static_assert(DL_BLOCK == SE_NT, "a step-end block is a k_joint block");
   if constexpr (SE) {
-    // the last block: the step's end sums (their inputs come from k_strut)
-    if (blockIdx.x == gridDim.x - 1) {
+    // the last se.nblocks blocks: the step's end sums (their inputs come
+    // from k_strut)
+    const int g = (int)blockIdx.x - ((int)gridDim.x - se.nblocks);
+    if (g >= 0) {
       __shared__ double red[2][SE_NT];
-      step_end_block(se, red);
+      if (se.nblocks == 1) step_end_block(se, red);
+      else step_end_one(se, g, red);
       return;
     }
   }
@@ -1039,7 +1054,7 @@ static void lj(const DevData &d, const StepParams &sp, const MilneDev &m, bool s
 // the step's joint kernel with the end-of-step sums as its last block
 static void lj_end(const DevData &d, const StepParams &sp, bool so3, const StepEnd &se,
                    hipStream_t s) {
-  const dim3 g(nblk(d.NJ) + 1), b(DL_BLOCK);
+  const dim3 g(nblk(d.NJ) + se.nblocks), b(DL_BLOCK);
   const MilneDev m{};
   if (so3) hipLaunchKernelGGL((k_joint<MODE_STEP, true, true>), g, b, 0, s, d, sp, m, se);
   else hipLaunchKernelGGL((k_joint<MODE_STEP, false, true>), g, b, 0, s, d, sp, m, se);
@@ -1067,14 +1082,14 @@ void launch_force_extras(const DevData &d, bool so3, bool gyro, hipStream_t s) {
   else hipLaunchKernelGGL(k_force_extras<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, 1);
 }
 
-bool step_end_fusable(int32_t ngroups, bool diss) { return ngroups + (diss ? 1 : 0) <= 8; }
-
 StepEnd make_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
                       int64_t ndb, double *diss_total, double *hist_react, double *hist_diss,
-                      int32_t *step, bool tail) {
+                      int32_t *step, int32_t *done, bool tail) {
   if (tail) diss = false;
-  return StepEnd{(const double *)d.rbuf, rgptr, ngroups, ngroups + (diss ? 1 : 0),
-                 (const double *)d.dbuf, ndb, diss_total, hist_react, hist_diss, step, tail ? 1 : 0};
+  const int32_t nsums = ngroups + (diss ? 1 : 0);
+  return StepEnd{(const double *)d.rbuf, rgptr, ngroups, nsums, (const double *)d.dbuf, ndb,
+                 diss_total, hist_react, hist_diss, step, tail ? 1 : 0,
+                 nsums <= 8 ? 1 : nsums, done};
 }
 
 void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bool diss,
@@ -1083,9 +1098,9 @@ void launch_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, bo
   if (tail) diss = false;
   const int nb = ngroups + (diss ? 1 : 0);
   if (tail && nb == 0) return;
-  if (step_end_fusable(ngroups, diss)) {
+  if (nb <= 8) {
     const StepEnd se = make_step_end(d, rgptr, ngroups, diss, ndb, diss_total, hist_react,
-                                     hist_diss, step, tail);
+                                     hist_diss, step, nullptr, tail);
     hipLaunchKernelGGL(k_step_end1, dim3(1), dim3(SE_NT), 0, s, se);
     return;
   }

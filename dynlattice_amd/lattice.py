@@ -418,18 +418,27 @@ def bcc_adjoint_check(radius=None, dt=None) -> Problem:
     return p
 
 
+C4_STEPS = 10_000   # steps of every design-sweep instance
+C4_STRAIN = 0.2     # nominal strain every instance reaches in those steps
+
+
 def c4_instance(i: int) -> Problem:
     """Design-sweep instance i (seed 1000 + i): n ~ U{4..16}, m ~ U{2..6},
-    r ~ U[0.2, 0.8] mm, yield ~ U[150, 350] MPa; elastoplastic BCC crush."""
+    r ~ U[0.2, 0.8] mm, yield ~ U[150, 350] MPa; elastoplastic BCC crush at
+    its own stable dt (0.5 x the estimate), the top face moving so that the
+    C4_STEPS steps take every instance to C4_STRAIN nominal strain (a fixed
+    plate speed would crush the small lattices flat long before 10,000 steps
+    and leave the large ones barely loaded)."""
     u = uniform01(1000 + i, 4)
     n = 4 + int(u[0] * 13)
     m = 2 + int(u[1] * 5)
     r = 0.2e-3 + 0.6e-3 * u[2]
     sy = 150e6 + 200e6 * u[3]
     R = 8 * n ** 3
-    p = bcc_lattice(n, m, 10e-3, np.full(R, r), 70e9, 0.3, 2700.0)
+    cell = 10e-3
+    p = bcc_lattice(n, m, cell, np.full(R, r), 70e9, 0.3, 2700.0)
     p.plastic = Plastic.resultant_capacities(np.full(R, r), sy, kin_ratio=0.01, exponent=2.0)
-    crush_bcs(p, -20.0)
     p.dt = stable_dt(p, 0.5)
+    crush_bcs(p, -C4_STRAIN * n * cell / (C4_STEPS * p.dt))
     p.name = f"sweep-{i}-n{n}-m{m}"
     return p
