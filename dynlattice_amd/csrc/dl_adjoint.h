@@ -61,49 +61,56 @@ DL_HD void qexp_vjp(const double t[3], const Quat qb, double tb[3]) {
   for (int i = 0; i < 3; i++) tb[i] = qb.w * s0 * t[i] + f * qv[i] + g * dv * t[i];
 }
 
-// element internal force (same computation as elem_kinematics + elem_forces
-// with the elastic material) and its VJP.  Inputs: node displacements, node
-// quaternions (the nodal rotation state), constant frames/strain; given
-// fbar_a, fbar_b (6 each) returns ubar_a/b, qbar_a/b (node quaternions) and
-// Cbar (material stiffness).
+// element internal force (same computation as elem_kinematics + elem_forces)
+// and its VJP, in three parts so that the elastic and the elastoplastic
+// material share them: the forward intermediates (elem_vjp_forward), the
+// adjoint of one material stress from the adjoint of the nodal force it
+// makes (elem_vjp_stress; the force is linear in the stress, so several
+// stress sources add up), and the adjoint of the strain back to the node
+// displacements and quaternions (elem_vjp_strain).
 struct ElemVJP {
   double ua[3], ub[3];
   Quat qa, qb;
   double C[6];
 };
 
-DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna, const Quat qNa,
-                          const double Xb[3], const double ub[3], const Quat qnb, const Quat qNb,
-                          const double eps0[6], const double C[6], const double fba[6],
-                          const double fbb[6], ElemVJP &out) {
-  // ---- forward (as in elem_kinematics / elem_forces)
-  const Quat qa = qmul(qna, qNa), qb = qmul(qnb, qNb);
+struct ElemFwd {
   ElemKin k;
-  elem_kinematics(Xa, ua, Xb, ub, qa, qb, k);
+  Quat qa, qb, qr, qh, qm;
+  double sgn, s, s2, hn;
+  double strain[6]; // material strain minus the initial strain
+};
+
+DL_HD void elem_vjp_forward(const double Xa[3], const double ua[3], const Quat qna, const Quat qNa,
+                            const double Xb[3], const double ub[3], const Quat qnb, const Quat qNb,
+                            const double eps0[6], ElemFwd &F) {
+  F.qa = qmul(qna, qNa);
+  F.qb = qmul(qnb, qNb);
+  elem_kinematics(Xa, ua, Xb, ub, F.qa, F.qb, F.k);
   // intermediate quantities of the rotation interpolation
-  Quat qr = qcmul(qa, qb);
-  const double sgn = (qr.w < 0.0) ? -1.0 : 1.0;
-  qr.w *= sgn; qr.x *= sgn; qr.y *= sgn; qr.z *= sgn;
-  const double s2 = qr.x * qr.x + qr.y * qr.y + qr.z * qr.z;
-  const double s = sqrt(s2);
-  const double hn = 1.0 / sqrt(2.0 * (1.0 + qr.w));
-  Quat qh;
-  qh.w = (1.0 + qr.w) * hn; qh.x = qr.x * hn; qh.y = qr.y * hn; qh.z = qr.z * hn;
-  const Quat qm = qmul(qa, qh);
-  double strain[6], sm[6];
+  Quat qr = qcmul(F.qa, F.qb);
+  F.sgn = (qr.w < 0.0) ? -1.0 : 1.0;
+  qr.w *= F.sgn; qr.x *= F.sgn; qr.y *= F.sgn; qr.z *= F.sgn;
+  F.qr = qr;
+  F.s2 = qr.x * qr.x + qr.y * qr.y + qr.z * qr.z;
+  F.s = sqrt(F.s2);
+  F.hn = 1.0 / sqrt(2.0 * (1.0 + qr.w));
+  F.qh.w = (1.0 + qr.w) * F.hn; F.qh.x = qr.x * F.hn; F.qh.y = qr.y * F.hn; F.qh.z = qr.z * F.hn;
+  F.qm = qmul(F.qa, F.qh);
 #pragma unroll
-  for (int i = 0; i < 6; i++) {
-    strain[i] = k.eps[i] - eps0[i];
-    sm[i] = C[i] * strain[i];
-  }
+  for (int i = 0; i < 6; i++) F.strain[i] = F.k.eps[i] - eps0[i];
+}
+
+// force (fa, fb) = XI^T (R sm) w: given its adjoint (fba, fbb), the adjoint
+// of the material stress sm (smb) and the accumulations into the adjoints
+// of Lambda (Rb) and phi' (pPb)
+DL_HD void elem_vjp_stress(const ElemFwd &F, const double sm[6], const double fba[6],
+                           const double fbb[6], double Rb[9], double pPb[3], double smb[6]) {
+  const ElemKin &k = F.k;
   const double *R = k.R;
-  double n[3], m[3];
+  double n[3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    n[i] = R[3 * i] * sm[0] + R[3 * i + 1] * sm[1] + R[3 * i + 2] * sm[2];
-    m[i] = R[3 * i] * sm[3] + R[3 * i + 1] * sm[4] + R[3 * i + 2] * sm[5];
-  }
-  // ---- reverse
+  for (int i = 0; i < 3; i++) n[i] = R[3 * i] * sm[0] + R[3 * i + 1] * sm[1] + R[3 * i + 2] * sm[2];
   double nb[3], mb[3], cb[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -112,30 +119,33 @@ DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna
     cb[i] = -0.5 * k.w * (fba[3 + i] + fbb[3 + i]);
   }
   // c = phiP x n
-  double pPb[3];
-  pPb[0] = n[1] * cb[2] - n[2] * cb[1];
-  pPb[1] = n[2] * cb[0] - n[0] * cb[2];
-  pPb[2] = n[0] * cb[1] - n[1] * cb[0];
+  pPb[0] += n[1] * cb[2] - n[2] * cb[1];
+  pPb[1] += n[2] * cb[0] - n[0] * cb[2];
+  pPb[2] += n[0] * cb[1] - n[1] * cb[0];
   nb[0] += cb[1] * k.phiP[2] - cb[2] * k.phiP[1];
   nb[1] += cb[2] * k.phiP[0] - cb[0] * k.phiP[2];
   nb[2] += cb[0] * k.phiP[1] - cb[1] * k.phiP[0];
   // n = R sm_t, m = R sm_r
-  double Rb[9], smb[6];
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) Rb[3 * i + j] = nb[i] * sm[j] + mb[i] * sm[3 + j];
+    for (int j = 0; j < 3; j++) Rb[3 * i + j] += nb[i] * sm[j] + mb[i] * sm[3 + j];
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     smb[j] = R[j] * nb[0] + R[3 + j] * nb[1] + R[6 + j] * nb[2];
     smb[3 + j] = R[j] * mb[0] + R[3 + j] * mb[1] + R[6 + j] * mb[2];
   }
-  double stb[6];
-#pragma unroll
-  for (int i = 0; i < 6; i++) {
-    out.C[i] = smb[i] * strain[i];
-    stb[i] = C[i] * smb[i];
-  }
+}
+
+// strain = (R^T phiP, phi cL) - eps0: from its adjoint stb (and Rb, pPb
+// accumulated by elem_vjp_stress) the adjoints of the node displacements
+// and quaternions (out.ua, ub, qa, qb)
+DL_HD void elem_vjp_strain(const ElemFwd &F, const double stb[6], double Rb[9], double pPb[3],
+                           const Quat qNa, const Quat qNb, ElemVJP &out) {
+  const ElemKin &k = F.k;
+  const double *R = k.R;
+  const Quat qr = F.qr;
+  const double hn = F.hn, s = F.s, s2 = F.s2;
   // eps_t = R^T phiP
 #pragma unroll
   for (int kk = 0; kk < 3; kk++)
@@ -148,9 +158,9 @@ DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna
   const double cL = -0.5 * k.g0 + 0.5 * k.g1;
   double phib[3] = {cL * stb[3], cL * stb[4], cL * stb[5]};
   // R = qmat(qm); qm = qa * qh
-  const Quat qmb = qmat_vjp(qm, Rb);
-  Quat qab = qmul(qmb, qconj(qh));
-  const Quat qhb = qmul(qconj(qa), qmb);
+  const Quat qmb = qmat_vjp(F.qm, Rb);
+  Quat qab = qmul(qmb, qconj(F.qh));
+  const Quat qhb = qmul(qconj(F.qa), qmb);
   // qh = ((1+w) hn, v hn), hn = (2(1+w))^-1/2
   double wb = qhb.w * (0.5 * hn) - (qhb.x * qr.x + qhb.y * qr.y + qhb.z * qr.z) * hn * hn * hn;
   double vb[3] = {hn * qhb.x, hn * qhb.y, hn * qhb.z};
@@ -177,9 +187,9 @@ DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna
   }
   // qr = sgn * conj(qa) * qb
   Quat qrb;
-  qrb.w = sgn * wb; qrb.x = sgn * vb[0]; qrb.y = sgn * vb[1]; qrb.z = sgn * vb[2];
-  qab = qadd(qab, qmul(qb, qconj(qrb)));
-  const Quat qbb = qmul(qa, qrb);
+  qrb.w = F.sgn * wb; qrb.x = F.sgn * vb[0]; qrb.y = F.sgn * vb[1]; qrb.z = F.sgn * vb[2];
+  qab = qadd(qab, qmul(F.qb, qconj(qrb)));
+  const Quat qbb = qmul(F.qa, qrb);
   // qa = qna * qNa
   out.qa = qmul(qab, qconj(qNa));
   out.qb = qmul(qbb, qconj(qNb));
@@ -188,6 +198,221 @@ DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna
     out.ua[i] = k.g0 * pPb[i];
     out.ub[i] = k.g1 * pPb[i];
   }
+}
+
+// elastic material: inputs node displacements, node quaternions (the nodal
+// rotation state), constant frames/strain; given fbar_a, fbar_b (6 each)
+// returns ubar_a/b, qbar_a/b (node quaternions) and Cbar (material stiffness)
+DL_HD void elem_force_vjp(const double Xa[3], const double ua[3], const Quat qna, const Quat qNa,
+                          const double Xb[3], const double ub[3], const Quat qnb, const Quat qNb,
+                          const double eps0[6], const double C[6], const double fba[6],
+                          const double fbb[6], ElemVJP &out) {
+  ElemFwd F;
+  elem_vjp_forward(Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, F);
+  double sm[6], smb[6], stb[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) sm[i] = C[i] * F.strain[i];
+  double Rb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, pPb[3] = {0, 0, 0};
+  elem_vjp_stress(F, sm, fba, fbb, Rb, pPb, smb);
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    out.C[i] = smb[i] * F.strain[i];
+    stb[i] = C[i] * smb[i];
+  }
+  elem_vjp_strain(F, stb, Rb, pPb, qNa, qNb, out);
+}
+
+// ---------------------------------------------------------------------
+// elastoplastic material: the cutting plane of return_map reversed
+// ---------------------------------------------------------------------
+// second derivative g''(|x|) of the yield family's term g(|x|) = |x|^p
+template <int EXPK>
+DL_HD double yield_pderiv2(double x, double p) {
+  const double ax = fabs(x);
+  if (EXPK == 1) return 0.0;
+  if (EXPK == 2) return 2.0;
+  if (p == 1.0) return 0.0;
+  if (p == 2.0) return 2.0;
+  return ax > 0.0 ? p * (p - 1.0) * pow(ax, p - 2.0) : 0.0;
+}
+
+// Parameter adjoints of the material: stiffness, yield weights, kinematic
+// moduli (accumulated).
+struct MatBar {
+  double C[6], w[6], Hk[6];
+};
+
+// One correction of return_map<EXPK> taken at the plastic state (ep, al),
+// reversed.  epb, alb: adjoints of the corrected state on entry, of (ep, al)
+// on return; strb and the parameter adjoints accumulate.  The forward part
+// repeats return_map's operations for one pass of its loop.
+template <int EXPK>
+DL_HD void rm_correction_vjp(const PlasticParams &P, const double C[6], const double w[6],
+                             const double Hk[6], const double strain[6], const double ep[6],
+                             const double al[7], double epb[6], double alb[7], double strb[6],
+                             MatBar &mb) {
+  const double tiny = 2.220446049250313e-16;
+  double st[6], q[7], a[6];
+  q[0] = -1.0 * (P.has_iso ? P.iso_modulus * al[0] : 0.0);
+  double f = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    st[i] = C[i] * (strain[i] - ep[i]);
+    q[1 + i] = -1.0 * (P.has_kin ? Hk[i] * al[1 + i] : 0.0);
+    a[i] = st[i] + (P.has_kin ? P.kin_coupling[i] * q[1 + i] : 0.0);
+    f += yield_pterm<EXPK>(w[i] * a[i], P.exponent[i]);
+  }
+  f -= P.yield_level * (1.0 + (P.has_iso ? P.iso_coupling * q[0] : 0.0));
+  double gs[6], gq[7];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const double dd = yield_pderiv<EXPK>(w[i] * a[i], P.exponent[i]) * w[i] * dl_sgn(a[i]);
+    gs[i] = dd;
+    gq[1 + i] = P.has_kin ? dd * P.kin_coupling[i] : 0.0;
+    if (fabs(st[i]) < tiny) gs[i] = 0.0;
+  }
+  gq[0] = P.has_iso ? -P.yield_level * P.iso_coupling : 0.0;
+  double den = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) den += gs[i] * (C[i] * gs[i]);
+  double b = 0.0;
+  if (P.has_iso) b += gq[0] * (P.iso_modulus * gq[0]);
+  if (P.has_kin)
+#pragma unroll
+    for (int i = 0; i < 6; i++) b += gq[1 + i] * (Hk[i] * gq[1 + i]);
+  den += b;
+  const double ddl = f / den;
+  // ---- reverse: ep' = ep + ddl gs, al' = al + ddl gq
+  double lb = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) lb += epb[i] * gs[i];
+  if (P.has_iso) lb += alb[0] * gq[0];
+  if (P.has_kin)
+#pragma unroll
+    for (int i = 0; i < 6; i++) lb += alb[1 + i] * gq[1 + i];
+  // ddl = f / den
+  const double fb = lb / den, denb = -lb * ddl / den;
+  double q0b = P.has_iso ? -P.yield_level * P.iso_coupling * fb : 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    // den = sum gs C gs + sum gq Hk gq (+ the isotropic constant)
+    const double gsb = ddl * epb[i] + 2.0 * C[i] * gs[i] * denb;
+    mb.C[i] += gs[i] * gs[i] * denb;
+    double gqb = 0.0;
+    if (P.has_kin) {
+      gqb = ddl * alb[1 + i] + 2.0 * Hk[i] * gq[1 + i] * denb;
+      mb.Hk[i] += gq[1 + i] * gq[1 + i] * denb;
+    }
+    // gs = dd (zeroed where |stress| is tiny), gq = dd ck;
+    // dd = g'(|x|) w sgn(a), x = w a, d|x| = sgn(x) (w da + a dw)
+    const double ddb = (fabs(st[i]) < tiny ? 0.0 : gsb) + (P.has_kin ? P.kin_coupling[i] * gqb : 0.0);
+    const double x = w[i] * a[i], sx = dl_sgn(x), sa = dl_sgn(a[i]);
+    const double g1 = yield_pderiv<EXPK>(x, P.exponent[i]);
+    const double g2 = yield_pderiv2<EXPK>(x, P.exponent[i]);
+    // f = sum g(|x|) - ...
+    const double ab = ddb * g2 * sx * w[i] * w[i] * sa + fb * g1 * sx * w[i];
+    mb.w[i] += ddb * (g2 * sx * a[i] * w[i] * sa + g1 * sa) + fb * g1 * sx * a[i];
+    // a = st + ck q_kin, q_kin = -Hk al_kin, st = C (strain - ep)
+    if (P.has_kin) {
+      const double qkb = P.kin_coupling[i] * ab;
+      alb[1 + i] += -Hk[i] * qkb;
+      mb.Hk[i] += -al[1 + i] * qkb;
+    }
+    mb.C[i] += ab * (strain[i] - ep[i]);
+    strb[i] += C[i] * ab;
+    epb[i] -= C[i] * ab;
+  }
+  if (P.has_iso) alb[0] += -P.iso_modulus * q0b;
+}
+
+// Reverse of return_map<EXPK>(..., inelastic = 1) from the committed state
+// (ep0, al0) that took n >= 0 corrections and ended at plastic strain epn:
+// given the adjoint of the returned stress (sb) and of the returned state
+// (epb, alb; on return the adjoints of ep0, al0), accumulates the adjoints
+// of the strain and of the material parameters.  The state before
+// correction k is recomputed by return_map itself capped at k corrections,
+// so it has the forward's bits.
+template <int EXPK>
+DL_HD void return_map_vjp(const PlasticParams &P, const double C[6], const double w[6],
+                          const double Hk[6], const double strain[6], const double ep0[6],
+                          const double al0[7], int n, const double epn[6], const double sb[6],
+                          double epb[6], double alb[7], double strb[6], MatBar &mb) {
+  // stress = C (strain - ep_n)
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    mb.C[i] += sb[i] * (strain[i] - epn[i]);
+    strb[i] += C[i] * sb[i];
+    epb[i] -= C[i] * sb[i];
+  }
+  PlasticParams Pk = P;
+#pragma unroll 1
+  for (int k = n - 1; k >= 0; k--) {
+    double ep[6], al[7], st[6], dfl;
+#pragma unroll
+    for (int i = 0; i < 6; i++) ep[i] = ep0[i];
+#pragma unroll
+    for (int i = 0; i < 7; i++) al[i] = al0[i];
+    Pk.max_iter = k;
+    return_map<EXPK>(Pk, C, w, Hk, strain, ep, al, st, &dfl, 1);
+    rm_correction_vjp<EXPK>(P, C, w, Hk, strain, ep, al, epb, alb, strb, mb);
+  }
+}
+
+// Element force of the elastoplastic material, reversed.  Two stress
+// sources share the kinematics:
+//  - the step's internal force (unless seed): the return map from the
+//    committed state (ep0, al0), with force adjoint (fba, fbb);
+//  - the GroupOutput rows (when has_r): the output load case, the elastic
+//    predictor C (strain - ep0), with force adjoint (rba, rbb).
+// epb, alb: on entry the adjoints of the plastic state the step commits
+// (ignored with seed), on return those of the committed state (ep0, al0).
+// Returns the number of corrections, or -1 at the iteration cap.
+template <int EXPK>
+DL_HD int elem_force_vjp_pl(const double Xa[3], const double ua[3], const Quat qna,
+                            const Quat qNa, const double Xb[3], const double ub[3],
+                            const Quat qnb, const Quat qNb, const double eps0[6],
+                            const PlasticParams &P, const double C[6], const double w[6],
+                            const double Hk[6], const double ep0[6], const double al0[7],
+                            bool seed, const double fba[6], const double fbb[6], bool has_r,
+                            const double rba[6], const double rbb[6], double epb[6],
+                            double alb[7], ElemVJP &out, MatBar &mb) {
+  ElemFwd F;
+  elem_vjp_forward(Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, F);
+  double Rb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, pPb[3] = {0, 0, 0};
+  double stb[6] = {0, 0, 0, 0, 0, 0};
+  int n = 0;
+  if (seed) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) epb[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 7; i++) alb[i] = 0.0;
+  } else {
+    double ep[6], al[7], sm[6], smb[6], dfl;
+#pragma unroll
+    for (int i = 0; i < 6; i++) ep[i] = ep0[i];
+#pragma unroll
+    for (int i = 0; i < 7; i++) al[i] = al0[i];
+    n = return_map<EXPK>(P, C, w, Hk, F.strain, ep, al, sm, &dfl, 1);
+    if (n < 0) return n;
+    elem_vjp_stress(F, sm, fba, fbb, Rb, pPb, smb);
+    return_map_vjp<EXPK>(P, C, w, Hk, F.strain, ep0, al0, n, ep, smb, epb, alb, stb, mb);
+  }
+  if (has_r) {
+    double so[6], sob[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) so[i] = C[i] * (F.strain[i] - ep0[i]);
+    elem_vjp_stress(F, so, rba, rbb, Rb, pPb, sob);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      mb.C[i] += sob[i] * (F.strain[i] - ep0[i]);
+      stb[i] += C[i] * sob[i];
+      epb[i] -= C[i] * sob[i];
+    }
+  }
+  elem_vjp_strain(F, stb, Rb, pPb, qNa, qNb, out);
+#pragma unroll
+  for (int i = 0; i < 6; i++) out.C[i] = 0.0;
+  return n;
 }
 
 // VJP of node_advance<true> (SO(3) mode).  Inputs are those of the forward

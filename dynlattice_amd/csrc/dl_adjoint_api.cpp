@@ -9,9 +9,12 @@
 //   k_adj_node (gather of the previous reversal's element ends + node VJP)
 //   -> k_adj_elem -> [contact list refresh + pair VJPs].
 // Parameter adjoints (per-element stiffness, per-node mass and inertia,
-// per-element contact distance) accumulate over all steps and are mapped to
-// the per-strut section data on the host, through the same set-up functions
-// that built the device tables from the sections.
+// per-element contact distance, per-element yield weights) accumulate over
+// all steps and are mapped to the per-strut section data on the host,
+// through the same set-up functions that built the device tables from the
+// sections.  With the elastoplastic material the committed plastic state is
+// checkpointed and taped too, and its adjoint travels back step by step
+// beside the node state's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,11 +39,28 @@ static int state_copy(dl_ctx *c, double *buf, bool save) {
   return DL_OK;
 }
 
+// copies the committed plastic state (the [E][12] records, then [E] alpha_0)
+// between the context and a [13 E] buffer
+static int plastic_copy(dl_ctx *c, double *buf, bool save) {
+  const int64_t E = c->d.E;
+  const size_t b = sizeof(double);
+  if (save) {
+    HIPCHK(hipMemcpyAsync(buf, c->d.pst, 12 * E * b, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(buf + 12 * E, c->d.aiso, E * b, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(c->d.pst, buf, 12 * E * b, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d.aiso, buf + 12 * E, E * b, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return DL_OK;
+}
+
+// return_map's exponent kind for the context's yield family
+static int plastic_expk(const dl_ctx *c) { return c->mat_kind == 3 ? 0 : c->mat_kind; }
+
 static int adjoint_supported(dl_ctx *c) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (!c->initialized) return fail(DL_ERR_STATE, "adjoint before INIT");
   if (!c->so3) return fail(DL_ERR_STATE, "the adjoint supports the SO(3) rotation update only");
-  if (c->plastic) return fail(DL_ERR_STATE, "the adjoint supports the elastic material only");
   if (c->pd.nprog > 0) return fail(DL_ERR_STATE, "the adjoint does not support time programs");
   if (c->imp.ngroups > 0) return fail(DL_ERR_STATE, "the adjoint does not support impact groups");
   if (c->n_mirrored > 0)
@@ -67,14 +87,35 @@ extern "C" int dl_adjoint_forward(dl_ctx *c, int64_t nsteps, double dt, int32_t 
     A.ckpt_slots = nseg;
   }
   if (!A.final_state && (rc = dalloc(c, &A.final_state, 13 * (size_t)N))) return rc;
+  const int64_t E = c->d.E;
+  if (c->plastic) {
+    if (nseg > A.ckpt_pl_slots) {
+      dfree(c, A.ckpt_pl);
+      A.ckpt_pl = nullptr;
+      A.ckpt_pl_slots = 0;
+      if ((rc = dalloc(c, &A.ckpt_pl, (size_t)nseg * 13 * E))) return rc;
+      A.ckpt_pl_slots = nseg;
+    }
+    if (!A.final_pl && (rc = dalloc(c, &A.final_pl, 13 * (size_t)E))) return rc;
+    if (!A.final_active) {
+      double *p = nullptr;
+      if ((rc = dalloc(c, &p, (E + 7) / 8))) return rc;
+      A.final_active = (uint8_t *)p;
+    }
+  }
   if ((rc = prepare_history(c, nsteps))) return rc;
   const StepParams sp{dt, 1, 0};
   for (int64_t seg = 0; seg < nseg; seg++) {
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, true))) return rc;
+    if (c->plastic && (rc = plastic_copy(c, A.ckpt_pl + seg * 13 * E, true))) return rc;
     if ((rc = advance_steps(c, std::min<int64_t>(every, nsteps - seg * every), sp))) return rc;
   }
   if ((rc = launch_reaction_tail(c))) return rc;
   if ((rc = state_copy(c, A.final_state, true))) return rc;
+  if (c->plastic) {
+    if ((rc = plastic_copy(c, A.final_pl, true))) return rc;
+    HIPCHK(hipMemcpyAsync(A.final_active, c->d.active, E, hipMemcpyDeviceToDevice, c->stream));
+  }
   HIPCHK(hipGetLastError());
   c->hist_n = nsteps;
   if ((rc = check_err(c))) return rc;
@@ -118,6 +159,19 @@ static int adjoint_buffers(dl_ctx *c) {
     if ((rc = upload(c, &A.nlist, nlist))) return rc;
     A.ready = true;
   }
+  if (c->plastic && !A.pbar) {
+    if ((rc = dalloc(c, &A.pbar, 13 * E))) return rc;
+    if ((rc = dalloc(c, &A.Wbar, 6 * E))) return rc;
+    if ((rc = dalloc(c, &A.rnode, N))) return rc;
+    if ((rc = dalloc(c, &A.fstep_r, N))) return rc;
+  }
+  if (c->plastic && A.ptape_rows < A.every) {
+    dfree(c, A.ptape);
+    A.ptape = nullptr;
+    A.ptape_rows = 0;
+    if ((rc = dalloc(c, &A.ptape, (size_t)A.every * 13 * E))) return rc;
+    A.ptape_rows = A.every;
+  }
   // reaction slot -> group (the groups may have changed since the last call)
   std::vector<int32_t> sg(std::max<int64_t>(c->n_react, 1), 0);
   for (int32_t g = 0; g < c->n_groups; g++)
@@ -150,8 +204,12 @@ static int compose_gradients(dl_ctx *c, dl_section_grads *g) {
   const AdjState &A = c->adj;
   const int64_t N = m.N, E = m.E;
   const int32_t R = m.R;
-  std::vector<double> Cb(6 * E), mtb(N), J0b(9 * N), crb(E);
+  std::vector<double> Cb(6 * E), mtb(N), J0b(9 * N), crb(E), Wb;
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->plastic) {
+    Wb.resize(6 * E);
+    HIPCHK(hipMemcpy(Wb.data(), A.Wbar, Wb.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
   HIPCHK(hipMemcpy(Cb.data(), A.Cbar, Cb.size() * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(mtb.data(), A.mtbar, mtb.size() * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(J0b.data(), A.J0bar, J0b.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -225,6 +283,16 @@ static int compose_gradients(dl_ctx *c, dl_section_grads *g) {
   put(g->shear_modulus, gG);
   put(g->density, grho);
   put(g->contact_radius, gcr);
+  if (g->yield_weight) {
+    // yield weights per strut, [R][6] like the material's table
+    for (int32_t s = 0; s < R; s++)
+      for (int i = 0; i < 6; i++) {
+        double t = 0.0;
+        if (c->plastic)
+          for (int64_t e = m.eptr[s]; e < m.eptr[s + 1]; e++) t += Wb[i * E + e];
+        g->yield_weight[6 * s + i] = t;
+      }
+  }
   return DL_OK;
 }
 
@@ -253,6 +321,16 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
   a.ubar = A.ubar; a.qbar = A.qbar; a.vbar = A.vbar; a.fbar = A.fbar; a.ebuf = A.ebuf;
   a.Cbar = A.Cbar; a.mtbar = A.mtbar; a.J0bar = A.J0bar; a.crbar = A.crbar;
   a.nptr = A.nptr; a.nlist = A.nlist; a.slot_group = A.slot_group;
+  const bool pl = c->plastic;
+  if (pl) {
+    HIPCHK(hipMemsetAsync(A.Wbar, 0, 6 * E * sizeof(double), s));
+    a.pbar = A.pbar; a.Wbar = A.Wbar; a.rnode = A.rnode; a.fstep_r = A.fstep_r;
+  }
+  // the element force's reverse for the context's material
+  auto adj_elem = [&](const AdjDev &aa) {
+    if (pl) launch_adj_elem_pl(c->d, aa, c->pp, plastic_expk(c), s);
+    else launch_adj_elem(c->d, aa, s);
+  };
   const StepParams sp{A.dt, 1, 0};
   const int64_t nseg = (A.nsteps + A.every - 1) / A.every;
   // Row t of the reaction history is the GroupOutput sum at the state after
@@ -263,7 +341,15 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
   a.rbar = A.rbar + (A.nsteps - 1) * ng;
   a.tape = A.final_state; // rows 0..6: u, rotation (the tape's layout)
   launch_adj_seed(c->d, a, s);
-  launch_adj_elem(c->d, a, s);
+  if (pl) {
+    // the closing row is the output load case at the final state: elastic
+    // predictor from the final committed plastic state, which the seed
+    // evaluation also gives its first adjoint (a.pbar)
+    a.seed = 1;
+    a.ptape = A.final_pl;
+  }
+  adj_elem(a);
+  a.seed = 0;
   if (c->contact) {
     contact_invalidate(c->cs, s);
     DevData dn = c->d;
@@ -273,11 +359,14 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
   for (int64_t seg = nseg - 1; seg >= 0; seg--) {
     const int64_t t0 = seg * A.every, len = std::min<int64_t>(A.every, A.nsteps - t0);
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, false))) return rc;
+    if (pl && (rc = plastic_copy(c, A.ckpt_pl + seg * 13 * E, false))) return rc;
     if (c->contact) contact_invalidate(c->cs, s);
     // recompute the segment, taping every step's start state and forces
+    // (and the committed plastic state the step starts from)
     for (int64_t j = 0; j < len; j++) {
       DevData dj = c->d;
       dj.tape = A.tape + j * 19 * N;
+      if (pl && (rc = plastic_copy(c, A.ptape + j * 13 * E, true))) return rc;
       if (c->contact && (rc = contact_step(c->cs, dj, s, nullptr, nullptr, nullptr))) return rc;
       launch_strut_pp(dj, sp, c->pp, MODE_STEP, true, c->mat_kind, c->contact, s);
       launch_joint(dj, sp, MODE_STEP, true, s);
@@ -288,8 +377,9 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
       a.tape = A.tape + j * 19 * N;
       // the start-state force of step t is row t - 1's reaction
       a.rbar = (t0 + j >= 1) ? A.rbar + (t0 + j - 1) * ng : nullptr;
+      if (pl) a.ptape = A.ptape + j * 13 * E;
       launch_adj_node(c->d, a, A.dt, true, s);
-      launch_adj_elem(c->d, a, s);
+      adj_elem(a);
       if (c->contact) {
         DevData dn = c->d;
         dn.u = (double *)a.tape; // rows 0..2 of the tape: the step's start displacement
@@ -299,6 +389,10 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
     HIPCHK(hipGetLastError());
   }
   if ((rc = state_copy(c, A.final_state, false))) return rc;
+  if (pl) {
+    if ((rc = plastic_copy(c, A.final_pl, false))) return rc;
+    HIPCHK(hipMemcpyAsync(c->d.active, A.final_active, E, hipMemcpyDeviceToDevice, s));
+  }
   if (c->contact) contact_invalidate(c->cs, s);
   if ((rc = check_err(c))) return rc;
   return compose_gradients(c, grads);

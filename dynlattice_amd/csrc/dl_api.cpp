@@ -618,6 +618,24 @@ extern "C" int dl_set_sections(dl_ctx *c, const dl_sections *sec, const double *
   return DL_OK;
 }
 
+extern "C" int dl_set_yield_weights(dl_ctx *c, const double *weights) {
+  if (!c || !weights) return fail(DL_ERR_INPUT, "NULL argument");
+  if (!c->plastic) return fail(DL_ERR_STATE, "yield weights need the elastoplastic material");
+  const int32_t R = c->mesh.R;
+  for (int64_t k = 0; k < 6 * (int64_t)R; k++)
+    if (!(weights[k] >= 0.0) || !std::isfinite(weights[k]))
+      return fail(DL_ERR_INPUT, "yield weights must be finite and >= 0");
+  hipSetDevice(c->device);
+  c->adj.valid = false;
+  c->pl_weights.assign(weights, weights + 6 * (size_t)R);
+  std::vector<double> pw(6 * (size_t)R);
+  for (int32_t s = 0; s < R; s++)
+    for (int i = 0; i < 6; i++) pw[i * (size_t)R + s] = weights[6 * s + i];
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy((void *)c->d.pw, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice));
+  return DL_OK;
+}
+
 extern "C" int dl_sync(dl_ctx *c) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   hipSetDevice(c->device);

@@ -69,6 +69,16 @@ struct AdjState {
   int32_t *nptr = nullptr, *nlist = nullptr, *slot_group = nullptr;
   double *rbar = nullptr;
   int64_t rbar_cap = 0;
+  // elastoplastic material: committed plastic state (the [E][12] records,
+  // then [E] alpha_0) at every checkpoint, at the end of the forward run and
+  // at the start of every step of the segment being reversed
+  int64_t ckpt_pl_slots = 0;
+  double *ckpt_pl = nullptr;   // [slots][13 E]
+  double *final_pl = nullptr;  // [13 E]
+  uint8_t *final_active = nullptr; // [E] plastic active flags after the forward run
+  int64_t ptape_rows = 0;
+  double *ptape = nullptr;     // [rows][13 E]
+  double *pbar = nullptr, *Wbar = nullptr, *rnode = nullptr, *fstep_r = nullptr;
 };
 
 // boundary-data time programs (host copy)

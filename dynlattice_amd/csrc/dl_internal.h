@@ -120,9 +120,10 @@ struct StepParams {
 // into the force evaluation
 enum { MODE_STEP = 0, MODE_FORCE = 1, MODE_MPRED = 2, MODE_MCORR = 3 };
 
-// Reverse sweep of the explicit step (SO(3) mode, elastic material).
-// Adjoints ("bar") of the state after the step being reversed are turned
-// into those of the state before it; parameter adjoints accumulate.
+// Reverse sweep of the explicit step (SO(3) mode, elastic or elastoplastic
+// material).  Adjoints ("bar") of the state after the step being reversed
+// are turned into those of the state before it; parameter adjoints
+// accumulate.
 struct AdjDev {
   const double *tape;   // [19][N] start state of the step: u 3, rot 4, v 6, f 6
   double *ubar;         // [3][N]
@@ -138,6 +139,16 @@ struct AdjDev {
   const int32_t *nlist; // (element << 1) | end
   const int32_t *slot_group; // reaction slot -> group
   const double *rbar;   // [groups] objective adjoint of this step's reactions
+  // elastoplastic material (null for the elastic one)
+  const double *ptape;  // committed plastic state at the step's start: the
+                        // [E][12] records of DevData::pst, then [E] alpha_0
+  double *pbar;         // [13][E] adjoint of the committed plastic state
+                        // (eps_p 6, alpha 7): of the step's end state on
+                        // entry to the element kernel, of its start on exit
+  double *Wbar;         // [6][E] yield-weight adjoint, summed over steps
+  double *rnode;        // [N] objective weight on the node's reaction dof
+  double *fstep_r;      // [N] the step force's adjoint on that dof (without it)
+  int32_t seed;         // closing evaluation: the output load case only
 };
 
 // boundary-data time programs: scales evaluated on the device from the
@@ -305,6 +316,10 @@ void launch_state_io(const DevData &d, bool so3, bool to_dof, double *disp, doub
 // adjoint (dl_adjoint.hip): [gather of the element ends +] node update, element force
 void launch_adj_node(const DevData &d, const AdjDev &a, double dt, bool gather, hipStream_t s);
 void launch_adj_elem(const DevData &d, const AdjDev &a, hipStream_t s);
+// elastoplastic element force (return map and output load case), reversed;
+// expk: the yield family's exponent kind (return_map's template argument)
+void launch_adj_elem_pl(const DevData &d, const AdjDev &a, const PlasticParams &pp, int expk,
+                        hipStream_t s);
 // objective weight of the call's closing reaction row: fbar = rbar at the
 // reaction groups' dofs, zero elsewhere
 void launch_adj_seed(const DevData &d, const AdjDev &a, hipStream_t s);

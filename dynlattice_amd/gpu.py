@@ -38,7 +38,7 @@ EXPORTS = (
     "dl_set_sections", "dl_adjoint_forward", "dl_adjoint_backward", "dl_milne_step",
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
     "dl_get_table", "dl_get_energies", "dl_set_impact_groups", "dl_explicit_step_noalloc",
-    "dl_contact_list_stats", "dl_set_plastic_state",
+    "dl_contact_list_stats", "dl_set_plastic_state", "dl_set_yield_weights",
 )
 #: DL_NOALLOC_SCRATCH_DOUBLES of include/dl_gpu.h
 NOALLOC_SCRATCH_DOUBLES = 1 << 18
@@ -99,7 +99,7 @@ class dl_params(C.Structure):
 
 
 class dl_section_grads(C.Structure):
-    _fields_ = [(f, dp) for f in SECTION_FIELDS] + [("contact_radius", dp)]
+    _fields_ = [(f, dp) for f in SECTION_FIELDS] + [("contact_radius", dp), ("yield_weight", dp)]
 
 
 class dl_step_out(C.Structure):
@@ -167,6 +167,7 @@ def load():
     L.dl_explicit_step_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, dp, dp, ip64]
     L.dl_contact_list_stats.argtypes = [C.c_void_p, ip64, ip64]
     L.dl_set_plastic_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+    L.dl_set_yield_weights.argtypes = [C.c_void_p, dp]
     L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
     L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
     L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
@@ -528,6 +529,13 @@ class GpuModel:
         self._check(self.L.dl_set_sections(self.h, C.byref(sec) if sec is not None else None,
                                            _p(cr)))
 
+    def set_yield_weights(self, weights):
+        """Replace the elastoplastic material's yield weights ([n_struts, 6])."""
+        w = _c(weights).reshape(-1)
+        if w.size != 6 * self.p.n_struts:
+            raise ValueError(f"yield weights must have {self.p.n_struts} x 6 entries")
+        self._check(self.L.dl_set_yield_weights(self.h, _p(w)))
+
     def adjoint_forward(self, nsteps, dt=None, checkpoint_every=100):
         """nsteps explicit steps with checkpoints; returns the reaction
         history [nsteps, n_groups]."""
@@ -541,14 +549,17 @@ class GpuModel:
 
     def adjoint_backward(self, reaction_bar):
         """Gradients of the section data (dict of [n_struts] arrays, plus
-        contact_radius) for the objective adjoint reaction_bar [nsteps, n_groups]."""
+        contact_radius and, [n_struts, 6], yield_weight) for the objective
+        adjoint reaction_bar [nsteps, n_groups]."""
         ng = max(getattr(self, "_ngroups", 1), 1)
         rb = _c(reaction_bar).reshape(-1)
         if rb.size != self._adj_steps * ng:
             raise ValueError(f"reaction_bar must have {self._adj_steps} x {ng} entries")
         R = self.p.n_struts
         out = {f: np.zeros(R) for f in SECTION_FIELDS + ("contact_radius",)}
-        g = dl_section_grads(*[_p(out[f]) for f in SECTION_FIELDS + ("contact_radius",)])
+        out["yield_weight"] = np.zeros((R, 6))
+        g = dl_section_grads(*[_p(out[f]) for f in SECTION_FIELDS + ("contact_radius",)],
+                             _p(out["yield_weight"]))
         self._check(self.L.dl_adjoint_backward(self.h, _p(rb), C.byref(g)))
         return out
 

@@ -399,13 +399,14 @@ def bcc_c5(n: int = 20) -> Problem:
     return p
 
 
-def bcc_adjoint_check(radius=None, dt=None) -> Problem:
+def bcc_adjoint_check(radius=None, dt=None, sigma_y=None) -> Problem:
     """Small adjoint check case: 2^3 BCC cells of 10 mm, 3 elements per
     strut (64 struts, 192 elements), r = 1.2 mm (1 + 0.1 U[-1, 1]), seed 5
     (or the given per-strut radii), E = 70 GPa, nu = 0.3, rho = 2700,
     elastic, penalty contact 1e6 N/m with contact radius = strut radius,
     bottom clamped, top at -20 m/s.  Within 1600 steps about 30 contact pairs
-    close."""
+    close.  With sigma_y (a scalar or per strut): elastoplastic, quadratic
+    interaction of the resultant capacities, kinematic hardening H = C/100."""
     n, m, cell = 2, 3, 10e-3
     R = 8 * n ** 3
     r = 1.2e-3 * (1.0 + 0.1 * (2.0 * uniform01(5, R) - 1.0)) if radius is None else \
@@ -415,6 +416,9 @@ def bcc_adjoint_check(radius=None, dt=None) -> Problem:
     crush_bcs(p, -20.0)
     p.dt = stable_dt(p, 0.5) if dt is None else dt
     p.name = "bcc-2x2x2-m3-elastic-contact"
+    if sigma_y is not None:
+        p.plastic = Plastic.resultant_capacities(r, sigma_y, kin_ratio=0.01, exponent=2.0)
+        p.name = "bcc-2x2x2-m3-elastoplastic-contact"
     return p
 
 

@@ -308,11 +308,20 @@ int dl_set_timing(dl_ctx *ctx, int32_t enabled);
  * blacklist keeps the radii of the last INIT. */
 int dl_set_sections(dl_ctx *ctx, const dl_sections *sec, const double *contact_radius);
 
+/* Replaces the yield weights of the elastoplastic material ([n_struts][6],
+ * the table dl_material.weights gave at create: the per-rod-model yield
+ * function's coefficients, ElastoPlasticRodMaterial.cpp:131-141), e.g. after
+ * a change of the strut radii or yield stresses. */
+int dl_set_yield_weights(dl_ctx *ctx, const double *weights);
+
 /* Gradients of the per-strut section data (each array [n_struts] or NULL to
- * skip it). */
+ * skip it) and, with the elastoplastic material, of the yield weights
+ * ([n_struts][6] like dl_material.weights, or NULL; zero for the elastic
+ * material). */
 typedef struct {
   double *area, *I1, *I2, *J, *shear_factor, *young, *shear_modulus, *density;
   double *contact_radius;
+  double *yield_weight;
 } dl_section_grads;
 
 /* Adjoint design gradients (no counterpart in the reference, which has no
@@ -325,11 +334,17 @@ typedef struct {
  * returns the gradient of the objective with respect to the section data by
  * a reverse sweep: every segment is recomputed from its checkpoint, then
  * reversed through hand-written adjoint kernels of the node update, the
- * element internal force (elastic material) and the contact pair forces;
- * the last row's weight enters through the internal force's adjoint at the
- * final state.
- * SO(3) rotation update and elastic material only.  The forward run's final
- * state is restored afterwards; backward may be called again with another
+ * element internal force and the contact pair forces; the last row's weight
+ * enters through the internal force's adjoint at the final state.
+ * Elastoplastic material (ElastoPlasticRodMaterial.cpp:198-269): the
+ * committed plastic state is checkpointed and taped with the node state,
+ * the cutting-plane return map is reversed through every correction it
+ * took (the derivative of the algorithm as executed: exact wherever the
+ * number of corrections does not change), and the reaction rows are
+ * reversed through the output load case (the elastic predictor) that
+ * GroupOutput records.
+ * SO(3) rotation update only.  The forward run's final state (and plastic
+ * state) is restored afterwards; backward may be called again with another
  * reaction_bar until the state, boundary data or sections change. */
 int dl_adjoint_forward(dl_ctx *ctx, int64_t nsteps, double dt, int32_t checkpoint_every,
                        double *reaction);

@@ -237,3 +237,168 @@ def test_contact_pair_vjp_open_pair(lib):
     z = np.zeros(6)
     assert lib.hc_pair_vjp(ptr(pA), ptr(pB), 1, 2e-3, 1e6, 1e6, ptr(z), ptr(z), ptr(out)) == 0
     assert (out == 3.0).all()
+
+
+# ------------------------------------------------------------------ plastic
+# The cutting plane (return_map) is differentiated as executed: through each
+# correction it takes.  Central differences are valid where the number of
+# corrections is the same on both sides of the difference; every check
+# asserts that.
+def _plib(lib):
+    P = C.POINTER(C.c_double)
+    lib.hc_return_map.argtypes = [C.c_int] + [P] * 8
+    lib.hc_return_map.restype = C.c_int
+    lib.hc_return_map_vjp.argtypes = [C.c_int] + [P] * 11
+    lib.hc_return_map_vjp.restype = C.c_int
+    lib.hc_elem_force_pl.argtypes = [C.c_int] + [P] * 16
+    lib.hc_elem_force_pl.restype = C.c_int
+    lib.hc_elem_force_vjp_pl.argtypes = [C.c_int] + [P] * 15 + [C.c_int] + [P] * 5
+    lib.hc_elem_force_vjp_pl.restype = C.c_int
+    lib.hc_elem_strain.argtypes = [P] * 10
+    return lib
+
+
+def _params(p, iso, kin, max_iter=20):
+    """flat PlasticParams: yield_level, kin_coupling 6, iso_coupling,
+    iso_modulus, precision, exponent 6, kin_const 6, kin_ratio, has_iso,
+    has_kin, max_iter"""
+    return np.r_[1.0, np.full(6, 1.0), 0.4 if iso else 0.0, 50.0 if iso else 0.0, 1e-5,
+                 np.full(6, float(p)), np.zeros(6), 0.01, float(iso), float(kin), float(max_iter)]
+
+
+RM_CASES = [  # (expk, p, iso, kin, overshoot, balanced stiffness)
+    (2, 2.0, 0, 1, 1.2, False),   # the crush's quadratic interaction, kinematic hardening
+    (2, 2.0, 1, 1, 1.1, False),   # + isotropic hardening
+    (1, 1.0, 0, 1, 1.3, True),    # sum-abs
+    (0, 1.5, 1, 0, 1.2, False),   # general exponent, isotropic only
+    (2, 2.0, 0, 1, 0.9, False),   # inside the surface: elastic, no correction
+]
+# The reference's cutting plane converges only from near the surface (a
+# quadratic trial state 1.5 times outside already cycles past the 20
+# iterations with these stiffness ratios, and the sum-abs surface diverges
+# once a component changes sign), so the trial states lie where it converges
+# -- as in an explicit step, whose strain increments are small.
+
+
+def _rm_inputs(rng, p, over, balanced):
+    Cm = (np.ones(6) if balanced else np.array([2e4, 2e4, 6e4, 0.3, 0.3, 0.25])) * \
+        (1 + rng.random(6))
+    ey = 1e-3 * (1 + 0.3 * rng.random(6))
+    w = 1.0 / (Cm * ey)
+    Hk = 0.01 * Cm * (1 + rng.random(6))
+    ep0 = rng.normal(size=6) * 0.1 * ey
+    al0 = np.r_[abs(rng.normal()) * 1e-3, rng.normal(size=6) * 0.1 * ey]
+    # w |C (strain - ep0)| = over |d_i| / ||d||_p, so that sum |.|^p = over^p
+    d = (0.5 + 0.5 * rng.random(6)) * rng.choice([-1.0, 1.0], 6)
+    strain = ep0 + over * ey * d / (np.abs(d) ** p).sum() ** (1.0 / p)
+    return Cm, w, Hk, strain, ep0, al0
+
+
+@pytest.mark.parametrize("case", RM_CASES)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_return_map_vjp(lib, case, seed):
+    lib = _plib(lib)
+    expk, p, iso, kin, over, bal = case
+    rng = np.random.default_rng(100 * seed + RM_CASES.index(case))
+    pa = _params(p, iso, kin)
+    Cm, w, Hk, strain, ep0, al0 = _rm_inputs(rng, p, over, bal)
+    sb, epb, alb = rng.normal(size=6) * 1e-3, rng.normal(size=6) * 1e3, rng.normal(size=7) * 1e3
+
+    def run(x):
+        ep, al, st = x[6:12].copy(), x[12:19].copy(), np.zeros(6)
+        n = lib.hc_return_map(expk, ptr(pa), ptr(x[19:25]), ptr(x[25:31]), ptr(x[31:37]),
+                              ptr(x[0:6]), ptr(ep), ptr(al), ptr(st))
+        return n, float(sb @ st + epb @ ep + alb @ al)
+
+    x0 = np.r_[strain, ep0, al0, Cm, w, Hk]
+    n0, _ = run(x0)
+    out = np.zeros(37)
+    n = lib.hc_return_map_vjp(expk, ptr(pa), ptr(Cm), ptr(w), ptr(Hk), ptr(strain), ptr(ep0),
+                              ptr(al0), ptr(sb), ptr(epb), ptr(alb), ptr(out))
+    assert n == n0 and n >= 0
+    if over > 1.0:
+        assert n >= 1
+    else:
+        assert n == 0
+    h = np.abs(x0) * 1e-7 + 1e-14
+
+    def J_of(x):
+        nx, J = run(x)
+        assert nx == n0, "correction count changed inside the difference"
+        return J
+
+    # out = strain 6, ep0 6, al0 7, C 6, w 6, Hk 6: the order of x0
+    fd_check(J_of, x0, out, h, rtol=2e-5, atol_frac=1e-8)
+
+
+def test_return_map_cases_take_several_corrections(lib):
+    """the quadratic and general-exponent cases above reverse more than one
+    correction (so the recomputation of the intermediate states is tested),
+    the sum-abs case exactly one (its surface is linear between sign changes)"""
+    lib = _plib(lib)
+    for case in RM_CASES:
+        expk, p, iso, kin, over, bal = case
+        rng = np.random.default_rng(RM_CASES.index(case))
+        Cm, w, Hk, strain, ep0, al0 = _rm_inputs(rng, p, over, bal)
+        ep, al, st = ep0.copy(), al0.copy(), np.zeros(6)
+        n = lib.hc_return_map(expk, ptr(_params(p, iso, kin)), ptr(Cm), ptr(w), ptr(Hk),
+                              ptr(strain), ptr(ep), ptr(al), ptr(st))
+        expect = 0 if over < 1 else (1 if expk == 1 else 2)
+        assert n >= expect and (expk != 1 or n == 1), (case, n)
+
+
+def _pl_element(lib, rng, over):
+    Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, Cm = _element(rng, 0.7)
+    ep0 = rng.normal(size=6) * 1e-4
+    strain = np.zeros(6)
+    lib.hc_elem_strain(ptr(Xa), ptr(ua), ptr(qna), ptr(qNa), ptr(Xb), ptr(ub), ptr(qnb), ptr(qNb),
+                       ptr(eps0), ptr(strain))
+    # weights that put the trial state `over` times outside the quadratic surface
+    w = over / (np.sqrt(6.0) * np.abs(Cm * (strain - ep0)))
+    Hk = 0.01 * Cm
+    al0 = np.r_[0.0, rng.normal(size=6) * 1e-5]
+    return Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, Cm, w, Hk, ep0, al0
+
+
+@pytest.mark.parametrize("seed,over,has_r,seed_mode", [(0, 1.2, False, 0), (1, 1.1, True, 0),
+                                                       (2, 0.6, True, 0), (3, 1.15, True, 1)])
+def test_element_force_vjp_plastic(lib, seed, over, has_r, seed_mode):
+    """the element with the return map (step force), the output load case
+    (elastic predictor from the committed state) and the returned plastic
+    state, all reversed together; seed_mode: the output load case alone"""
+    lib = _plib(lib)
+    rng = np.random.default_rng(50 + seed)
+    pa = _params(2.0, 0, 1)
+    Xa, ua, qna, qNa, Xb, ub, qnb, qNb, eps0, Cm, w, Hk, ep0, al0 = _pl_element(lib, rng, over)
+    fbar = rng.normal(size=12) * (0.0 if seed_mode else 1.0)
+    rbar = rng.normal(size=12) if has_r else np.zeros(12)
+    epb = rng.normal(size=6) * (0.0 if seed_mode else 1e2)
+    alb = rng.normal(size=7) * (0.0 if seed_mode else 1e2)
+
+    def run(x):
+        out = np.zeros(37)
+        n = lib.hc_elem_force_pl(2, ptr(pa), ptr(Xa), ptr(x[0:3]), ptr(x[3:7]), ptr(qNa), ptr(Xb),
+                                 ptr(x[7:10]), ptr(x[10:14]), ptr(qNb), ptr(eps0), ptr(x[27:33]),
+                                 ptr(x[33:39]), ptr(x[39:45]), ptr(x[14:20]), ptr(x[20:27]),
+                                 ptr(out))
+        return n, float(fbar @ out[0:12] + rbar @ out[12:24] + epb @ out[24:30] + alb @ out[30:37])
+
+    x0 = np.r_[ua, qna, ub, qnb, ep0, al0, Cm, w, Hk]
+    n0, _ = run(x0)
+    assert (n0 >= 1) == (over > 1.0)
+    out = np.zeros(45)
+    n = lib.hc_elem_force_vjp_pl(2, ptr(pa), ptr(Xa), ptr(ua), ptr(qna), ptr(qNa), ptr(Xb),
+                                 ptr(ub), ptr(qnb), ptr(qNb), ptr(eps0), ptr(Cm), ptr(w), ptr(Hk),
+                                 ptr(ep0), ptr(al0), seed_mode, ptr(fbar),
+                                 ptr(rbar) if has_r else None, ptr(epb), ptr(alb), ptr(out))
+    assert n == (0 if seed_mode else n0)
+
+    def J_of(x):
+        nx, J = run(x)
+        assert nx == n0, "correction count changed inside the difference"
+        return J
+
+    h = np.r_[np.full(3, 1e-9), np.full(4, 1e-8), np.full(3, 1e-9), np.full(4, 1e-8),
+              np.full(6, 1e-9), np.full(7, 1e-9), Cm * 1e-7, w * 1e-7,
+              Hk * 1e-7]
+    fd_check(J_of, x0, out, h, rtol=2e-5, atol_frac=1e-8)

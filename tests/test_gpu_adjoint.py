@@ -10,6 +10,14 @@
   differences of the oracle (fixture tests/golden/adjoint_fd_bcc2.json, made
   by tests/golden/make_adjoint_fd.py) on all 64 struts of the contact case,
   for J = absorbed energy + peak plate force.
+* Elastoplastic material: the checkpointed forward is the explicit step
+  (plastic state included) and backward restores it; section and
+  yield-weight gradients equal central differences of the GPU forward
+  (1e-5); radius and yield-stress gradients equal the oracle's central
+  differences (fixture tests/golden/adjoint_fd_bcc2_plastic.json, made by
+  tests/golden/make_adjoint_fd_plastic.py) within 1e-4, without contact on
+  every strut, with contact on the struts whose differences at h and 2h
+  agree (the others straddle a jump of the reference's contact branches).
 """
 import json
 import os
@@ -171,3 +179,142 @@ def test_gradients_with_init_blacklist_and_crowded_joints():
              ("contact_radius", 18)]
     assert _fd_check(p, K, 50, picks, contact=True, rtol=1e-4,
                      max_pairs=400 * p.n_elems) == len(picks)
+
+
+# ------------------------------------------------------------------ plastic
+# The elastoplastic material: the reverse sweep runs through the cutting-plane
+# return map (every correction it took), the committed plastic state's
+# adjoint travels back step by step, and the reaction rows (GroupOutput's
+# output load case: the elastic predictor) are reversed as such.
+SIGMA_Y = 250e6
+
+
+def _plastic_case(contact=True, precision=None):
+    p = L.bcc_adjoint_check(sigma_y=SIGMA_Y)
+    if not contact:
+        p.contact = None
+    if precision is not None:
+        p.plastic.precision = precision
+    return p
+
+
+def test_plastic_checkpointed_forward_is_the_explicit_step():
+    """reaction history, final state and plastic state of the checkpointed
+    forward equal the explicit step's bit for bit; backward restores them
+    (plastic state and active set included) and repeats bit for bit"""
+    from dynlattice_amd.gpu import GpuModel
+    p = _plastic_case()
+    n = 700
+    sim = _sim(p, n, 64)
+    r = sim.run(_sections(p), p.contact.radius)
+    d_fw, v_fw = sim.model.get_state()
+    ep_fw, al_fw = sim.model.plastic_state()
+    act_fw = sim.model.plastic_active()
+    assert act_fw.mean() > 0.2
+    g = GpuModel(p)
+    g.set_reaction_group(p.groups["top"], 2)
+    g.step(n)
+    d, v = g.get_state()
+    assert np.array_equal(r[:, 0], g.history(n)["reaction"])
+    assert np.array_equal(d, d_fw) and np.array_equal(v, v_fw)
+    ep, al = g.plastic_state()
+    assert np.array_equal(ep, ep_fw) and np.array_equal(al, al_fw)
+    w = np.random.default_rng(0).normal(size=r.shape)
+    g1 = sim.gradients(w)
+    d2, v2 = sim.model.get_state()
+    assert np.array_equal(d2, d_fw) and np.array_equal(v2, v_fw)
+    ep2, al2 = sim.model.plastic_state()
+    assert np.array_equal(ep2, ep_fw) and np.array_equal(al2, al_fw)
+    assert np.array_equal(sim.model.plastic_active(), act_fw)
+    g2 = sim.gradients(w)
+    for k in g1:
+        assert np.array_equal(g1[k], g2[k]), k
+    assert np.abs(g1["yield_weight"]).max() > 0
+    # a second forward run starts again from the zero plastic state
+    r2 = sim.run(_sections(p), p.contact.radius)
+    assert np.array_equal(r, r2)
+    g.close()
+    sim.close()
+
+
+def test_plastic_section_gradients_match_gpu_finite_differences():
+    """section and yield-weight gradients of a yielding crush (no contact)
+    against central differences of the GPU forward; the return map's
+    precision property at 1e-12, so that its stopping test does not make J
+    jump inside the difference"""
+    p = _plastic_case(contact=False, precision=1e-12)
+    nsteps, every = 800, 64
+    rng = np.random.default_rng(4)
+    sim = _sim(p, nsteps, every)
+    base = _sections(p)
+    yw0 = p.plastic.weights.copy()
+    r = sim.run(base, None, yw0)
+    assert sim.model.plastic_active().mean() > 0.2
+    w = rng.normal(size=r.shape)
+    g = sim.gradients(w)
+    J_abs = float(np.abs(w * r).sum())
+    picks = [("area", 3), ("I1", 21), ("J", 11), ("young", 17), ("shear_modulus", 40),
+             ("density", 8), ("yield_weight", (5, 2)), ("yield_weight", (21, 3)),
+             ("yield_weight", (40, 0)), ("yield_weight", (13, 5))]
+    for field, s in picks:
+        x0 = yw0[s] if field == "yield_weight" else base[field][s]
+        h = 1e-6 * abs(x0)
+        vals = []
+        for sgn in (1, -1):
+            sec = {k: v.copy() for k, v in base.items()}
+            yw = yw0.copy()
+            if field == "yield_weight":
+                yw[s] += sgn * h
+            else:
+                sec[field][s] += sgn * h
+            vals.append(float((w * sim.run(sec, None, yw)).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        ad = g[field][s]
+        assert abs(fd - ad) <= 1e-5 * abs(fd) + 1e-7 * J_abs / abs(x0), (field, s, fd, ad)
+    sim.close()
+
+
+def _oracle_fd_check(name, contact):
+    """radius and yield-stress gradients through the torch path against the
+    oracle's central differences, on the variables whose differences at h
+    and 2h agree within 1e-4 (the others straddle a jump of J)"""
+    import torch
+    from dynlattice_amd.adjoint import impact_objective, response_from_radius
+    fx = json.load(open(os.path.join(HERE, "golden", "adjoint_fd_bcc2_plastic.json")))
+    case = fx[name]
+    p = _plastic_case(contact=contact)
+    assert abs(p.dt - fx["dt"]) == 0.0
+    sim = _sim(p, fx["nsteps"], 100)
+    r = torch.tensor(L.bcc_adjoint_check().contact.radius, dtype=torch.float64,
+                     requires_grad=True)
+    sy = torch.full_like(r, fx["sigma_y"]).requires_grad_(True)
+    reaction = response_from_radius(sim, r, 70e9, 0.3, 2700.0, sigma_y=sy)
+    J = impact_objective(reaction, p.dt, fx["v_plate"])
+    assert abs(J.item() - case["objective"]) <= 1e-8 * abs(case["objective"])
+    J.backward()
+    out = {}
+    x0 = {"dJ_dr": r.detach().numpy(), "dJ_dsy": sy.detach().numpy()}
+    for key, grad, idx in (("dJ_dr", r.grad.numpy(), fx["struts"]),
+                           ("dJ_dsy", sy.grad.numpy(), fx["sy_struts"])):
+        f1, f2 = (np.array(v) for v in case[key])
+        ad = grad[idx]
+        # rounding floor of a central difference of J over a step h x0
+        noise = 1e-13 * abs(case["objective"]) / (fx["h_rel"][0] * x0[key][idx])
+        clean = np.abs(f1 - f2) <= 1e-4 * np.abs(f1) + noise
+        # (a strut that never yields has no yield-stress derivative: both 0)
+        err = np.abs(ad - f1) / (1e-4 * np.abs(f1) + noise)
+        assert err[clean].max() <= 1.0, (key, np.array(idx)[clean][np.argmax(err[clean])],
+                                         float(err[clean].max()))
+        out[key] = int(clean.sum())
+    sim.close()
+    return out
+
+
+def test_plastic_gradients_match_oracle_finite_differences():
+    n = _oracle_fd_check("nocontact", contact=False)
+    assert n["dJ_dr"] >= 50 and n["dJ_dsy"] >= 12, n
+
+
+def test_plastic_contact_gradients_match_oracle_finite_differences():
+    n = _oracle_fd_check("contact", contact=True)
+    assert n["dJ_dr"] >= 12, n
