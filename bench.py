@@ -305,22 +305,24 @@ def sweep(args):
 
 def adjoint(args):
     """Adjoint design gradient of the 20^3 BCC impact with contact (see
-    adjoint_measure), with a central finite-difference spot check."""
+    adjoint_measure), with a central finite-difference spot check; with
+    --adjoint-sigma-y the elastoplastic variant."""
     print(json.dumps(adjoint_measure(args.adjoint_n, args.adjoint_steps, args.checkpoint_every,
-                                     args.fd_struts)), flush=True)
+                                     args.fd_struts, args.adjoint_sigma_y or None)), flush=True)
 
 
-def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0):
+def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0, sigma_y=None):
     """Adjoint design gradient of the n_cells^3 BCC impact with contact:
     gradient of J = absorbed energy + peak plate force with respect to all
     strut radii, by the checkpointed forward run and the reverse sweep;
     reported as (forward with checkpoints + reverse sweep) / plain forward
     time, with a central finite-difference spot check of the GPU forward run
-    on fd_struts struts."""
+    on fd_struts struts.  With sigma_y the elastoplastic variant, whose
+    radius gradient includes the yield weights' dependence on the radius."""
     from dynlattice_amd import lattice as L
     from dynlattice_amd.adjoint import ImpactSimulation
     from dynlattice_amd.gpu import SECTION_FIELDS
-    p = L.bcc_c5(n_cells)
+    p = L.bcc_c5(n_cells, sigma_y)
     print(f"adjoint: {p.name}: {p.n_elems} elements, {p.n_struts} struts, {n} steps, "
           f"checkpoint every {every}", file=sys.stderr, flush=True)
     sim = ImpactSimulation(p, n, every)
@@ -357,6 +359,11 @@ def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0):
     pi = math.pi
     dJdr = (g["area"] * 2 * pi * r0 + (g["I1"] + g["I2"]) * pi * r0 ** 3 +
             g["J"] * 2 * pi * r0 ** 3 + g["contact_radius"])
+    if p.plastic is not None:
+        # weights 1/capacity: capacities go with r^2 (shear, axial), r^3 (moments)
+        k = np.array([2.0, 2.0, 2.0, 3.0, 3.0, 3.0])
+        dJdr += (g["yield_weight"] * (-k * p.plastic.weights / r0[:, None])).sum(1)
+    active = float(m.plastic_active().mean()) if p.plastic is not None else 0.0
     # finite-difference spot check on the GPU forward run
     fd = []
     for s in np.random.default_rng(5).choice(p.n_struts, fd_struts, replace=False):
@@ -366,7 +373,10 @@ def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0):
             r = r0.copy()
             r[s] += sg * h
             S = L.Sections.circle(r, p.sections.young, 0.3, p.sections.density)
-            Js.append(objective(sim.run({f: getattr(S, f) for f in SECTION_FIELDS}, r))[0])
+            yw = None
+            if p.plastic is not None:
+                yw = L.Plastic.resultant_capacities(r, sigma_y, 0.01, 2.0).weights
+            Js.append(objective(sim.run({f: getattr(S, f) for f in SECTION_FIELDS}, r, yw))[0])
         fdv = (Js[0] - Js[1]) / (2 * h)
         fd.append({"strut": int(s), "fd": fdv, "adjoint": float(dJdr[s]),
                    "rel_err": abs(fdv - dJdr[s]) / abs(fdv)})
@@ -378,6 +388,10 @@ def adjoint_measure(n_cells=20, n=5000, every=100, fd_struts=0):
                       "steps": n, "checkpoint_every": every, "design_variables": p.n_struts},
            "t_forward_plain_s": t_plain, "t_forward_checkpointed_s": t_fwd,
            "t_reverse_s": t_bwd, "objective": J0, "fd_check": fd}
+    if p.plastic is not None:
+        res["config"]["yield"] = "quadratic resultant interaction (p=2)"
+        res["config"]["sigma_y"] = sigma_y
+        res["plastic_active_fraction_end"] = active
     return res
 
 
@@ -610,6 +624,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="adjoint gradient of the 20^3 impact: adjoint/forward time ratio")
     ap.add_argument("--adjoint-n", type=int, default=20)
+    ap.add_argument("--adjoint-sigma-y", type=float, default=0.0,
+                    help="with --adjoint: elastoplastic lattice of this yield stress")
     ap.add_argument("--adjoint-steps", type=int, default=5000)
     ap.add_argument("--checkpoint-every", type=int, default=100)
     ap.add_argument("--fd-struts", type=int, default=4)
@@ -792,6 +808,12 @@ def main():
         res["adjoint_forward_ratio"] = round(a["value"], 3)
         res["adjoint"] = {k: a[k] for k in ("config", "t_forward_plain_s",
                                             "t_forward_checkpointed_s", "t_reverse_s")}
+        # the same gradient through the elastoplastic material's return map
+        a = adjoint_measure(20, 5000, 100, 0, sigma_y=250e6)
+        res["adjoint_plastic_forward_ratio"] = round(a["value"], 3)
+        res["adjoint_plastic"] = {k: a[k] for k in ("config", "t_forward_plain_s",
+                                                    "t_forward_checkpointed_s", "t_reverse_s",
+                                                    "plastic_active_fraction_end")}
     print(json.dumps(res), flush=True)
 
 

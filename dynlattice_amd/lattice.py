@@ -385,9 +385,11 @@ def bcc_c3(n: int = 40, contact: bool = True) -> Problem:
     return p
 
 
-def bcc_c5(n: int = 20) -> Problem:
+def bcc_c5(n: int = 20, sigma_y=None) -> Problem:
     """Adjoint case: n^3 cells (n = 20: 256,000 elements), elastic with
-    contact, r = 0.5 mm (1 + 0.1 U[-1, 1]), seed 5."""
+    contact, r = 0.5 mm (1 + 0.1 U[-1, 1]), seed 5.  With sigma_y: the
+    elastoplastic variant (quadratic interaction of the resultant
+    capacities, kinematic hardening H = C/100)."""
     m, cell = 4, 10e-3
     R = 8 * n ** 3
     r = 0.5e-3 * (1.0 + 0.1 * (2.0 * uniform01(5, R) - 1.0))
@@ -396,6 +398,9 @@ def bcc_c5(n: int = 20) -> Problem:
     crush_bcs(p, -20.0)
     p.dt = stable_dt(p, 0.5)
     p.name = f"bcc-{n}x{n}x{n}-m4-elastic-contact"
+    if sigma_y is not None:
+        p.plastic = Plastic.resultant_capacities(r, sigma_y, kin_ratio=0.01, exponent=2.0)
+        p.name = f"bcc-{n}x{n}x{n}-m4-elastoplastic-contact"
     return p
 
 
