@@ -12,7 +12,12 @@ case describe:
   ``all``; GroupInputModule.cpp:167-330), plus the built-in group ``all``;
 * ``model.model``: the Lattice rod mesh with its ElasticRod material
   (cross sections circle | square | rectangle | explicit area and moments,
-  ElasticRodMaterial.cpp:65-190), Dirichlet (INCREMENT or under LoadScale;
+  ElasticRodMaterial.cpp:65-190) or ElastoPlasticRod material (the same
+  sections; yieldCond read into the device's power-sum family by
+  :mod:`.yieldfn`, isotropicCoefficient, a diagonal kinematicTensor,
+  maxIter, precision; ElastoPlasticRodMaterial.cpp:55-147), its RodContact
+  child ``contact`` (penaltySTS / penaltyNTS or penalty, radius;
+  RodContactModel.cpp:60-98), Dirichlet (INCREMENT or under LoadScale;
   the value is a prescribed acceleration in dynamics), Neumann (loadIncr /
   initLoad, or under LoadScale), LoadScale (scaleFunc of t), InitLoad
   (initial displacements and velocities), Mass (point masses), ImpactBC
@@ -173,10 +178,85 @@ def _sections(mat, R):
                       inertia_correct=P.number(mat.get("inertia_correct", 1.0)))
 
 
+def _plastic(mat, R):
+    """ElastoPlasticRod (ElastoPlasticRodMaterial.cpp:55-147): the yield
+    condition as the device's power-sum family, the hardening moduli
+    (isotropicCoefficient; kinematicTensor, 36 values row by row, diagonal
+    only), maxIter and precision (defaults 20 and 1e-5, :38-46)"""
+    from .yieldfn import YieldError, to_family
+    has_iso = "isotropicCoefficient" in mat
+    kin = mat.get("kinematicTensor")
+    has_kin = kin is not None
+    if "yieldCond" not in mat:
+        raise CaseError("Expected a yield function for an elasto-plastic material!")
+    if "yieldDeriv" in mat:
+        raise CaseError("yieldDeriv: the device differentiates the yield family analytically; "
+                        "give yieldCond only")
+    try:
+        fam = to_family(mat["yieldCond"], has_iso, has_kin)
+    except YieldError as e:
+        raise CaseError(str(e)) from None
+    kc = np.zeros(6)
+    if has_kin:
+        H = np.array([P.number(x) for x in (kin if isinstance(kin, list) else [kin])])
+        if H.size != 36:
+            raise CaseError("kinematicTensor: 36 values (6 x 6, row by row) expected")
+        H = H.reshape(6, 6)
+        if np.abs(H - np.diag(np.diag(H))).max() > 0.0:
+            raise CaseError("kinematicTensor: only a diagonal hardening tensor is supported")
+        kc = np.diag(H).copy()
+    return L.Plastic(weights=np.tile(fam["weights"], (R, 1)), yield_level=fam["yield_level"],
+                     kin_coupling=tuple(float(x) for x in fam["kin_coupling"]),
+                     iso_coupling=float(fam["iso_coupling"]), has_iso=has_iso,
+                     iso_modulus=P.number(mat["isotropicCoefficient"]) if has_iso else 0.0,
+                     has_kin=has_kin, kin_const=tuple(float(x) for x in kc), kin_ratio=0.0,
+                     precision=P.number(mat.get("precision", 1e-5)),
+                     max_iter=int(P.number(mat.get("maxIter", 20))),
+                     exponent=tuple(float(x) for x in fam["exponent"]))
+
+
+def _contact(spec, R):
+    """RodContact (RodContactModel.cpp:60-98): penaltySTS and penaltyNTS, or
+    penalty for both; the scalar radius (pairs close at twice it); every rod
+    of the lattice.  Its updateWhen only lets the output load case reuse the
+    last pairs; the device searches at every evaluation, i.e. the default
+    condition, and other conditions are refused."""
+    if spec.get("type") != "RodContact":
+        raise CaseError(f"contact model {spec.get('type')} is not supported")
+    if "penaltySTS" in spec and "penaltyNTS" in spec:
+        ks, kn = P.number(spec["penaltySTS"]), P.number(spec["penaltyNTS"])
+    elif "penalty" in spec:
+        ks = kn = P.number(spec["penalty"])
+    else:
+        raise CaseError("RodContact: penalty (or penaltySTS and penaltyNTS) required")
+    if "radius" not in spec:
+        raise CaseError("RodContact: radius required")
+    uw = spec.get("updateWhen", True)
+    if not (uw is True or (isinstance(uw, str) and uw.strip() in ("true", "1"))):
+        raise CaseError("RodContact updateWhen: only the default (every evaluation) is supported")
+    return L.Contact(radius=np.full(R, P.number(spec["radius"])), penalty_sts=ks,
+                     penalty_nts=kn)
+
+
 def _dof_index(names_t, names_r):
     idx = {n: i for i, n in enumerate(names_t)}
     idx.update({n: 3 + i for i, n in enumerate(names_r)})
     return idx
+
+
+def _model_root(props) -> str:
+    """path of the Multi model that holds the Lattice and the boundary
+    models: below ``model``, through Debug / Matrix wrappers (the transient
+    cases use model.model, the plastic and contact ones model.model.model)"""
+    path = "model"
+    for _ in range(8):
+        node = P.get(props, path, {}) or {}
+        if node.get("type") == "Multi":
+            return path
+        if node.get("type") not in ("Debug", "Matrix") or "model" not in node:
+            break
+        path += ".model"
+    raise CaseError("no Multi model below 'model' (through Debug / Matrix wrappers)")
 
 
 def _apply_models(props, p, dofidx, mdl_root="model.model"):
@@ -305,21 +385,26 @@ def load(pro_path: str, case_name: str | None = None) -> Case:
     pts, lines = G.read(geo_path, onelab)
     coords, en, es, _ = G.mesh(pts, lines, int(inp.get("order", 1)))
     # the rod model and its material
-    root = P.get(props, "model.model", {})
+    mroot = _model_root(props)
+    root = P.get(props, mroot, {})
     rod = next((root[m] for m in root.get("models", []) if root[m].get("type") == "Lattice"), None)
     if rod is None:
         raise CaseError("no Lattice rod model")
     child = rod["child"]
     mat = child["material"]
-    if mat.get("type") != "ElasticRod":
+    if mat.get("type") not in ("ElasticRod", "ElastoPlasticRod"):
         raise CaseError(f"material {mat.get('type')} is not supported by the case reader")
     R = len(lines)
     p = L.Problem(coords=coords, elem_nodes=en, elem_strut=es, n_struts=R,
                   sections=_sections(mat, R)).finalize()
+    if mat.get("type") == "ElastoPlasticRod":
+        p.plastic = _plastic(mat, R)
+    if "contact" in rod:
+        p.contact = _contact(rod["contact"], R)
     p.groups = _groups(props, coords, en)
     dofidx = _dof_index(child.get("dofNamesTrans", ["dx", "dy", "dz"]),
                         child.get("dofNamesRot", ["rx", "ry", "rz"]))
-    _apply_models(props, p, dofidx)
+    _apply_models(props, p, dofidx, mroot)
     integ = P.get(props, "Solver.integrator", {})
     kind = integ.get("type", "MilneDevice")
     if kind not in ("MilneDevice", "LeapFrog"):
@@ -416,9 +501,11 @@ def run(case: Case, out_dir: str = ".", precision: float | None = None, chunk: i
     the final time and the number of steps taken."""
     import time as _time
 
-    from .gpu import GpuModel, milne_opts
+    from .gpu import DL_TRACK_DISSIPATION, GpuModel, milne_opts
     p = case.problem
-    g = GpuModel(p, device=device)
+    # dissipatedEnergy (the material's energyDiss_ summed at COMMIT) needs the
+    # step's dissipation sums
+    g = GpuModel(p, device=device, flags=DL_TRACK_DISSIPATION if p.plastic is not None else 0)
     rows = {o.name: [] for o in case.outputs}
     dof_of = {n: k for k, n in enumerate(case.dof_names)}
     t_start = _time.perf_counter()

@@ -195,3 +195,108 @@ def test_oracle_elbow_energy_criterion():
     print("max rise", rise, "E_tot after unloading", after[0], after[-1])
     assert rise < 1e-5
     assert 0.0 < (after[0] - after[-1]) / after[0] < 0.05
+
+
+# ------------------------------------------------------- ElastoPlasticRod, RodContact
+# the yieldCond strings of the reference's plastic cases (data, quoted from
+# tests/plastic/test1.pro:22-28, test2a.pro:22, test2b.pro:22, test3.pro:22-28)
+YIELD_T1 = ("  abs(dx) + abs(dy) + abs(dz+0*h_dz) + abs(rx/0.12) + abs(ry/0.12) + abs(rz/0.12) "
+            "- 10 * (1+0*h_0)")
+YIELD_T2A = "abs(dz+0*h_dz) - 10 * (1+1*h_0)"
+YIELD_T2B = "abs(dz+20*h_dz) - 10 * (1+0*h_0)"
+YIELD_T3 = ("  abs(dx/( 700-h_dx))^2.04 + abs(dy/( 700-h_dy))^2.04 + abs(dz/(1470-h_dz))^1.76 "
+            "+ abs(rx/(0.62-h_rx))^2.09 + abs(ry/(0.62-h_ry))^2.09 + abs(rz/(0.56-h_rz))^1.73 - 1")
+
+
+def test_reference_yield_conditions_read_into_the_family():
+    from dynlattice_amd.yieldfn import YieldError, to_family
+    f = to_family(YIELD_T1, True, True)
+    assert np.allclose(f["weights"], [1, 1, 1, 1 / 0.12, 1 / 0.12, 1 / 0.12], rtol=1e-12)
+    assert np.array_equal(f["exponent"], np.ones(6))
+    assert abs(f["yield_level"] - 10.0) < 1e-12 and abs(f["iso_coupling"]) < 1e-12
+    assert np.abs(f["kin_coupling"]).max() < 1e-12
+    f = to_family(YIELD_T2A, True, True)
+    assert np.allclose(f["weights"], [0, 0, 1, 0, 0, 0]) and abs(f["iso_coupling"] - 1) < 1e-12
+    f = to_family(YIELD_T2B, True, True)
+    assert np.allclose(f["kin_coupling"], [0, 0, 20, 0, 0, 0], rtol=1e-12)
+    # hardening inside a denominator is no power-sum term: refused
+    with pytest.raises(YieldError):
+        to_family(YIELD_T3, False, True)
+    # general exponents, isotropic coupling of either sign
+    f = to_family("(dx/2)^2 + (dz/3)^2 + abs(rz+0.5*h_rz)^1.5 - 4*(1-0.2*h_0)", True, True)
+    assert np.allclose(f["weights"], [0.5, 0, 1 / 3, 0, 0, 1], rtol=1e-12)
+    assert np.allclose(f["exponent"], [2, 1, 2, 1, 1, 1.5], rtol=1e-12)
+    assert abs(f["iso_coupling"] + 0.2) < 1e-12 and abs(f["kin_coupling"][5] - 0.5) < 1e-12
+    with pytest.raises(YieldError):
+        to_family("dx^2 + dy - 1", False, False)  # a linear term without abs
+    with pytest.raises(YieldError):
+        to_family("abs(dx) - 1 + h_0", False, False)  # h_0 without isotropic hardening
+
+
+CRUSH = os.path.join(HERE, "golden", "crush", "crush.pro")
+
+
+def _crush_by_hand():
+    R = 64
+    r = np.full(R, 1.2e-3)
+    p = L.bcc_lattice(2, 3, 10e-3, r, 70e9, 0.3, 2700.0)
+    p.plastic = L.Plastic.resultant_capacities(r, 250e6, kin_ratio=0.01, exponent=2.0)
+    p.contact = L.Contact(radius=r.copy(), penalty_sts=1e6, penalty_nts=1e6)
+    L.crush_bcs(p, -20.0)
+    p.dt = 2.5e-7
+    return p
+
+
+def test_crush_case_reads_material_and_contact():
+    """tests/golden/crush/crush.pro (an explicit crush in the reference's
+    property layout, Debug / Matrix / Multi): ElastoPlasticRod with the
+    quadratic capacity interaction and a diagonal kinematicTensor, a
+    RodContact child, LeapFrog with the SO(3) update"""
+    c = CS.load(CRUSH)
+    p, h = c.problem, _crush_by_hand()
+    assert c.integrator == "LeapFrog" and p.so3 and p.dt == 2.5e-7
+    assert (p.n_elems, p.n_nodes, p.n_struts) == (h.n_elems, h.n_nodes, h.n_struts)
+    pl = p.plastic
+    assert np.allclose(pl.weights, h.plastic.weights, rtol=1e-12, atol=0)
+    assert pl.exponent == (2.0,) * 6 and pl.has_kin and not pl.has_iso
+    assert np.allclose(pl.kin_coupling, 1.0, rtol=1e-12) and abs(pl.yield_level - 1) < 1e-12
+    Cd = np.array([0.9 * p.sections.shear_modulus[0] * p.sections.area[0]] * 2 +
+                  [p.sections.young[0] * p.sections.area[0], p.sections.young[0] * p.sections.I1[0],
+                   p.sections.young[0] * p.sections.I2[0], p.sections.shear_modulus[0] * p.sections.J[0]])
+    assert np.allclose(pl.kin_const, Cd / 100.0, rtol=1e-12) and pl.kin_ratio == 0.0
+    assert np.array_equal(p.contact.radius, h.contact.radius)
+    assert p.contact.penalty_sts == p.contact.penalty_nts == 1e6
+    assert len(p.groups["bottom"]) == len(p.groups["top"]) == 9
+    assert (p.velo0[p.groups["top"], 2] == -20.0).all()
+
+
+def test_crush_case_runs_like_the_hand_built_problem_on_the_oracle():
+    """the case's problem and the hand-built one (other node numbering, same
+    physics) give the same bottom reaction and dissipation on the oracle"""
+    from oracle.ref_cpu import RefModel
+    c = CS.load(CRUSH)
+    outs = []
+    for p in (c.problem, _crush_by_hand()):
+        ref = RefModel(p, grid=True)
+        o = ref.leapfrog(300, group=p.groups["bottom"], group_dof=2)
+        outs.append((o["reaction"], ref.plastic_state()[3].mean()))
+    (ra, aa), (rb, ab) = outs
+    assert aa > 0.05 and aa == ab
+    assert np.abs(ra - rb).max() <= 1e-9 * np.abs(rb).max()
+
+
+def test_crush_case_refuses_what_the_device_cannot_do(tmp_path):
+    import shutil
+    for name in ("crush.pro", "crush.geo"):
+        shutil.copy(os.path.join(HERE, "golden", "crush", name), tmp_path / name)
+    base = (tmp_path / "crush.pro").read_text()
+    for extra, msg in (
+            ('model.model.model.lattice.child.material.yieldCond = "abs(dx/(1-h_dx))^2 - 1";',
+             "yield condition"),
+            ("model.model.model.lattice.child.material.kinematicTensor = [ 1., 1., 0., 0., 0., 0., "
+             "0., 1., 0., 0., 0., 0., 0., 0., 1., 0., 0., 0., 0., 0., 0., 1., 0., 0., 0., 0., 0., "
+             "0., 1., 0., 0., 0., 0., 0., 0., 1. ];", "diagonal"),
+            ('model.model.model.lattice.contact.updateWhen = "i%10 < 1";', "updateWhen")):
+        (tmp_path / "crush.pro").write_text(base + extra + "\n")
+        with pytest.raises(CS.CaseError, match=msg):
+            CS.load(str(tmp_path / "crush.pro"))

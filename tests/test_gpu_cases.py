@@ -145,3 +145,28 @@ def test_transient_test3_case_on_gpu(tmp_path):
                         tip["Y"].values)
     print("max energy rise", rise, "literature distance elbow", e_elbow, "tip", e_tip)
     assert rise < 1e-5
+
+
+def test_crush_case_on_gpu_matches_oracle(tmp_path):
+    """tests/golden/crush/crush.pro (ElastoPlasticRod with the quadratic
+    capacity interaction, RodContact, LeapFrog, SO(3)) run by case.run for
+    its 400 steps: the Sample rows' bottom.resp.dz (GroupOutput: output load
+    case, contact included) and dissipatedEnergy against the oracle on the
+    case's own problem"""
+    from dynlattice_amd import case as CS
+    from oracle.ref_cpu import RefModel
+    c = CS.load(os.path.join(HERE, "golden", "crush", "crush.pro"))
+    print(f"crush case: {c.problem.n_elems} elements", flush=True)
+    paths, t, i = CS.run(c, str(tmp_path))
+    assert i == 400
+    rows = np.loadtxt(paths["resp"], delimiter=",", ndmin=2)
+    assert rows.shape == (400, 5)
+    assert np.array_equal(rows[:, 0], np.arange(1, 401))
+    ref = RefModel(c.problem, grid=True)
+    o = ref.leapfrog(400, group=c.problem.groups["bottom"], group_dof=2, energy=True)
+    r = o["reaction"]
+    assert np.abs(rows[:, 2] - r).max() <= 1e-9 * np.abs(r).max()
+    ed = o["energies"][:, 1]  # the oracle's rows: potential, dissipated, kinetic
+    assert ed[-1] > 0
+    assert np.abs(rows[:, 4] - ed).max() <= 1e-8 * ed[-1]
+    assert ref.plastic_state()[3].mean() > 0.05
