@@ -29,6 +29,8 @@ case describe:
 * ``Solver.integrator``: MilneDevice or LeapFrog with deltaTime, precision,
   min/maxDtime, stepSaftey, increase/decreaseFactor, lengthScale; the
   SO(3) update when ``dofs_SO3`` is given (ExplicitModule.cpp:94-106);
+  updateWhen, the condition under which a step re-assembles the mass
+  (ExplicitModule.cpp:119-126, 256-266);
 * ``control.runWhile`` ("t <= T", "t < T", "i < N");
 * ``Output``: CSVOutput modules of the state vectors with saveWhen
   (CSVDataPrinter.cpp: header "time,label,<dof>[<node>]...", one row per
@@ -89,6 +91,7 @@ class Case:
     run_while: str
     outputs: list = field(default_factory=list)
     dof_names: tuple = DOF_NAMES
+    mass_update: str | None = None  # updateWhen of the integrator (None: every step)
 
 
 def _groups(props, coords, en):
@@ -440,7 +443,15 @@ def load(pro_path: str, case_name: str | None = None) -> Case:
             if label not in ("disp", "velo"):
                 raise CaseError(f"CSVOutput vector {v!r} is not supported")
             outs.append(Output(m, spec["file"], label, spec.get("saveWhen", "1")))
+    uw = integ.get("updateWhen")
+    if isinstance(uw, bool) or uw is None:
+        if uw is False:
+            raise CaseError("updateWhen = false: the mass would never be assembled")
+        uw = None
+    elif isinstance(uw, (int, float)):
+        uw = None if uw != 0 else "0"
     return Case(name=name, problem=p, integrator=kind, dt=dt, milne=milne,
+                mass_update=uw,
                 run_while=P.get(props, "control.runWhile", "t <= 0"), outputs=outs,
                 dof_names=tuple(list(child.get("dofNamesTrans", ["dx", "dy", "dz"])) +
                                 list(child.get("dofNamesRot", ["rx", "ry", "rz"]))))
@@ -506,6 +517,8 @@ def run(case: Case, out_dir: str = ".", precision: float | None = None, chunk: i
     # dissipatedEnergy (the material's energyDiss_ summed at COMMIT) needs the
     # step's dissipation sums
     g = GpuModel(p, device=device, flags=DL_TRACK_DISSIPATION if p.plastic is not None else 0)
+    if case.mass_update is not None:
+        g.set_mass_update(case.mass_update)
     rows = {o.name: [] for o in case.outputs}
     dof_of = {n: k for k, n in enumerate(case.dof_names)}
     t_start = _time.perf_counter()

@@ -62,6 +62,8 @@ static int adjoint_supported(dl_ctx *c) {
   if (!c->initialized) return fail(DL_ERR_STATE, "adjoint before INIT");
   if (!c->so3) return fail(DL_ERR_STATE, "the adjoint supports the SO(3) rotation update only");
   if (c->pd.nprog > 0) return fail(DL_ERR_STATE, "the adjoint does not support time programs");
+  if (c->d.Js)
+    return fail(DL_ERR_STATE, "the adjoint does not support an updateWhen mass schedule");
   if (c->imp.ngroups > 0) return fail(DL_ERR_STATE, "the adjoint does not support impact groups");
   if (c->n_mirrored > 0)
     return fail(DL_ERR_STATE, "the adjoint does not support -I strut frames (struts along -e3)");

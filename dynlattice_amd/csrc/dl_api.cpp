@@ -697,6 +697,8 @@ extern "C" int dl_set_state(dl_ctx *c, const double *disp, const double *velo, i
   if (e1 == hipSuccess && e2 == hipSuccess)
     launch_state_io(c->d, c->so3, false, disp ? tmp : nullptr, velo ? tmp + ndof : nullptr, c->stream);
   if (c->contact) contact_invalidate(c->cs, c->stream);
+  // updateWhen: the next step assembles the mass at the new state
+  if (c->mu.force) launch_set_i32(c->mu.force, 1, c->stream);
   hipError_t e3 = hipStreamSynchronize(c->stream);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
     return fail(DL_ERR_HIP, "dl_set_state copy failed");
@@ -1025,6 +1027,8 @@ extern "C" int dl_take_action(dl_ctx *c, int32_t action, dl_params *p) {
     // re-capture after every INIT
     c->gen++;
     launch_init_strain(c->d, c->stream);
+    // ExplicitModule::init ends with updateMass_: the next step assembles
+    if (c->mu.force) launch_set_i32(c->mu.force, 1, c->stream);
     // no output has run yet: ImpactBC dofs start free
     if (c->imp.ready) launch_set_i32(c->imp.ready, 0, c->stream);
     if (c->contact) contact_invalidate(c->cs, c->stream);
@@ -1149,6 +1153,8 @@ extern "C" int dl_phase_time(dl_ctx *c, const char *phase, double *ms, int64_t *
 // one explicit step: kernels only (capturable into a hipGraph)
 int launch_one_step(dl_ctx *c, const StepParams &sp, hipEvent_t *ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  // updateWhen: the mass of the step (ExplicitModule::advance)
+  launch_mass_update(c->d, c->mu, c->so3, c->stream);
   // boundary values at the end time of the step (after advance)
   launch_bc_program(c->pd, sp.dt, nullptr, c->stream);
   if (c->contact) {

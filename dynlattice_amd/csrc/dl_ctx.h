@@ -170,6 +170,8 @@ struct dl_ctx {
   // boundary programs, point masses (per node, added to mt and J0's diagonal)
   ProgState prog;
   ProgDev pd{};
+  // updateWhen of the explicit modules (dl_set_mass_update)
+  MassUpd mu{};
   ImpDev imp{};
   std::vector<double> pm_mass, pm_inertia;
 };

@@ -326,6 +326,21 @@ DL_HD void spatial_inertia(const double R[9], const double J0p[6], double J[9]) 
       J[3 * i + j] = T[3 * i] * R[3 * j] + T[3 * i + 1] * R[3 * j + 1] + T[3 * i + 2] * R[3 * j + 2];
 }
 
+// rotational inertia of a node: J0 turned by the node rotation q, or, when
+// the mass is only updated now and then (updateWhen), the spatial inertia
+// of the last update, given packed like J0 (Jsp)
+DL_HD void node_inertia(const Quat q, const double J0p[6], bool Jsp, double J[9]) {
+  if (Jsp) {
+    J[0] = J0p[0]; J[1] = J0p[3]; J[2] = J0p[4];
+    J[3] = J0p[3]; J[4] = J0p[1]; J[5] = J0p[5];
+    J[6] = J0p[4]; J[7] = J0p[5]; J[8] = J0p[2];
+  } else {
+    double R[9];
+    qmat(q, R);
+    spatial_inertia(R, J0p, J);
+  }
+}
+
 // solve J a = f for the free components (mask bit i set = constrained with
 // value c[i]) -- the reduced system of a Dirichlet-constrained block solve
 DL_HD void solve3_masked(const double J[9], const double f[3], unsigned mask, const double c[3],
@@ -375,10 +390,9 @@ DL_HD void solve3_masked(const double J[9], const double f[3], unsigned mask, co
 template <bool SO3>
 DL_HD void node_accel(double f[6], const Quat q, const double v[6], double mt,
                       const double J0[6], double gm, unsigned mask, const double ac[6],
-                      const double fe[6], double a[6]) {
-  double R[9], J[9];
-  qmat(q, R);
-  spatial_inertia(R, J0, J);
+                      const double fe[6], double a[6], bool Jsp = false) {
+  double J[9];
+  node_inertia(q, J0, Jsp, J);
   if (SO3) {
     // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
     const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
@@ -415,10 +429,9 @@ DL_HD void rot_advance(double r[4], const Quat q, const double w[3], double dt) 
 template <bool SO3>
 DL_HD void node_advance(double f[6], const Quat q, double u[3], double v[6], double r[4],
                         double mt, const double J0[6], double gm, unsigned mask,
-                        const double ac[6], const double fe[6], double dt) {
-  double R[9], J[9];
-  qmat(q, R);
-  spatial_inertia(R, J0, J);
+                        const double ac[6], const double fe[6], double dt, bool Jsp = false) {
+  double J[9];
+  node_inertia(q, J0, Jsp, J);
   if (SO3) {
     // assembleGyro_ (SpecialCosseratRodModel.cpp:951-967): w x (J w), once per strut holding the node
     const double jw0 = J[0] * v[3] + J[1] * v[4] + J[2] * v[5];
@@ -456,9 +469,9 @@ template <bool SO3>
 DL_HD void node_milne_pred(double f[6], const Quat q, const double u[3], const double v[6],
                            const double r[4], double mt, const double J0[6], double gm,
                            unsigned mask, const double ac[6], const double fe[6], double dt,
-                           double u_p[3], double v_p[6], double r_p[4]) {
+                           double u_p[3], double v_p[6], double r_p[4], bool Jsp = false) {
   double a[6];
-  node_accel<SO3>(f, q, v, mt, J0, gm, mask, ac, fe, a);
+  node_accel<SO3>(f, q, v, mt, J0, gm, mask, ac, fe, a, Jsp);
   for (int c = 0; c < 6; c++) v_p[c] = v[c] + dt * a[c];
   for (int c = 0; c < 3; c++) u_p[c] = u[c] + dt * v[c];
   for (int c = 0; c < 4; c++) r_p[c] = r[c];

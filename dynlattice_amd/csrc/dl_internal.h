@@ -24,6 +24,10 @@ struct DevData {
   const double *mt;  // translational lumped mass
   const double *ndt; // per-node time step (independent batched instances) or null
   const double *J0;  // 6: rotational inertia at theta = 0 (xx yy zz xy xz yz)
+  // updateWhen: the spatial inertia R J0 R^T of the last mass update, packed
+  // like J0, which the node updates use instead of turning J0 by the
+  // current rotation (null: the mass follows the rotation at every step)
+  const double *Js;
   const double *gm;  // gyro multiplicity (number of struts holding the node)
   const int32_t *bc; // -1 or record index
   // boundary records
@@ -169,6 +173,7 @@ struct ProgDev {
   int32_t nbc;
   double *clock_t;       // [1] time
   int64_t *clock_k;      // [1] steps taken
+  int32_t clock_on;      // the clock moves without programs too (updateWhen)
 };
 
 // ImpactBC groups (ImpactModel.cpp:74-104): the group's dof gets the
@@ -307,6 +312,18 @@ StepEnd make_step_end(const DevData &d, const int32_t *rgptr, int32_t ngroups, b
 // of this state): rbuf[k] = the dof share of entry k
 void launch_react(const DevData &d, bool so3, bool plastic, bool contact, hipStream_t s);
 void launch_set_i32(int32_t *p, int32_t v, hipStream_t s);
+// updateWhen (ExplicitModule.cpp:256-266): at the start of a step or
+// attempt, Js = R J0 R^T at the current rotations where the condition
+// program holds at the clock (t, k) or *force is set (then cleared)
+struct MassUpd {
+  const ProgOp *code;
+  int32_t nops;
+  const double *clock_t;
+  const int64_t *clock_k;
+  int32_t *force;
+  double *Js;
+};
+void launch_mass_update(const DevData &d, const MassUpd &mu, bool so3, hipStream_t s);
 // p[0 .. n) = 0 by a kernel (no memset node when a caller captures the stream)
 void launch_zero_f64(double *p, int64_t n, hipStream_t s);
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s);

@@ -68,8 +68,9 @@ __device__ __forceinline__ void load_node_data(const DevData &d, int32_t n, Node
 #pragma unroll
   for (int c = 0; c < (SO3 ? 4 : 3); c++) nd.rr[c] = d.rot[c * N + n];
   if (!SO3) nd.rr[3] = 0.0;
+  const double *J0 = d.Js ? d.Js : d.J0;
 #pragma unroll
-  for (int c = 0; c < 6; c++) nd.J0[c] = d.J0[c * N + n];
+  for (int c = 0; c < 6; c++) nd.J0[c] = J0[c * N + n];
   nd.mt = d.mt[n];
   nd.gm = SO3 ? d.gm[n] : 0.0;
 }
@@ -136,11 +137,12 @@ __device__ __forceinline__ void node_update(const DevData &d, const StepParams &
   }
   const double mt = pre ? pre->mt : d.mt[n];
   double J0[6];
+  const double *J0d = d.Js ? d.Js : d.J0;
 #pragma unroll
-  for (int c = 0; c < 6; c++) J0[c] = pre ? pre->J0[c] : d.J0[c * N + n];
+  for (int c = 0; c < 6; c++) J0[c] = pre ? pre->J0[c] : J0d[c * N + n];
   const double dt = d.ndt ? d.ndt[n] : sp.dt;
   node_advance<SO3>(f, q, u, v, rr, mt, J0, pre ? pre->gm : (SO3 ? d.gm[n] : 0.0), mask, ac, fe,
-                    dt);
+                    dt, d.Js != nullptr);
 #pragma unroll
   for (int c = 0; c < 6; c++) d.v[c * N + n] = v[c];
 #pragma unroll
@@ -713,11 +715,13 @@ __global__ __launch_bounds__(DL_BLOCK) void k_force_extras(DevData d, int gyro) 
     double r[4];
     Quat q;
     load_rot<SO3>(d, (int32_t)n, r, q);
-    double R[9], J[9], J0[6], w[3];
-    for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+    // the gyroscopic term with the mass in use (the matrix passed with
+    // GET_INT_VECTOR: the last update's when updateWhen is set)
+    double J[9], J0[6], w[3];
+    const double *J0d = d.Js ? d.Js : d.J0;
+    for (int c = 0; c < 6; c++) J0[c] = J0d[c * N + n];
     for (int c = 0; c < 3; c++) w[c] = d.v[(3 + c) * N + n];
-    qmat(q, R);
-    spatial_inertia(R, J0, J);
+    node_inertia(q, J0, d.Js != nullptr, J);
     const double jw0 = J[0] * w[0] + J[1] * w[1] + J[2] * w[2];
     const double jw1 = J[3] * w[0] + J[4] * w[1] + J[5] * w[2];
     const double jw2 = J[6] * w[0] + J[7] * w[1] + J[8] * w[2];
@@ -927,6 +931,25 @@ __global__ __launch_bounds__(DL_BLOCK) void k_commit(DevData d, double *epot, do
 }
 
 // ------------------------------------------------------------------
+// updateWhen: the mass update at the start of a step / attempt
+template <bool SO3>
+__global__ __launch_bounds__(DL_BLOCK) void k_mass_update(DevData d, MassUpd mu) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= d.N) return;
+  const bool upd = *mu.force != 0 ||
+                   prog_eval_expr(mu.code, mu.nops, *mu.clock_t, (double)*mu.clock_k) != 0.0;
+  if (!upd) return;
+  const int64_t N = d.N;
+  double r[4], R[9], J[9], J0[6];
+  Quat q;
+  load_rot<SO3>(d, (int32_t)n, r, q);
+  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  qmat(q, R);
+  spatial_inertia(R, J0, J);
+  const double pk[6] = {J[0], J[4], J[8], J[1], J[2], J[5]};
+  for (int c = 0; c < 6; c++) mu.Js[c * N + n] = pk[c];
+}
+
 // GET_MATRIX2: 6x6 block per node, row-major [N][36]
 // ------------------------------------------------------------------
 template <bool SO3>
@@ -1157,8 +1180,15 @@ void launch_impact_sum(const ImpDev &I, const DevData &d, hipStream_t s) {
 }
 
 void launch_clock_advance(const ProgDev &P, double dt, hipStream_t s) {
-  if (P.nprog == 0) return;
+  if (P.nprog == 0 && !P.clock_on) return;
   hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(1), 0, s, P.clock_t, P.clock_k, dt);
+}
+
+void launch_mass_update(const DevData &d, const MassUpd &mu, bool so3, hipStream_t s) {
+  if (!mu.Js) return;
+  if (so3) hipLaunchKernelGGL(k_mass_update<true>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, mu);
+  else hipLaunchKernelGGL(k_mass_update<false>, dim3(nblk(d.N)), dim3(DL_BLOCK), 0, s, d, mu);
+  hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, mu.force, 0);
 }
 
 void launch_commit_pl(const DevData &d, bool plastic, double *epot, double *ediss, hipStream_t s) {

@@ -73,8 +73,10 @@ static int launch_attempt(dl_ctx *c, const MilneCtl &mc, hipEvent_t *ev) {
     m.nblk = (int32_t)((d.N + 255) / 256);
   }
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
+  // updateWhen: the mass of the attempt, at its start state
+  launch_mass_update(d, c->mu, c->so3, s);
   // boundary values at the end time of the attempt; the clock moves on accept
-  if (c->pd.nprog > 0) {
+  if (c->pd.nprog > 0 || c->pd.clock_on) {
     launch_bc_program(c->pd, 0.0, m.dt, s);
     m.clock_t = c->pd.clock_t;
     m.clock_k = c->pd.clock_k;

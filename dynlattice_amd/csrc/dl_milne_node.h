@@ -83,11 +83,12 @@ __device__ __forceinline__ void milne_pred_node(const DevData &d, const MilneDev
   Quat q;
   load_node<SO3>(m.u0, m.r0, m.v0, N, n, u, r, v, q);
   double J0[6];
+  const double *J0d = d.Js ? d.Js : d.J0;
 #pragma unroll
-  for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
+  for (int c = 0; c < 6; c++) J0[c] = J0d[c * N + n];
   double up[3], rp[4], vp[6];
   node_milne_pred<SO3>(f, q, u, v, r, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, *m.dt,
-                       up, vp, rp);
+                       up, vp, rp, d.Js != nullptr);
   store_node<SO3>(m.u1, m.r1, m.v1, N, n, up, rp, vp);
   list_trigger(d, n, up);
 }
@@ -123,9 +124,10 @@ __device__ __forceinline__ void milne_corr_node(const DevData &d, const MilneDev
   for (int c = 0; c < 6; c++) vp[c] = m.v1[c * N + n];
   {
     double J0[6];
+    const double *J0d = d.Js ? d.Js : d.J0;
 #pragma unroll
-    for (int c = 0; c < 6; c++) J0[c] = d.J0[c * N + n];
-    node_accel<SO3>(f, q, vp, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, a);
+    for (int c = 0; c < 6; c++) J0[c] = J0d[c * N + n];
+    node_accel<SO3>(f, q, vp, d.mt[n], J0, SO3 ? d.gm[n] : 0.0, mask, ac, fe, a, d.Js != nullptr);
   }
   // velocities: v_new = v + dt a_pre
   double dvel[6];

@@ -337,6 +337,47 @@ extern "C" int dl_set_programs(dl_ctx *c, const dl_program *progs, int32_t n_pro
   return prog_entries(c);
 }
 
+extern "C" int dl_set_mass_update(dl_ctx *c, const char *cond) {
+  if (!c) return fail(DL_ERR_INPUT, "NULL context");
+  hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the inertia pointer and the condition live in captured step graphs
+  c->gen++;
+  c->adj.valid = false;
+  MassUpd &M = c->mu;
+  for (void *q : {(void *)M.code, (void *)M.force, (void *)M.Js}) dfree(c, q);
+  M = MassUpd{};
+  c->d.Js = nullptr;
+  c->pd.clock_on = 0;
+  if (!cond || !*cond) return DL_OK;
+  std::vector<ProgOp> code;
+  std::string err;
+  if (!prog_compile(cond, code, err)) return fail(DL_ERR_INPUT, "updateWhen: " + err);
+  ProgDev &P = c->pd;
+  int rc;
+  if (!P.clock_t) {
+    if ((rc = dalloc(c, &P.clock_t, 1))) return rc;
+    if ((rc = dalloc(c, &P.clock_k, 1))) return rc;
+    HIPCHK(hipMemset(P.clock_t, 0, sizeof(double)));
+    HIPCHK(hipMemset(P.clock_k, 0, sizeof(int64_t)));
+  }
+  ProgOp *po;
+  if ((rc = upload(c, &po, code))) return rc;
+  M.code = po;
+  M.nops = (int32_t)code.size();
+  int32_t *f;
+  if ((rc = upload(c, &f, std::vector<int32_t>{1}))) return rc; // first step: assemble
+  M.force = f;
+  double *js;
+  if ((rc = dalloc(c, &js, 6 * c->d.N))) return rc;
+  M.Js = js;
+  M.clock_t = P.clock_t;
+  M.clock_k = P.clock_k;
+  c->d.Js = js;
+  P.clock_on = 1;
+  return DL_OK;
+}
+
 extern "C" int dl_set_time(dl_ctx *c, double t, int64_t step) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (!std::isfinite(t) || step < 0) return fail(DL_ERR_INPUT, "finite t and step >= 0 required");

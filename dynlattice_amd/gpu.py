@@ -39,6 +39,7 @@ EXPORTS = (
     "dl_set_programs", "dl_set_time", "dl_get_time", "dl_program_eval", "dl_set_point_masses",
     "dl_get_table", "dl_get_energies", "dl_set_impact_groups", "dl_explicit_step_noalloc",
     "dl_contact_list_stats", "dl_set_plastic_state", "dl_set_yield_weights",
+    "dl_set_mass_update",
 )
 #: DL_NOALLOC_SCRATCH_DOUBLES of include/dl_gpu.h
 NOALLOC_SCRATCH_DOUBLES = 1 << 18
@@ -168,6 +169,7 @@ def load():
     L.dl_contact_list_stats.argtypes = [C.c_void_p, ip64, ip64]
     L.dl_set_plastic_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
     L.dl_set_yield_weights.argtypes = [C.c_void_p, dp]
+    L.dl_set_mass_update.argtypes = [C.c_void_p, C.c_char_p]
     L.dl_get_history.argtypes = [C.c_void_p, dp, dp, ip64, C.c_int64, ip64]
     L.dl_contact_pairs.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
     L.dl_contact_candidates.argtypes = [C.c_void_p, ip32, ip32, C.c_int64, ip64]
@@ -528,6 +530,12 @@ class GpuModel:
         cr = _c(contact_radius) if contact_radius is not None else None
         self._check(self.L.dl_set_sections(self.h, C.byref(sec) if sec is not None else None,
                                            _p(cr)))
+
+    def set_mass_update(self, cond):
+        """updateWhen of the explicit modules: the mass is re-assembled at the
+        start of the steps (attempts) where cond(t, i) holds, and after INIT
+        or set_state (None: every step)"""
+        self._check(self.L.dl_set_mass_update(self.h, None if cond is None else cond.encode()))
 
     def set_yield_weights(self, weights):
         """Replace the elastoplastic material's yield weights ([n_struts, 6])."""

@@ -426,6 +426,18 @@ int dl_set_programs(dl_ctx *ctx, const dl_program *progs, int32_t n_progs, const
                     int64_t ndof);
 /* The programs' clock: time t and steps taken. */
 int dl_set_time(dl_ctx *ctx, double t, int64_t step);
+
+/* updateWhen of the explicit solver modules (ExplicitModule.cpp:119-126,
+ * 256-266): the block-lumped mass (and with it the rotational inertia of the
+ * solve and of the gyroscopic term) is re-assembled at the current rotations
+ * only at the start of the steps (MilneDevice: attempts) where cond(t, i)
+ * holds on the context's clock (dl_set_time; it moves by every step, and by
+ * every accepted attempt), and at the first step after INIT or
+ * dl_set_state; in between the last assembled mass is used.  cond is a
+ * program expression of t and i (dl_set_programs' syntax); NULL or "" gives
+ * the default, an update at every step.  GET_MATRIX2 still assembles at the
+ * current state (the reference's action does). */
+int dl_set_mass_update(dl_ctx *ctx, const char *cond);
 int dl_get_time(dl_ctx *ctx, double *t, int64_t *step);
 /* Evaluates a program expression on the host with the compiler and
  * evaluator the device uses (to check an expression before a run). */

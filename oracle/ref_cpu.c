@@ -253,6 +253,12 @@ struct ref_model {
   double *cval0, *fext0;
   double clock_t;
   int64_t clock_k;
+  /* updateWhen of the explicit modules (ExplicitModule.cpp:119-126,
+   * 256-266): the mass is assembled at the start of a step only when the
+   * condition holds or no valid mass exists (after INIT) */
+  char *mu_cond;      /* NULL: default, every step */
+  int mu_valid;
+  double *mu_blocks;  /* [36N] the mass in use */
   /* point masses (MassModel) */
   double *pm_mass, *pm_inertia;
   /* ImpactBC groups */
@@ -426,6 +432,7 @@ void ref_destroy(ref_model *m) {
   free(m->curr_ep); free(m->old_ep); free(m->curr_al); free(m->old_al);
   free(m->dflow); free(m->ediss); free(m->ehard); free(m->active);
   free(m->crad); free(m->black); free(m->actA); free(m->actB);
+  free(m->mu_cond); free(m->mu_blocks);
   free(m->cmask); free(m->cval); free(m->fext);
   for (int q = 0; q < m->n_prog; q++) free(m->prog[q].expr);
   free(m->dof_prog); free(m->cval0); free(m->fext0);
@@ -687,6 +694,43 @@ static int prog_apply(ref_model *m, double t_end, double *scale) {
     if (m->cmask[i]) m->cval[i] = m->cval0[i] * scale[q];
     else m->fext[i] = m->fext0[i] * scale[q];
   }
+  return REF_OK;
+}
+
+/* ExplicitModule::advance (ExplicitModule.cpp:256-266): valid_ = valid_ &&
+ * !updCond(t, i); the mass is re-assembled (updateMass_) when not valid, at
+ * the step's start state; the clock is the one before the step's time
+ * advance.  Without a condition the mass is assembled every step. */
+int ref_set_mass_update(ref_model *m, const char *cond) {
+  free(m->mu_cond);
+  m->mu_cond = NULL;
+  m->mu_valid = 0;
+  if (!cond) return REF_OK;
+  double v;
+  if (ref_expr_eval(cond, 0., 0., &v)) return REF_ERR_INPUT;
+  m->mu_cond = (char *)malloc(strlen(cond) + 1);
+  strcpy(m->mu_cond, cond);
+  if (!m->mu_blocks) m->mu_blocks = (double *)malloc(sizeof(double) * 36 * m->N);
+  return REF_OK;
+}
+
+static int mass_in_use(ref_model *m, const double *disp, double *blocks) {
+  if (!m->mu_cond) return ref_mass_blocks(m, disp, blocks, !m->so3);
+  double c = 1.;
+  if (m->mu_valid && ref_expr_eval(m->mu_cond, m->clock_t, (double)m->clock_k, &c))
+    return REF_ERR_INPUT;
+  if (!m->mu_valid || c != 0.) {
+    ref_mass_blocks(m, disp, m->mu_blocks, !m->so3);
+    m->mu_valid = 1;
+  }
+  memcpy(blocks, m->mu_blocks, sizeof(double) * 36 * m->N);
+  return REF_OK;
+}
+
+/* the mass of the last update ([36N] blocks), for the tests */
+int ref_mass_in_use(ref_model *m, double *blocks) {
+  if (!m->mu_cond || !m->mu_valid) return REF_ERR_INPUT;
+  memcpy(blocks, m->mu_blocks, sizeof(double) * 36 * m->N);
   return REF_OK;
 }
 
@@ -1495,6 +1539,7 @@ int64_t ref_blacklist(ref_model *m, int32_t *a, int32_t *b, int64_t cap) {
 void ref_set_grid(ref_model *m, int use_grid) { m->use_grid = use_grid; }
 
 int ref_init(ref_model *m, const double *disp) {
+  m->mu_valid = 0; /* ExplicitModule::init ends with updateMass_ */
   for (int32_t s = 0; s < m->R; s++) init_rotation(m, &m->st[s]);
   /* initStrain_: strains at null displacement, Lambda = LambdaN */
   for (int64_t e = 0; e < m->E; e++) {
@@ -1941,7 +1986,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
     /* advance: boundary values at the new time; updCond default true ->
      * updateMass_ at the current state */
     if (m->n_prog && (rc = prog_apply(m, m->clock_t + dt, scale))) break;
-    ref_mass_blocks(m, disp, blocks, !m->so3);
+    if ((rc = mass_in_use(m, disp, blocks))) break;
     if (!m->so3) {
       for (int64_t n = 0; n < N; n++)
         for (int i = 0; i < 6; i++)
@@ -2009,7 +2054,7 @@ int ref_leapfrog(ref_model *m, double *disp, double *velo, double dt, int64_t ns
     } else {
       apply_deform(m);
     }
-    if (m->n_prog) prog_commit(m, dt, scale);
+    if (m->n_prog || m->mu_cond) prog_commit(m, dt, scale);
     m->imp_ready = 1;
     /* output modules after the solver's commit */
     if (react) output_reaction(m, disp, group, group_n, group_dof, react + st);
@@ -2155,7 +2200,7 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
   for (int64_t k = 0; k < n_attempts && !rc; k++) {
     /* advance: boundary values at the attempt's end time */
     if (m->n_prog && (rc = prog_apply(m, m->clock_t + dt, scale))) break;
-    ref_mass_blocks(m, disp, blocks, !m->so3);
+    if ((rc = mass_in_use(m, disp, blocks))) break;
     int64_t nc = 0;
     if ((rc = milne_forces(m, disp, velo, blocks, fint, fg, &nc))) break;
     if (m->imp_n) impact_constraints(m, fint);
@@ -2190,7 +2235,7 @@ int ref_milne(ref_model *m, double *disp, double *velo, const ref_milne_opts *o,
       apply_deform(m);
       memcpy(disp, u_new, sizeof(double) * nd);
       memcpy(velo, v_new, sizeof(double) * nd);
-      if (m->n_prog) prog_commit(m, dt, scale);
+      if (m->n_prog || m->mu_cond) prog_commit(m, dt, scale);
       m->imp_ready = 1;
     } else {
       double t = o->safety * dt_opt;

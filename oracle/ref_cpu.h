@@ -160,6 +160,9 @@ typedef struct {
 int ref_set_programs(ref_model *m, int n, const int32_t *kind, const double *abc,
                      const char *const *exprs, const int32_t *dof_prog);
 int ref_set_time(ref_model *m, double t, int64_t k);
+/* updateWhen of the explicit modules: cond(t, i) or NULL (every step) */
+int ref_set_mass_update(ref_model *m, const char *cond);
+int ref_mass_in_use(ref_model *m, double *blocks);
 int ref_get_time(const ref_model *m, double *t, int64_t *k);
 int ref_expr_eval(const char *expr, double t, double i, double *value);
 /* GET_TABLE at disp: kind 0 strain, 1 mat_strain, 2 stress, 3 mat_stress

@@ -107,6 +107,8 @@ def lib(omp=False, path=None):
                             C.c_int64, C.c_int32, dp]
     L.ref_set_programs.argtypes = [C.c_void_p, C.c_int, ip32, dp, C.POINTER(C.c_char_p), ip32]
     L.ref_set_time.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+    L.ref_set_mass_update.argtypes = [C.c_void_p, C.c_char_p]
+    L.ref_mass_in_use.argtypes = [C.c_void_p, dp]
     L.ref_get_time.argtypes = [C.c_void_p, dp, ip64]
     L.ref_expr_eval.argtypes = [C.c_char_p, C.c_double, C.c_double, dp]
     L.ref_set_point_masses.argtypes = [C.c_void_p, dp, dp]
@@ -309,6 +311,17 @@ class RefModel:
 
     def set_time(self, t, k=0):
         self._check(self.L.ref_set_time(self.h, float(t), int(k)))
+
+    def set_mass_update(self, cond):
+        """updateWhen of the explicit modules: the mass is re-assembled at the
+        start of the steps where cond(t, i) holds (None: every step)"""
+        self._check(self.L.ref_set_mass_update(self.h, None if cond is None else cond.encode()))
+
+    def mass_in_use(self):
+        """[N, 6, 6] blocks of the last mass update"""
+        b = np.zeros(36 * self.p.n_nodes)
+        self._check(self.L.ref_mass_in_use(self.h, _p(b)))
+        return b.reshape(-1, 6, 6)
 
     def get_time(self):
         t = C.c_double()
