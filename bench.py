@@ -194,7 +194,8 @@ def sweep(args):
     """Configuration C4: the 256 seeded elastoplastic BCC lattices (per-instance
     dt, each crushed to 20 % nominal strain in its 10,000 steps) as one
     batched context whose steps replay captured hipGraphs, with the schedule
-    (instance order x graph length) chosen by batch.autotune; the whole
+    (instance order x graph length, then the assembly layout) chosen by
+    batch.autotune; the whole
     10,000-step sweep is timed from the initial state.  Against it: the same
     instances one after another in their own contexts (a sample of steps
     each), and the oracle on the median instance (all cores, one core)."""
@@ -207,10 +208,10 @@ def sweep(args):
     flags = DL_TRACK_DISSIPATION
     E = sum(p.n_elems for p in probs)
     print(f"sweep: {n_inst} instances, {E} elements", file=sys.stderr, flush=True)
-    order, K, tune = BT.autotune(probs, steps=args.sweep_tune_steps, flags=flags, log=sys.stderr)
-    perm = BT.instance_order(probs, order)
-    bt = BT.concat([probs[i] for i in perm])
-    g = BT.make_model(bt, flags=flags, graph_steps=K)
+    order, K, layout, tune = BT.autotune(probs, steps=args.sweep_tune_steps, flags=flags,
+                                         log=sys.stderr)
+    bt = BT.ordered_batch(probs, order)
+    g = BT.make_model(bt, flags=flags | BT.LAYOUTS[layout], graph_steps=K)
     t0 = time.perf_counter()
     g.step(nsteps, out=False)
     g.sync()
@@ -266,9 +267,9 @@ def sweep(args):
                       "dt_range": [float(bt.dts.min()), float(bt.dts.max())],
                       "steps_per_instance": nsteps, "nominal_strain": L.C4_STRAIN,
                       "yield": yield_family(bt.problem)},
-           "schedule": {"instance_order": order, "graph_steps": K,
-                        "autotune_ms_per_step": {f"{o}/{k}": round(1e3 * t, 5)
-                                                 for (o, k), t in tune.items()}},
+           "schedule": {"instance_order": order, "graph_steps": K, "layout": layout,
+                        "autotune_ms_per_step": {f"{o}/{lay}/{k}": round(1e3 * t, 5)
+                                                 for (o, lay, k), t in tune.items()}},
            "wall_s": t_batch,
            "phase_ms": {k: round(v, 5) for k, v in phases.items()},
            "roofline": {"bound": "hbm", "kernel": "k_strut", "achieved": round(a_strut, 1),

@@ -10,8 +10,13 @@ whose mesh is the disjoint union of all instances:
 * instances are translated apart along x, so that even with contact enabled
   no two instances can touch;
 * the step sequence is replayed from captured hipGraphs (dl_set_graph); the
-  graph length and the order of the instances in the union are chosen by
-  :func:`autotune` on the real workload.
+  graph length, the order of the instances in the union and the assembly
+  layout are chosen by :func:`autotune` on the real workload.
+
+A persistent kernel looping over the steps with grid-wide barriers between
+the element and joint phases is not used: a cooperative grid barrier costs
+7.5 us with 64 resident blocks, 25 us with 256 and 78 us with 768 on an
+MI355X, against about 1.5 us per kernel node of a replayed graph.
 
 Every instance computes exactly what it would compute alone: the kernels
 never mix data of two instances (nodes, elements and struts are disjoint).
@@ -39,9 +44,23 @@ class Batch:
     dts: np.ndarray          # [B]
 
 
-def concat(problems: List[L.Problem], gap: float = 0.1, group: str = "bottom") -> Batch:
+def x_positions(problems: List[L.Problem], gap: float = 0.1) -> np.ndarray:
+    """x at which each instance starts when placed side by side in the given
+    order, gap apart"""
+    x, out = 0.0, []
+    for p in problems:
+        out.append(x)
+        x += p.coords[:, 0].max() - p.coords[:, 0].min() + gap
+    return np.array(out)
+
+
+def concat(problems: List[L.Problem], gap: float = 0.1, group: str = "bottom",
+           x0s=None) -> Batch:
     """Disjoint union of problems (all elastic or all elastoplastic with the
-    same global yield constants; per-strut data may differ)."""
+    same global yield constants; per-strut data may differ).  x0s: where each
+    instance starts along x (default: side by side in this order); giving
+    every instance the same place whatever the order of the union keeps its
+    coordinates, hence its results, the same bits."""
     B = len(problems)
     nn = np.array([p.n_nodes for p in problems])
     ne = np.array([p.n_elems for p in problems])
@@ -50,11 +69,10 @@ def concat(problems: List[L.Problem], gap: float = 0.1, group: str = "bottom") -
     elem_off = np.r_[0, np.cumsum(ne)]
     strut_off = np.r_[0, np.cumsum(ns)]
     coords, en, es = [], [], []
-    x0 = 0.0
+    xs = x_positions(problems, gap) if x0s is None else np.asarray(x0s, np.float64)
     for b, p in enumerate(problems):
         c = p.coords.copy()
-        c[:, 0] += x0 - c[:, 0].min()
-        x0 = c[:, 0].max() + gap
+        c[:, 0] += xs[b] - c[:, 0].min()
         coords.append(c)
         en.append(p.elem_nodes + node_off[b])
         es.append(p.elem_strut + strut_off[b])
@@ -127,6 +145,14 @@ def autotune_graph(model, candidates=(0, 8, 32, 128), steps=256, dt=None):
 ORDERS = ("given", "size-descending", "chain-length")
 
 
+def ordered_batch(problems: List[L.Problem], order: str) -> Batch:
+    """the union in the given order; every instance at its place of the
+    given order (so its results do not depend on the order)"""
+    perm = instance_order(problems, order)
+    xs = x_positions(problems)
+    return concat([problems[i] for i in perm], x0s=xs[perm])
+
+
 def instance_order(problems: List[L.Problem], order: str) -> np.ndarray:
     n = len(problems)
     if order == "given":
@@ -140,35 +166,56 @@ def instance_order(problems: List[L.Problem], order: str) -> np.ndarray:
     raise ValueError(f"unknown order {order}")
 
 
+# assembly layouts that autotune tries (dl_options.flags; every one gives the
+# same bits): joints whose chains all sit in one wavefront summed in LDS
+# inside the strut kernel, or every shared node through the slot buffer and
+# the joint kernel; inner elements' end-b frames from the next lane, or both
+# end frames read per element
+LAYOUTS = {"wavefront-joints": 0, "joints-in-memory": 4, "frames-per-element": 2}
+
+
+def _time_graphs(g, graph_lengths, steps, label, times, log):
+    for k in graph_lengths:
+        g.set_graph(k)
+        g.step(max(steps, k), out=False)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(max(steps, k), out=False)
+        g.sync()
+        times[label + (k,)] = (time.perf_counter() - t0) / max(steps, k)
+        if log:
+            print(f"autotune: {' '.join(str(x) for x in label)}, graph {k}: "
+                  f"{1e3 * times[label + (k,)]:.4f} ms/step", file=log, flush=True)
+
+
 def autotune(problems: List[L.Problem], orders=ORDERS, graph_lengths=(0, 10, 100), steps=100,
-             flags=0, log=None):
-    """Schedule of a batched sweep, chosen by measurement: for every order of
-    the instances in the union (a different mesh numbering, hence lane
-    packing, memory layout and block -> instance mapping) and every
-    captured-graph length, ``steps`` steps after one warm-up pass of the same
-    length (which also captures the graph).  The instances advance during
-    tuning, from their initial state for every order.  Returns
-    (order, graph length, {(order, length): seconds per step})."""
+             flags=0, log=None, layouts=tuple(LAYOUTS)):
+    """Schedule of a batched sweep, chosen by measurement in two stages.
+    (1) For every order of the instances in the union (a different mesh
+    numbering, hence lane packing, memory layout and block -> instance
+    mapping) and every captured-graph length, ``steps`` steps after one
+    warm-up pass of the same length (which also captures the graph).
+    (2) At the best order, every assembly layout of :data:`LAYOUTS` with
+    every graph length.  The instances advance during tuning, from their
+    initial state for every context.  Returns (order, graph length, layout,
+    {(order, layout, length): seconds per step})."""
     times = {}
+    base = layouts[0]
     for order in orders:
-        perm = instance_order(problems, order)
-        bt = concat([problems[i] for i in perm])
-        g = make_model(bt, flags=flags)
-        for k in graph_lengths:
-            g.set_graph(k)
-            g.step(max(steps, k), out=False)
-            g.sync()
-            t0 = time.perf_counter()
-            g.step(max(steps, k), out=False)
-            g.sync()
-            times[(order, k)] = (time.perf_counter() - t0) / max(steps, k)
-            if log:
-                print(f"autotune: order {order}, graph {k}: "
-                      f"{1e3 * times[(order, k)]:.4f} ms/step", file=log, flush=True)
+        bt = ordered_batch(problems, order)
+        g = make_model(bt, flags=flags | LAYOUTS[base])
+        _time_graphs(g, graph_lengths, steps, (order, base), times, log)
         g.sync()  # raises a pending device error
         g.close()
+    best_order = min(times, key=times.get)[0]
+    bt = ordered_batch(problems, best_order)
+    for lay in layouts[1:]:
+        g = make_model(bt, flags=flags | LAYOUTS[lay])
+        _time_graphs(g, graph_lengths, steps, (best_order, lay), times, log)
+        g.sync()
+        g.close()
     best = min(times, key=times.get)
-    return best[0], best[1], times
+    return best[0], best[2], best[1], times
 
 
 def c4_batch(n_instances=256):

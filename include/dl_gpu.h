@@ -58,11 +58,13 @@ extern "C" {
 #define DL_FRAMES_PER_ELEMENT 2u /* the element kernel reads both end frames of
                                   * every element instead of taking an inner
                                   * element's end-b frame from its neighbour
-                                  * (same values; for testing) */
+                                  * (same values; a layout the sweep
+                                  * autotuner tries) */
 #define DL_JOINTS_IN_MEMORY 4u   /* every shared node goes through the slot
                                   * buffer and the joint kernel, also those
                                   * whose chains all sit in one wavefront
-                                  * (same bits; for testing) */
+                                  * (same bits; a layout the sweep autotuner
+                                  * tries, faster for some instance mixes) */
 
 typedef struct dl_ctx dl_ctx;
 

@@ -148,3 +148,32 @@ def test_c4_sweep_of_256_instances():
               "reaction", rel_inf(h[:, b], o["reaction"]))
         assert rel_inf(d[lo:hi], r.disp.reshape(-1, 6)) <= 1e-8
         assert rel_inf(h[:, b], o["reaction"]) <= 1e-9
+
+
+def test_autotune_schedules_give_the_same_bits():
+    """every schedule the sweep autotuner can choose (instance order, graph
+    length, assembly layout) computes the same per-instance results: the
+    chosen one only changes the speed"""
+    ps = _instances()
+    order, K, layout, times = BT.autotune(ps, graph_lengths=(0, 7), steps=20, flags=1)
+    assert len(times) == len(BT.ORDERS) * 2 + (len(BT.LAYOUTS) - 1) * 2
+    assert layout in BT.LAYOUTS and order in BT.ORDERS
+    ref = None
+    for order in BT.ORDERS:
+        perm = BT.instance_order(ps, order)
+        inv = np.argsort(perm)
+        for lay, fl in BT.LAYOUTS.items():
+            for k in (0, 7):
+                bt = BT.ordered_batch(ps, order)
+                g = BT.make_model(bt, flags=1 | fl, graph_steps=k)
+                g.step(60, out=False)
+                d, v = g.get_state()
+                h = g.group_history(60)
+                g.close()
+                # per instance, in the given order
+                per = [(d[bt.node_off[b]:bt.node_off[b + 1]], h[:, b]) for b in inv]
+                if ref is None:
+                    ref = per
+                else:
+                    for (da, ha), (db, hb) in zip(per, ref):
+                        assert np.array_equal(da, db) and np.array_equal(ha, hb), (order, lay, k)
