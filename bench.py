@@ -86,25 +86,16 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(p, budget_s, steps_gpu, omp, model=None):
+def cpu_baseline(p, budget_s, steps_gpu, omp, snap=None):
     """The oracle (C restatement, oracle/ref_cpu.c) on this host, bounded
     sample: the same lattice, K steps with K chosen so the timed part takes
     about budget_s.  omp: the all-cores build (struts and pairs threaded
-    with OpenMP) on cpu_threads() threads; else the serial build.  model: a
-    GpuModel whose current state (displacements, velocities, plastic state)
-    the oracle starts from, i.e. the state the GPU steps were timed at."""
-    from oracle.ref_cpu import RefModel
+    with OpenMP) on cpu_threads() threads; else the serial build.  snap: a
+    state snapshot (displacements, velocities, plastic state) the oracle
+    starts from, i.e. the state the GPU steps were timed at."""
     t_all = time.perf_counter()
-    ref = RefModel(p, grid=True, omp=omp)
-    state = "the start of the run"
-    if model is not None:
-        d, v = model.get_state()
-        ref.disp[:] = d.reshape(-1)
-        ref.velo[:] = v.reshape(-1)
-        if p.plastic is not None:
-            ep, al = model.plastic_state()
-            ref.set_plastic_state(ep, al)
-        state = "the GPU's timed state"
+    ref = oracle_from(p, snap, omp)
+    state = "the start of the run" if snap is None else "the GPU's timed state"
     t0 = time.perf_counter()
     ref.leapfrog(1)
     t1 = time.perf_counter() - t0
@@ -191,25 +182,29 @@ def calibrate_copy(nbytes=1 << 30):
 
 
 def sweep(args):
+    print(json.dumps(sweep_measure(args.sweep_instances, args.sweep_steps,
+                                   args.sweep_tune_steps, args.sweep_seq_steps, args.warmup,
+                                   0.0 if args.no_cpu else args.cpu_budget)), flush=True)
+
+
+def sweep_measure(n_inst=256, nsteps=10_000, tune_steps=50, seq_steps=200, warmup=20,
+                  cpu_budget=15.0):
     """Configuration C4: the 256 seeded elastoplastic BCC lattices (per-instance
     dt, each crushed to 20 % nominal strain in its 10,000 steps) as one
     batched context whose steps replay captured hipGraphs, with the schedule
     (instance order x graph length, then the assembly layout) chosen by
-    batch.autotune; the whole
-    10,000-step sweep is timed from the initial state.  Against it: the same
-    instances one after another in their own contexts (a sample of steps
-    each), and the oracle on the median instance (all cores, one core)."""
+    batch.autotune; nsteps steps of the sweep are timed from the initial
+    state (10,000: the whole sweep).  Against it: the same instances one
+    after another in their own contexts (seq_steps steps each), and, with a
+    cpu_budget, the oracle on the median instance (all cores, one core)."""
     from dynlattice_amd import batch as BT
     from dynlattice_amd import lattice as L
     from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
-    n_inst = args.sweep_instances
-    nsteps = args.sweep_steps
     probs = [L.c4_instance(i) for i in range(n_inst)]
     flags = DL_TRACK_DISSIPATION
     E = sum(p.n_elems for p in probs)
     print(f"sweep: {n_inst} instances, {E} elements", file=sys.stderr, flush=True)
-    order, K, layout, tune = BT.autotune(probs, steps=args.sweep_tune_steps, flags=flags,
-                                         log=sys.stderr)
+    order, K, layout, tune = BT.autotune(probs, steps=tune_steps, flags=flags, log=sys.stderr)
     bt = BT.ordered_batch(probs, order)
     g = BT.make_model(bt, flags=flags | BT.LAYOUTS[layout], graph_steps=K)
     t0 = time.perf_counter()
@@ -237,12 +232,12 @@ def sweep(args):
     hip = C.CDLL("libamdhip64.so")
     stream = C.c_void_p()
     assert hip.hipStreamCreateWithFlags(C.byref(stream), 1) == 0
-    k_seq = args.sweep_seq_steps
+    k_seq = seq_steps
     singles = []
     for p in probs:
         m = GpuModel(p, stream=stream, flags=flags)
         m.set_reaction_group(p.groups["bottom"], 2)
-        m.step(args.warmup, p.dt, out=False)
+        m.step(warmup, p.dt, out=False)
         singles.append(m)
     for m in singles:
         m.sync()
@@ -281,7 +276,7 @@ def sweep(args):
                                      float(np.abs(hist).max(0).max())],
            "sequential_value": sequential, "sequential_steps_per_instance": k_seq,
            "batched_over_sequential": batched / sequential}
-    if not args.no_cpu:
+    if cpu_budget > 0:
         from oracle.ref_cpu import RefModel
         b = int(np.argsort([p.n_elems for p in probs])[n_inst // 2])
         q = probs[b]
@@ -290,7 +285,7 @@ def sweep(args):
             t1 = time.perf_counter()
             ref.leapfrog(1)
             t1 = time.perf_counter() - t1
-            k = int(max(1, min(nsteps, args.cpu_budget / max(t1, 1e-9))))
+            k = int(max(1, min(nsteps, cpu_budget / max(t1, 1e-9))))
             t1 = time.perf_counter()
             ref.leapfrog(k)
             t1 = time.perf_counter() - t1
@@ -301,7 +296,7 @@ def sweep(args):
                                   f"instance {q.name} ({q.n_elems} elements), {k} LeapFrog "
                                   f"steps after 1, {t1:.1f} s"}
         res["speedup_vs_cpu"] = batched / res["cpu_baseline"]["value"]
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def adjoint(args):
@@ -486,8 +481,47 @@ def profiler_region(on):
     (lib.roctxProfilerResume if on else lib.roctxProfilerPause)(C.c_uint64(0))
 
 
+def snapshot(g, p):
+    """the model's state (displacements, velocities, plastic state) on the host"""
+    d, v = g.get_state()
+    snap = {"disp": d, "velo": v}
+    if p.plastic is not None:
+        snap["eps_p"], snap["alpha"] = g.plastic_state()
+    return snap
+
+
+def oracle_from(p, snap, omp):
+    from oracle.ref_cpu import RefModel
+    ref = RefModel(p, grid=True, omp=omp)
+    if snap is not None:
+        ref.disp[:] = snap["disp"].reshape(-1)
+        ref.velo[:] = snap["velo"].reshape(-1)
+        if p.plastic is not None:
+            ref.set_plastic_state(snap["eps_p"], snap["alpha"])
+    return ref
+
+
+def window_cpu(p, w, snap, cpu_budget, window):
+    """the all-cores oracle timed from a window's state (a sample of about
+    cpu_budget seconds)"""
+    t_cpu = time.perf_counter()
+    ref = oracle_from(p, snap, True)
+    t1 = time.perf_counter()
+    ref.leapfrog(1)
+    t1 = time.perf_counter() - t1
+    k = int(max(1, min(window, cpu_budget / max(t1, 1e-9))))
+    t1 = time.perf_counter()
+    ref.leapfrog(k)
+    t1 = time.perf_counter() - t1
+    w["cpu_baseline"] = {"value": p.n_elems * k / t1, "unit": "element-GP updates/s",
+                         "cores": cpu_threads(), "kind": "port",
+                         "sample": f"oracle, all cores, {k} steps from this state"}
+    w["speedup_vs_cpu"] = w["value"] / w["cpu_baseline"]["value"]
+    w["cpu_s"] = time.perf_counter() - t_cpu
+
+
 def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0, acc=None,
-                  profile=False):
+                  profile=False, deferred=None):
     """Runs the crush (top plate at its prescribed velocity) through the given
     nominal strains and times a window of `window` steps at each: wall time
     per step, kernel phases (HIP events), plastic-active fraction, active
@@ -496,7 +530,10 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
     copied over) on a sample of about cpu_budget seconds.  start: steps the
     model has taken; acc: {"steps", "wall_s"} accumulating the ordinary
     step calls (not the event-timed passes).  profile: the windows' timed
-    steps are the regions a rocprofv3 --selected-regions run collects."""
+    steps are the regions a rocprofv3 --selected-regions run collects.
+    deferred: a list that takes (window, state snapshot) for the oracle
+    timing instead of running it between the GPU's timed work (a host
+    pause of seconds lets the GPU clocks drop before the next timed steps)."""
     H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
     v = float(np.abs(p.velo0[p.groups["top"], 2]).max())
     out = []
@@ -544,26 +581,10 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
              "list_entries": int(g.contact_list_stats()[0]) if p.contact is not None else 0,
              "dissipation_J": float(h["e_diss"][-1])}
         if cpu_budget > 0:
-            from oracle.ref_cpu import RefModel
-            t_cpu = time.perf_counter()
-            d, vv = g.get_state()
-            ep, al = g.plastic_state()
-            ref = RefModel(p, grid=True, omp=True)
-            ref.disp[:] = d.reshape(-1)
-            ref.velo[:] = vv.reshape(-1)
-            ref.set_plastic_state(ep, al)
-            t1 = time.perf_counter()
-            ref.leapfrog(1)
-            t1 = time.perf_counter() - t1
-            k = int(max(1, min(window, cpu_budget / max(t1, 1e-9))))
-            t1 = time.perf_counter()
-            ref.leapfrog(k)
-            t1 = time.perf_counter() - t1
-            w["cpu_baseline"] = {"value": p.n_elems * k / t1, "unit": "element-GP updates/s",
-                                 "cores": cpu_threads(), "kind": "port",
-                                 "sample": f"oracle, all cores, {k} steps from this state"}
-            w["speedup_vs_cpu"] = w["value"] / w["cpu_baseline"]["value"]
-            w["cpu_s"] = time.perf_counter() - t_cpu
+            if deferred is not None:
+                deferred.append((w, snapshot(g, p)))
+            else:
+                window_cpu(p, w, snapshot(g, p), cpu_budget, window)
         print(f"bench: strain {eps:.2f} (step {done}): {w['ms_per_step']:.4f} ms/step, "
               f"plastic {w['plastic_active_fraction']:.3f}, pairs {w['active_pairs_per_step']:.0f}, "
               f"rebuilds {w['rebuilds_per_window']}", file=log, flush=True)
@@ -659,6 +680,8 @@ def main():
                     help="skip the per-kernel event timing pass")
     ap.add_argument("--no-crush", action="store_true",
                     help="time the steps at the start of the run instead of in the crush")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the short batched-sweep comparison of the default line")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward time ratio (20^3 lattice, 5000 steps)")
     ap.add_argument("--window", type=int, default=200, help="steps per crush window")
@@ -697,10 +720,15 @@ def main():
     crushing = p.contact is not None and p.plastic is not None and not args.no_crush
     windows = []
     acc = {}
+    # the oracle timings wait for the end: all GPU timing runs back to back
+    # (a host pause of seconds lets the GPU clocks drop before the next
+    # timed steps)
+    deferred = []
     if crushing:
         # the timed steps sit in the crush where it works: 20 % nominal strain
         windows += crush_windows(g, p, [0.1, 0.2], args.window,
-                                 0.0 if args.no_cpu else args.cpu_budget / 2, acc=acc)
+                                 0.0 if args.no_cpu else args.cpu_budget / 2, acc=acc,
+                                 deferred=deferred)
     t0 = time.perf_counter()
     g.step(args.warmup, out=False)
     g.sync()
@@ -710,6 +738,7 @@ def main():
     g.sync()
     wall = time.perf_counter() - t0
     o = g.step(0)  # error check only
+    timed_state = snapshot(g, p) if not args.no_cpu else None
     acc["steps"] = acc.get("steps", 0) + args.warmup + args.steps
     acc["wall_s"] = acc.get("wall_s", 0.0) + t_w + wall
     value = p.n_elems * args.steps / wall
@@ -782,19 +811,12 @@ def main():
         "phase_ms": {k: round(v, 5) for k, v in phases.items()},
         "plastic_active_fraction": float(h_act),
     }
-    if not args.no_cpu:
-        cb = cpu_baseline(p, args.cpu_budget, args.steps, omp=True, model=g)
-        c1 = cpu_baseline(p, args.cpu_budget, args.steps, omp=False, model=g)
-        res["cpu_baseline"] = cb
-        res["speedup_vs_cpu"] = value / cb["value"]
-        res["cpu_baseline_serial"] = c1
-        res["speedup_vs_cpu_serial"] = value / c1["value"]
     if crushing:
         try:
             start = windows[-1]["step"] + args.warmup + (1 if args.no_timing else 2) * args.steps
             windows += crush_windows(g, p, [0.4], args.window,
                                      0.0 if args.no_cpu else args.cpu_budget / 2, start=start,
-                                     acc=acc)
+                                     acc=acc, deferred=deferred)
             # the crush from 0 to 40 % strain: the ordinary step calls (list
             # rebuilds and windows included, event-timed passes excluded)
             res["whole_crush"] = {"steps": acc["steps"], "wall_s": round(acc["wall_s"], 3),
@@ -815,6 +837,26 @@ def main():
         res["adjoint_plastic"] = {k: a[k] for k in ("config", "t_forward_plain_s",
                                                     "t_forward_checkpointed_s", "t_reverse_s",
                                                     "plastic_active_fraction_end")}
+    if not args.no_sweep:
+        # C4: the batched sweep against the same instances one by one (its
+        # first 1000 steps; bench.py --sweep times all 10,000)
+        w = sweep_measure(256, 1000, 10, 100, args.warmup, 0.0)
+        res["sweep_batched_over_sequential"] = round(w["batched_over_sequential"], 3)
+        res["sweep"] = {k: w[k] for k in ("value", "ms_per_step", "steps", "sequential_value",
+                                          "sequential_steps_per_instance", "roofline")}
+        res["sweep"]["config"] = {k: w["config"][k] for k in ("workload", "instances", "elements")}
+        res["sweep"]["schedule"] = {k: w["schedule"][k]
+                                    for k in ("instance_order", "graph_steps", "layout")}
+    # the oracle on the host cores, from the states the GPU was timed at
+    if not args.no_cpu:
+        cb = cpu_baseline(p, args.cpu_budget, args.steps, omp=True, snap=timed_state)
+        c1 = cpu_baseline(p, args.cpu_budget, args.steps, omp=False, snap=timed_state)
+        res["cpu_baseline"] = cb
+        res["speedup_vs_cpu"] = value / cb["value"]
+        res["cpu_baseline_serial"] = c1
+        res["speedup_vs_cpu_serial"] = value / c1["value"]
+        for w, snap in deferred:
+            window_cpu(p, w, snap, args.cpu_budget / 2, args.window)
     print(json.dumps(res), flush=True)
 
 
