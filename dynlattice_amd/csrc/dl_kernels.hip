@@ -571,13 +571,20 @@ __device__ __forceinline__ void step_end_sums(int g0, int nsums, const double *r
       n[q] = ndb;
     }
   }
-  // thread t sums in[t], in[t + SE_NT], ... in that order, four loads per
-  // sum in flight at a time
+  // thread t sums in[t], in[t + SE_NT], ... in that order, eight (then four)
+  // loads per sum in flight at a time
   double s[P];
 #pragma unroll
   for (int q = 0; q < P; q++) {
     s[q] = 0.0;
     int64_t i = threadIdx.x;
+    for (; i + 7 * SE_NT < n[q]; i += 8 * SE_NT) {
+      double a[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) a[k] = in[q][i + k * SE_NT];
+#pragma unroll
+      for (int k = 0; k < 8; k++) s[q] += a[k];
+    }
     for (; i + 3 * SE_NT < n[q]; i += 4 * SE_NT) {
       const double a0 = in[q][i], a1 = in[q][i + SE_NT], a2 = in[q][i + 2 * SE_NT],
                    a3 = in[q][i + 3 * SE_NT];
@@ -631,7 +638,7 @@ __global__ __launch_bounds__(SE_NT) void k_step_end(const double *rbuf, const in
 }
 
 // few sums: one block does them two at a time and then moves the step
-// counter; tail: the reaction row only.  Run by k_step_end1 or by the last
+// counter; tail: the reaction row only.  Run by k_step_end1 or by the first
 // block of the step's k_joint launch (no other block of it reads the counter)
 __device__ __forceinline__ void step_end_block(const StepEnd &se, double (*red)[SE_NT]) {
   const int64_t st = *se.step;
@@ -663,17 +670,18 @@ template <int MODE, bool SO3, bool SE>
 __global__ __launch_bounds__(DL_BLOCK) void k_joint(DevData d, StepParams sp, MilneDev m, StepEnd se) {
   static_assert(DL_BLOCK == SE_NT, "a step-end block is a k_joint block");
   if constexpr (SE) {
-    // the last se.nblocks blocks: the step's end sums (their inputs come
-    // from k_strut)
-    const int g = (int)blockIdx.x - ((int)gridDim.x - se.nblocks);
-    if (g >= 0) {
+    // the first se.nblocks blocks: the step's end sums (their inputs come
+    // from k_strut).  Dispatched first, their serial sums overlap the joint
+    // blocks instead of trailing them.
+    const int g = (int)blockIdx.x;
+    if (g < se.nblocks) {
       __shared__ double red[2][SE_NT];
       if (se.nblocks == 1) step_end_block(se, red);
       else step_end_one(se, g, red);
       return;
     }
   }
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t j = ((int32_t)blockIdx.x - (SE ? se.nblocks : 0)) * blockDim.x + threadIdx.x;
   if (MODE == MODE_MCORR) {
     // every thread joins the block's error partial sums
     double em[2] = {0.0, 0.0};
@@ -1074,7 +1082,7 @@ static void lj(const DevData &d, const StepParams &sp, const MilneDev &m, bool s
   else hipLaunchKernelGGL((k_joint<MODE, false, false>), g, b, 0, s, d, sp, m, se);
 }
 
-// the step's joint kernel with the end-of-step sums as its last block
+// the step's joint kernel with the end-of-step sums as its first block(s)
 static void lj_end(const DevData &d, const StepParams &sp, bool so3, const StepEnd &se,
                    hipStream_t s) {
   const dim3 g(nblk(d.NJ) + se.nblocks), b(DL_BLOCK);
