@@ -986,6 +986,7 @@ extern "C" int dl_set_node_dt(dl_ctx *c, const double *dt, int64_t n_nodes) {
   if (!c) return fail(DL_ERR_INPUT, "NULL context");
   if (dt && c->pd.nprog > 0) return fail(DL_ERR_STATE, "per-node time steps with time programs");
   if (dt && c->imp.ngroups > 0) return fail(DL_ERR_STATE, "per-node time steps with impact groups");
+  if (dt && c->mu.code) return fail(DL_ERR_STATE, "per-node time steps with a mass update schedule");
   c->adj.valid = false;
   if (!dt) {
     c->d.ndt = nullptr;

@@ -350,6 +350,9 @@ extern "C" int dl_set_mass_update(dl_ctx *c, const char *cond) {
   c->d.Js = nullptr;
   c->pd.clock_on = 0;
   if (!cond || !*cond) return DL_OK;
+  // the schedule runs on the context's clock, which per-node time steps do
+  // not define
+  if (c->d.ndt) return fail(DL_ERR_STATE, "a mass update schedule with per-node time steps");
   std::vector<ProgOp> code;
   std::string err;
   if (!prog_compile(cond, code, err)) return fail(DL_ERR_INPUT, "updateWhen: " + err);

@@ -192,8 +192,17 @@ def compile_condition(text):
     strings is one string: JEM's += on a string property appends)"""
     if isinstance(text, (list, tuple)):
         text = "".join(text)
-    tree = _Parser(text).parse()
-    return lambda env: _ev(tree, env)
+    try:
+        tree = _Parser(text).parse()
+    except RecursionError:
+        raise YieldError("yield condition: nested too deeply") from None
+
+    def f(env):
+        try:
+            return _ev(tree, env)
+        except RecursionError:
+            raise YieldError("yield condition: nested too deeply") from None
+    return f
 
 
 def to_family(text, has_iso: bool, has_kin: bool, tol: float = 1e-10):

@@ -438,7 +438,8 @@ int dl_set_time(dl_ctx *ctx, double t, int64_t step);
  * dl_set_state; in between the last assembled mass is used.  cond is a
  * program expression of t and i (dl_set_programs' syntax); NULL or "" gives
  * the default, an update at every step.  GET_MATRIX2 still assembles at the
- * current state (the reference's action does). */
+ * current state (the reference's action does).  A schedule and per-node time
+ * steps (dl_set_node_dt) exclude each other: DL_ERR_STATE. */
 int dl_set_mass_update(dl_ctx *ctx, const char *cond);
 int dl_get_time(dl_ctx *ctx, double *t, int64_t *step);
 /* Evaluates a program expression on the host with the compiler and

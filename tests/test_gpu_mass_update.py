@@ -111,3 +111,21 @@ def test_crush_with_schedule_matches_oracle():
     assert rel_inf(g.history(300)["reaction"], o["reaction"]) <= 1e-9
     assert np.array_equal(g.plastic_active(), r.plastic_state()[3])
     g.close()
+
+
+def test_schedule_and_node_time_steps_exclude_each_other():
+    """A schedule runs on the context's clock, which per-node time steps
+    (batched sweeps) do not define: either order of the two calls is refused."""
+    from dynlattice_amd.gpu import DL_ERR_STATE, DLError, GpuModel
+    p = _tumbling_rod()
+    g = GpuModel(p)
+    g.set_mass_update("i % 5 < 1")
+    with pytest.raises(DLError) as e:
+        g.set_node_dt(np.full(p.n_nodes, p.dt))
+    assert e.value.code == DL_ERR_STATE
+    g.set_mass_update(None)
+    g.set_node_dt(np.full(p.n_nodes, p.dt))
+    with pytest.raises(DLError) as e:
+        g.set_mass_update("i % 5 < 1")
+    assert e.value.code == DL_ERR_STATE
+    g.close()
