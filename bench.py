@@ -536,6 +536,9 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
     pause of seconds lets the GPU clocks drop before the next timed steps)."""
     H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
     v = float(np.abs(p.velo0[p.groups["top"], 2]).max())
+    # k_strut's algorithmic bytes per launch (the headline roofline's figure)
+    b_strut = (B_ELEM + (B_ELEM_PLASTIC if p.plastic is not None else 0.0)) * p.n_elems + \
+        B_NODE * private_node_count(p)
     out = []
     done = start
     acc = {} if acc is None else acc
@@ -579,7 +582,8 @@ def crush_windows(g, p, strains, window, cpu_budget=0.0, log=sys.stderr, start=0
              "rebuilds_per_window": int(o.n_rebuilds),
              "candidates_last_step": int(o.n_candidates),
              "list_entries": int(g.contact_list_stats()[0]) if p.contact is not None else 0,
-             "dissipation_J": float(h["e_diss"][-1])}
+             "dissipation_J": float(h["e_diss"][-1]),
+             "strut_roofline_frac": round(b_strut / (ph["strut"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if cpu_budget > 0:
             if deferred is not None:
                 deferred.append((w, snapshot(g, p)))

@@ -106,11 +106,45 @@ extern "C" int dl_adjoint_forward(dl_ctx *c, int64_t nsteps, double dt, int32_t 
     }
   }
   if ((rc = prepare_history(c, nsteps))) return rc;
+  const bool lists = c->contact && c->cs.c.skin > 0.0;
+  if (lists) {
+    // log the steps that build the candidate lists (the kernels take the
+    // log's pointer by value: captured step graphs are re-captured)
+    if (A.blog_cap < nsteps) {
+      dfree(c, A.blog);
+      A.blog = nullptr;
+      A.blog_cap = 0;
+      double *p = nullptr;
+      if ((rc = dalloc(c, &p, (nsteps + 7) / 8))) return rc;
+      A.blog = (uint8_t *)p;
+      A.blog_cap = nsteps;
+    }
+    HIPCHK(hipMemsetAsync(A.blog, 0, nsteps, c->stream));
+    c->cs.c.blog = A.blog;
+    c->cs.c.bstep = c->step_ctr;
+    c->gen++;
+  }
   const StepParams sp{dt, 1, 0};
   for (int64_t seg = 0; seg < nseg; seg++) {
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, true))) return rc;
     if (c->plastic && (rc = plastic_copy(c, A.ckpt_pl + seg * 13 * E, true))) return rc;
-    if ((rc = advance_steps(c, std::min<int64_t>(every, nsteps - seg * every), sp))) return rc;
+    // every segment starts with lists built at its checkpoint, as its
+    // recomputation does
+    if (lists) contact_invalidate(c->cs, c->stream);
+    rc = advance_steps(c, std::min<int64_t>(every, nsteps - seg * every), sp);
+    if (rc) {
+      if (lists) {
+        c->cs.c.blog = nullptr;
+        c->gen++;
+      }
+      return rc;
+    }
+  }
+  if (lists) {
+    c->cs.c.blog = nullptr;
+    c->gen++;
+    A.blog_h.resize(nsteps);
+    HIPCHK(hipMemcpyAsync(A.blog_h.data(), A.blog, nsteps, hipMemcpyDeviceToHost, c->stream));
   }
   if ((rc = launch_reaction_tail(c))) return rc;
   if ((rc = state_copy(c, A.final_state, true))) return rc;
@@ -358,21 +392,35 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
     dn.u = A.final_state;
     if ((rc = contact_adjoint(c->cs, dn, a, s))) return rc;
   }
+  // candidate lists: the forward run's build log says at which steps they
+  // were built (always at a segment's first step); the recomputation builds
+  // them at the same steps from the same states, and the reverse sweep
+  // rebuilds, from the taped state, the lists each step was evaluated with
+  const bool lists = c->contact && c->cs.c.skin > 0.0 &&
+                     (int64_t)A.blog_h.size() >= A.nsteps;
   for (int64_t seg = nseg - 1; seg >= 0; seg--) {
     const int64_t t0 = seg * A.every, len = std::min<int64_t>(A.every, A.nsteps - t0);
     if ((rc = state_copy(c, A.ckpt + seg * 13 * N, false))) return rc;
     if (pl && (rc = plastic_copy(c, A.ckpt_pl + seg * 13 * E, false))) return rc;
-    if (c->contact) contact_invalidate(c->cs, s);
+    if (c->contact && !lists) contact_invalidate(c->cs, s);
+    // the build in effect at each step of the segment
+    std::vector<int64_t> built(len, 0);
+    if (lists)
+      for (int64_t j = 1; j < len; j++) built[j] = A.blog_h[t0 + j] ? j : built[j - 1];
     // recompute the segment, taping every step's start state and forces
     // (and the committed plastic state the step starts from)
     for (int64_t j = 0; j < len; j++) {
       DevData dj = c->d;
       dj.tape = A.tape + j * 19 * N;
       if (pl && (rc = plastic_copy(c, A.ptape + j * 13 * E, true))) return rc;
-      if (c->contact && (rc = contact_step(c->cs, dj, s, nullptr, nullptr, nullptr))) return rc;
+      const int refresh = !lists ? REFRESH_GATED : (built[j] == j ? REFRESH_FORCE : REFRESH_SKIP);
+      if (c->contact && (rc = contact_step(c->cs, dj, s, nullptr, nullptr, nullptr, nullptr,
+                                           refresh)))
+        return rc;
       launch_strut_pp(dj, sp, c->pp, MODE_STEP, true, c->mat_kind, c->contact, s);
       launch_joint(dj, sp, MODE_STEP, true, s);
     }
+    int64_t cur = built[len - 1]; // the lists the recomputation left
     // reverse it; the node kernel first gathers the element ends of the
     // step reversed before (before the last step: the closing evaluation)
     for (int64_t j = len - 1; j >= 0; j--) {
@@ -385,7 +433,14 @@ extern "C" int dl_adjoint_backward(dl_ctx *c, const double *reaction_bar, dl_sec
       if (c->contact) {
         DevData dn = c->d;
         dn.u = (double *)a.tape; // rows 0..2 of the tape: the step's start displacement
-        if ((rc = contact_adjoint(c->cs, dn, a, s))) return rc;
+        if (lists && built[j] != cur) {
+          DevData db = c->d;
+          db.u = A.tape + built[j] * 19 * N;
+          contact_rebuild(c->cs, db, s);
+          cur = built[j];
+        }
+        if ((rc = contact_adjoint(c->cs, dn, a, s, lists ? REFRESH_SKIP : REFRESH_GATED)))
+          return rc;
       }
     }
     HIPCHK(hipGetLastError());

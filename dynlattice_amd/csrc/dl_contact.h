@@ -73,6 +73,10 @@ struct ContactDev {
   unsigned long long *cnew; // [(E+63)/64]
   int32_t *err;         // device error flag (capacity)
   unsigned long long *nbuild; // [1] list builds so far
+  // while set (the adjoint's checkpointed forward run): blog[*bstep] = 1 at
+  // every list build, the step's entry of a build log
+  uint8_t *blog;
+  const int32_t *bstep;
 };
 
 struct ContactState {
@@ -99,15 +103,23 @@ int contact_init_blacklist(ContactState &cs, const DevData &d, hipStream_t s);
 int contact_force(ContactState &cs, const DevData &d, hipStream_t s);
 // evaluation inside the explicit step: per-element forces + counts added to
 // n_active[*step], n_cand[*step] (kernels only: safe inside a captured graph)
+// refresh: REFRESH_GATED rebuilds the lists when the trigger has fired;
+// REFRESH_FORCE rebuilds them; REFRESH_SKIP launches no rebuild stage (the
+// caller knows the lists hold: the adjoint's recorded build schedule)
+enum { REFRESH_GATED = 0, REFRESH_FORCE = 1, REFRESH_SKIP = 2 };
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh = nullptr);
+                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh = nullptr,
+                 int refresh = REFRESH_GATED);
 // candidate-list entries summed over the elements (after a stream sync)
 int contact_list_entries(ContactState &cs, hipStream_t s, int64_t *n);
 // adjoint of the contact forces at the state d (the taped start state of a
 // step): broadphase, then per closed pair the VJP into a.ebuf and a.crbar
-int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s);
+int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s,
+                    int refresh = REFRESH_GATED);
 // marks the candidate lists stale (the state was set from outside the step)
 void contact_invalidate(ContactState &cs, hipStream_t s);
+// rebuilds the candidate lists at the state d (kernels only)
+void contact_rebuild(ContactState &cs, const DevData &d, hipStream_t s);
 // candidate-list builds since the context was created (after a stream sync)
 int contact_rebuilds(ContactState &cs, hipStream_t s, int64_t *n);
 // replaces the per-strut contact radii (host array of n_struts values)

@@ -537,7 +537,10 @@ __device__ __forceinline__ int32_t list_partner(int32_t v) { return v < 0 ? ~v :
 
 __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const int32_t *gate) {
   GATED(gate);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(c.nbuild, 1ull);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(c.nbuild, 1ull);
+    if (c.blog) c.blog[*c.bstep] = 1;
+  }
   GRID_STRIDE(k, c.E) {
     const CRec x = ((const CRec *)c.rec)[k];
     const int32_t X = x.e;
@@ -1101,7 +1104,7 @@ static void list_refresh(ContactState &cs, const DevData &d, hipStream_t s) {
 }
 
 int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_active,
-                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh) {
+                 int64_t *n_cand, const int32_t *step, hipEvent_t after_refresh, int refresh) {
   ContactDev &c = cs.c;
   if (!(c.skin > 0.0)) {
     // no lists: full broadphase and cell scan every step
@@ -1113,7 +1116,8 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
     return DL_OK;
   }
   // list rebuild, gated on the trigger (no-ops while the lists are valid)
-  list_refresh(cs, d, s);
+  if (refresh == REFRESH_FORCE) contact_invalidate(cs, s);
+  if (refresh != REFRESH_SKIP) list_refresh(cs, d, s);
   if (after_refresh) CHK(hipEventRecord(after_refresh, s));
   const int64_t nthr = (c.E + VN_PER - 1) / VN_PER;
   const unsigned gm = (unsigned)std::min<int64_t>(VM_GRID, (c.E + CB - 1) / CB);
@@ -1122,6 +1126,12 @@ int contact_step(ContactState &cs, const DevData &d, hipStream_t s, int64_t *n_a
   hipLaunchKernelGGL(kc_vsum, dim3((unsigned)((nthr + CB - 1) / CB)), dim3(CB), 0, s, c, d);
   CHK(hipGetLastError());
   return DL_OK;
+}
+
+void contact_rebuild(ContactState &cs, const DevData &d, hipStream_t s) {
+  if (!(cs.c.skin > 0.0)) return;
+  contact_invalidate(cs, s);
+  list_refresh(cs, d, s);
 }
 
 void contact_invalidate(ContactState &cs, hipStream_t s) {
@@ -1165,13 +1175,15 @@ int contact_read_pairs(ContactState &cs, hipStream_t s, bool active, int32_t *a,
   return DL_OK;
 }
 
-int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s) {
+int contact_adjoint(ContactState &cs, const DevData &d, const AdjDev &a, hipStream_t s,
+                    int refresh) {
   ContactDev &c = cs.c;
   if (!(c.skin > 0.0)) {
     broadphase(cs, d, s);
     hipLaunchKernelGGL(kc_adj_narrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
   } else {
-    list_refresh(cs, d, s);
+    if (refresh == REFRESH_FORCE) contact_invalidate(cs, s);
+    if (refresh != REFRESH_SKIP) list_refresh(cs, d, s);
     hipLaunchKernelGGL(kc_adj_vnarrow, dim3(cs.nblk_e), dim3(CB), 0, s, c, d, a);
   }
   CHK(hipGetLastError());

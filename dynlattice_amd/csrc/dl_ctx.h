@@ -79,6 +79,12 @@ struct AdjState {
   int64_t ptape_rows = 0;
   double *ptape = nullptr;     // [rows][13 E]
   double *pbar = nullptr, *Wbar = nullptr, *rnode = nullptr, *fstep_r = nullptr;
+  // contact candidate lists: the steps of the forward run that built them
+  // (every checkpoint builds), so that the reverse sweep rebuilds them only
+  // there, from the taped state of the build
+  int64_t blog_cap = 0;
+  uint8_t *blog = nullptr;     // [nsteps] device
+  std::vector<uint8_t> blog_h; // [nsteps] host copy
 };
 
 // boundary-data time programs (host copy)
