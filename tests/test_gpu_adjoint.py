@@ -76,6 +76,33 @@ def test_checkpointed_forward_is_the_explicit_step():
     sim.close()
 
 
+def test_gradients_do_not_depend_on_the_list_skin():
+    """The reverse sweep rebuilds the contact candidate lists only at the
+    forward run's logged builds, from the taped state of the build in effect.
+    The lists fix the pairs, not the build time, so a small skin (builds
+    inside the segments, and the reverse crossing back over them) and a wide
+    one (a build per segment) give the same gradient bits."""
+    import copy
+    p0 = L.bcc_adjoint_check()
+    r = float(p0.contact.radius.max())
+    n, every = 1600, 100
+    grads, builds = [], []
+    w = np.random.default_rng(4).normal(size=(n, 1))
+    for skin in (0.02 * r, 2.0 * r):
+        p = copy.deepcopy(p0)
+        p.contact.skin = skin
+        sim = _sim(p, n, every, max_pairs=400 * p.n_elems)
+        sim.run(_sections(p), p.contact.radius)
+        builds.append(sim.model.contact_list_stats()[1])
+        grads.append(sim.gradients(w))
+        sim.close()
+    print("list builds per run:", builds)
+    nseg = n // every
+    assert builds[0] > 2 * nseg  # builds inside the segments
+    for k in grads[0]:
+        assert np.array_equal(grads[0][k], grads[1][k]), k
+
+
 def test_backward_requires_a_forward_run():
     from dynlattice_amd.gpu import DL_ERR_STATE, DLError
     p = _no_contact_case()
