@@ -8,7 +8,9 @@ depend on the size:
   force equals the sum of the oracle's element forces at the same state,
   within the 1e-11 bar of the force parity tests, on 3000 seeded nodes;
 * evaluations and steps are bitwise reproducible, and a run replayed from
-  captured graphs equals direct launches bit for bit after 24 steps.
+  captured graphs equals direct launches bit for bit after 24 steps;
+* from the crush at 25 % and at 40 % strain (tests at the end of the file),
+  the device's steps equal the oracle's.
 The state is the rest state perturbed by seeded displacements (1 um) and
 rotations (1e-4 rad), which keeps contact and plastic flow off, so that the
 sampled nodes see element forces only."""
@@ -162,3 +164,54 @@ def test_mid_crush_state_matches_oracle(c3):
     assert rel_inf(rg, o["reaction"]) <= 1e-9
     assert np.array_equal(ng, o["n_contacts"])
     assert np.array_equal(ag.astype(bool), act_r.astype(bool))
+
+
+def test_dense_crush_state_matches_oracle(c3):
+    """The crush carried to 40 % nominal strain, where the lattice has
+    densified (tens of thousands of closed contact pairs, millions of list
+    entries, list rebuilds every few dozen steps), then 20 steps on the
+    device and on the oracle from that state: equal active contact pairs per
+    step, displacements and velocities within 1e-8, the GroupOutput reaction
+    within 1e-9 and equal plastic active sets."""
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
+    from oracle.ref_cpu import RefModel
+    from tests.conftest import rel_inf
+    p = c3
+    bot, top = p.groups["bottom"], p.groups["top"]
+    g = GpuModel(p, flags=DL_TRACK_DISSIPATION)
+    g.set_reaction_group(bot, 2)
+    g.set_graph(10)
+    H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
+    target = int(round(0.4 * H / abs(float(p.velo0[top[0], 2])) / p.dt))
+    done = 0
+    while done < target:
+        k = min(5000, target - done)
+        g.step(k)
+        done += k
+    d0, v0 = g.get_state()
+    ep0, al0 = g.plastic_state()
+    K = 20
+    g.step(K)
+    dg, vg = g.get_state()
+    h = g.history(K)
+    ag = g.plastic_active()
+    entries = g.contact_list_stats()[0]
+    g.close()
+    r = RefModel(p, grid=True, omp=True)
+    r.disp[:] = d0.reshape(-1)
+    r.velo[:] = v0.reshape(-1)
+    r.set_plastic_state(ep0, al0)
+    o = r.leapfrog(K, group=bot, group_dof=2)
+    dr, vr = r.disp.reshape(-1, 6), r.velo.reshape(-1, 6)
+    act_r = r.plastic_state()[3].astype(bool)
+    print("dense crush: step", done, "closed pairs", h["n_contacts"].min(), "-",
+          h["n_contacts"].max(), "list entries", entries, "plastic", ag.mean(),
+          "active-set mismatches", int((ag.astype(bool) != act_r).sum()),
+          "disp", rel_inf(dg, dr), "velo", rel_inf(vg, vr),
+          "reaction", rel_inf(h["reaction"], o["reaction"]))
+    assert h["n_contacts"].min() > 10_000 and entries > 1_000_000
+    assert np.array_equal(h["n_contacts"], o["n_contacts"])
+    assert rel_inf(dg, dr) <= 1e-8
+    assert rel_inf(vg, vr) <= 1e-8
+    assert rel_inf(h["reaction"], o["reaction"]) <= 1e-9
+    assert np.array_equal(ag.astype(bool), act_r)
