@@ -783,7 +783,8 @@ def main():
             roof["fp64_TFLOPs"] = round(tf, 3)
             roof["fp64_frac"] = round(tf / FP64_PEAK_TFLOPS, 4)
         # the whole step against the same bound: every element's and node's
-        # algorithmic bytes (SURVEY figures) over the measured step time
+        # algorithmic bytes (the per-element and per-node figures above) over
+        # the measured step time
         b_step = b_elem * p.n_elems + B_NODE * p.n_nodes
         a_step = b_step / (ms_per_step * 1e-3) / 1e9
         step_roof = {"bound": "hbm", "achieved": round(a_step, 1), "peak": HBM_PEAK_GBS,
