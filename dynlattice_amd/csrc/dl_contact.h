@@ -62,11 +62,12 @@ struct ContactDev {
   int32_t *vcount;      // [E] partners listed
   int32_t *vlist;       // [E][vk] partner ids
   int32_t vk;           // list capacity per element
-  // the step's pair pass: every list entry's position X * vk + t (the
-  // entries of one element contiguous, elements in build order), a bit per
-  // closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32, so an
+  // the step's pair pass: a 32-byte record per list entry (the entries of
+  // one element contiguous, elements in build order): the entry's bit
+  // X * vkb + t, its list code, both struts and both elements' nodes; a bit
+  // per closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32, so an
   // element owns whole words) and a bit per element with a closed pair
-  uint32_t *ventry;     // [E * vk]
+  int32_t *vrec;        // [E * vk][8] (bit, code, sX, sY, xa, xb, ya, yb)
   int32_t *nentries;    // [1]
   int32_t vkb;
   uint32_t *vmark;      // [E * vkb / 32]

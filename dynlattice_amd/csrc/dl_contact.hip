@@ -563,10 +563,19 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     if (n > cap) *c.err = DL_ERR_CAPACITY;
     c.vcount[X] = n < cap ? n : cap;
     if (n > 0) {
-      // the entry table of kc_vmark (the order of the elements is irrelevant)
+      // the entry records of kc_vmark (the order of the elements is irrelevant)
       const int32_t m = n < cap ? n : cap;
       const int32_t base = atomicAdd(c.nentries, m);
-      for (int32_t t = 0; t < m; t++) c.ventry[base + t] = (uint32_t)X * (uint32_t)cap + (uint32_t)t;
+      // the entry's record: its bit, the list code and both elements' nodes
+      // and struts, so that the pair pass starts from one 32-byte load
+      // instead of a chain of lookups
+      const int32_t xa = c.en0[X], xb = c.en1[X], sX = x.s;
+      for (int32_t t = 0; t < m; t++) {
+        const int32_t code = lst[t], Y = list_partner(code);
+        int4 *r = (int4 *)(c.vrec + 8 * (int64_t)(base + t));
+        r[0] = make_int4((int32_t)((int64_t)X * c.vkb + t), code, sX, c.es[Y]);
+        r[1] = make_int4(xa, xb, c.en0[Y], c.en1[Y]);
+      }
     }
   }
   // the lists are built: remember the displacement they were built at (no
@@ -578,7 +587,7 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 
 // ------------------------------------------------------------------
 // The step's pair pass from the lists, in two kernels.
-//   kc_vmark  parallel over list entries (the entry table written by the
+//   kc_vmark  parallel over list entries (the entry records written by the
 //             list build): the exact boxes of X and of the listed Y, the
 //             candidate and active counts, and for every closed pair its
 //             bit in vmark and X's bit in cnew.  A thread takes one entry
@@ -603,19 +612,17 @@ __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
   const int lane = threadIdx.x & 63;
   const int64_t N = c.N;
   const int64_t ne = *c.nentries;
-  const uint32_t vk = (uint32_t)c.vk;
   unsigned long long nact = 0, ncand = 0;
   for (int64_t q = (int64_t)blockIdx.x * CB + threadIdx.x; q < ne; q += (int64_t)gridDim.x * CB) {
-    const uint32_t pos = c.ventry[q];
-    const int32_t Xq = (int32_t)(pos / vk);
-    const int32_t t = (int32_t)(pos - (uint32_t)Xq * vk);
-    const int32_t code = c.vlist[pos];
-    const int32_t Y = list_partner(code);
-    const int32_t sX = c.es[Xq], sY = c.es[Y];
+    const int4 r0 = ((const int4 *)c.vrec)[2 * q], r1 = ((const int4 *)c.vrec)[2 * q + 1];
+    const int64_t bit = r0.x;
+    const int32_t Xq = (int32_t)(bit / c.vkb);
+    const int32_t code = r0.y, sX = r0.z, sY = r0.w;
+    const int32_t xa = r1.x, xb = r1.y, ya = r1.z, yb = r1.w;
     const double rX = c.crad[sX], rY = c.crad[sY];
     double xlo[3], xhi[3], ylo[3], yhi[3];
-    elem_box(d, N, c.en0[Xq], c.en1[Xq], rX, xlo, xhi);
-    elem_box(d, N, c.en0[Y], c.en1[Y], rY, ylo, yhi);
+    elem_box(d, N, xa, xb, rX, xlo, xhi);
+    elem_box(d, N, ya, yb, rY, ylo, yhi);
     bool ov = true;
     for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
     if (ov) {
@@ -623,13 +630,12 @@ __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
       if (XisA) ncand++;
       if (code >= 0) { // not blacklisted at INIT
         double pX[2][3], pY[2][3], g0[3], g1[3];
-        elem_pos(d, c.en0[Xq], c.en1[Xq], pX);
-        elem_pos(d, c.en0[Y], c.en1[Y], pY);
+        elem_pos(d, xa, xb, pX);
+        elem_pos(d, ya, yb, pY);
         const double R = rX + rY;
         const bool closed = XisA ? cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
                                  : cm_narrowphase<false, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
         if (closed) {
-          const int64_t bit = (int64_t)Xq * c.vkb + t;
           atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
           atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
           if (XisA) nact++;
@@ -952,9 +958,10 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     if ((rc = calloc_dev(cs, &c.vcount, E))) return rc;
     if ((rc = calloc_dev(cs, &c.vlist, (size_t)E * c.vk))) return rc;
     c.vkb = (c.vk + 31) & ~31;
-    if ((uint64_t)E * (uint64_t)c.vk > (uint64_t)UINT32_MAX)
-      return dl_set_error(DL_ERR_INPUT, "n_elems * (max_pairs / n_elems) must stay below 2^32");
-    if ((rc = calloc_dev(cs, &c.ventry, (size_t)E * c.vk))) return rc;
+    if ((uint64_t)E * (uint64_t)c.vkb > (uint64_t)INT32_MAX)
+      return dl_set_error(DL_ERR_INPUT,
+                          "n_elems * (max_pairs / n_elems rounded up to 32) must stay below 2^31");
+    if ((rc = calloc_dev(cs, &c.vrec, 8 * (size_t)E * c.vk))) return rc;
     if ((rc = calloc_dev(cs, &c.nentries, 1))) return rc;
     if ((rc = calloc_dev(cs, &c.vmark, (size_t)E * (c.vkb / 32)))) return rc;
     if ((rc = calloc_dev(cs, &c.cnew, (E + 63) / 64))) return rc;
