@@ -166,20 +166,14 @@ def test_mid_crush_state_matches_oracle(c3):
     assert np.array_equal(ag.astype(bool), act_r.astype(bool))
 
 
-def test_dense_crush_state_matches_oracle(c3):
-    """The crush carried to 40 % nominal strain, where the lattice has
-    densified (tens of thousands of closed contact pairs, millions of list
-    entries, list rebuilds every few dozen steps), then 20 steps on the
-    device and on the oracle from that state: equal active contact pairs per
-    step, displacements and velocities within 1e-8, the GroupOutput reaction
-    within 1e-9 and equal plastic active sets."""
+@pytest.fixture(scope="module")
+def dense_state(c3):
+    """the crush carried to 40 % nominal strain on the device: (disp, velo,
+    eps_p, alpha, steps)"""
     from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
-    from oracle.ref_cpu import RefModel
-    from tests.conftest import rel_inf
     p = c3
-    bot, top = p.groups["bottom"], p.groups["top"]
+    top = p.groups["top"]
     g = GpuModel(p, flags=DL_TRACK_DISSIPATION)
-    g.set_reaction_group(bot, 2)
     g.set_graph(10)
     H = float(p.coords[:, 2].max() - p.coords[:, 2].min())
     target = int(round(0.4 * H / abs(float(p.velo0[top[0], 2])) / p.dt))
@@ -190,6 +184,28 @@ def test_dense_crush_state_matches_oracle(c3):
         done += k
     d0, v0 = g.get_state()
     ep0, al0 = g.plastic_state()
+    g.close()
+    return d0, v0, ep0, al0, done
+
+
+def test_dense_crush_state_matches_oracle(c3, dense_state):
+    """The crush carried to 40 % nominal strain, where the lattice has
+    densified (tens of thousands of closed contact pairs, millions of list
+    entries, list rebuilds every few dozen steps), then 20 steps on the
+    device and on the oracle from that state: equal active contact pairs per
+    step, displacements and velocities within 1e-8, the GroupOutput reaction
+    within 1e-9 and equal plastic active sets."""
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel
+    from oracle.ref_cpu import RefModel
+    from tests.conftest import rel_inf
+    p = c3
+    bot = p.groups["bottom"]
+    d0, v0, ep0, al0, done = dense_state
+    g = GpuModel(p, flags=DL_TRACK_DISSIPATION)
+    g.set_reaction_group(bot, 2)
+    g.set_graph(10)
+    g.set_state(d0, v0)
+    g.set_plastic_state(ep0, al0)
     K = 20
     g.step(K)
     dg, vg = g.get_state()
@@ -215,3 +231,45 @@ def test_dense_crush_state_matches_oracle(c3):
     assert rel_inf(vg, vr) <= 1e-8
     assert rel_inf(h["reaction"], o["reaction"]) <= 1e-9
     assert np.array_equal(ag.astype(bool), act_r)
+
+
+def test_dense_crush_milne_matches_oracle(c3, dense_state):
+    """MilneDevice from the 40 % crush state: 6 adaptive attempts (precision
+    1e-4, starting from the LeapFrog step) on the device and on the oracle's
+    ref_milne: the same accept/reject sequence, step sizes and GroupOutput
+    reactions within 1e-8, equal closed-pair counts per attempt, the final
+    state within 1e-8 and equal plastic active sets."""
+    from dynlattice_amd.gpu import DL_TRACK_DISSIPATION, GpuModel, milne_opts
+    from oracle.ref_cpu import RefModel
+    from oracle.ref_cpu import milne_opts as ref_opts
+    from tests.conftest import rel_inf
+    p = c3
+    bot = p.groups["bottom"]
+    d0, v0, ep0, al0, done = dense_state
+    n, prec = 6, 1e-4
+    g = GpuModel(p, flags=DL_TRACK_DISSIPATION)
+    g.set_reaction_group(bot, 2)
+    g.set_state(d0, v0)
+    g.set_plastic_state(ep0, al0)
+    b = g.milne(n, milne_opts(p.dt, prec))
+    dg, vg = g.get_state()
+    ag = g.plastic_active().astype(bool)
+    g.close()
+    r = RefModel(p, grid=True, omp=True)
+    r.disp[:] = d0.reshape(-1)
+    r.velo[:] = v0.reshape(-1)
+    r.set_plastic_state(ep0, al0)
+    a = r.milne(n, ref_opts(p.dt, prec), group=bot)
+    dr = r.disp.reshape(-1, 6)
+    act_r = r.plastic_state()[3].astype(bool)
+    print("dense crush milne: accepted", a["accepted"].astype(int).tolist(), "dt",
+          rel_inf(b["dt"], a["dt"]), "reaction", rel_inf(b["reaction"], a["reaction"]),
+          "pairs", b["n_contacts"].tolist(), "disp", rel_inf(dg, dr),
+          "active-set mismatches", int((ag != act_r).sum()))
+    assert np.array_equal(b["accepted"], a["accepted"])
+    assert rel_inf(b["dt"], a["dt"]) <= 1e-8
+    assert rel_inf(b["reaction"], a["reaction"]) <= 1e-8
+    assert np.array_equal(b["n_contacts"], a["n_contacts"])
+    assert min(b["n_contacts"]) > 10_000
+    assert rel_inf(dg, dr) <= 1e-8
+    assert np.array_equal(ag, act_r)
