@@ -68,6 +68,8 @@ struct ContactDev {
   // per closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32, so an
   // element owns whole words) and a bit per element with a closed pair
   int32_t *vrec;        // [E * vk][8] (bit, code, sX, sY, xa, xb, ya, yb)
+  int32_t *vbase;       // [E] an element's first entry record
+  double *fpair;        // [E * vk][6] force of a closed entry on X's nodes a, b
   int32_t *nentries;    // [1]
   int32_t vkb;
   uint32_t *vmark;      // [E * vkb / 32]

@@ -566,6 +566,7 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
       // the entry records of kc_vmark (the order of the elements is irrelevant)
       const int32_t m = n < cap ? n : cap;
       const int32_t base = atomicAdd(c.nentries, m);
+      c.vbase[X] = base;
       // the entry's record: its bit, the list code and both elements' nodes
       // and struts, so that the pair pass starts from one 32-byte load
       // instead of a chain of lookups
@@ -590,16 +591,17 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 //   kc_vmark  parallel over list entries (the entry records written by the
 //             list build): the exact boxes of X and of the listed Y, the
 //             candidate and active counts, and for every closed pair its
-//             bit in vmark and X's bit in cnew.  A thread takes one entry
+//             force on X's two nodes (stored per entry), its bit in vmark
+//             and X's bit in cnew.  A thread takes one entry
 //             at a time, so the work follows the number of entries and not
 //             the longest list, and a light load spreads over as many
 //             wavefronts as it has entries.
 //   kc_vsum   per element with a closed pair (16 elements per lane, the
-//             wavefront owning 1024 elements = 16 mask words): its closed
-//             pairs recomputed and summed in list order, i.e. by partner id
-//             -- the operations and the order of one serial pass over the
-//             list, so the force does not depend on the cell layout, on when
-//             the lists were built or on the split into two kernels.  It
+//             wavefront owning 1024 elements = 16 mask words): the stored
+//             forces of its closed pairs summed in list order, i.e. by
+//             partner id -- the operations and the order of one serial pass
+//             over the list, so the force does not depend on the cell layout,
+//             on when the lists were built or on the split into two kernels.  It
 //             writes the element's force row, stores the wavefront's contact
 //             mask words whole (k_strut reads them) and clears the pair and
 //             element bits it used, so both arrays are zero between steps.
@@ -629,13 +631,18 @@ __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
       const bool XisA = sX < sY;
       if (XisA) ncand++;
       if (code >= 0) { // not blacklisted at INIT
-        double pX[2][3], pY[2][3], g0[3], g1[3];
+        double pX[2][3], pY[2][3], g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
         elem_pos(d, xa, xb, pX);
         elem_pos(d, ya, yb, pY);
         const double R = rX + rY;
-        const bool closed = XisA ? cm_narrowphase<false, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                                 : cm_narrowphase<false, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+        const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
+                                 : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
         if (closed) {
+          // the pair's force on X's nodes, summed by kc_vsum in list order
+          double2 *fp = (double2 *)(c.fpair + 6 * q);
+          fp[0] = make_double2(g0[0], g0[1]);
+          fp[1] = make_double2(g0[2], g1[0]);
+          fp[2] = make_double2(g1[1], g1[2]);
           atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
           atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
           if (XisA) nact++;
@@ -654,16 +661,12 @@ __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
   }
 }
 
-// the force row of X from its closed listed pairs, in list order (the pairs
-// kc_vmark marked; every marked pair is closed, so no test is repeated)
+// the force row of X from its closed listed pairs, in list order: the pairs
+// kc_vmark marked, whose forces it stored per entry
 __device__ __forceinline__ void vsum_elem(const ContactDev &c, const DevData &d, int64_t X) {
   const int64_t E = c.E;
-  const int32_t sX = c.es[X];
-  const double rX = c.crad[sX];
-  double pX[2][3];
-  elem_pos(d, c.en0[X], c.en1[X], pX);
   double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0};
-  const int32_t *lst = c.vlist + X * c.vk;
+  const double2 *fp = (const double2 *)(c.fpair + 6 * (int64_t)c.vbase[X]);
   uint32_t *mk = c.vmark + X * (c.vkb / 32);
   const int32_t t1 = c.vcount[X];
   for (int32_t w0 = 0; w0 < t1; w0 += 32) {
@@ -673,13 +676,8 @@ __device__ __forceinline__ void vsum_elem(const ContactDev &c, const DevData &d,
     while (m) {
       const int t = w0 + __ffs(m) - 1;
       m &= m - 1u;
-      const int32_t Y = lst[t];
-      const int32_t sY = c.es[Y];
-      double pY[2][3], g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
-      elem_pos(d, c.en0[Y], c.en1[Y], pY);
-      const double R = rX + c.crad[sY];
-      if (sX < sY) cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1);
-      else cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
+      const double2 a = fp[3 * t], b = fp[3 * t + 1], e = fp[3 * t + 2];
+      const double g0[3] = {a.x, a.y, b.x}, g1[3] = {b.y, e.x, e.y};
       for (int i = 0; i < 3; i++) {
         fa[i] += g0[i];
         fb[i] += g1[i];
@@ -962,6 +960,8 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
       return dl_set_error(DL_ERR_INPUT,
                           "n_elems * (max_pairs / n_elems rounded up to 32) must stay below 2^31");
     if ((rc = calloc_dev(cs, &c.vrec, 8 * (size_t)E * c.vk))) return rc;
+    if ((rc = calloc_dev(cs, &c.vbase, E))) return rc;
+    if ((rc = calloc_dev(cs, &c.fpair, 6 * (size_t)E * c.vk))) return rc;
     if ((rc = calloc_dev(cs, &c.nentries, 1))) return rc;
     if ((rc = calloc_dev(cs, &c.vmark, (size_t)E * (c.vkb / 32)))) return rc;
     if ((rc = calloc_dev(cs, &c.cnew, (E + 63) / 64))) return rc;
