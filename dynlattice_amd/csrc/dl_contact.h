@@ -62,15 +62,15 @@ struct ContactDev {
   int32_t *vcount;      // [E] partners listed
   int32_t *vlist;       // [E][vk] partner ids
   int32_t vk;           // list capacity per element
-  // the step's pair pass: a 32-byte record per list entry (the entries of
-  // one element contiguous, elements in build order): the entry's bit
-  // X * vkb + t, its list code, both struts and both elements' nodes; a bit
-  // per closed listed pair ([E][vkb] bits, vkb = vk rounded up to 32, so an
-  // element owns whole words) and a bit per element with a closed pair
-  int32_t *vrec;        // [E * vk][8] (bit, code, sX, sY, xa, xb, ya, yb)
-  int32_t *vbase;       // [E] an element's first entry record
-  double *fpair;        // [E * vk][6] force of a closed entry on X's nodes a, b
-  int32_t *nentries;    // [1]
+  // the step's pair pass: a 32-byte record per listed pair, on its A side
+  // (the element of the lower strut id): A's entry bit A * vkb + t (~bit
+  // when blacklisted at INIT), B, both struts and both elements' nodes; a
+  // bit per closed list entry ([E][vkb] bits, vkb = vk rounded up to 32, so
+  // an element owns whole words) and a bit per element with a closed pair
+  int32_t *vrec;        // [E * vk][8] (bitA, B, sA, sB, a0, a1, b0, b1)
+  int32_t *vbase;       // [E] an element's first entry in fpair
+  double *fpair;        // [E * vk][6] force of a closed entry on its element's nodes a, b
+  int32_t *nentries;    // [2] list entries, pair records
   int32_t vkb;
   uint32_t *vmark;      // [E * vkb / 32]
   unsigned long long *cnew; // [(E+63)/64]

@@ -25,9 +25,10 @@
 //
 // The explicit step does not run this per step: it keeps per-element
 // candidate lists built by the same broadphase with boxes inflated by a skin
-// (kc_vbuild) and rebuilds them only when a node has moved by nearly the
-// skin (kc_vmark and kc_vsum evaluate the listed pairs; see the list section
-// below).
+// (kc_vbuild) and rebuilds them only when a node has moved by
+// nearly the skin (kc_vmark evaluates each listed pair once for both its
+// elements, kc_vsum sums each element's pairs in list order; see the list
+// section below).
 // The action path (GET_INT_VECTOR, pair listings, blacklist) and the adjoint
 // without lists use the cell scan directly.
 #include <hip/hip_runtime.h>
@@ -249,7 +250,7 @@ __device__ __forceinline__ void place_one(const ContactDev &c, int64_t e) {
 // zero: a counter the list build that follows appends to (or null)
 __global__ __launch_bounds__(CB) void kc_place(ContactDev c, const int32_t *gate, int32_t *zero) {
   GATED(gate);
-  if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+  if (zero && blockIdx.x == 0 && threadIdx.x == 0) zero[0] = zero[1] = 0;
   GRID_STRIDE(e, c.E) place_one(c, e);
 }
 
@@ -563,19 +564,25 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
     if (n > cap) *c.err = DL_ERR_CAPACITY;
     c.vcount[X] = n < cap ? n : cap;
     if (n > 0) {
-      // the entry records of kc_vmark (the order of the elements is irrelevant)
       const int32_t m = n < cap ? n : cap;
-      const int32_t base = atomicAdd(c.nentries, m);
-      c.vbase[X] = base;
-      // the entry's record: its bit, the list code and both elements' nodes
-      // and struts, so that the pair pass starts from one 32-byte load
-      // instead of a chain of lookups
-      const int32_t xa = c.en0[X], xb = c.en1[X], sX = x.s;
-      for (int32_t t = 0; t < m; t++) {
-        const int32_t code = lst[t], Y = list_partner(code);
-        int4 *r = (int4 *)(c.vrec + 8 * (int64_t)(base + t));
-        r[0] = make_int4((int32_t)((int64_t)X * c.vkb + t), code, sX, c.es[Y]);
-        r[1] = make_int4(xa, xb, c.en0[Y], c.en1[Y]);
+      c.vbase[X] = atomicAdd(c.nentries, m); // the element's fpair rows
+      // the pair records of kc_vmark, one per listed pair on its A side (the
+      // element of the lower strut id; the order of the records is
+      // irrelevant): the entry's bit, B, both struts and both elements' nodes
+      const int32_t sX = x.s;
+      int32_t na = 0;
+      for (int32_t t = 0; t < m; t++) na += sX < c.es[list_partner(lst[t])] ? 1 : 0;
+      if (na > 0) {
+        int64_t q = atomicAdd(c.nentries + 1, na);
+        const int32_t xa = c.en0[X], xb = c.en1[X];
+        for (int32_t t = 0; t < m; t++) {
+          const int32_t code = lst[t], Y = list_partner(code), sY = c.es[Y];
+          if (!(sX < sY)) continue;
+          const int32_t bit = (int32_t)((int64_t)X * c.vkb + t);
+          int4 *r = (int4 *)(c.vrec + 8 * q++);
+          r[0] = make_int4(code < 0 ? ~bit : bit, Y, sX, sY);
+          r[1] = make_int4(xa, xb, c.en0[Y], c.en1[Y]);
+        }
       }
     }
   }
@@ -588,14 +595,15 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 
 // ------------------------------------------------------------------
 // The step's pair pass from the lists, in two kernels.
-//   kc_vmark  parallel over list entries (the entry records written by the
-//             list build): the exact boxes of X and of the listed Y, the
-//             candidate and active counts, and for every closed pair its
-//             force on X's two nodes (stored per entry), its bit in vmark
-//             and X's bit in cnew.  A thread takes one entry
-//             at a time, so the work follows the number of entries and not
-//             the longest list, and a light load spreads over as many
-//             wavefronts as it has entries.
+//   kc_vmark  parallel over listed pairs (the records of kc_vbuild, one per
+//             pair, each pair listed on both its elements): the exact boxes
+//             of A and B, the candidate and active counts, and for every
+//             closed pair one evaluation that gives its force on A's two
+//             nodes and on B's, stored in the fpair rows of both entries,
+//             with both entries' bits in vmark and both elements' bits in
+//             cnew.  A thread takes one pair at a time, so the work follows
+//             the number of pairs and not the longest list, and a light load
+//             spreads over as many wavefronts as it has pairs.
 //   kc_vsum   per element with a closed pair (16 elements per lane, the
 //             wavefront owning 1024 elements = 16 mask words): the stored
 //             forces of its closed pairs summed in list order, i.e. by
@@ -606,49 +614,105 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 //             mask words whole (k_strut reads them) and clears the pair and
 //             element bits it used, so both arrays are zero between steps.
 // ------------------------------------------------------------------
+// the force of a closed entry on its element's nodes, stored in the entry's
+// fpair row, and the entry's bit and its element's bit
+__device__ __forceinline__ void vstore(const ContactDev &c, int32_t bit, const double g0[3],
+                                       const double g1[3]) {
+  const int32_t X = bit / c.vkb, t = bit - X * c.vkb;
+  double2 *fp = (double2 *)(c.fpair + 6 * ((int64_t)c.vbase[X] + t));
+  fp[0] = make_double2(g0[0], g0[1]);
+  fp[1] = make_double2(g0[2], g1[0]);
+  fp[2] = make_double2(g1[1], g1[2]);
+  atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
+  atomicOr(&c.cnew[X >> 6], 1ull << (X & 63));
+}
+
+// the pair's force on the nodes of both elements (one evaluation), stored
+// in the fpair rows of both entries; each row is summed by kc_vsum in its
+// element's list order
+__device__ __forceinline__ bool vpair_force(const ContactDev &c, const DevData &d, int64_t q) {
+  const int4 r0 = ((const int4 *)c.vrec)[2 * q], r1 = ((const int4 *)c.vrec)[2 * q + 1];
+  const int32_t bitA = r0.x, B = r0.y; // bitA >= 0: not blacklisted
+  double pA[2][3], pB[2][3];
+  double g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
+  double h0[3] = {0.0, 0.0, 0.0}, h1[3] = {0.0, 0.0, 0.0};
+  elem_pos(d, r1.x, r1.y, pA);
+  elem_pos(d, r1.z, r1.w, pB);
+  const double R = c.crad[r0.z] + c.crad[r0.w];
+  if (!cm_narrowphase<true, 2>(pA, pB, R, c.kS, c.kN, g0, g1, h0, h1)) return false;
+  vstore(c, bitA, g0, g1);
+  // B's entry of the pair: A in B's list, which is sorted by partner id.
+  // The candidate test is symmetric, so B's list holds A unless an overflow
+  // truncated it (reported by the list build).
+  const int32_t A = bitA / c.vkb;
+  const int32_t *lb = c.vlist + (int64_t)B * c.vk;
+  const int32_t mb = c.vcount[B];
+  int32_t lo = 0, hi = mb;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (list_partner(lb[mid]) < A) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < mb && list_partner(lb[lo]) == A) vstore(c, (int32_t)((int64_t)B * c.vkb + lo), h0, h1);
+  return true;
+}
+
+// A block takes 256 records at a time: the box test of each, then the
+// pairs that need the contact evaluation (boxes overlap, not blacklisted)
+// join a queue in LDS, which is evaluated 256 at a time, so that the
+// evaluation runs on full wavefronts although only a fraction of the listed
+// pairs overlap.
 #define VM_GRID 2048
 __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
                                                unsigned long long *cnt_act,
                                                unsigned long long *cnt_cand,
                                                const int32_t *step) {
+  __shared__ int32_t qq[2 * CB];
+  __shared__ int32_t qn;
   const int lane = threadIdx.x & 63;
   const int64_t N = c.N;
-  const int64_t ne = *c.nentries;
+  const int64_t ne = c.nentries[1];
   unsigned long long nact = 0, ncand = 0;
-  for (int64_t q = (int64_t)blockIdx.x * CB + threadIdx.x; q < ne; q += (int64_t)gridDim.x * CB) {
-    const int4 r0 = ((const int4 *)c.vrec)[2 * q], r1 = ((const int4 *)c.vrec)[2 * q + 1];
-    const int64_t bit = r0.x;
-    const int32_t Xq = (int32_t)(bit / c.vkb);
-    const int32_t code = r0.y, sX = r0.z, sY = r0.w;
-    const int32_t xa = r1.x, xb = r1.y, ya = r1.z, yb = r1.w;
-    const double rX = c.crad[sX], rY = c.crad[sY];
-    double xlo[3], xhi[3], ylo[3], yhi[3];
-    elem_box(d, N, xa, xb, rX, xlo, xhi);
-    elem_box(d, N, ya, yb, rY, ylo, yhi);
-    bool ov = true;
-    for (int i = 0; i < 3; i++) ov = ov && (xhi[i] >= ylo[i]) && (yhi[i] >= xlo[i]);
-    if (ov) {
-      const bool XisA = sX < sY;
-      if (XisA) ncand++;
-      if (code >= 0) { // not blacklisted at INIT
-        double pX[2][3], pY[2][3], g0[3] = {0.0, 0.0, 0.0}, g1[3] = {0.0, 0.0, 0.0};
-        elem_pos(d, xa, xb, pX);
-        elem_pos(d, ya, yb, pY);
-        const double R = rX + rY;
-        const bool closed = XisA ? cm_narrowphase<true, 0>(pX, pY, R, c.kS, c.kN, g0, g1)
-                                 : cm_narrowphase<true, 1>(pY, pX, R, c.kS, c.kN, g0, g1);
-        if (closed) {
-          // the pair's force on X's nodes, summed by kc_vsum in list order
-          double2 *fp = (double2 *)(c.fpair + 6 * q);
-          fp[0] = make_double2(g0[0], g0[1]);
-          fp[1] = make_double2(g0[2], g1[0]);
-          fp[2] = make_double2(g1[1], g1[2]);
-          atomicOr(&c.vmark[bit >> 5], 1u << (bit & 31));
-          atomicOr(&c.cnew[Xq >> 6], 1ull << (Xq & 63));
-          if (XisA) nact++;
-        }
+  int32_t n = 0; // queue length, the same in every thread
+  if (threadIdx.x == 0) qn = 0;
+  __syncthreads();
+  for (int64_t q0 = (int64_t)blockIdx.x * CB;; q0 += (int64_t)gridDim.x * CB) {
+    const bool more = q0 < ne; // the same in every thread
+    if (more) {
+      const int64_t q = q0 + threadIdx.x;
+      bool ev = false;
+      if (q < ne) {
+        const int4 r0 = ((const int4 *)c.vrec)[2 * q], r1 = ((const int4 *)c.vrec)[2 * q + 1];
+        const double rA = c.crad[r0.z], rB = c.crad[r0.w];
+        double alo[3], ahi[3], blo[3], bhi[3];
+        elem_box(d, N, r1.x, r1.y, rA, alo, ahi);
+        elem_box(d, N, r1.z, r1.w, rB, blo, bhi);
+        bool ov = true;
+        for (int i = 0; i < 3; i++) ov = ov && (ahi[i] >= blo[i]) && (bhi[i] >= alo[i]);
+        if (ov) ncand++;
+        ev = ov && r0.x >= 0;
       }
+      // append to the queue (n < CB here, so at most 2 CB - 1 entries)
+      const unsigned long long bal = __ballot(ev);
+      int32_t wb = 0;
+      if (lane == 0 && bal) wb = atomicAdd(&qn, __popcll(bal));
+      wb = __shfl(wb, 0, 64);
+      if (ev) qq[n + wb + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)q;
+      __syncthreads();
+      n += qn;
+      __syncthreads();
+      if (threadIdx.x == 0) qn = 0;
     }
+    // a full wave set of pairs, or the rest after the last records
+    if (n >= CB || (!more && n > 0)) {
+      const int32_t k = n < CB ? n : CB;
+      if (threadIdx.x < k && vpair_force(c, d, qq[threadIdx.x])) nact++;
+      __syncthreads();
+      if (threadIdx.x < n - k) qq[threadIdx.x] = qq[k + threadIdx.x];
+      n -= k;
+    }
+    __syncthreads();
+    if (!more) break;
   }
   for (int o = 32; o > 0; o >>= 1) {
     nact += __shfl_xor(nact, o, 64);
@@ -962,10 +1026,10 @@ int contact_create(ContactState &cs, DevData &d, const dl_mesh *mesh, const dl_c
     if ((rc = calloc_dev(cs, &c.vrec, 8 * (size_t)E * c.vk))) return rc;
     if ((rc = calloc_dev(cs, &c.vbase, E))) return rc;
     if ((rc = calloc_dev(cs, &c.fpair, 6 * (size_t)E * c.vk))) return rc;
-    if ((rc = calloc_dev(cs, &c.nentries, 1))) return rc;
+    if ((rc = calloc_dev(cs, &c.nentries, 2))) return rc;
     if ((rc = calloc_dev(cs, &c.vmark, (size_t)E * (c.vkb / 32)))) return rc;
     if ((rc = calloc_dev(cs, &c.cnew, (E + 63) / 64))) return rc;
-    CHK(hipMemset(c.nentries, 0, sizeof(int32_t)));
+    CHK(hipMemset(c.nentries, 0, 2 * sizeof(int32_t)));
     CHK(hipMemset(c.vmark, 0, sizeof(uint32_t) * (size_t)E * (c.vkb / 32)));
     CHK(hipMemset(c.cnew, 0, sizeof(unsigned long long) * ((E + 63) / 64)));
     CHK(hipMemset(c.vcount, 0, sizeof(int32_t) * E));

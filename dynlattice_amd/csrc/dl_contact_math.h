@@ -129,13 +129,57 @@ DL_HD bool cm_nts(const T s[3], const T pM[2][3], const T &uM, const T &R, doubl
   return true;
 }
 
+// One speculative node-to-segment contact of cm_narrowphase: end node NODE
+// of A onto B (FROM_A) or of B onto A, its contributions added to the rows
+// that SIDE selects (see cm_narrowphase).  Returns whether it is closed.
+template <bool F, int SIDE, int NODE, bool FROM_A, typename T>
+DL_HD bool cm_spec_nts(const T pA[2][3], const T pB[2][3], const T &R, double kN, T f0[3],
+                       T f1[3], T *h0, T *h1) {
+  constexpr bool WA = SIDE != 1, WB = SIDE != 0;
+  const T *node = FROM_A ? pA[NODE] : pB[NODE];
+  const T u = FROM_A ? cm_closest_point(node, pB) : cm_closest_point(node, pA);
+  if (!cm_contains(u)) return false;
+  T g[9] = {};
+  if (!(FROM_A ? cm_nts(node, pB, u, R, kN, F ? g : nullptr)
+               : cm_nts(node, pA, u, R, kN, F ? g : nullptr)))
+    return false;
+  if (F)
+    for (int i = 0; i < 3; i++) {
+      // the node's force goes to its own element's row NODE, the segment's
+      // to the other element's two rows
+      if (WA) {
+        if (FROM_A) (NODE == 0 ? f0 : f1)[i] += g[i];
+        else {
+          f0[i] += g[3 + i];
+          f1[i] += g[6 + i];
+        }
+      }
+      if (WB) {
+        if (FROM_A) {
+          (SIDE == 2 ? h0 : f0)[i] += g[3 + i];
+          (SIDE == 2 ? h1 : f1)[i] += g[6 + i];
+        } else if (NODE == 0) (SIDE == 2 ? h0 : f0)[i] += g[i];
+        else (SIDE == 2 ? h1 : f1)[i] += g[i];
+      }
+    }
+  return true;
+}
+
 // computeContacts_ per candidate pair (A, B) after the shared-node and
 // blacklist filters.  Returns whether the contact is closed.  When F is true
 // the translational contributions are added: SIDE 0 -> (A0, A1) into (f0, f1),
-// SIDE 1 -> (B0, B1) into (f0, f1).  All indices are static so nothing spills.
+// SIDE 1 -> (B0, B1) into (f0, f1), SIDE 2 -> (A0, A1) into (f0, f1) and
+// (B0, B1) into (h0, h1).  Each side's sums are the same operations in the
+// same order for every SIDE, so one SIDE 2 evaluation gives both sides the
+// bits of a SIDE 0 and a SIDE 1 evaluation.  All indices are static so
+// nothing spills.
 template <bool F, int SIDE, typename T>
 DL_HD bool cm_narrowphase(const T pA[2][3], const T pB[2][3], const T &R, double kS, double kN,
-                          T f0[3], T f1[3]) {
+                          T f0[3], T f1[3], T *h0 = nullptr, T *h1 = nullptr) {
+  constexpr bool WA = SIDE != 1, WB = SIDE != 0; // the sides written
+  // B's rows: (h0, h1) for SIDE 2, else (f0, f1)
+#define CM_B0(i) (SIDE == 2 ? h0[i] : f0[i])
+#define CM_B1(i) (SIDE == 2 ? h1[i] : f1[i])
   T uA, uB;
   cm_closest_points(pA, pB, &uA, &uB);
   const bool inA = cm_contains(uA), inB = cm_contains(uB);
@@ -144,8 +188,14 @@ DL_HD bool cm_narrowphase(const T pA[2][3], const T pB[2][3], const T &R, double
     if (!cm_sts(pA, pB, uA, uB, R, kS, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
-        f0[i] += g[6 * SIDE + i];
-        f1[i] += g[6 * SIDE + 3 + i];
+        if (WA) {
+          f0[i] += g[i];
+          f1[i] += g[3 + i];
+        }
+        if (WB) {
+          CM_B0(i) += g[6 + i];
+          CM_B1(i) += g[9 + i];
+        }
       }
     return true;
   }
@@ -162,12 +212,13 @@ DL_HD bool cm_narrowphase(const T pA[2][3], const T pB[2][3], const T &R, double
     if (!cm_nts(sA, pB, u, R, kN, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
-        if (SIDE == 0) {
+        if (WA) {
           f0[i] += i1 ? T(0.0) : g[i];
           f1[i] += i1 ? g[i] : T(0.0);
-        } else {
-          f0[i] += g[3 + i];
-          f1[i] += g[6 + i];
+        }
+        if (WB) {
+          CM_B0(i) += g[3 + i];
+          CM_B1(i) += g[6 + i];
         }
       }
     return true;
@@ -182,55 +233,26 @@ DL_HD bool cm_narrowphase(const T pA[2][3], const T pB[2][3], const T &R, double
     if (!cm_nts(sB, pA, u, R, kN, F ? g : nullptr)) return false;
     if (F)
       for (int i = 0; i < 3; i++) {
-        if (SIDE == 1) {
-          f0[i] += i1 ? T(0.0) : g[i];
-          f1[i] += i1 ? g[i] : T(0.0);
-        } else {
+        if (WB) {
+          CM_B0(i) += i1 ? T(0.0) : g[i];
+          CM_B1(i) += i1 ? g[i] : T(0.0);
+        }
+        if (WA) {
           f0[i] += g[3 + i];
           f1[i] += g[6 + i];
         }
       }
     return true;
   }
-  // speculative NTS from all four end nodes
-  bool closed = false;
-#pragma unroll
-  for (int iA = 0; iA < 2; iA++) {
-    const T u = cm_closest_point(pA[iA], pB);
-    if (!cm_contains(u)) continue;
-    T g[9] = {};
-    if (cm_nts(pA[iA], pB, u, R, kN, F ? g : nullptr)) {
-      if (F)
-        for (int i = 0; i < 3; i++) {
-          if (SIDE == 0) {
-            if (iA == 0) f0[i] += g[i]; else f1[i] += g[i];
-          } else {
-            f0[i] += g[3 + i];
-            f1[i] += g[6 + i];
-          }
-        }
-      closed = true;
-    }
-  }
-#pragma unroll
-  for (int iB = 0; iB < 2; iB++) {
-    const T u = cm_closest_point(pB[iB], pA);
-    if (!cm_contains(u)) continue;
-    T g[9] = {};
-    if (cm_nts(pB[iB], pA, u, R, kN, F ? g : nullptr)) {
-      if (F)
-        for (int i = 0; i < 3; i++) {
-          if (SIDE == 0) {
-            f0[i] += g[3 + i];
-            f1[i] += g[6 + i];
-          } else {
-            if (iB == 0) f0[i] += g[i]; else f1[i] += g[i];
-          }
-        }
-      closed = true;
-    }
-  }
+  // speculative NTS from all four end nodes, A's then B's (the order of
+  // the sums)
+  bool closed = cm_spec_nts<F, SIDE, 0, true>(pA, pB, R, kN, f0, f1, h0, h1);
+  closed = cm_spec_nts<F, SIDE, 1, true>(pA, pB, R, kN, f0, f1, h0, h1) || closed;
+  closed = cm_spec_nts<F, SIDE, 0, false>(pA, pB, R, kN, f0, f1, h0, h1) || closed;
+  closed = cm_spec_nts<F, SIDE, 1, false>(pA, pB, R, kN, f0, f1, h0, h1) || closed;
   return closed;
+#undef CM_B0
+#undef CM_B1
 }
 
 #endif
