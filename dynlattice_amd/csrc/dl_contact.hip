@@ -661,8 +661,11 @@ __device__ __forceinline__ bool vpair_force(const ContactDev &c, const DevData &
 // pairs that need the contact evaluation (boxes overlap, not blacklisted)
 // join a queue in LDS, which is evaluated 256 at a time, so that the
 // evaluation runs on full wavefronts although only a fraction of the listed
-// pairs overlap.
-#define VM_GRID 2048
+// pairs overlap.  512 blocks (two wavefronts per SIMD) loop over the
+// records: on the 40 %-strain crush state (1.9 M records) 512 blocks ran the
+// step 8 % faster than 2,048 and 2-5 % faster than 1,024 or 256, and it
+// matched 2,048 on the light load at 20 % strain.
+#define VM_GRID 512
 __global__ __launch_bounds__(CB) void kc_vmark(ContactDev c, DevData d,
                                                unsigned long long *cnt_act,
                                                unsigned long long *cnt_cand,
