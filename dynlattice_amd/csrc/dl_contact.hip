@@ -604,8 +604,8 @@ __global__ __launch_bounds__(CB) void kc_vbuild(ContactDev c, DevData d, const i
 //             cnew.  A thread takes one pair at a time, so the work follows
 //             the number of pairs and not the longest list, and a light load
 //             spreads over as many wavefronts as it has pairs.
-//   kc_vsum   per element with a closed pair (16 elements per lane, the
-//             wavefront owning 1024 elements = 16 mask words): the stored
+//   kc_vsum   per element with a closed pair (4 elements per lane, the
+//             wavefront owning 256 elements = 4 mask words): the stored
 //             forces of its closed pairs summed in list order, i.e. by
 //             partner id -- the operations and the order of one serial pass
 //             over the list, so the force does not depend on the cell layout,
@@ -757,7 +757,9 @@ __device__ __forceinline__ void vsum_elem(const ContactDev &c, const DevData &d,
   }
 }
 
-#define VN_PER 16
+// elements per lane: 4 ran the 40 %-strain crush state 1 % faster than 8
+// and 16, and 32 slower; the same at 20 %
+#define VN_PER 4
 __global__ __launch_bounds__(CB) void kc_vsum(ContactDev c, DevData d) {
   __shared__ int32_t qe[CB / 64][64 * VN_PER];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -769,7 +771,7 @@ __global__ __launch_bounds__(CB) void kc_vsum(ContactDev c, DevData d) {
   if (T == 0 && *c.err != DL_ERR_CAPACITY) *c.gate = 0;
   const int64_t xw = (T - lane) * VN_PER; // the wavefront's first element
   const int64_t nw = (c.E + 63) / 64;
-  // the wavefront's 16 mask words: lane k < 16 holds word k
+  // the wavefront's VN_PER mask words: lane k < VN_PER holds word k
   unsigned long long word = 0ull;
   const int64_t wi = (xw >> 6) + lane;
   if (lane < VN_PER && wi < nw) word = c.cnew[wi];
